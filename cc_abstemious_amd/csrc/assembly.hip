@@ -1,0 +1,359 @@
+// Influence-matrix assembly on gfx950.
+//
+// Implements the Helmholtz kernel of influence_matrix.rs:12-60 (one-point
+// centroid collocation, the flat-disk singular diagonal, Burton-Miller
+// coupling with the reference's dh sign at influence_matrix.rs:30) and the
+// dense assembly of influence_matrix.rs:64-98, fused with the BC algebra of
+// solve.rs:24-71.
+//
+// Layout: matrices are column-major (entry (i,j) at i + j*ld).  A workgroup
+// of 256 lanes owns 256 consecutive rows i and a chunk of CJ columns j; each
+// lane keeps its row's collocation point in registers and walks the columns,
+// whose geometry is wave-uniform (scalar loads).  Every store of a wavefront
+// is 64 consecutive entries of one column: 512 B (c64) / 1 KiB (c128), full
+// 128-byte lines.  The kernel is HBM-store bound: 8 B (c64) or 16 B (c128)
+// per stored entry and matrix.
+#include "ccab_internal.h"
+
+namespace {
+
+constexpr int kRows = 256;  // rows per workgroup (4 wavefronts)
+constexpr int kColChunk = 64;
+
+struct Soa {
+    const double *x, *y, *z, *nx, *ny, *nz, *area, *wt;
+};
+
+__device__ __forceinline__ Soa soa_of(const double* base, int64_t n) {
+    return Soa{base, base + n, base + 2 * n, base + 3 * n,
+               base + 4 * n, base + 5 * n, base + 6 * n, base + 7 * n};
+}
+
+__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+__device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
+__device__ __forceinline__ float sqrt_t(float a) { return sqrtf(a); }
+
+// Off-diagonal kernel pair (g, h) for source point j (x, n_x) and collocation
+// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.
+template <typename T, bool BM>
+__device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx, T nxy, T nxz,
+                                       T nyx, T nyy, T nyz, Cx<T>& g, Cx<T>& h) {
+    const T rd = sqrt_t(rx * rx + ry * ry + rz * rz);
+    const T inv_r = T(1) / rd;
+    const T erx = rx * inv_r, ery = ry * inv_r, erz = rz * inv_r;
+    T s, c;
+    sincos_t(k * rd, &s, &c);
+    const T gs = inv_r * T(1.0 / (4.0 * kPi));
+    g = Cx<T>{c * gs, s * gs};
+    const Cx<T> f1{-inv_r, k};
+    const T rdoty = erx * nyx + ery * nyy + erz * nyz;
+    const Cx<T> gf1 = g * f1;
+    h = gf1 * (-rdoty);
+    if (BM) {
+        const T rdotx = erx * nxx + ery * nxy + erz * nxz;
+        const Cx<T> dg = gf1 * rdotx;
+        const T nn = nyx * nxx + nyy * nxy + nyz * nxz;
+        const T rr = rdotx * rdoty;
+        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)
+        const Cx<T> term{-f1.re * nn - (k * k * rd + T(3) * f1.re) * rr,
+                         -f1.im * nn - T(3) * f1.im * rr};
+        const Cx<T> dh = (g * inv_r) * term;
+        g += beta * dg;
+        h += beta * dh;
+    }
+}
+
+// Singular diagonal (influence_matrix.rs:42-50) incl. the global shifts of
+// get_dense_surface_matrices (influence_matrix.rs:78-79).  Always FP64.
+__device__ __forceinline__ void diag_entry(const SysParams& p, double area, double wt, c128& g,
+                                           c128& h) {
+    const double b = sqrt(area / kPi);
+    const double wa = wt * area;
+    g = c128{1.0 / (2.0 * kPi * b) * wa, 0.0};
+    h = c128{0.0, 0.0};
+    if (p.bm) {
+        const double s = -1.0 / (2.0 * kPi * (b * b * b));
+        h = c128{p.beta.re * s * wa, p.beta.im * s * wa};
+    }
+    g += p.gdiag;
+    h += p.hdiag;
+}
+
+template <typename T>
+__device__ __forceinline__ void store(void* base, int64_t off, c128 v) {
+    if constexpr (sizeof(T) == 16) {
+        reinterpret_cast<c128*>(base)[off] = v;
+    } else {
+        reinterpret_cast<c64*>(base)[off] = c64{static_cast<float>(v.re), static_cast<float>(v.im)};
+    }
+}
+
+// MODE ASM_HG: out0 = H, out1 = G (either may be null), element type OutT.
+// MODE ASM_LHS: out0 = LHS c128 (or null), out1 = LHS c64 (or null).
+template <typename T, bool BM, int MODE, typename OutT>
+__global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __restrict__ params,
+                                                         const double* __restrict__ soa_base,
+                                                         int64_t n, void* out0, void* out1,
+                                                         int64_t ld, int64_t stride,
+                                                         double* __restrict__ rowsum_part,
+                                                         int64_t nchunks) {
+    const int64_t b = blockIdx.z;
+    const SysParams& p = params[b];
+    const Soa s = soa_of(soa_base, n);
+    const int64_t i = int64_t(blockIdx.x) * kRows + threadIdx.x;
+    const int64_t j0 = int64_t(blockIdx.y) * kColChunk;
+    const int64_t j1 = min(j0 + kColChunk, n);
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    const double yx = s.x[ic], yy = s.y[ic], yz = s.z[ic];
+    const T nyx = T(s.nx[ic]), nyy = T(s.ny[ic]), nyz = T(s.nz[ic]);
+    const T k = T(p.k);
+    const Cx<T> beta{T(p.beta.re), T(p.beta.im)};
+    const int64_t sys_off = b * stride;
+    double rs_re = 0.0, rs_im = 0.0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double wa = s.wt[j] * s.area[j];
+        c128 G, H;
+        if (j == ic) {
+            diag_entry(p, s.area[j], s.wt[j], G, H);
+        } else {
+            Cx<T> g, h;
+            greens<T, BM>(k, beta, T(s.x[j] - yx), T(s.y[j] - yy), T(s.z[j] - yz), T(s.nx[j]),
+                          T(s.ny[j]), T(s.nz[j]), nyx, nyy, nyz, g, h);
+            G = c128{double(g.re) * wa, double(g.im) * wa};
+            H = c128{double(h.re) * wa, double(h.im) * wa};
+        }
+        if (!active) continue;
+        const int64_t off = sys_off + i + j * ld;
+        if constexpr (MODE == ASM_HG) {
+            if (out0) store<OutT>(out0, off, H);
+            if (out1) store<OutT>(out1, off, G);
+        } else {
+            const c128 L = p.lhs_a * H + p.lhs_b * G;
+            if (out0) store<c128>(out0, off, L);
+            if (out1) store<c64>(out1, off, L);
+            if (p.rowsum_of == 1) {
+                rs_re += G.re;
+                rs_im += G.im;
+            } else if (p.rowsum_of == 2) {
+                rs_re += H.re;
+                rs_im += H.im;
+            }
+        }
+    }
+    if (MODE == ASM_LHS && rowsum_part != nullptr && active && p.rowsum_of != 0) {
+        double* dst = rowsum_part + 2 * ((b * nchunks + blockIdx.y) * n + i);
+        dst[0] = rs_re;
+        dst[1] = rs_im;
+    }
+}
+
+// rhs = rhs_inc_sign * rhs_inc + rhs_scale * rowsum   (deterministic chunk order)
+__global__ void rhs_finish_kernel(const SysParams* __restrict__ params, int64_t n,
+                                  const double* __restrict__ part, int64_t nchunks,
+                                  const ccab_c128* __restrict__ rhs_inc, ccab_c128* rhs) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SysParams& p = params[b];
+    c128 acc{0.0, 0.0};
+    if (p.rowsum_of != 0 && part != nullptr) {
+        for (int64_t c = 0; c < nchunks; ++c) {
+            const double* src = part + 2 * ((b * nchunks + c) * n + i);
+            acc.re += src[0];
+            acc.im += src[1];
+        }
+        acc = p.rhs_scale * acc;
+    }
+    const ccab_c128 ri = rhs_inc[b * n + i];
+    rhs[b * n + i] = ccab_c128{acc.re + p.rhs_inc_sign * ri.re, acc.im + p.rhs_inc_sign * ri.im};
+}
+
+__global__ void cpts_to_soa_kernel(const double* __restrict__ aos, int64_t n, double* soa) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) soa[f * n + i] = aos[8 * i + f];
+}
+
+// incident_wave.rs:7-63
+__global__ void incident_kernel(SysParams p, int kind, double d0, double d1, double d2,
+                                c128 amp, const double* __restrict__ soa_base, int64_t n,
+                                ccab_c128* rhs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Soa s = soa_of(soa_base, n);
+    const double cx_ = s.x[i], cy = s.y[i], cz = s.z[i];
+    c128 out;
+    if (kind == CCAB_WAVE_PLANE) {
+        double sn, cs;
+        sincos(p.k * (d0 * cx_ + d1 * cy + d2 * cz), &sn, &cs);
+        const c128 phi = amp * c128{cs, sn};
+        out = phi;
+        if (p.bm) {
+            const double dn = d0 * s.nx[i] + d1 * s.ny[i] + d2 * s.nz[i];
+            const c128 vn = (phi * c128{0.0, p.k}) * dn;
+            out += p.beta * vn;
+        }
+    } else {
+        const double rx = cx_ - d0, ry = cy - d1, rz = cz - d2;
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        double sn, cs;
+        sincos(p.k * r, &sn, &cs);
+        const double inv = 1.0 / (4.0 * kPi * r);
+        const c128 phi = (amp * c128{cs, sn}) * inv;
+        out = phi;
+        if (p.bm) {
+            const double en = (rx * s.nx[i] + ry * s.ny[i] + rz * s.nz[i]) / r;
+            const c128 vn = (phi * c128{-1.0 / r, p.k}) * en;
+            out += p.beta * vn;
+        }
+    }
+    rhs[i] = ccab_c128{out.re, out.im};
+}
+
+// influence_matrix.rs:170-231.  Diagonal shift at the first position t whose
+// (i, j) pair is diagonal.
+template <bool BM>
+__global__ void rowcol_kernel(SysParams p, const double* __restrict__ soa_base, int64_t n,
+                              const int64_t* __restrict__ di, int64_t ni,
+                              const int64_t* __restrict__ dj, int64_t nj, int side,
+                              ccab_c128* out) {
+    const int64_t cnt = ni == 1 ? nj : ni;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    const Soa s = soa_of(soa_base, n);
+    const int64_t i = ni == 1 ? di[0] : di[t];
+    const int64_t j = ni == 1 ? dj[t] : dj[0];
+    c128 G, H;
+    if (i == j) {
+        // values without the shift, then shift only at the first diagonal hit
+        diag_entry(p, s.area[j], s.wt[j], G, H);
+        G -= p.gdiag;
+        H -= p.hdiag;
+        bool first = true;
+        for (int64_t u = 0; u < t; ++u) {
+            const int64_t iu = ni == 1 ? di[0] : di[u];
+            const int64_t ju = ni == 1 ? dj[u] : dj[0];
+            if (iu == ju) {
+                first = false;
+                break;
+            }
+        }
+        if (first) {
+            G += p.gdiag;
+            H += p.hdiag;
+        }
+    } else {
+        c128 g, h;
+        greens<double, BM>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
+                           s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);
+        const double wa = s.wt[j] * s.area[j];
+        G = g * wa;
+        H = h * wa;
+    }
+    const c128 a = side == CCAB_LHS ? p.lhs_a : p.rhs_a;
+    const c128 bb = side == CCAB_LHS ? p.lhs_b : p.rhs_b;
+    const c128 v = bb * G + a * H;
+    out[t] = ccab_c128{v.re, v.im};
+}
+
+template <typename T, bool BM, int MODE, typename OutT>
+void launch_one(dim3 grid, hipStream_t st, const SysParams* d_params, const double* soa,
+                int64_t n, void* o0, void* o1, int64_t ld, int64_t stride, double* part,
+                int64_t nchunks) {
+    hipLaunchKernelGGL((assemble_kernel<T, BM, MODE, OutT>), grid, dim3(kRows), 0, st, d_params,
+                       soa, n, o0, o1, ld, stride, part, nchunks);
+}
+
+}  // namespace
+
+int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d_soa) {
+    hipLaunchKernelGGL(cpts_to_soa_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                       d_cpts, n, d_soa);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
+                    int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
+                    void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
+                    int64_t* nchunks_out) {
+    const int64_t nchunks = (n + kColChunk - 1) / kColChunk;
+    if (nchunks_out) *nchunks_out = nchunks;
+    CCAB_CHECK(batch >= 1 && batch <= 65535, "batch out of range");
+    const bool bm = h_params[0].bm != 0;
+    for (int64_t b = 1; b < batch; ++b)
+        CCAB_CHECK((h_params[b].bm != 0) == bm, "all systems of a batch share one method");
+    dim3 grid((unsigned)((n + kRows - 1) / kRows), (unsigned)nchunks, (unsigned)batch);
+    hipStream_t st = ctx->stream;
+    double bytes_per_entry = 0.0;
+    if (mode == ASM_HG) {
+        const double e = prec_compute == PREC_F64 ? 16.0 : 8.0;
+        bytes_per_entry = (out0 ? e : 0.0) + (out1 ? e : 0.0);
+    } else {
+        bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? 8.0 : 0.0);
+    }
+    ProfScope ps(ctx, PROF_ASSEMBLY, double(n) * double(n) * double(batch) * bytes_per_entry);
+    if (mode == ASM_HG) {
+        if (prec_compute == PREC_F64) {
+            if (bm) launch_one<double, true, ASM_HG, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);
+            else launch_one<double, false, ASM_HG, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);
+        } else {
+            if (bm) launch_one<float, true, ASM_HG, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);
+            else launch_one<float, false, ASM_HG, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);
+        }
+    } else {
+        if (prec_compute == PREC_F32) {
+            CCAB_CHECK(out0 == nullptr, "FP32 assembly writes only the c64 LHS");
+            if (bm) launch_one<float, true, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+            else launch_one<float, false, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+        } else {
+            if (bm) launch_one<double, true, ASM_LHS, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+            else launch_one<double, false, ASM_LHS, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+        }
+    }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, int64_t n,
+                      const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
+                      ccab_c128* d_rhs) {
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)batch);
+    hipLaunchKernelGGL(rhs_finish_kernel, grid, dim3(256), 0, ctx->stream, d_params, n,
+                       d_rowsum_part, nchunks, d_rhs_inc, d_rhs);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_incident(ccab_ctx* ctx, const SysParams* hp, const ccab_physics& ph,
+                    const ccab_wave& w, const double* d_soa, int64_t n, ccab_c128* d_rhs) {
+    double d[3] = {w.vec[0], w.vec[1], w.vec[2]};
+    const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(nn > 0.0)) return ctx_fail(ctx, CCAB_ENAN, "incident wave vector has zero length");
+    for (double& v : d) v /= nn;  // normalize(); spherical origin moves to the unit sphere
+    const c128 pamp{w.amplitude[0], w.amplitude[1]};
+    const c128 amp = cdiv(pamp, c128{0.0, hp->omega * ph.mass_density});
+    hipLaunchKernelGGL(incident_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, *hp,
+                       w.kind, d[0], d[1], d[2], amp, d_soa, n, d_rhs);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_rowcol(ccab_ctx* ctx, const SysParams& p, const double* d_soa, int64_t n,
+                  const int64_t* d_i, int64_t ni, const int64_t* d_j, int64_t nj, int side,
+                  ccab_c128* d_out) {
+    const int64_t cnt = ni == 1 ? nj : ni;
+    if (cnt == 0) return CCAB_OK;
+    dim3 grid((unsigned)((cnt + 255) / 256));
+    if (p.bm)
+        hipLaunchKernelGGL(rowcol_kernel<true>, grid, dim3(256), 0, ctx->stream, p, d_soa, n, d_i,
+                           ni, d_j, nj, side, d_out);
+    else
+        hipLaunchKernelGGL(rowcol_kernel<false>, grid, dim3(256), 0, ctx->stream, p, d_soa, n,
+                           d_i, ni, d_j, nj, side, d_out);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}

@@ -1,0 +1,201 @@
+// Level-1/2 kernels of the solvers (FP64 accumulation throughout):
+// batched GEMV over a c128 or c64 column-major matrix (the GMRES matvec of
+// gmres.rs:64-69 and the FP64 refinement residual), vector norms, the
+// Frobenius norm of gmres.rs:75-79, precision conversions and the
+// modified Gram-Schmidt step of gmres.rs:158-176.
+//
+// The GEMV is HBM-bound: n*n*sizeof(entry) bytes per product.  A workgroup
+// of 256 lanes owns 256 rows; lanes walk a chunk of columns whose x[j] is
+// wave-uniform.  Column chunks are spread over grid.y so that even a single
+// system fills the 256 CUs; partial sums are combined in fixed order.
+#include "ccab_internal.h"
+
+namespace {
+
+template <typename TA>
+__device__ __forceinline__ c128 ld_c(const TA* p, int64_t off);
+template <>
+__device__ __forceinline__ c128 ld_c<c128>(const c128* p, int64_t off) {
+    return p[off];
+}
+template <>
+__device__ __forceinline__ c128 ld_c<c64>(const c64* p, int64_t off) {
+    const c64 v = p[off];
+    return c128{double(v.re), double(v.im)};
+}
+
+template <typename TA>
+__global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, const TA* __restrict__ A,
+                                                        int64_t lda, int64_t sA,
+                                                        const c128* __restrict__ x, int64_t sx,
+                                                        int64_t chunk, c128* __restrict__ part,
+                                                        int64_t nsplit) {
+    const int64_t b = blockIdx.z;
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t j0 = int64_t(blockIdx.y) * chunk;
+    const int64_t j1 = min(j0 + chunk, N);
+    if (i >= M) return;
+    const TA* a = A + b * sA + i;
+    const c128* xb = x + b * sx;
+    double re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+    int64_t j = j0;
+    for (; j + 1 < j1; j += 2) {
+        const c128 a0 = ld_c(a, j * lda), a1 = ld_c(a, (j + 1) * lda);
+        const c128 x0 = xb[j], x1 = xb[j + 1];
+        re0 += a0.re * x0.re - a0.im * x0.im;
+        im0 += a0.re * x0.im + a0.im * x0.re;
+        re1 += a1.re * x1.re - a1.im * x1.im;
+        im1 += a1.re * x1.im + a1.im * x1.re;
+    }
+    if (j < j1) {
+        const c128 a0 = ld_c(a, j * lda);
+        const c128 x0 = xb[j];
+        re0 += a0.re * x0.re - a0.im * x0.im;
+        im0 += a0.re * x0.im + a0.im * x0.re;
+    }
+    part[(b * nsplit + blockIdx.y) * M + i] = c128{re0 + re1, im0 + im1};
+}
+
+__global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int64_t nsplit,
+                                   c128 alpha, c128 beta, c128* y, int64_t sy) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    c128 s{0.0, 0.0};
+    for (int64_t p = 0; p < nsplit; ++p) s += part[(b * nsplit + p) * M + i];
+    c128* yy = y + b * sy + i;
+    const bool zero_beta = beta.re == 0.0 && beta.im == 0.0;
+    *yy = alpha * s + (zero_beta ? c128{0.0, 0.0} : beta * *yy);
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < BS / 64; ++w) s += red[w];
+    return s;
+}
+
+__global__ __launch_bounds__(256) void nrm2_kernel(int64_t n, const c128* __restrict__ v,
+                                                   int64_t stride, double* out) {
+    __shared__ double red[4];
+    const c128* p = v + blockIdx.x * stride;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += p[i].re * p[i].re + p[i].im * p[i].im;
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(s);
+}
+
+template <typename TA>
+__global__ __launch_bounds__(256) void sumsq_kernel(int64_t n, const TA* __restrict__ A,
+                                                    int64_t lda, double* part) {
+    __shared__ double red[4];
+    // one column per block row
+    const int64_t j = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const c128 z = ld_c(A, i + j * lda);
+        s += z.re * z.re + z.im * z.im;
+    }
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) part[j] = s;
+}
+
+__global__ __launch_bounds__(256) void sum_sqrt_kernel(int64_t n, const double* part, double* out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) *out = sqrt(s);
+}
+
+__global__ void c128_to_c64_kernel(int64_t count, const c128* __restrict__ s, c64* __restrict__ d) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) d[i] = c64{float(s[i].re), float(s[i].im)};
+}
+__global__ void c64_to_c128_kernel(int64_t count, const c64* __restrict__ s, c128* __restrict__ d) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < count) d[i] = c128{double(s[i].re), double(s[i].im)};
+}
+
+}  // namespace
+
+size_t gemv_workspace_bytes(int64_t n, int64_t batch) { return 0; }
+
+int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
+                 int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
+                 c128 beta, c128* d_y, int64_t stridey) {
+    if (M <= 0 || batch <= 0) return CCAB_OK;
+    const int64_t rowblocks = (M + 255) / 256;
+    int64_t nsplit = (1024 + rowblocks * batch - 1) / (rowblocks * batch);
+    nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
+    const int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
+    nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;
+    c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(batch) * nsplit * M * 16));
+    if (!part) return CCAB_ENOMEM;
+    dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)batch);
+    ProfScope ps(ctx, PROF_GEMV,
+                 double(M) * double(N) * double(batch) * (prec == CCAB_C128 ? 16.0 : 8.0));
+    if (N > 0) {
+        if (prec == CCAB_C128)
+            hipLaunchKernelGGL(gemv_part_kernel<c128>, grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit);
+        else
+            hipLaunchKernelGGL(gemv_part_kernel<c64>, grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit);
+    } else {
+        nsplit = 0;
+    }
+    hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)rowblocks, (unsigned)batch), dim3(256),
+                       0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_A, int64_t lda,
+                 int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha, c128 beta,
+                 c128* d_y, int64_t stridey) {
+    return gemv_general(ctx, prec, n, n, batch, d_A, lda, strideA, d_x, stridex, alpha, beta, d_y,
+                        stridey);
+}
+
+int nrm2_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* d_v, int64_t stride,
+                 double* d_out) {
+    hipLaunchKernelGGL(nrm2_kernel, dim3((unsigned)batch), dim3(256), 0, ctx->stream, n, d_v,
+                       stride, d_out);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int frob_norm(ccab_ctx* ctx, int prec, int64_t n, const void* d_A, int64_t lda, double* d_out) {
+    double* part = static_cast<double*>(ctx_buf(ctx, SLOT_NORM, size_t(n + 1) * 8));
+    if (!part) return CCAB_ENOMEM;
+    if (prec == CCAB_C128)
+        hipLaunchKernelGGL(sumsq_kernel<c128>, dim3((unsigned)n), dim3(256), 0, ctx->stream, n,
+                           static_cast<const c128*>(d_A), lda, part);
+    else
+        hipLaunchKernelGGL(sumsq_kernel<c64>, dim3((unsigned)n), dim3(256), 0, ctx->stream, n,
+                           static_cast<const c64*>(d_A), lda, part);
+    hipLaunchKernelGGL(sum_sqrt_kernel, dim3(1), dim3(256), 0, ctx->stream, n, part, d_out);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int convert_c128_to_c64(ccab_ctx* ctx, int64_t count, const c128* src, c64* dst) {
+    hipLaunchKernelGGL(c128_to_c64_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                       ctx->stream, count, src, dst);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+int convert_c64_to_c128(ccab_ctx* ctx, int64_t count, const c64* src, c128* dst) {
+    hipLaunchKernelGGL(c64_to_c128_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                       ctx->stream, count, src, dst);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}

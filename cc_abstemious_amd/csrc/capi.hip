@@ -1,0 +1,608 @@
+// extern "C" entry points of libccab (declared in include/ccab.h).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "ccab_internal.h"
+
+int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, c128* d_x,
+              const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st);
+int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A64, int64_t lda,
+              int64_t strideA, c128* d_x, double ir_tol, int ir_max, ccab_solve_stats* st);
+int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const void* A,
+                 int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc);
+
+// ---------------------------------------------------------------------------
+// context plumbing
+// ---------------------------------------------------------------------------
+int ctx_fail(ccab_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes) {
+    if (slot < 0 || slot >= SLOT_COUNT) return nullptr;
+    bytes = ws_align(std::max<size_t>(bytes, 256));
+    if (ctx->buf_bytes[slot] >= bytes) return ctx->buf[slot];
+    if (ctx->buf[slot]) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipFree(ctx->buf[slot]);
+        ctx->buf[slot] = nullptr;
+        ctx->buf_bytes[slot] = 0;
+        if (slot == SLOT_LU_LINV || slot == SLOT_LU_UINV || slot == SLOT_LU_PERM)
+            ctx->lu_key = nullptr;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx_fail(ctx, CCAB_ENOMEM, "hipMalloc of %zu bytes failed (slot %d)", bytes, slot);
+        return nullptr;
+    }
+    ctx->buf[slot] = p;
+    ctx->buf_bytes[slot] = bytes;
+    return p;
+}
+
+int pinned_reserve(ccab_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes >= bytes) return CCAB_OK;
+    if (ctx->pinned) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+    }
+    bytes = std::max<size_t>(bytes, 1 << 16);
+    CCAB_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
+    ctx->pinned_bytes = bytes;
+    return CCAB_OK;
+}
+
+static hipEvent_t pool_get(ccab_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void prof_begin(ccab_ctx* ctx, hipEvent_t* ev) {
+    *ev = pool_get(ctx);
+    if (*ev) (void)hipEventRecord(*ev, ctx->stream);
+}
+
+void prof_end(ccab_ctx* ctx, hipEvent_t a, int cls, double work) {
+    hipEvent_t b = pool_get(ctx);
+    if (!b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->prof_pending.push_back({a, b, cls, work});
+}
+
+static void prof_resolve(ccab_ctx* ctx) {
+    if (ctx->prof_pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& r : ctx->prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess && r.cls >= 0 && r.cls < 16) {
+            ctx->prof_ms[r.cls] += ms;
+            ctx->prof_work[r.cls] += r.work;
+            ctx->prof_count[r.cls] += 1;
+        }
+        ctx->event_pool.push_back(r.a);
+        ctx->event_pool.push_back(r.b);
+    }
+    ctx->prof_pending.clear();
+}
+
+SysParams make_params(const ccab_physics& ph) {
+    SysParams p{};
+    p.omega = 2.0 * kPi * ph.frequency;
+    p.k = p.omega / ph.sound_speed;
+    p.beta = c128{0.0, 1.0 / p.k};
+    p.bm = ph.method == CCAB_BURTON_MILLER ? 1 : 0;
+    p.hdiag = ph.problem_type == CCAB_EXTERIOR ? c128{-0.5, 0.0} : c128{0.0, 0.0};
+    p.gdiag = p.bm ? c128{0.5 * p.beta.re, 0.5 * p.beta.im} : c128{0.0, 0.0};
+    p.bcv = c128{ph.bc_value[0], ph.bc_value[1]};
+    const bool bc_zero = ph.bc_value[0] == 0.0 && ph.bc_value[1] == 0.0;
+    switch (ph.bc_type) {
+    case CCAB_BC_PRESSURE:
+        p.lhs_a = {0, 0}; p.lhs_b = {1, 0}; p.rhs_a = {-1, 0}; p.rhs_b = {0, 0};
+        p.rowsum_of = bc_zero ? 0 : 2;
+        p.rhs_scale = c128{-p.bcv.re, -p.bcv.im};
+        p.rhs_inc_sign = 1.0;
+        break;
+    case CCAB_BC_NORMAL_VELOCITY:
+        p.lhs_a = {1, 0}; p.lhs_b = {0, 0}; p.rhs_a = {0, 0}; p.rhs_b = {1, 0};
+        p.rowsum_of = bc_zero ? 0 : 1;
+        p.rhs_scale = p.bcv;
+        p.rhs_inc_sign = -1.0;
+        break;
+    default:  // impedance: (H + f G) phi = -rhs_inc, f = -i w rho / Z
+        p.lhs_a = {1, 0};
+        p.lhs_b = cdiv(c128{0.0, -p.omega * ph.mass_density}, p.bcv);
+        p.rhs_a = {0, 0}; p.rhs_b = {0, 0};
+        p.rowsum_of = 0;
+        p.rhs_scale = {0, 0};
+        p.rhs_inc_sign = -1.0;
+    }
+    return p;
+}
+
+int check_physics(ccab_ctx* ctx, const ccab_physics& ph) {
+    CCAB_CHECK(ph.frequency > 0.0 && std::isfinite(ph.frequency), "frequency must be > 0");
+    CCAB_CHECK(ph.sound_speed > 0.0 && std::isfinite(ph.sound_speed), "sound_speed must be > 0");
+    CCAB_CHECK(std::isfinite(ph.mass_density), "mass_density must be finite");
+    CCAB_CHECK(ph.method == CCAB_CLASSICAL || ph.method == CCAB_BURTON_MILLER, "bad method");
+    CCAB_CHECK(ph.problem_type == CCAB_EXTERIOR || ph.problem_type == CCAB_INTERIOR,
+               "bad problem type");
+    CCAB_CHECK(ph.bc_type >= CCAB_BC_PRESSURE && ph.bc_type <= CCAB_BC_IMPEDANCE, "bad bc type");
+    if (ph.bc_type == CCAB_BC_IMPEDANCE)
+        CCAB_CHECK(ph.bc_value[0] != 0.0 || ph.bc_value[1] != 0.0, "impedance must be non-zero");
+    return CCAB_OK;
+}
+
+namespace {
+
+__global__ void fill_kernel(int64_t n, c128 v, ccab_c128* dst, int64_t stride) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[blockIdx.y * stride + i] = ccab_c128{v.re, v.im};
+}
+
+__global__ void scale_copy_kernel(int64_t n, const SysParams* __restrict__ params,
+                                  const ccab_c128* __restrict__ src, ccab_c128* dst) {
+    // vn = f * phi for the impedance BC (solve.rs:67)
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const c128 f = params[b].lhs_b;
+    const ccab_c128 s = src[b * n + i];
+    const c128 v = f * c128{s.re, s.im};
+    dst[b * n + i] = ccab_c128{v.re, v.im};
+}
+
+__global__ void copy_vec_kernel(int64_t n, const ccab_c128* __restrict__ src, ccab_c128* dst) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// SoA copy of the collocation points (8 n doubles) for the current call
+int stage_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double** soa) {
+    *soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(n) * 64));
+    if (!*soa) return CCAB_ENOMEM;
+    return launch_cpts_to_soa(ctx, d_cpts, n, *soa);
+}
+
+int stage_params(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch, std::vector<SysParams>& hp,
+                 SysParams** dp) {
+    hp.resize(batch);
+    for (int64_t b = 0; b < batch; ++b) {
+        CCAB_TRY(check_physics(ctx, ph[b]));
+        hp[b] = make_params(ph[b]);
+    }
+    *dp = static_cast<SysParams*>(ctx_buf(ctx, SLOT_PARAMS, sizeof(SysParams) * batch));
+    if (!*dp) return CCAB_ENOMEM;
+    CCAB_TRY(pinned_reserve(ctx, sizeof(SysParams) * batch));
+    // stage through pinned memory; the copy completes before pinned is reused
+    std::memcpy(ctx->pinned, hp.data(), sizeof(SysParams) * batch);
+    CCAB_HIP(hipMemcpyAsync(*dp, ctx->pinned, sizeof(SysParams) * batch, hipMemcpyHostToDevice,
+                            ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    return CCAB_OK;
+}
+
+// phi / vn from the solved unknown x per BC (solve.rs:36-69)
+int finish_surface(ccab_ctx* ctx, const SysParams* hp, const SysParams* dp, int bc, int64_t n,
+                   int64_t batch, const ccab_c128* x, ccab_c128* phi, ccab_c128* vn) {
+    const dim3 g((unsigned)((n + 255) / 256));
+    for (int64_t b = 0; b < batch; ++b) {
+        const ccab_c128* xb = x + b * n;
+        ccab_c128* pb = phi + b * n;
+        ccab_c128* vb = vn + b * n;
+        if (bc == CCAB_BC_PRESSURE) {
+            hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, hp[b].bcv, pb, 0);
+            if (vb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, vb);
+        } else if (bc == CCAB_BC_NORMAL_VELOCITY) {
+            if (pb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, pb);
+            hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, hp[b].bcv, vb, 0);
+        } else {
+            if (pb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, pb);
+        }
+    }
+    if (bc == CCAB_BC_IMPEDANCE)
+        hipLaunchKernelGGL(scale_copy_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
+                           dim3(256), 0, ctx->stream, n, dp, phi, vn);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ccab_version(void) { return CCAB_VERSION; }
+
+int ccab_create(int device, ccab_ctx** out) {
+    if (!out) return CCAB_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CCAB_EHIP;
+    if (device < 0 || device >= ndev) return CCAB_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return CCAB_EHIP;
+    ccab_ctx* ctx = new ccab_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return CCAB_EHIP;
+    }
+    ctx->stream = ctx->own_stream;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return CCAB_OK;
+}
+
+int ccab_profile_enable(ccab_ctx* ctx, int on) {
+    if (!ctx) return CCAB_EINVAL;
+    prof_resolve(ctx);
+    ctx->prof_on = on != 0;
+    return CCAB_OK;
+}
+
+int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double* work,
+                      int reset) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(cls >= 0 && cls < PROF_NCLASS, "bad profile class %d", cls);
+    prof_resolve(ctx);
+    if (count) *count = ctx->prof_count[cls];
+    if (ms) *ms = ctx->prof_ms[cls];
+    if (work) *work = ctx->prof_work[cls];
+    if (reset) {
+        ctx->prof_count[cls] = 0;
+        ctx->prof_ms[cls] = 0.0;
+        ctx->prof_work[cls] = 0.0;
+    }
+    return CCAB_OK;
+}
+
+int ccab_destroy(ccab_ctx* ctx) {
+    if (!ctx) return CCAB_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    prof_resolve(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (int s = 0; s < SLOT_COUNT; ++s)
+        if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+    return CCAB_OK;
+}
+
+const char* ccab_last_error(const ccab_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ccab_set_stream(ccab_ctx* ctx, void* hip_stream) {
+    if (!ctx) return CCAB_EINVAL;
+    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+    return CCAB_OK;
+}
+
+int ccab_synchronize(ccab_ctx* ctx) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    return CCAB_OK;
+}
+
+// elements.rs:49-181 + preprocess.rs:126-153 (host)
+int ccab_collocation_points(const double* nodes, const int64_t* conn, const int32_t* nverts,
+                            int64_t nelem, double* cpts) {
+    if (!nodes || !conn || !nverts || !cpts || nelem < 0) return CCAB_EINVAL;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int64_t* cn = conn + 4 * e;
+        double* out = cpts + 8 * e;
+        double x[3] = {0, 0, 0}, nu[3];
+        if (nverts[e] == 3) {
+            const double xi = 1.0 / 3.0, eta = 1.0 / 3.0;
+            const double N[3] = {1.0 - xi - eta, xi, eta};
+            for (int a = 0; a < 3; ++a)
+                for (int d = 0; d < 3; ++d) x[d] += N[a] * nodes[3 * cn[a] + d];
+            const double* p0 = nodes + 3 * cn[0];
+            const double* p1 = nodes + 3 * cn[1];
+            const double* p2 = nodes + 3 * cn[2];
+            const double a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const double b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            nu[0] = a[1] * b[2] - a[2] * b[1];
+            nu[1] = a[2] * b[0] - a[0] * b[2];
+            nu[2] = a[0] * b[1] - a[1] * b[0];
+            const double nn = std::sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
+            out[6] = 0.5 * nn;
+            out[7] = 1.0;
+            for (int d = 0; d < 3; ++d) out[3 + d] = nu[d] / nn;
+        } else if (nverts[e] == 4) {
+            // QUADGP1 with the reference's detj = 0.25 |dxi x deta| (elements.rs:168)
+            const double dN[4][2] = {{-0.25, -0.25}, {0.25, -0.25}, {0.25, 0.25}, {-0.25, 0.25}};
+            double dxi[3] = {0, 0, 0}, deta[3] = {0, 0, 0};
+            for (int a = 0; a < 4; ++a) {
+                const double* p = nodes + 3 * cn[a];
+                for (int d = 0; d < 3; ++d) {
+                    x[d] += 0.25 * p[d];
+                    dxi[d] += dN[a][0] * p[d];
+                    deta[d] += dN[a][1] * p[d];
+                }
+            }
+            nu[0] = dxi[1] * deta[2] - dxi[2] * deta[1];
+            nu[1] = dxi[2] * deta[0] - dxi[0] * deta[2];
+            nu[2] = dxi[0] * deta[1] - dxi[1] * deta[0];
+            const double nn = std::sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
+            out[6] = 0.25 * nn;
+            out[7] = 4.0;
+            for (int d = 0; d < 3; ++d) out[3 + d] = nu[d] / nn;
+        } else {
+            return CCAB_EINVAL;
+        }
+        if (!(out[6] > 0.0)) return CCAB_ENAN;  // degenerate element: b = 0 on the diagonal
+        out[0] = x[0];
+        out[1] = x[1];
+        out[2] = x[2];
+    }
+    return CCAB_OK;
+}
+
+int ccab_incident_surface(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
+                          const double* d_cpts, int64_t n, ccab_c128* d_rhs) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(ph && w && d_cpts && d_rhs && n > 0, "null argument or n <= 0");
+    CCAB_CHECK(w->kind == CCAB_WAVE_PLANE || w->kind == CCAB_WAVE_SPHERICAL, "bad wave kind");
+    CCAB_TRY(check_physics(ctx, *ph));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    const SysParams p = make_params(*ph);
+    return launch_incident(ctx, &p, *ph, *w, soa, n, d_rhs);
+}
+
+int ccab_dense_surface_matrices(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                                int64_t n, int prec, void* d_H, void* d_G, int64_t ld) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(ph && d_cpts && n > 0 && ld >= n, "invalid arguments");
+    CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "bad precision");
+    std::vector<SysParams> hp;
+    SysParams* dp;
+    CCAB_TRY(stage_params(ctx, ph, 1, hp, &dp));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    return launch_assemble(ctx, dp, hp.data(), 1, soa, n, prec == CCAB_C128 ? PREC_F64 : PREC_F32,
+                           ASM_HG, d_H, d_G, ld, 0, nullptr, nullptr);
+}
+
+int ccab_assemble_system(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch,
+                         const double* d_cpts, int64_t n, ccab_c128* d_lhs128, ccab_c64* d_lhs64,
+                         int64_t ld, int64_t stride, const ccab_c128* d_rhs_inc,
+                         ccab_c128* d_rhs) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(ph && d_cpts && n > 0 && batch > 0 && ld >= n, "invalid arguments");
+    CCAB_CHECK(batch == 1 || stride >= ld * n, "stride too small");
+    CCAB_CHECK(d_lhs128 || d_lhs64, "no output matrix");
+    std::vector<SysParams> hp;
+    SysParams* dp;
+    CCAB_TRY(stage_params(ctx, ph, batch, hp, &dp));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    bool need_rowsum = false;
+    for (auto& p : hp) need_rowsum |= p.rowsum_of != 0;
+    const int64_t nchunks = (n + 63) / 64;
+    double* part = nullptr;
+    if (need_rowsum && d_rhs) {
+        part = static_cast<double*>(ctx_buf(ctx, SLOT_ROWSUM, size_t(batch) * nchunks * n * 16));
+        if (!part) return CCAB_ENOMEM;
+    }
+    const int pc = d_lhs128 ? PREC_F64 : PREC_F64;  // FP64 arithmetic for every fused output
+    int64_t nch = 0;
+    CCAB_TRY(launch_assemble(ctx, dp, hp.data(), batch, soa, n, pc, ASM_LHS, d_lhs128, d_lhs64, ld,
+                             stride, part, &nch));
+    if (d_rhs) {
+        CCAB_CHECK(d_rhs_inc != nullptr, "rhs requested without rhs_inc");
+        CCAB_TRY(launch_rhs_finish(ctx, dp, batch, n, part, nch, d_rhs_inc, d_rhs));
+    }
+    return CCAB_OK;
+}
+
+int ccab_surface_row_or_column(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                               int64_t n, int side, const int64_t* d_i, int64_t ni,
+                               const int64_t* d_j, int64_t nj, ccab_c128* d_out) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK((ni == 1) != (nj == 1), "Invalid call to get_surface_row_or_column: i or j must be singleton");
+    CCAB_CHECK(ph && d_cpts && d_i && d_j && d_out && n > 0, "null argument");
+    CCAB_CHECK(side == CCAB_LHS || side == CCAB_RHS, "bad side");
+    CCAB_TRY(check_physics(ctx, *ph));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    return launch_rowcol(ctx, make_params(*ph), soa, n, d_i, ni, d_j, nj, side, d_out);
+}
+
+int ccab_lu_factor_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, void* d_A,
+                           int64_t lda, int64_t strideA, int32_t* d_ipiv) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A && d_ipiv, "null argument");
+    return lu_factor(ctx, prec, n, batch, d_A, lda, strideA, d_ipiv);
+}
+
+int ccab_lu_solve_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_LU,
+                          int64_t lda, int64_t strideA, const int32_t* d_ipiv, void* d_B,
+                          int64_t ldb, int64_t strideB, int64_t nrhs) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_LU && d_ipiv && d_B, "null argument");
+    return lu_solve(ctx, prec, n, batch, d_LU, lda, strideA, d_ipiv, d_B, ldb, strideB, nrhs);
+}
+
+int ccab_solve_lu(ccab_ctx* ctx, int64_t n, const ccab_c128* d_A, int64_t lda, ccab_c128* d_x) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A && d_x && n > 0 && lda >= n, "invalid arguments");
+    // a_lu = a.clone().lu(): factor a private copy
+    c128* W = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_A, size_t(n) * n * 16));
+    int32_t* piv = static_cast<int32_t*>(ctx_buf(ctx, SLOT_SOLVE_PIV, size_t(n) * 4));
+    if (!W || !piv) return CCAB_ENOMEM;
+    CCAB_HIP(hipMemcpy2DAsync(W, n * 16, d_A, lda * 16, n * 16, n, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    CCAB_TRY(lu_factor(ctx, CCAB_C128, n, 1, W, n, n * n, piv));
+    return lu_solve(ctx, CCAB_C128, n, 1, W, n, n * n, piv, d_x, n, n, 1);
+}
+
+int ccab_solve_mixed_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const ccab_c128* d_A128,
+                             ccab_c64* d_A64, int64_t lda, int64_t strideA, ccab_c128* d_x,
+                             double ir_tol, int ir_max_steps, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A128 && d_A64 && d_x && n > 0 && batch > 0 && lda >= n, "invalid arguments");
+    CCAB_CHECK(batch == 1 || strideA >= lda * n, "stride too small");
+    return mixed_run(ctx, n, batch, reinterpret_cast<const c128*>(d_A128),
+                     reinterpret_cast<c64*>(d_A64), lda, strideA, reinterpret_cast<c128*>(d_x),
+                     ir_tol, ir_max_steps, stats);
+}
+
+int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda, ccab_c128* d_x,
+                     const ccab_c128* d_x0, const ccab_gmres_params* p, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A && d_x && p && n > 0 && lda >= n, "invalid arguments");
+    CCAB_CHECK(p->matrix_prec == CCAB_C128 || p->matrix_prec == CCAB_C64, "bad matrix precision");
+    return gmres_run(ctx, n, p->matrix_prec, d_A, lda, reinterpret_cast<c128*>(d_x),
+                     reinterpret_cast<const c128*>(d_x0), *p, stats);
+}
+
+}  // extern "C"
+
+// Shared by ccab_solve_surface (one frequency, given rhs_inc) and ccab_sweep
+// (incident wave evaluated per frequency).
+static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                      const ccab_c128* d_rhs_inc_given, const double* d_cpts, int64_t n,
+                      const ccab_solver_cfg* cfg, ccab_c128* d_phi, ccab_c128* d_vn,
+                      ccab_solve_stats* stats) {
+    CCAB_CHECK(ph && d_cpts && cfg && d_phi && d_vn && n > 0 && nfreq > 0, "invalid arguments");
+    const int bc = ph[0].bc_type;
+    for (int64_t f = 1; f < nfreq; ++f) {
+        CCAB_CHECK(ph[f].bc_type == bc && ph[f].method == ph[0].method,
+                   "a sweep shares one BC type and method");
+    }
+    std::vector<SysParams> hp;
+    SysParams* dp;
+    CCAB_TRY(stage_params(ctx, ph, nfreq, hp, &dp));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    // right-hand sides: rhs_inc per frequency, then the fused BC algebra
+    c128* rhs = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_RHS, size_t(nfreq) * n * 16));
+    if (!rhs) return CCAB_ENOMEM;
+    ccab_c128* rhs_c = reinterpret_cast<ccab_c128*>(rhs);
+    if (d_rhs_inc_given) {
+        CCAB_HIP(hipMemcpyAsync(rhs, d_rhs_inc_given, size_t(n) * 16 * nfreq,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+        for (int64_t f = 0; f < nfreq; ++f)
+            CCAB_TRY(launch_incident(ctx, &hp[f], ph[f], *w, soa, n, rhs_c + f * n));
+    }
+    const int solver = cfg->solver;
+    const size_t n2 = size_t(n) * n;
+    // process frequencies in chunks that fit a memory budget
+    size_t per = 0;
+    if (solver == CCAB_SOLVER_DIRECT) per = n2 * 16;
+    else if (solver == CCAB_SOLVER_DIRECT_MIXED) per = n2 * 24;
+    else if (solver == CCAB_SOLVER_ITERATIVE)
+        per = n2 * (cfg->gmres.matrix_prec == CCAB_C64 ? 8 : 16);
+    else return ctx_fail(ctx, CCAB_EINVAL, "solver %d not available on the dense path", solver);
+    size_t budget = size_t(96) << 30;
+    size_t freemem = 0, total = 0;
+    if (hipMemGetInfo(&freemem, &total) == hipSuccess) budget = std::min(budget, freemem / 10 * 7);
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nfreq, int64_t(budget / per)));
+    bool need_rowsum = false;
+    for (auto& p : hp) need_rowsum |= p.rowsum_of != 0;
+    const int64_t nchunks = (n + 63) / 64;
+    double* part = nullptr;
+    if (need_rowsum) {
+        part = static_cast<double*>(ctx_buf(ctx, SLOT_ROWSUM, size_t(chunk) * nchunks * n * 16));
+        if (!part) return CCAB_ENOMEM;
+    }
+    void* A = nullptr;
+    c64* A64 = nullptr;
+    if (solver == CCAB_SOLVER_DIRECT || solver == CCAB_SOLVER_DIRECT_MIXED ||
+        (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C128)) {
+        A = ctx_buf(ctx, SLOT_SOLVE_A, size_t(chunk) * n2 * 16);
+        if (!A) return CCAB_ENOMEM;
+    }
+    if (solver == CCAB_SOLVER_DIRECT_MIXED ||
+        (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C64)) {
+        A64 = static_cast<c64*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(chunk) * n2 * 8));
+        if (!A64) return CCAB_ENOMEM;
+    }
+    int32_t* piv = nullptr;
+    if (solver == CCAB_SOLVER_DIRECT) {
+        piv = static_cast<int32_t*>(ctx_buf(ctx, SLOT_SOLVE_PIV, size_t(chunk) * n * 4));
+        if (!piv) return CCAB_ENOMEM;
+    }
+    for (int64_t f0 = 0; f0 < nfreq; f0 += chunk) {
+        const int64_t nb = std::min(chunk, nfreq - f0);
+        int64_t nch = 0;
+        CCAB_TRY(launch_assemble(ctx, dp + f0, hp.data() + f0, nb, soa, n, PREC_F64, ASM_LHS, A,
+                                 A64, n, int64_t(n2), part, &nch));
+        ccab_c128* x = rhs_c + f0 * n;
+        CCAB_TRY(launch_rhs_finish(ctx, dp + f0, nb, n, part, nch, x, x));
+        if (solver == CCAB_SOLVER_DIRECT) {
+            CCAB_TRY(lu_factor(ctx, CCAB_C128, n, nb, A, n, int64_t(n2), piv));
+            CCAB_TRY(lu_solve(ctx, CCAB_C128, n, nb, A, n, int64_t(n2), piv, x, n, n, 1));
+            if (stats)
+                for (int64_t b = 0; b < nb; ++b) {
+                    stats[f0 + b] = ccab_solve_stats{};
+                    stats[f0 + b].converged = 1;
+                    stats[f0 + b].backward_error = -1.0;
+                    stats[f0 + b].rel_residual = -1.0;
+                }
+        } else if (solver == CCAB_SOLVER_DIRECT_MIXED) {
+            CCAB_TRY(mixed_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
+                               reinterpret_cast<c128*>(x), cfg->ir_tolerance, cfg->ir_max_steps,
+                               stats ? stats + f0 : nullptr));
+        } else {
+            const void* op = cfg->gmres.matrix_prec == CCAB_C64 ? static_cast<const void*>(A64) : A;
+            const size_t esz = cfg->gmres.matrix_prec == CCAB_C64 ? 8 : 16;
+            for (int64_t b = 0; b < nb; ++b) {
+                ccab_solve_stats st{};
+                CCAB_TRY(gmres_run(ctx, n, cfg->gmres.matrix_prec,
+                                   static_cast<const char*>(op) + b * n2 * esz, n,
+                                   reinterpret_cast<c128*>(x + b * n), nullptr, cfg->gmres, &st));
+                if (stats) stats[f0 + b] = st;
+            }
+        }
+        CCAB_TRY(finish_surface(ctx, hp.data() + f0, dp + f0, bc, n, nb, x, d_phi + f0 * n,
+                                d_vn + f0 * n));
+    }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+extern "C" int ccab_sweep(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                          const double* d_cpts, int64_t n, const ccab_solver_cfg* cfg,
+                          ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(w != nullptr, "null wave");
+    CCAB_CHECK(w->kind == CCAB_WAVE_PLANE || w->kind == CCAB_WAVE_SPHERICAL, "bad wave kind");
+    return sweep_impl(ctx, ph, nfreq, w, nullptr, d_cpts, n, cfg, d_phi, d_vn, stats);
+}
+
+extern "C" int ccab_solve_surface(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                                  int64_t n, const ccab_c128* d_rhs_inc,
+                                  const ccab_solver_cfg* cfg, ccab_c128* d_phi, ccab_c128* d_vn,
+                                  ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_rhs_inc != nullptr, "null rhs_inc");
+    return sweep_impl(ctx, ph, 1, nullptr, d_rhs_inc, d_cpts, n, cfg, d_phi, d_vn, stats);
+}
+
+extern "C" int ccab_cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K,
+                                 const void* d_A, int64_t lda, const void* d_B, int64_t ldb,
+                                 void* d_C, int64_t ldc) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "bad precision");
+    CCAB_CHECK(lda >= M && ldb >= K && ldc >= M, "bad leading dimensions");
+    return cgemm_update(ctx, prec, M, N, K, d_A, lda, d_B, ldb, d_C, ldc);
+}

@@ -1,0 +1,247 @@
+// Internal declarations shared by the HIP translation units of libccab.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/ccab.h"
+
+// ---------------------------------------------------------------------------
+// Complex arithmetic (interleaved re/im, same layout as ccab_c128 / ccab_c64)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Cx {
+    T re, im;
+};
+using c128 = Cx<double>;
+using c64 = Cx<float>;
+
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> cx(T r, T i) {
+    return Cx<T>{r, i};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator+(Cx<T> a, Cx<T> b) {
+    return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator-(Cx<T> a, Cx<T> b) {
+    return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator-(Cx<T> a) {
+    return {-a.re, -a.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator*(Cx<T> a, Cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator*(Cx<T> a, T s) {
+    return {a.re * s, a.im * s};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> operator*(T s, Cx<T> a) {
+    return {a.re * s, a.im * s};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T>& operator+=(Cx<T>& a, Cx<T> b) {
+    a.re += b.re;
+    a.im += b.im;
+    return a;
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T>& operator-=(Cx<T>& a, Cx<T> b) {
+    a.re -= b.re;
+    a.im -= b.im;
+    return a;
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> conj(Cx<T> a) {
+    return {a.re, -a.im};
+}
+template <typename T>
+__host__ __device__ __forceinline__ T norm_sqr(Cx<T> a) {
+    return a.re * a.re + a.im * a.im;
+}
+// num-complex division: (a * conj(b)) / |b|^2
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> cdiv(Cx<T> a, Cx<T> b) {
+    T d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+template <typename T>
+__host__ __device__ __forceinline__ Cx<T> crecip(Cx<T> b) {
+    T d = b.re * b.re + b.im * b.im;
+    return {b.re / d, -b.im / d};
+}
+template <typename To, typename Ti>
+__host__ __device__ __forceinline__ Cx<To> cvt(Cx<Ti> a) {
+    return {static_cast<To>(a.re), static_cast<To>(a.im)};
+}
+
+static_assert(sizeof(c128) == sizeof(ccab_c128), "layout");
+static_assert(sizeof(c64) == sizeof(ccab_c64), "layout");
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct ccab_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    std::string err;
+    // grow-only device buffers, one per slot (see enum BufSlot)
+    void* buf[32] = {};
+    size_t buf_bytes[32] = {};
+    // identity of the factors whose diagonal-block inverses and row
+    // permutation are held in the LU slots
+    const void* lu_key = nullptr;
+    int lu_prec = -1;
+    int64_t lu_n = 0, lu_batch = 0, lu_lda = 0, lu_stride = 0;
+    // pinned host staging for small readbacks
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    int num_cus = 256;
+    // in-process kernel timing (HIP events on ctx->stream), see prof_begin
+    bool prof_on = false;
+    struct ProfRec {
+        hipEvent_t a, b;
+        int cls;
+        double work;
+    };
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> event_pool;
+    double prof_ms[16] = {};
+    double prof_work[16] = {};
+    int64_t prof_count[16] = {};
+};
+
+// kernel classes of the profiler (ccab_profile_read); work is algorithmic:
+// flops for the LU classes, bytes for HBM-bound classes
+enum ProfClass {
+    PROF_ASSEMBLY = 0,   // bytes stored
+    PROF_LU_PANEL = 1,   // flops
+    PROF_LU_TRSM = 2,    // flops
+    PROF_LU_UPDATE = 3,  // flops (trailing complex GEMM)
+    PROF_LU_LASWP = 4,
+    PROF_LU_TRINV = 5,
+    PROF_TRSV = 6,       // bytes of L/U read
+    PROF_GEMV = 7,       // bytes of matrix read
+    PROF_MGS = 8,
+    PROF_NCLASS = 9
+};
+void prof_begin(ccab_ctx* ctx, hipEvent_t* ev);
+void prof_end(ccab_ctx* ctx, hipEvent_t ev, int cls, double work);
+struct ProfScope {
+    ccab_ctx* ctx;
+    hipEvent_t ev = nullptr;
+    int cls;
+    double work;
+    ProfScope(ccab_ctx* c, int k, double w) : ctx(c), cls(k), work(w) {
+        if (ctx->prof_on) prof_begin(ctx, &ev);
+    }
+    ~ProfScope() {
+        if (ctx->prof_on && ev) prof_end(ctx, ev, cls, work);
+    }
+};
+
+int ctx_fail(ccab_ctx* ctx, int code, const char* fmt, ...);
+
+#define CCAB_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ctx_fail(ctx, CCAB_EHIP, "%s failed: %s (%s:%d)", #call,             \
+                            hipGetErrorString(e_), __FILE__, __LINE__);                 \
+    } while (0)
+
+#define CCAB_CHECK(cond, ...)                                                           \
+    do {                                                                                \
+        if (!(cond)) return ctx_fail(ctx, CCAB_EINVAL, __VA_ARGS__);                    \
+    } while (0)
+
+#define CCAB_TRY(expr)                                                                  \
+    do {                                                                                \
+        int rc_ = (expr);                                                               \
+        if (rc_ != CCAB_OK) return rc_;                                                 \
+    } while (0)
+
+// Persistent per-slot device buffers.  A slot is owned by one routine; the
+// buffer only grows (growing synchronises the stream first).  Returns
+// nullptr and records the error on failure.
+enum BufSlot {
+    SLOT_SOA = 0, SLOT_PARAMS, SLOT_ROWSUM, SLOT_LU_LINV, SLOT_LU_UINV, SLOT_LU_PERM,
+    SLOT_LU_INFO, SLOT_LU_Y, SLOT_GEMV_PART, SLOT_NORM, SLOT_MIXED_R, SLOT_MIXED_D,
+    SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
+    SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_COUNT
+};
+void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
+inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
+int pinned_reserve(ccab_ctx* ctx, size_t bytes);
+
+// ---------------------------------------------------------------------------
+// Physics constants per system (preprocess.rs:41-74, influence_matrix.rs:130-162)
+// ---------------------------------------------------------------------------
+struct SysParams {
+    double k;            // wavenumber
+    double omega;
+    c128 beta;           // Burton-Miller coupling i/k
+    c128 hdiag, gdiag;   // global diagonal shifts
+    c128 lhs_a, lhs_b;   // LHS = a H + b G
+    c128 rhs_a, rhs_b;   // RHS factors (get_rhs_factors)
+    c128 bcv;            // BC value
+    c128 rhs_scale;      // multiplies the row sum of the RHS-side matrix
+    int32_t bm;
+    int32_t rowsum_of;   // 0 none, 1 sum of G, 2 sum of H
+    double rhs_inc_sign; // sign applied to rhs_inc
+};
+SysParams make_params(const ccab_physics& ph);
+int check_physics(ccab_ctx* ctx, const ccab_physics& ph);
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Internal launchers (all enqueue on ctx->stream)
+// ---------------------------------------------------------------------------
+// assembly.hip
+int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
+                    int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
+                    void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
+                    int64_t* nchunks_out);
+int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, int64_t n,
+                      const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
+                      ccab_c128* d_rhs);
+int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d_soa);
+int launch_incident(ccab_ctx* ctx, const SysParams* h_params, const ccab_physics& ph,
+                    const ccab_wave& w, const double* d_soa, int64_t n, ccab_c128* d_rhs);
+int launch_rowcol(ccab_ctx* ctx, const SysParams& p, const double* d_soa, int64_t n,
+                  const int64_t* d_i, int64_t ni, const int64_t* d_j, int64_t nj, int side,
+                  ccab_c128* d_out);
+
+enum { ASM_HG = 0, ASM_LHS = 1 };
+enum { PREC_F64 = 0, PREC_F32 = 1, PREC_MIXED = 2 };
+
+// lu.hip
+int lu_factor(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, void* d_A, int64_t lda,
+              int64_t strideA, int32_t* d_ipiv);
+int lu_solve(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_LU, int64_t lda,
+             int64_t strideA, const int32_t* d_ipiv, void* d_B, int64_t ldb, int64_t strideB,
+             int64_t nrhs);
+
+// blas.hip
+// y_b = alpha * A_b x_b + beta * y_b for each system b (A c128 or c64, x,y c128)
+int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_A,
+                 int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
+                 c128 beta, c128* d_y, int64_t stridey);
+size_t gemv_workspace_bytes(int64_t n, int64_t batch);
+// out[b] = ||v_b||_2 (FP64), written to device
+int nrm2_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* d_v, int64_t stride,
+                 double* d_out);
+int frob_norm(ccab_ctx* ctx, int prec, int64_t n, const void* d_A, int64_t lda, double* d_out);
+int convert_c128_to_c64(ccab_ctx* ctx, int64_t count, const c128* src, c64* dst);
+int convert_c64_to_c128(ccab_ctx* ctx, int64_t count, const c64* src, c128* dst);

@@ -1,0 +1,611 @@
+// Batched dense complex LU with partial pivoting on gfx950.
+//
+// Replaces nalgebra's LU behind solve.rs:166-171 (solve_lu).  Right-looking,
+// blocked with panel width NB = 64:
+//   1. panel_kernel    one workgroup per system factors the (n-k0) x NB panel
+//                      column by column; pivot = first row maximising
+//                      |re|+|im| (nalgebra icamax on norm1); an all-zero
+//                      column is skipped as nalgebra does.
+//   2. trinv_kernel    inverts the NB x NB unit-lower L11 and upper U11 (kept
+//                      for the blocked triangular solves).
+//   3. laswp_kernel    applies the panel's row swaps to the other columns.
+//   4. cgemm (overwrite) U12 = inv(L11) A12.
+//   5. cgemm (update)  A22 -= L21 U12  -- the O(n^3) part, on MFMA.
+// The complex GEMM runs as a real GEMM of twice the size on the 16x16x4 MFMA
+// (f32 for c64, f64 for c128):  [C_re; C_im] interleaved = A' B' where B' and
+// C' are the interleaved memory itself and A' is the 2x2 real expansion
+// [[Re, -Im], [Im, Re]] built on the fly from LDS.
+#include "ccab_internal.h"
+
+namespace {
+
+constexpr int NB = 64;         // panel width
+constexpr int kPanelThreads = 1024;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<float> {
+    using acc_t = floatx4;
+    static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mfma<double> {
+    using acc_t = doublex4;
+    static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ float norm1(c64 z) { return fabsf(z.re) + fabsf(z.im); }
+__device__ __forceinline__ double norm1(c128 z) { return fabs(z.re) + fabs(z.im); }
+
+// ---------------------------------------------------------------------------
+// Panel factorisation: one workgroup per system.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void panel_kernel(Cx<T>* __restrict__ A, int64_t lda,
+                                                               int64_t strideA, int64_t n,
+                                                               int64_t k0, int nb,
+                                                               int32_t* __restrict__ ipiv,
+                                                               int32_t* __restrict__ perm,
+                                                               int32_t* __restrict__ info) {
+    using C = Cx<T>;
+    const int64_t b = blockIdx.x;
+    C* a = A + b * strideA;
+    int32_t* piv = ipiv + b * n;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    __shared__ T red_v[kPanelThreads / 64];
+    __shared__ int64_t red_i[kPanelThreads / 64];
+    __shared__ int64_t s_piv;
+    __shared__ C s_row[NB];
+    __shared__ C s_inv;
+    __shared__ int s_zero;
+
+    for (int jj = 0; jj < nb; ++jj) {
+        const int64_t c = k0 + jj;
+        // --- pivot search: max norm1 over rows c..n-1, first index on ties
+        T best = T(-1);
+        int64_t bi = n;
+        for (int64_t r = c + tid; r < n; r += kPanelThreads) {
+            const T v = norm1(a[r + c * lda]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(best, off, 64);
+            const int64_t oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            red_v[wave] = best;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            T bv = red_v[0];
+            int64_t bx = red_i[0];
+            for (int w = 1; w < kPanelThreads / 64; ++w) {
+                if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bx)) {
+                    bv = red_v[w];
+                    bx = red_i[w];
+                }
+            }
+            const bool zero = !(bv > T(0));
+            s_zero = zero ? 1 : 0;
+            s_piv = zero ? c : bx;
+            piv[c] = int32_t(zero ? c : bx);
+            if (zero) {
+                info[b] = 1;
+            } else if (bx != c) {
+                int32_t* pm = perm + b * n;
+                const int32_t t = pm[c];
+                pm[c] = pm[bx];
+                pm[bx] = t;
+            }
+        }
+        __syncthreads();
+        if (s_zero) continue;  // nalgebra skips an all-zero column
+        const int64_t p = s_piv;
+        // --- swap rows c and p within the panel columns
+        if (p != c) {
+            for (int t = tid; t < nb; t += kPanelThreads) {
+                const int64_t col = k0 + t;
+                const C x = a[c + col * lda];
+                a[c + col * lda] = a[p + col * lda];
+                a[p + col * lda] = x;
+            }
+        }
+        __syncthreads();
+        // --- pivot row of the remaining panel columns to LDS, reciprocal
+        for (int t = tid; t < nb; t += kPanelThreads) s_row[t] = a[c + (k0 + t) * lda];
+        if (tid == 0) s_inv = crecip(a[c + c * lda]);
+        __syncthreads();
+        const C inv = s_inv;
+        // --- scale column c below the diagonal, rank-1 update of the panel
+        for (int64_t r = c + 1 + tid; r < n; r += kPanelThreads) {
+            const C l = a[r + c * lda] * inv;
+            a[r + c * lda] = l;
+            for (int t = jj + 1; t < nb; ++t) {
+                C& dst = a[r + (k0 + t) * lda];
+                dst = dst - l * s_row[t];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inverses of the diagonal block: Linv (unit lower) and Uinv (upper), NB x NB
+// column-major, zero padded when nb < NB.  One wavefront-column per thread.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NB) void trinv_kernel(const Cx<T>* __restrict__ A, int64_t lda,
+                                                   int64_t strideA, int64_t k0, int nb,
+                                                   Cx<T>* __restrict__ Linv,
+                                                   Cx<T>* __restrict__ Uinv, int64_t inv_stride) {
+    using C = Cx<T>;
+    const int64_t b = blockIdx.x;
+    const C* a = A + b * strideA + k0 + k0 * lda;
+    C* li = Linv + b * inv_stride;
+    C* ui = Uinv + b * inv_stride;
+    __shared__ C blk[NB * NB];
+    const int c = threadIdx.x;
+    for (int t = 0; t < NB; ++t) {
+        blk[c + t * NB] = (c < nb && t < nb) ? a[c + t * lda] : C{T(0), T(0)};
+    }
+    __syncthreads();
+    // Linv column c: solve L x = e_c (unit diagonal)
+    for (int r = 0; r < NB; ++r) {
+        C x{T(0), T(0)};
+        if (c < nb && r >= c && r < nb) {
+            if (r == c) {
+                x = C{T(1), T(0)};
+            } else {
+                C s{T(0), T(0)};
+                for (int t = c; t < r; ++t) s += blk[r + t * NB] * li[t + c * NB];
+                x = -s;
+            }
+        }
+        li[r + c * NB] = x;
+    }
+    // Uinv column c: solve U y = e_c, rows c down to 0
+    for (int r = NB - 1; r >= 0; --r) {
+        C y{T(0), T(0)};
+        if (c < nb && r <= c) {
+            C s = (r == c) ? C{T(1), T(0)} : C{T(0), T(0)};
+            for (int t = r + 1; t <= c; ++t) s -= blk[r + t * NB] * ui[t + c * NB];
+            const C d = blk[r + r * NB];
+            y = (d.re == T(0) && d.im == T(0)) ? C{T(0), T(0)} : cdiv(s, d);
+        }
+        ui[r + c * NB] = y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row swaps of one panel applied to every column outside the panel.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void laswp_kernel(Cx<T>* __restrict__ A, int64_t lda, int64_t strideA, int64_t n,
+                             int64_t k0, int nb, const int32_t* __restrict__ ipiv) {
+    const int64_t b = blockIdx.y;
+    const int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= n || (col >= k0 && col < k0 + nb)) return;
+    Cx<T>* a = A + b * strideA + col * lda;
+    const int32_t* piv = ipiv + b * n;
+    for (int j = 0; j < nb; ++j) {
+        const int64_t r = k0 + j;
+        const int64_t p = piv[r];
+        if (p != r) {
+            const Cx<T> x = a[r];
+            a[r] = a[p];
+            a[p] = x;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Complex GEMM on MFMA.  MODE 0: C -= A B.  MODE 1: C = A B.
+// Workgroup tile 64 (complex rows) x 64 (cols), 4 wavefronts in 2x2, each
+// wave 32 complex rows (= 64 real rows, 4 MFMA tiles) x 32 cols (2 tiles).
+// K is walked in chunks of KC complex through LDS.  With MODE 1 and
+// M <= 64, C may alias B: a workgroup reads its whole B column block before
+// its first store.
+// ---------------------------------------------------------------------------
+constexpr int TM = 64;
+constexpr int TN = 64;
+constexpr int KC = 16;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void cgemm_kernel(int64_t M, int64_t N, int64_t K,
+                                                    const Cx<T>* __restrict__ A, int64_t lda,
+                                                    int64_t sA, const Cx<T>* B, int64_t ldb,
+                                                    int64_t sB, Cx<T>* C, int64_t ldc,
+                                                    int64_t sC) {
+    using Cxt = Cx<T>;
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc_t;
+    const int64_t bz = blockIdx.z;
+    const int64_t m0 = int64_t(blockIdx.x) * TM;
+    const int64_t n0 = int64_t(blockIdx.y) * TN;
+    const Cxt* a = A + bz * sA;
+    const Cxt* bmat = B + bz * sB;
+    Cxt* c = C + bz * sC;
+
+    __shared__ Cxt As[KC][TM + 1];  // [k][row]
+    __shared__ Cxt Bs[KC][TN + 1];  // [k][col]
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wr = wave & 1;   // 32-complex-row half
+    const int wc = wave >> 1;  // 32-col half
+    const int l16 = lane & 15;
+    const int lq = lane >> 4;  // 0..3: real k index within the MFMA step
+
+    acc_t acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = MF::zero();
+
+    for (int64_t kc0 = 0; kc0 < K; kc0 += KC) {
+        // A chunk: TM rows x KC cols (column-major source, rows contiguous)
+        for (int e = tid; e < TM * KC; e += 256) {
+            const int r = e % TM, kk = e / TM;
+            const int64_t gr = m0 + r, gk = kc0 + kk;
+            As[kk][r] = (gr < M && gk < K) ? a[gr + gk * lda] : Cxt{T(0), T(0)};
+        }
+        // B chunk: KC rows x TN cols
+        for (int e = tid; e < TN * KC; e += 256) {
+            const int kk = e % KC, cc = e / KC;
+            const int64_t gk = kc0 + kk, gc = n0 + cc;
+            Bs[kk][cc] = (gk < K && gc < N) ? bmat[gk + gc * ldb] : Cxt{T(0), T(0)};
+        }
+        __syncthreads();
+        // 2*KC real k values, 4 per MFMA
+#pragma unroll
+        for (int ks = 0; ks < 2 * KC; ks += 4) {
+            const int rk = ks + lq;     // real k of this lane
+            const int lk = rk >> 1;     // complex k
+            const int tk = rk & 1;      // 0: multiplies Re(B), 1: multiplies Im(B)
+            T av[4], bv[2];
+#pragma unroll
+            for (int tr = 0; tr < 4; ++tr) {
+                const int rrow = wr * 64 + tr * 16 + l16;  // real row in tile
+                const Cxt z = As[lk][rrow >> 1];
+                const int s = rrow & 1;
+                // A'[2i+s][2l+t]: s=0: (Re, -Im), s=1: (Im, Re)
+                av[tr] = s == 0 ? (tk == 0 ? z.re : -z.im) : (tk == 0 ? z.im : z.re);
+            }
+#pragma unroll
+            for (int tc = 0; tc < 2; ++tc) {
+                const Cxt z = Bs[lk][wc * 32 + tc * 16 + l16];
+                bv[tc] = tk == 0 ? z.re : z.im;
+            }
+#pragma unroll
+            for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+                for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = MF::mma(av[tr], bv[tc], acc[tr][tc]);
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds real rows 4*lq + v of each 16-row tile
+#pragma unroll
+    for (int tr = 0; tr < 4; ++tr) {
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) {
+            const int64_t col = n0 + wc * 32 + tc * 16 + l16;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t row = m0 + (wr * 64 + tr * 16) / 2 + 2 * lq + h;
+                if (row < M && col < N) {
+                    const Cxt v{acc[tr][tc][2 * h], acc[tr][tc][2 * h + 1]};
+                    Cxt* dst = c + row + col * ldc;
+                    if (MODE == 0) {
+                        *dst = *dst - v;
+                    } else {
+                        *dst = v;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int MODE>
+int gemm_launch(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, int64_t lda,
+                int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C, int64_t ldc,
+                int64_t sC, int64_t batch) {
+    if (M <= 0 || N <= 0 || K <= 0) return CCAB_OK;
+    dim3 grid((unsigned)((M + TM - 1) / TM), (unsigned)((N + TN - 1) / TN), (unsigned)batch);
+    hipLaunchKernelGGL((cgemm_kernel<T, MODE>), grid, dim3(256), 0, ctx->stream, M, N, K, A, lda,
+                       sA, B, ldb, sB, C, ldc, sC);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked triangular solves with the stored diagonal-block inverses.
+// Forward (unit L): X[kb] = Linv_kb * Y[kb]; Y[rows below] -= L[below, kb] X[kb]
+// Backward (U):     X[kb] = Uinv_kb * Y[kb]; Y[rows above] -= U[above, kb] X[kb]
+// Y is read-only for block kb in its step, so every workgroup recomputes
+// X[kb] from it and workgroup 0 stores it.
+// ---------------------------------------------------------------------------
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void trsv_step_kernel(const Cx<T>* __restrict__ LU, int64_t lda,
+                                                        int64_t strideA, int64_t n, int64_t k0,
+                                                        int nb, const Cx<T>* __restrict__ inv,
+                                                        int64_t inv_stride, int64_t blk,
+                                                        Cx<T>* Y, Cx<T>* X, int64_t ldb,
+                                                        int64_t strideB, int64_t nrhs) {
+    using C = Cx<T>;
+    const int64_t b = blockIdx.y / nrhs;
+    const int64_t rhs = blockIdx.y % nrhs;
+    const C* a = LU + b * strideA;
+    const C* iv = inv + b * inv_stride + blk * NB * NB;
+    C* y = Y + b * strideB + rhs * ldb;
+    C* x = X + b * strideB + rhs * ldb;
+    __shared__ C ys[NB];
+    __shared__ C xs[NB];
+    const int tid = threadIdx.x;
+    if (tid < NB) ys[tid] = tid < nb ? y[k0 + tid] : C{T(0), T(0)};
+    __syncthreads();
+    if (tid < NB) {
+        C s{T(0), T(0)};
+        for (int t = 0; t < nb; ++t) s += iv[tid + t * NB] * ys[t];
+        xs[tid] = s;
+        if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = s;
+    }
+    __syncthreads();
+    // rows to update: FWD -> [k0+nb, n), BWD -> [0, k0)
+    const int64_t r0 = FWD ? k0 + nb : 0;
+    const int64_t r1 = FWD ? n : k0;
+    const int64_t r = r0 + int64_t(blockIdx.x) * 256 + tid;
+    if (r < r1) {
+        C s{T(0), T(0)};
+        for (int t = 0; t < nb; ++t) s += a[r + (k0 + t) * lda] * xs[t];
+        y[r] = y[r] - s;
+    }
+}
+
+template <typename T>
+__global__ void gather_rows_kernel(const int32_t* __restrict__ perm, int64_t n,
+                                   const Cx<T>* __restrict__ B, int64_t ldb, int64_t strideB,
+                                   Cx<T>* __restrict__ Y, int64_t nrhs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t b = blockIdx.y / nrhs, r = blockIdx.y % nrhs;
+    if (i < n) Y[(b * nrhs + r) * n + i] = B[b * strideB + r * ldb + perm[b * n + i]];
+}
+
+template <typename T>
+__global__ void copy_kernel(const Cx<T>* __restrict__ src, int64_t lds, int64_t ss, Cx<T>* dst,
+                            int64_t ldd, int64_t sd, int64_t n, int64_t nrhs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t b = blockIdx.y / nrhs, r = blockIdx.y % nrhs;
+    if (i < n) dst[b * sd + r * ldd + i] = src[b * ss + r * lds + i];
+}
+
+__global__ void init_perm_kernel(int32_t* perm, int64_t n, int32_t* info) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) perm[blockIdx.y * n + i] = int32_t(i);
+    if (i == 0) info[blockIdx.y] = 0;
+}
+
+// perm from LAPACK-order pivots (for factors produced elsewhere)
+__global__ void perm_from_ipiv_kernel(const int32_t* __restrict__ ipiv, int64_t n, int32_t* perm,
+                                      int64_t batch) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    int32_t* pm = perm + b * n;
+    const int32_t* pv = ipiv + b * n;
+    for (int64_t i = 0; i < n; ++i) pm[i] = int32_t(i);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t p = pv[i];
+        if (p != i) {
+            const int32_t t = pm[i];
+            pm[i] = pm[p];
+            pm[p] = t;
+        }
+    }
+}
+
+inline int64_t nblocks_of(int64_t n) { return (n + NB - 1) / NB; }
+
+template <typename T>
+int lu_aux(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>** linv, Cx<T>** uinv, int32_t** perm,
+           int32_t** info) {
+    const size_t inv = size_t(batch) * nblocks_of(n) * NB * NB * sizeof(Cx<T>);
+    *linv = static_cast<Cx<T>*>(ctx_buf(ctx, SLOT_LU_LINV, inv));
+    *uinv = static_cast<Cx<T>*>(ctx_buf(ctx, SLOT_LU_UINV, inv));
+    *perm = static_cast<int32_t*>(ctx_buf(ctx, SLOT_LU_PERM, size_t(batch) * n * 4));
+    *info = static_cast<int32_t*>(ctx_buf(ctx, SLOT_LU_INFO, size_t(batch) * 4));
+    if (!*linv || !*uinv || !*perm || !*info) return CCAB_ENOMEM;
+    return CCAB_OK;
+}
+
+template <typename T>
+int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, int64_t strideA,
+                int32_t* ipiv) {
+    Cx<T>*linv, *uinv;
+    int32_t *perm, *info;
+    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
+    const int64_t inv_stride = nblocks_of(n) * NB * NB;
+    hipLaunchKernelGGL(init_perm_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
+                       dim3(256), 0, ctx->stream, perm, n, info);
+    const double B = double(batch);
+    for (int64_t kb = 0; kb < nblocks_of(n); ++kb) {
+        const int64_t k0 = kb * NB;
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        double pflops = 0.0;
+        for (int j = 0; j < nb; ++j) pflops += 8.0 * double(n - k0 - j - 1) * double(nb - j - 1);
+        {
+            ProfScope ps(ctx, PROF_LU_PANEL, pflops * B);
+            hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0,
+                               ctx->stream, A, lda, strideA, n, k0, nb, ipiv, perm, info);
+        }
+        {
+            ProfScope ps(ctx, PROF_LU_TRINV, 0.0);
+            hipLaunchKernelGGL(trinv_kernel<T>, dim3((unsigned)batch), dim3(NB), 0, ctx->stream,
+                               A, lda, strideA, k0, nb, linv + kb * NB * NB, uinv + kb * NB * NB,
+                               inv_stride);
+        }
+        {
+            ProfScope ps(ctx, PROF_LU_LASWP, 0.0);
+            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
+                               dim3(256), 0, ctx->stream, A, lda, strideA, n, k0, nb, ipiv);
+        }
+        const int64_t rest = n - k0 - nb;
+        if (rest > 0) {
+            {
+                // U12 = Linv * A12 (in place: M = nb <= 64, one row tile)
+                ProfScope ps(ctx, PROF_LU_TRSM, 8.0 * nb * double(rest) * nb * B);
+                CCAB_TRY((gemm_launch<T, 1>(ctx, nb, rest, nb, linv + kb * NB * NB, NB,
+                                            inv_stride, A + k0 + (k0 + nb) * lda, lda, strideA,
+                                            A + k0 + (k0 + nb) * lda, lda, strideA, batch)));
+            }
+            {
+                // A22 -= L21 * U12
+                ProfScope ps(ctx, PROF_LU_UPDATE, 8.0 * double(rest) * double(rest) * nb * B);
+                CCAB_TRY((gemm_launch<T, 0>(ctx, rest, rest, nb, A + (k0 + nb) + k0 * lda, lda,
+                                            strideA, A + k0 + (k0 + nb) * lda, lda, strideA,
+                                            A + (k0 + nb) + (k0 + nb) * lda, lda, strideA,
+                                            batch)));
+            }
+        }
+    }
+    CCAB_HIP(hipGetLastError());
+    ctx->lu_key = A;
+    ctx->lu_prec = sizeof(T) == 16 ? CCAB_C128 : CCAB_C64;
+    ctx->lu_n = n;
+    ctx->lu_batch = batch;
+    ctx->lu_lda = lda;
+    ctx->lu_stride = strideA;
+    // singular check (one small readback)
+    CCAB_TRY(pinned_reserve(ctx, size_t(batch) * 4));
+    CCAB_HIP(hipMemcpyAsync(ctx->pinned, info, size_t(batch) * 4, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    const int32_t* hi = static_cast<const int32_t*>(ctx->pinned);
+    for (int64_t b = 0; b < batch; ++b)
+        if (hi[b]) return ctx_fail(ctx, CCAB_ESINGULAR, "zero pivot in system %lld", (long long)b);
+    return CCAB_OK;
+}
+
+template <typename T>
+int lu_solve_t(ccab_ctx* ctx, int64_t n, int64_t batch, const Cx<T>* LU, int64_t lda,
+               int64_t strideA, const int32_t* ipiv, Cx<T>* B, int64_t ldb, int64_t strideB,
+               int64_t nrhs) {
+    Cx<T>*linv, *uinv;
+    int32_t *perm, *info;
+    const bool cached = ctx->lu_key == LU && ctx->lu_n == n && ctx->lu_batch == batch &&
+                        ctx->lu_lda == lda && ctx->lu_stride == strideA &&
+                        ctx->lu_prec == (sizeof(T) == 16 ? CCAB_C128 : CCAB_C64);
+    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
+    const int64_t inv_stride = nblocks_of(n) * NB * NB;
+    const int64_t nblk = nblocks_of(n);
+    if (!cached) {
+        // factors from elsewhere: rebuild the diagonal inverses and permutation
+        hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3((unsigned)((batch + 63) / 64)), dim3(64), 0,
+                           ctx->stream, ipiv, n, perm, batch);
+        for (int64_t kb = 0; kb < nblk; ++kb) {
+            const int64_t k0 = kb * NB;
+            const int nb = (int)std::min<int64_t>(NB, n - k0);
+            hipLaunchKernelGGL(trinv_kernel<T>, dim3((unsigned)batch), dim3(NB), 0, ctx->stream,
+                               LU, lda, strideA, k0, nb, linv + kb * NB * NB, uinv + kb * NB * NB,
+                               inv_stride);
+        }
+        ctx->lu_key = LU;
+        ctx->lu_prec = sizeof(T) == 16 ? CCAB_C128 : CCAB_C64;
+        ctx->lu_n = n;
+        ctx->lu_batch = batch;
+        ctx->lu_lda = lda;
+        ctx->lu_stride = strideA;
+    }
+    const int64_t ystride = nrhs * n;
+    Cx<T>* Y = static_cast<Cx<T>*>(ctx_buf(ctx, SLOT_LU_Y, size_t(batch) * ystride * sizeof(Cx<T>)));
+    if (!Y) return CCAB_ENOMEM;
+    const unsigned nbr = (unsigned)(batch * nrhs);
+    const dim3 vgrid((unsigned)((n + 255) / 256), nbr);
+    hipLaunchKernelGGL(gather_rows_kernel<T>, vgrid, dim3(256), 0, ctx->stream, perm, n, B, ldb,
+                       strideB, Y, nrhs);
+    // forward: B <- L^{-1} Y  (block by block)
+    for (int64_t kb = 0; kb < nblk; ++kb) {
+        const int64_t k0 = kb * NB;
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        const int64_t rows = n - k0 - nb;
+        const unsigned gx = (unsigned)std::max<int64_t>(1, (rows + 255) / 256);
+        ProfScope ps(ctx, PROF_TRSV, double(rows) * nb * sizeof(Cx<T>) * nbr);
+        hipLaunchKernelGGL((trsv_step_kernel<T, true>), dim3(gx, nbr), dim3(256), 0, ctx->stream,
+                           LU, lda, strideA, n, k0, nb, linv, inv_stride, kb, Y, B, ldb, strideB,
+                           nrhs);
+    }
+    // Y <- B, backward: B <- U^{-1} Y
+    hipLaunchKernelGGL(copy_kernel<T>, vgrid, dim3(256), 0, ctx->stream, B, ldb, strideB, Y, n,
+                       ystride, n, nrhs);
+    for (int64_t kb = nblk - 1; kb >= 0; --kb) {
+        const int64_t k0 = kb * NB;
+        const int nb = (int)std::min<int64_t>(NB, n - k0);
+        const int64_t rows = k0;
+        const unsigned gx = (unsigned)std::max<int64_t>(1, (rows + 255) / 256);
+        ProfScope ps(ctx, PROF_TRSV, double(rows) * nb * sizeof(Cx<T>) * nbr);
+        hipLaunchKernelGGL((trsv_step_kernel<T, false>), dim3(gx, nbr), dim3(256), 0,
+                           ctx->stream, LU, lda, strideA, n, k0, nb, uinv, inv_stride, kb, Y, B,
+                           ldb, strideB, nrhs);
+    }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+}  // namespace
+
+int lu_factor(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, void* d_A, int64_t lda,
+              int64_t strideA, int32_t* d_ipiv) {
+    CCAB_CHECK(n > 0 && batch > 0 && lda >= n && (batch == 1 || strideA >= lda * n),
+               "invalid LU dimensions");
+    CCAB_CHECK(n < (int64_t(1) << 31), "n too large for 32-bit pivots");
+    if (prec == CCAB_C128)
+        return lu_factor_t<double>(ctx, n, batch, static_cast<c128*>(d_A), lda, strideA, d_ipiv);
+    if (prec == CCAB_C64)
+        return lu_factor_t<float>(ctx, n, batch, static_cast<c64*>(d_A), lda, strideA, d_ipiv);
+    return ctx_fail(ctx, CCAB_EINVAL, "unknown precision %d", prec);
+}
+
+int lu_solve(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_LU, int64_t lda,
+             int64_t strideA, const int32_t* d_ipiv, void* d_B, int64_t ldb, int64_t strideB,
+             int64_t nrhs) {
+    CCAB_CHECK(n > 0 && batch > 0 && nrhs > 0 && ldb >= n, "invalid LU solve dimensions");
+    if (prec == CCAB_C128)
+        return lu_solve_t<double>(ctx, n, batch, static_cast<const c128*>(d_LU), lda, strideA,
+                                  d_ipiv, static_cast<c128*>(d_B), ldb, strideB, nrhs);
+    if (prec == CCAB_C64)
+        return lu_solve_t<float>(ctx, n, batch, static_cast<const c64*>(d_LU), lda, strideA,
+                                 d_ipiv, static_cast<c64*>(d_B), ldb, strideB, nrhs);
+    return ctx_fail(ctx, CCAB_EINVAL, "unknown precision %d", prec);
+}
+
+
+// exposed for the GEMM unit test entry point
+int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const void* A,
+                 int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc) {
+    if (prec == CCAB_C128)
+        return gemm_launch<double, 0>(ctx, M, N, K, static_cast<const c128*>(A), lda, 0,
+                                      static_cast<const c128*>(B), ldb, 0,
+                                      static_cast<c128*>(C), ldc, 0, 1);
+    return gemm_launch<float, 0>(ctx, M, N, K, static_cast<const c64*>(A), lda, 0,
+                                 static_cast<const c64*>(B), ldb, 0, static_cast<c64*>(C), ldc,
+                                 0, 1);
+}

@@ -1,0 +1,142 @@
+"""Surface solve on the GPU -- mirror of src/solve.rs and src/solve/gmres.rs.
+
+solve_for_surface(predata, rhs_inc) -> (phi, vn)      solve.rs:11-71
+solve_lu(a, x)                                        solve.rs:166-171
+GMRES(max_it, thresh); .a = matrix; .solve(x)         gmres.rs:22-59
+sweep(predata) -> (phi, vn, stats)                    analysis.rs:86-111 (surface part)
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import torch
+
+from . import capi
+from .preprocess import PreData
+
+
+def _ptr(t) -> ct.c_void_p:
+    return ct.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _colmajor_ok(a: torch.Tensor) -> int:
+    """Leading dimension of a column-major 2-D tensor view, or raise."""
+    if a.dim() != 2 or a.stride(0) != 1 or a.stride(1) < a.shape[0]:
+        raise ValueError("matrix must be a column-major view (stride (1, ld))")
+    return a.stride(1)
+
+
+def solve_for_surface(predata: PreData, rhs_inc: torch.Tensor, ctx=None):
+    """Assemble and solve the surface system at the current frequency."""
+    ctx = ctx or capi.default_context(predata.device.index or 0)
+    n = predata.get_num_eqn()
+    phi = torch.empty(n, dtype=torch.complex128, device=predata.device)
+    vn = torch.empty(n, dtype=torch.complex128, device=predata.device)
+    st = capi.SolveStats()
+    cfg = predata.solver.to_c()
+    ri = rhs_inc.to(torch.complex128).contiguous()
+    ctx.call("ccab_solve_surface", ct.byref(predata.physics()), _ptr(predata.d_cpts), n, _ptr(ri),
+             ct.byref(cfg), _ptr(phi), _ptr(vn), ct.byref(st))
+    ctx.synchronize()
+    predata.last_stats = st.as_dict()
+    return phi, vn
+
+
+def sweep(predata: PreData, frequencies=None, ctx=None):
+    """Surface solution at every frequency: phi, vn of shape (nfreq, n)."""
+    ctx = ctx or capi.default_context(predata.device.index or 0)
+    fs = predata.get_frequencies() if frequencies is None else list(frequencies)
+    n = predata.get_num_eqn()
+    phi = torch.empty((len(fs), n), dtype=torch.complex128, device=predata.device)
+    vn = torch.empty((len(fs), n), dtype=torch.complex128, device=predata.device)
+    stats = (capi.SolveStats * len(fs))()
+    cfg = predata.solver.to_c()
+    w = predata.incident_wave.to_c()
+    ctx.call("ccab_sweep", predata.physics_array(fs), len(fs), ct.byref(w), _ptr(predata.d_cpts),
+             n, ct.byref(cfg), _ptr(phi), _ptr(vn), stats)
+    ctx.synchronize()
+    return phi, vn, [s.as_dict() for s in stats]
+
+
+def solve_lu(a: torch.Tensor, x: torch.Tensor, ctx=None):
+    """x <- a^{-1} x (a is not modified: the library factors a copy)."""
+    ctx = ctx or capi.default_context(a.device.index or 0)
+    lda = _colmajor_ok(a)
+    if a.dtype != torch.complex128 or x.dtype != torch.complex128 or not x.is_contiguous():
+        raise ValueError("solve_lu takes complex128 tensors and a contiguous x")
+    ctx.call("ccab_solve_lu", a.shape[0], _ptr(a), lda, _ptr(x))
+    ctx.synchronize()
+    return x
+
+
+def lu_factor_batched(A: torch.Tensor, ctx=None):
+    """In-place batched LU of A (batch, n, n) given as column-major systems
+    (A[b].stride() == (1, n)); returns ipiv (batch, n) int32 (0-based)."""
+    ctx = ctx or capi.default_context(A.device.index or 0)
+    nb, n, _ = A.shape
+    if A.stride(1) != 1 or A.stride(2) != n or A.stride(0) != n * n:
+        raise ValueError("expected column-major systems packed back to back")
+    prec = capi.C128 if A.dtype == torch.complex128 else capi.C64
+    ipiv = torch.empty((nb, n), dtype=torch.int32, device=A.device)
+    ctx.call("ccab_lu_factor_batched", prec, n, nb, _ptr(A), n, n * n, _ptr(ipiv))
+    ctx.synchronize()
+    return ipiv
+
+
+def lu_solve_batched(LU: torch.Tensor, ipiv: torch.Tensor, B: torch.Tensor, ctx=None):
+    """B (batch, n) or (batch, nrhs, n) overwritten with the solutions."""
+    ctx = ctx or capi.default_context(LU.device.index or 0)
+    nb, n, _ = LU.shape
+    prec = capi.C128 if LU.dtype == torch.complex128 else capi.C64
+    nrhs = 1 if B.dim() == 2 else B.shape[1]
+    if not B.is_contiguous():
+        raise ValueError("B must be contiguous")
+    ctx.call("ccab_lu_solve_batched", prec, n, nb, _ptr(LU), n, n * n, _ptr(ipiv), _ptr(B), n,
+             n * nrhs, nrhs)
+    ctx.synchronize()
+    return B
+
+
+def solve_mixed_batched(A128: torch.Tensor, A64: torch.Tensor, x: torch.Tensor, ir_tol=1e-12,
+                        ir_max_steps=10, ctx=None):
+    """c64 LU (destroys A64) + FP64 refinement against A128; x (batch, n) b -> x."""
+    ctx = ctx or capi.default_context(A128.device.index or 0)
+    nb, n, _ = A128.shape
+    stats = (capi.SolveStats * nb)()
+    ctx.call("ccab_solve_mixed_batched", n, nb, _ptr(A128), _ptr(A64), n, n * n, _ptr(x),
+             float(ir_tol), int(ir_max_steps), stats)
+    ctx.synchronize()
+    return x, [s.as_dict() for s in stats]
+
+
+class GMRES:
+    """Restarted GMRES with the reference's parameters (gmres.rs:12-59)."""
+
+    def __init__(self, max_it: int, thresh: float, restart: int = 0, ref_mgs: bool = True,
+                 stop_rule: int = capi.STOP_BACKWARD_ERROR):
+        self.max_it = max_it
+        self.thresh = thresh
+        self.restart = restart
+        self.ref_mgs = ref_mgs
+        self.stop_rule = stop_rule
+        self.a = None
+        self.num_mv = 0
+        self.stats = None
+
+    def solve(self, x: torch.Tensor, x0: torch.Tensor | None = None, ctx=None):
+        """x: b on entry, solution on exit (complex128, contiguous)."""
+        if self.a is None:
+            raise ValueError("GMRES needs an operator: set .a")
+        a = self.a
+        ctx = ctx or capi.default_context(a.device.index or 0)
+        lda = _colmajor_ok(a)
+        prec = capi.C128 if a.dtype == torch.complex128 else capi.C64
+        p = capi.gmres_params(self.max_it, self.thresh, self.restart, self.ref_mgs,
+                              self.stop_rule, prec)
+        st = capi.SolveStats()
+        ctx.call("ccab_gmres_dense", a.shape[0], _ptr(a), lda, _ptr(x),
+                 _ptr(x0) if x0 is not None else None, ct.byref(p), ct.byref(st))
+        ctx.synchronize()
+        self.stats = st.as_dict()
+        self.num_mv = st.num_mv
+        return x

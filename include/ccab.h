@@ -1,0 +1,221 @@
+/*
+ * ccab.h -- C ABI of the MI355X (gfx950) collocation-BEM hot path.
+ *
+ * Drop-in boundary for the Rust crate dbartilson/cc_abstemious v1.2.0.  Each
+ * entry point names the reference function it replaces (file:line in that
+ * crate).  Conventions:
+ *  - complex numbers are interleaved {re, im} (num-complex Complex<f64> /
+ *    Complex<f32> layout, #[repr(C)]-compatible); matrices are column-major
+ *    like nalgebra's DMatrix, with an explicit leading dimension;
+ *  - pointers named d_* are device (HBM) pointers, h_* are host pointers;
+ *  - every function returns CCAB_OK (0) or a negative status; the message of
+ *    the last failure is ccab_last_error(ctx).  Nothing aborts or throws
+ *    across the ABI;
+ *  - a context owns one HIP stream (or borrows the caller's, see
+ *    ccab_set_stream) and a device workspace; use one context per host thread.
+ *  - collocation points are passed as the 8-double record of
+ *    CollocationPoint (mesh_data.rs:71-77): x y z nx ny nz area wt.
+ */
+#ifndef CCAB_H
+#define CCAB_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCAB_VERSION 10000
+
+#define CCAB_OK 0
+#define CCAB_EINVAL (-1)   /* invalid argument (sizes, enums, singleton rule) */
+#define CCAB_EHIP (-2)     /* HIP runtime error */
+#define CCAB_ENOMEM (-3)   /* device allocation failed */
+#define CCAB_ENAN (-4)     /* NaN/Inf produced (zero-area element, origin source) */
+#define CCAB_ESINGULAR (-5) /* exact zero pivot in LU */
+
+typedef struct { double re, im; } ccab_c128;
+typedef struct { float re, im; } ccab_c64;
+typedef struct ccab_ctx ccab_ctx;
+
+/* input_data.rs:407-449 */
+enum { CCAB_CLASSICAL = 0, CCAB_BURTON_MILLER = 1 };
+enum { CCAB_EXTERIOR = 0, CCAB_INTERIOR = 1 };
+enum { CCAB_BC_PRESSURE = 0, CCAB_BC_NORMAL_VELOCITY = 1, CCAB_BC_IMPEDANCE = 2 };
+enum { CCAB_WAVE_PLANE = 0, CCAB_WAVE_SPHERICAL = 1 };
+/* influence_matrix.rs:164-167 */
+enum { CCAB_LHS = 0, CCAB_RHS = 1 };
+/* storage / arithmetic precision of a matrix */
+enum { CCAB_C128 = 0, CCAB_C64 = 1 };
+/* solver kinds, input_data.rs:419-424 (+ the mixed-precision LU) */
+enum { CCAB_SOLVER_DIRECT = 0, CCAB_SOLVER_ITERATIVE = 1, CCAB_SOLVER_HIERARCHICAL = 2,
+       CCAB_SOLVER_DIRECT_MIXED = 3, CCAB_SOLVER_ITERATIVE_MIXED = 4 };
+/* GMRES stopping rule */
+enum { CCAB_STOP_BACKWARD_ERROR = 0, /* gmres.rs:142-144 */
+       CCAB_STOP_RELATIVE_RESIDUAL = 1 };
+
+/* The physics of one frequency (UserInput fields + PreData frequency,
+ * input_data.rs:471-484, preprocess.rs:263-304). */
+typedef struct {
+    double frequency;     /* Hz; k = 2*pi*f / c */
+    double sound_speed;
+    double mass_density;
+    int32_t method;       /* CCAB_CLASSICAL / CCAB_BURTON_MILLER */
+    int32_t problem_type; /* CCAB_EXTERIOR / CCAB_INTERIOR */
+    int32_t bc_type;      /* CCAB_BC_* */
+    int32_t pad_;
+    double bc_value[2];   /* SurfaceBoundaryCondition.value */
+} ccab_physics;
+
+/* IncidentWaveInput (input_data.rs:426-436) */
+typedef struct {
+    int32_t kind;          /* CCAB_WAVE_PLANE / CCAB_WAVE_SPHERICAL */
+    int32_t pad_;
+    double vec[3];         /* plane: direction; spherical: origin */
+    double amplitude[2];   /* pressure amplitude */
+} ccab_wave;
+
+typedef struct {
+    int64_t max_iterations;  /* <= 0 -> 1000, capped at n (gmres.rs:41-48) */
+    double tolerance;        /* <= 0 -> 1e-5 */
+    int32_t restart;         /* 0 -> ceil(sqrt(max_iterations)) (gmres.rs:276) */
+    int32_t ref_mgs;         /* 1 -> Arnoldi coefficient qk1.dotc(q_i) (gmres.rs:168) */
+    int32_t stop_rule;       /* CCAB_STOP_* */
+    int32_t matrix_prec;     /* CCAB_C128 or CCAB_C64 operator */
+} ccab_gmres_params;
+
+typedef struct {
+    int64_t num_mv;          /* matrix-vector products (gmres.rs:66) */
+    int64_t iterations;      /* inner Arnoldi steps */
+    int64_t restarts;
+    int64_t ir_steps;        /* FP64 iterative-refinement corrections */
+    double backward_error;   /* ||b-Ax|| / (||A||_F ||x|| + ||b||) at exit */
+    double rel_residual;     /* ||b-Ax|| / ||b|| at exit (FP64 operator) */
+    int32_t converged;
+    int32_t pad_;
+} ccab_solve_stats;
+
+typedef struct {
+    int32_t solver;          /* CCAB_SOLVER_* */
+    int32_t pad_;
+    ccab_gmres_params gmres; /* ITERATIVE(_MIXED) */
+    double ir_tolerance;     /* mixed: stop when ||dx||/||x|| < this (<=0 -> 1e-12) */
+    int32_t ir_max_steps;    /* mixed: <=0 -> 10 */
+    int32_t pad2_;
+} ccab_solver_cfg;
+
+/* ---- context ---------------------------------------------------------- */
+int ccab_version(void);
+int ccab_create(int device, ccab_ctx** out);
+int ccab_destroy(ccab_ctx* ctx);
+const char* ccab_last_error(const ccab_ctx* ctx);
+/* Run on the caller's hipStream_t (NULL = the context's own stream). */
+int ccab_set_stream(ccab_ctx* ctx, void* hip_stream);
+int ccab_synchronize(ccab_ctx* ctx);
+
+/* In-process kernel timing with HIP events on the context's stream (no
+ * reference counterpart; the reference only logs).  Classes: 0 assembly
+ * (work = bytes stored), 1 LU panel, 2 LU triangular update, 3 LU trailing
+ * complex GEMM (work = flops), 4 row swaps, 5 diagonal-block inverses,
+ * 6 triangular solves, 7 GEMV (work = matrix bytes), 8 Gram-Schmidt. */
+int ccab_profile_enable(ccab_ctx* ctx, int on);
+int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double* work,
+                      int reset);
+
+/* ---- geometry (host) --------------------------------------------------- */
+/* One centroid collocation point per linear tri/quad element:
+ * elements.rs:49-181 (TRIGP1 / QUADGP1, detj incl. the quad 0.25 factor at
+ * elements.rs:168) + preprocess.rs:126-153.  conn: 4 node ids per element
+ * (-1 padded), nverts 3 or 4.  h_cpts: 8*nelem doubles. */
+int ccab_collocation_points(const double* h_nodes, const int64_t* h_conn,
+                            const int32_t* h_nverts, int64_t nelem, double* h_cpts);
+
+/* ---- incident wave (device) ---------------------------------------------- */
+/* incident_wave.rs:7-63 (get_incident_surface): d_rhs[n] */
+int ccab_incident_surface(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
+                          const double* d_cpts, int64_t n, ccab_c128* d_rhs);
+
+/* ---- influence matrices (device) ------------------------------------------ */
+/* influence_matrix.rs:64-98 (get_dense_surface_matrices) -> (H, G).  Writes
+ * the n x n matrices (either pointer may be NULL) in precision prec
+ * (CCAB_C128: FP64 arithmetic; CCAB_C64: FP32 arithmetic), leading dim ld. */
+int ccab_dense_surface_matrices(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                                int64_t n, int prec, void* d_H, void* d_G, int64_t ld);
+
+/* Fused assembly + BC algebra of solve.rs:24-71 for `batch` frequencies on
+ * one mesh: LHS_b = alpha_b H_b + beta_b G_b (get_lhs_factors,
+ * influence_matrix.rs:130-146) written as c128 (d_lhs128) and/or c64
+ * (d_lhs64, rounded from the same FP64 values), system b at offset b*stride,
+ * and the right-hand side of the BC algebra into d_rhs[b*n..]:
+ *   NormalVelocity: G v - rhs_inc, Pressure: rhs_inc - H p, Impedance: -rhs_inc.
+ * d_rhs_inc may equal d_rhs.  ph[batch] are host structs. */
+int ccab_assemble_system(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch,
+                         const double* d_cpts, int64_t n, ccab_c128* d_lhs128,
+                         ccab_c64* d_lhs64, int64_t ld, int64_t stride,
+                         const ccab_c128* d_rhs_inc, ccab_c128* d_rhs);
+
+/* influence_matrix.rs:170-231 (get_surface_row_or_column): one row (ni == 1)
+ * or one column (nj == 1) of alpha*H + beta*G for side CCAB_LHS / CCAB_RHS;
+ * the diagonal shifts are added at the first position holding the diagonal.
+ * Returns CCAB_EINVAL unless exactly one of ni, nj is 1 (the reference only
+ * logs that error, influence_matrix.rs:174-176). */
+int ccab_surface_row_or_column(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                               int64_t n, int side, const int64_t* d_i, int64_t ni,
+                               const int64_t* d_j, int64_t nj, ccab_c128* d_out);
+
+/* ---- dense LU (device) ------------------------------------------------------ */
+/* solve.rs:166-171 (solve_lu): factor a copy of A (c128, FP64 MFMA trailing
+ * updates) and overwrite d_x (= b on entry) with the solution. */
+int ccab_solve_lu(ccab_ctx* ctx, int64_t n, const ccab_c128* d_A, int64_t lda, ccab_c128* d_x);
+
+/* Batched right-looking partial-pivot LU in place, prec CCAB_C128 or CCAB_C64.
+ * d_ipiv[b*n + i] = 0-based row swapped with row i (LAPACK order). */
+int ccab_lu_factor_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, void* d_A,
+                           int64_t lda, int64_t strideA, int32_t* d_ipiv);
+/* Solve with the factors for nrhs right-hand sides per system (in place). */
+int ccab_lu_solve_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch,
+                          const void* d_LU, int64_t lda, int64_t strideA,
+                          const int32_t* d_ipiv, void* d_B, int64_t ldb, int64_t strideB,
+                          int64_t nrhs);
+
+/* Mixed precision (no reference counterpart; replaces solve_lu for the
+ * sweep): factor d_A64 in place in complex FP32 on MFMA, then FP64 iterative
+ * refinement with residuals against d_A128 until ||dx||/||x|| < ir_tol.
+ * d_x: b on entry, x on exit (n x batch, stride n).  stats[batch] or NULL. */
+int ccab_solve_mixed_batched(ccab_ctx* ctx, int64_t n, int64_t batch,
+                             const ccab_c128* d_A128, ccab_c64* d_A64, int64_t lda,
+                             int64_t strideA, ccab_c128* d_x, double ir_tol,
+                             int ir_max_steps, ccab_solve_stats* stats);
+
+/* ---- GMRES (device) --------------------------------------------------------- */
+/* gmres.rs:35-202 over a dense operator (c128 or c64 per p->matrix_prec).
+ * d_x: b on entry, x on exit; d_x0: initial guess or NULL (zero start as the
+ * reference, gmres.rs:38-39). */
+int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda,
+                     ccab_c128* d_x, const ccab_c128* d_x0, const ccab_gmres_params* p,
+                     ccab_solve_stats* stats);
+
+/* ---- the surface solve (device) ---------------------------------------------- */
+/* solve.rs:11-71 (solve_for_surface, dense branches): assembly + BC algebra +
+ * LU / GMRES / mixed LU with FP64 refinement; outputs surface phi and vn. */
+int ccab_solve_surface(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts, int64_t n,
+                       const ccab_c128* d_rhs_inc, const ccab_solver_cfg* cfg,
+                       ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats);
+
+/* analysis.rs:86-111 sweep loop (surface part) for nfreq frequencies at once:
+ * batched assembly + batched solve.  d_phi, d_vn: n x nfreq.  stats[nfreq]. */
+int ccab_sweep(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+               const double* d_cpts, int64_t n, const ccab_solver_cfg* cfg,
+               ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats);
+
+/* ---- building block ------------------------------------------------------- */
+/* C -= A B for complex column-major matrices (M x K times K x N) on MFMA; the
+ * trailing-update kernel of the LU (replaces the rank-1 updates of nalgebra's
+ * gauss_step behind solve.rs:169).  prec CCAB_C128 or CCAB_C64. */
+int ccab_cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K,
+                      const void* d_A, int64_t lda, const void* d_B, int64_t ldb, void* d_C,
+                      int64_t ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -370,8 +370,8 @@ def aca(tol: float, A: np.ndarray):
     m, n = A.shape
 
     def cb(user, is_row, idx, out):
-        src = A[idx, :] if is_row else A[:, idx]
-        ct.memmove(out, np.ascontiguousarray(src).ctypes.data, src.nbytes)
+        src = np.ascontiguousarray(A[idx, :] if is_row else A[:, idx])
+        ct.memmove(out, src.ctypes.data, src.nbytes)
 
     fn = ROWCOL_FN(cb)
     res = Aca()

@@ -1,0 +1,119 @@
+"""CPU-side tests of the product: the C ABI library loads and exports every
+declared symbol, host-side geometry matches the oracle, mesh generators and
+the VTK round trip.  No GPU calls."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cc_abstemious_amd import capi, mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "ccab.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ccab_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.lib()
+    names = _header_functions()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(L, name), name
+    # and the ctypes table covers exactly the header
+    assert sorted(capi.SIGNATURES) == names
+    assert L.ccab_version() == 10000
+
+
+def test_struct_layouts():
+    assert ct.sizeof(capi.Physics) == 56
+    assert ct.sizeof(capi.Wave) == 48
+    assert ct.sizeof(capi.GmresParams) == 32
+    assert ct.sizeof(capi.SolveStats) == 56
+    assert ct.sizeof(capi.SolverCfg) == 56
+
+
+@pytest.mark.parametrize("fname,body", [("sphere.vtk", 3), ("refined_sphere.vtk", 3),
+                                        ("sphere_quad.vtk", 2)])
+def test_host_geometry_matches_oracle(fname, body):
+    path = os.path.join(GOLD, fname)
+    m = mesh.read_vtk(path, body)
+    cp = m.collocation_points()
+    ref = O.mesh_cpts_from_vtk(path, body)
+    assert cp.shape == ref.shape
+    np.testing.assert_allclose(cp, ref, rtol=0, atol=1e-15)
+
+
+def test_degenerate_element_is_an_error():
+    nodes = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0]], float)
+    m = mesh.Mesh(nodes, np.array([[0, 1, 2, -1]]), np.array([3]))
+    with pytest.raises(capi.CcabError):
+        m.collocation_points()
+
+
+def test_vtk_reader_body_rules(tmp_path):
+    m = mesh.icosphere(1)
+    p = tmp_path / "ico.vtk"
+    mesh.write_vtk(str(p), m, body_index=3)
+    back = mesh.read_vtk(str(p), 3)
+    np.testing.assert_allclose(back.nodes, m.nodes)
+    assert (back.conn == m.conn).all()
+    # the oracle's independent reader agrees
+    ref = O.mesh_cpts_from_vtk(str(p), 3)
+    np.testing.assert_allclose(back.collocation_points(), ref, atol=1e-15)
+    with pytest.raises(ValueError):
+        mesh.read_vtk(str(p), 4)
+
+
+@pytest.mark.parametrize("level,count", [(0, 20), (1, 80), (3, 1280)])
+def test_icosphere(level, count):
+    m = mesh.icosphere(level)
+    cp = m.collocation_points()
+    assert cp.shape[0] == count
+    assert (np.einsum("ij,ij->i", cp[:, :3], cp[:, 3:6]) > 0).all()   # outward
+    area = (cp[:, 6] * cp[:, 7]).sum()
+    assert 4 * np.pi * {0: 0.6, 1: 0.9, 3: 0.99}[level] < area < 4 * np.pi
+
+
+def test_geodesic_and_cube_sphere():
+    g = mesh.geodesic(8)
+    assert g.num_elements == 1280
+    cs = mesh.cube_sphere(8)
+    assert cs.num_elements == 384
+    cp = cs.collocation_points()
+    assert (cp[:, 7] == 4.0).all()
+    assert (np.einsum("ij,ij->i", cp[:, :3], cp[:, 3:6]) > 0).all()
+    # quarter-area quirk of the reference quads: sum wt*area ~ (4 pi)/4
+    assert 0.9 * np.pi < (cp[:, 6] * cp[:, 7]).sum() < np.pi
+
+
+def test_perturbed_sphere_and_harmonics():
+    rng = np.random.default_rng(0)
+    c = rng.normal(0, 0.03, 81)
+    m = mesh.perturbed_sphere(8, c)
+    assert m.num_elements == 1280
+    r = np.linalg.norm(m.nodes, axis=1)
+    assert 0.7 < r.min() and r.max() < 1.3
+    # orthonormality of the real spherical harmonics (Lebedev-free check on a fine grid)
+    th = np.linspace(0, np.pi, 181)
+    ph = np.linspace(0, 2 * np.pi, 361)[:-1]
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    Y = mesh.real_sph_harm(3, T.ravel(), P.ravel())
+    w = (np.sin(T) * (np.pi / 180) * (2 * np.pi / 360)).ravel()
+    G = (Y * w) @ Y.T
+    np.testing.assert_allclose(G, np.eye(16), atol=2e-3)
+
+
+def test_tools_linspace_logspace():
+    from cc_abstemious_amd.preprocess import linspace, logspace
+    x = linspace(10.0, 1000.0, 50)
+    assert x[0] == 10.0 and x[-1] == 1000.0 and len(x) == 50
+    y = logspace(10.0, 1000.0, 3)
+    assert y == [10.0, pytest.approx(100.0), 1000.0]
