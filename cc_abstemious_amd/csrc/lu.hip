@@ -283,9 +283,15 @@ __global__ __launch_bounds__(256) void cgemm_kernel(int64_t M, int64_t N, int64_
             const int lk = rk >> 1;     // complex k
             const int tk = rk & 1;      // 0: multiplies Re(B), 1: multiplies Im(B)
             T av[4], bv[2];
+            // MFMA row l16 -> real row of the tile.  The f32 instruction returns
+            // row 4*(lane/16) + v in register v; the f64 one returns row
+            // 4*v + lane/16, so for f64 the A rows are fed transposed within
+            // each group of 16, which makes both precisions hand back real rows
+            // 4*lq + v (= re/im pairs of two complex rows) in registers v.
+            const int mrow = sizeof(T) == 8 ? 4 * (l16 & 3) + (l16 >> 2) : l16;
 #pragma unroll
             for (int tr = 0; tr < 4; ++tr) {
-                const int rrow = wr * 64 + tr * 16 + l16;  // real row in tile
+                const int rrow = wr * 64 + tr * 16 + mrow;  // real row in tile
                 const Cxt z = As[lk][rrow >> 1];
                 const int s = rrow & 1;
                 // A'[2i+s][2l+t]: s=0: (Re, -Im), s=1: (Im, Re)

@@ -69,13 +69,21 @@ def solve_lu(a: torch.Tensor, x: torch.Tensor, ctx=None):
     return x
 
 
+def _check_batch(A: torch.Tensor):
+    """A (batch, n, n) holds column-major systems back to back: A[b, i, j]
+    at offset b n^2 + i + j n, i.e. A = S.transpose(1, 2) for contiguous S."""
+    nb, n, m = A.shape
+    if n != m or A.stride(1) != 1 or A.stride(2) != n or (nb > 1 and A.stride(0) != n * n):
+        raise ValueError("expected column-major systems packed back to back "
+                         "(use S.transpose(1, 2) of a contiguous (batch, n, n) S)")
+    return nb, n
+
+
 def lu_factor_batched(A: torch.Tensor, ctx=None):
-    """In-place batched LU of A (batch, n, n) given as column-major systems
-    (A[b].stride() == (1, n)); returns ipiv (batch, n) int32 (0-based)."""
+    """In-place batched LU of A (batch, n, n), column-major systems;
+    returns ipiv (batch, n) int32 (0-based, LAPACK order)."""
     ctx = ctx or capi.default_context(A.device.index or 0)
-    nb, n, _ = A.shape
-    if A.stride(1) != 1 or A.stride(2) != n or A.stride(0) != n * n:
-        raise ValueError("expected column-major systems packed back to back")
+    nb, n = _check_batch(A)
     prec = capi.C128 if A.dtype == torch.complex128 else capi.C64
     ipiv = torch.empty((nb, n), dtype=torch.int32, device=A.device)
     ctx.call("ccab_lu_factor_batched", prec, n, nb, _ptr(A), n, n * n, _ptr(ipiv))
@@ -86,7 +94,7 @@ def lu_factor_batched(A: torch.Tensor, ctx=None):
 def lu_solve_batched(LU: torch.Tensor, ipiv: torch.Tensor, B: torch.Tensor, ctx=None):
     """B (batch, n) or (batch, nrhs, n) overwritten with the solutions."""
     ctx = ctx or capi.default_context(LU.device.index or 0)
-    nb, n, _ = LU.shape
+    nb, n = _check_batch(LU)
     prec = capi.C128 if LU.dtype == torch.complex128 else capi.C64
     nrhs = 1 if B.dim() == 2 else B.shape[1]
     if not B.is_contiguous():
@@ -101,7 +109,10 @@ def solve_mixed_batched(A128: torch.Tensor, A64: torch.Tensor, x: torch.Tensor, 
                         ir_max_steps=10, ctx=None):
     """c64 LU (destroys A64) + FP64 refinement against A128; x (batch, n) b -> x."""
     ctx = ctx or capi.default_context(A128.device.index or 0)
-    nb, n, _ = A128.shape
+    nb, n = _check_batch(A128)
+    _check_batch(A64)
+    if A128.dtype != torch.complex128 or A64.dtype != torch.complex64 or not x.is_contiguous():
+        raise ValueError("A128 complex128, A64 complex64, x contiguous complex128")
     stats = (capi.SolveStats * nb)()
     ctx.call("ccab_solve_mixed_batched", n, nb, _ptr(A128), _ptr(A64), n, n * n, _ptr(x),
              float(ir_tol), int(ir_max_steps), stats)

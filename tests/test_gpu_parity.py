@@ -122,7 +122,6 @@ def test_solve_lu_reference_fixture():
     # solve.rs:211-218 through the GPU LU (c128, FP64 MFMA)
     _note("solve_lu 100x100 fixture")
     a, b = O.generate_random_ab(100, 100, 10)
-    A = torch.from_numpy(np.asfortranarray(a)).cuda()
     A = colmajor(100, 100, torch.complex128, "cuda").copy_(torch.from_numpy(a))
     x = torch.from_numpy(b.copy()).cuda()
     solve.solve_lu(A, x)
@@ -140,7 +139,7 @@ def test_lu_batched_vs_oracle(n, batch, prec):
     dt = torch.complex128 if prec == capi.C128 else torch.complex64
     As = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in range(batch)]
     bs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(batch)]
-    A = torch.stack([torch.from_numpy(a.T.copy()) for a in As]).to(dt).cuda()  # column-major
+    A = torch.stack([torch.from_numpy(a.T.copy()) for a in As]).to(dt).cuda().transpose(1, 2)
     B = torch.stack([torch.from_numpy(b) for b in bs]).to(dt).cuda()
     ipiv = solve.lu_factor_batched(A)
     solve.lu_solve_batched(A, ipiv, B)
@@ -151,7 +150,7 @@ def test_lu_batched_vs_oracle(n, batch, prec):
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
         if prec == capi.C128:
             # same pivot sequence as the oracle (nalgebra rule) -> identical pivots
-            W = np.asfortranarray(As[k]).copy()
+            W = np.array(As[k], dtype=np.complex128, order="F")
             piv = np.zeros(n, np.int64)
             O.lib().orc_lu_factor(n, O._p(W), n, O._p(piv))
             assert (ipiv[k].cpu().numpy() == piv).all()
@@ -185,8 +184,9 @@ def test_mixed_precision_refinement():
     As = [np.eye(n) * 6 + rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
           for _ in range(nb)]
     bs = [rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(nb)]
-    A128 = torch.stack([torch.from_numpy(a.T.copy()) for a in As]).cuda()
-    A64 = A128.to(torch.complex64)
+    S = torch.stack([torch.from_numpy(a.T.copy()) for a in As]).cuda()
+    A128 = S.transpose(1, 2)
+    A64 = S.to(torch.complex64).transpose(1, 2)
     x = torch.stack([torch.from_numpy(b) for b in bs]).cuda()
     x, st = solve.solve_mixed_batched(A128, A64, x, 1e-13, 20)
     for k in range(nb):
@@ -274,7 +274,7 @@ def test_error_paths():
                  __import__("ctypes").byref(w), solve._ptr(d), 4, solve._ptr(out))
     assert e.value.code == capi.EINVAL
     # singular matrix -> ESINGULAR
-    A = torch.zeros((1, 8, 8), dtype=torch.complex128, device="cuda")
+    A = torch.zeros((1, 8, 8), dtype=torch.complex128, device="cuda").transpose(1, 2)
     with pytest.raises(capi.CcabError) as e:
         solve.lu_factor_batched(A)
     assert e.value.code == capi.ESINGULAR
