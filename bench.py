@@ -5,8 +5,11 @@ sphere (a = 1 m) meshed as an icosphere of 5120 triangles (n = 5120
 unknowns), Burton-Miller, plane wave of seeded direction, 64 frequencies
 with ka linearly spaced in [0.5, 8], c = 343 m/s, rho = 1.21 kg/m^3.
 One step = the whole sweep: batched fused assembly (FP64 arithmetic, c128 and
-c64 LHS) + complex-FP32 LU on MFMA + FP64 iterative refinement, inputs
-(collocation points) resident in HBM.
+c64 LHS) + batched GMRES on the c64 operator with FP64 iterative refinement
+against the c128 matrix, to an FP64 relative residual of 1e-10; inputs
+(collocation points) resident in HBM.  Also timed once: the plain FP64 dense
+LU sweep (c128, FP64 MFMA; the accuracy reference) and the c64-LU + FP64-IR
+sweep.
 
 Prints ONE JSON line.  value = sweep solves/s (frequencies solved per second).
 """
@@ -38,7 +41,8 @@ def parse():
     ap.add_argument("--nfreq", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--skip-gmres", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-10,
+                    help="FP64 relative residual at which a frequency counts as solved")
     ap.add_argument("--cpu-lu-n", type=int, default=2048)
     return ap.parse_args()
 
@@ -88,10 +92,12 @@ def main():
     ka = np.linspace(0.5, 8.0, args.nfreq)
     freqs = ka / a * c / (2 * math.pi)
     pd = PreData(m, freqs, c, rho, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
-                 (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)),
-                 Solver(kind=capi.SOLVER_DIRECT_MIXED, ir_tolerance=1e-12, ir_max_steps=10))
+                 (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)), None)
     n = pd.get_num_eqn()
     ctx = capi.default_context()
+    headline = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol, restart=50,
+                      ir_tolerance=1e-5, ir_max_steps=20, ref_mgs=False)
+    pd.solver = headline
     for _ in range(args.warmup):
         phi, vn, st = solve.sweep(pd, ctx=ctx)
     torch.cuda.synchronize()
@@ -107,41 +113,49 @@ def main():
     ctx.profile(False)
     solves = args.steps * args.nfreq
     value = solves / elapsed
-    ir_steps = max(s["ir_steps"] for s in st)
+    its = [s["iterations"] for s in st]
+    gmres_iters = {"mean": float(np.mean(its)), "max": int(max(its)), "min": int(min(its)),
+                   "ir_cycles_max": int(max(s["ir_steps"] for s in st)), "restart": 50,
+                   "inner_tol": 1e-5, "tol": args.tol, "operator": "c64 (FP64 residual c128)"}
 
-    # accuracy: FP64 dense LU on the GPU (c128 assembly + c128 LU, nalgebra pivoting)
-    rel_err = None
-    fp64_rate = None
-    if not args.skip_fp64:
-        pd.solver = Solver(kind=capi.SOLVER_DIRECT)
+    def timed_sweep(kind):
+        pd.solver = Solver(kind=kind, ir_tolerance=1e-12, ir_max_steps=10)
         torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
         t1 = time.perf_counter()
-        phi64, vn64, _ = solve.sweep(pd, ctx=ctx)
+        out = solve.sweep(pd, ctx=ctx)
         torch.cuda.synchronize()
-        fp64_rate = args.nfreq / (time.perf_counter() - t1)
-        num = torch.linalg.vector_norm(phi - phi64, dim=1)
-        den = torch.linalg.vector_norm(phi64, dim=1)
-        rel_err = float((num / den).max())
+        dt = time.perf_counter() - t1
+        pr = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        return out, args.nfreq / dt, pr
 
-    # GMRES iterations (textbook MGS, relative residual 1e-6) on 8 of the frequencies
-    gmres_iters = None
-    if not args.skip_gmres:
-        sub = list(freqs[:: max(1, args.nfreq // 8)])
-        pd.solver = Solver(kind=capi.SOLVER_ITERATIVE, tolerance=1e-6, max_iterations=n,
-                           restart=50, ref_mgs=False, stop_rule=capi.STOP_RELATIVE_RESIDUAL,
-                           matrix_prec=capi.C64)
-        _, _, gst = solve.sweep(pd, frequencies=sub, ctx=ctx)
-        gmres_iters = {"mean": float(np.mean([s["iterations"] for s in gst])),
-                       "max": int(max(s["iterations"] for s in gst)),
-                       "restart": 50, "tolerance": 1e-6, "operator": "c64",
-                       "frequencies": len(sub)}
+    def relerr(a, b):
+        return float((torch.linalg.vector_norm(a - b, dim=1) / torch.linalg.vector_norm(b, dim=1)).max())
 
-    upd = prof.get("lu_update", {"count": 0, "ms": float("nan"), "work": 0.0})
+    rel_err = fp64_rate = lu_rate = lu_err = None
+    lu_prof = {}
+    if not args.skip_fp64:
+        (phi64, _, _), fp64_rate, _ = timed_sweep(capi.SOLVER_DIRECT)
+        rel_err = relerr(phi, phi64)
+        (phim, _, stm), lu_rate, lu_prof = timed_sweep(capi.SOLVER_DIRECT_MIXED)
+        lu_err = relerr(phim, phi64)
+
+    gemv = prof.get("gemv", {"count": 0, "ms": float("nan"), "work": 0.0})
     asm = prof.get("assembly", {"count": 0, "ms": float("nan"), "work": 0.0})
-    achieved = upd["work"] / (upd["ms"] * 1e-3) / 1e12 if upd["count"] else None
+    achieved = gemv["work"] / (gemv["ms"] * 1e-3) / 1e9 if gemv["count"] else None
     asm_entries = asm["work"] / 24.0  # c128 + c64 stored per entry
     gentries = asm_entries / (asm["ms"] * 1e-3) / 1e9 if asm["count"] else None
     asm_gbs = asm["work"] / (asm["ms"] * 1e-3) / 1e9 if asm["count"] else None
+    upd = lu_prof.get("lu_update")
+    lu_roof = None
+    if upd:
+        tf = upd["work"] / (upd["ms"] * 1e-3) / 1e12
+        lu_roof = {"bound": "mfma", "kernel": "cgemm_kernel<float,0> (LU trailing update)",
+                   "achieved": tf, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
+                   "frac": tf / PEAK_FP32_MFMA_TFLOPS, "launches": upd["count"],
+                   "avg_launch_ms": upd["ms"] / upd["count"]}
 
     result = {
         "metric": METRIC,
@@ -157,23 +171,25 @@ def main():
         "dtype": "c64+f64ir",
         "data": "synthetic (deterministic icosphere, seeded plane-wave direction)",
         "config": {"workload": (f"rigid-sphere plane-wave scattering, icosphere {n} triangles, "
-                                f"Burton-Miller, {args.nfreq}-frequency sweep ka in [0.5, 8], "
-                                "c64 MFMA LU + FP64 iterative refinement"),
-                   "n": n, "nfreq": args.nfreq, "solver": "direct_mixed"},
+                                f"Burton-Miller, {args.nfreq}-frequency sweep ka in [0.5, 8]; "
+                                "batched c64 GMRES with FP64 iterative refinement"),
+                   "n": n, "nfreq": args.nfreq, "solver": "iterative_mixed"},
         "rel_err_vs_fp64_lu": rel_err,
-        "ir_steps": ir_steps,
         "assembly_gentries_per_s": gentries,
         "assembly_store_gbs": asm_gbs,
+        "gmres_iters": gmres_iters,
         "fp64_lu_solves_per_s": fp64_rate,
         "speedup_vs_fp64_lu": (value / fp64_rate) if fp64_rate else None,
-        "gmres_iters": gmres_iters,
+        "lu_mixed_solves_per_s": lu_rate,
+        "lu_mixed_rel_err_vs_fp64_lu": lu_err,
         "kernel_ms": {k: round(v["ms"] / args.steps, 3) for k, v in prof.items()},
-        "roofline": {"bound": "mfma", "kernel": "cgemm_kernel<float,0> (LU trailing update)",
-                     "achieved": achieved, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": (achieved / PEAK_FP32_MFMA_TFLOPS) if achieved else None,
-                     "launches": upd["count"],
-                     "avg_launch_ms": (upd["ms"] / upd["count"]) if upd["count"] else None,
+        "roofline": {"bound": "hbm", "kernel": "gemv_part_kernel (c64 GMRES matvec + c128 residual)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": (achieved / PEAK_HBM_GBS) if achieved else None,
+                     "launches": gemv["count"],
+                     "avg_launch_ms": (gemv["ms"] / gemv["count"]) if gemv["count"] else None,
                      "traffic": None},
+        "roofline_lu": lu_roof,
     }
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pd.cpts, (freqs[-1], c, rho, 1, 0, 1, (0.0, 0.0)),

@@ -102,6 +102,7 @@ SIGNATURES = {
     "ccab_synchronize": (_I, [_P]),
     "ccab_profile_enable": (_I, [_P, _I]),
     "ccab_profile_read": (_I, [_P, _I, _P, _P, _P, _I]),
+    "ccab_set_tuning": (_I, [_P, _I, _I, _I]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_incident_surface": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
     "ccab_dense_surface_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _P, _P, _I64]),
@@ -120,6 +121,10 @@ SIGNATURES = {
     "ccab_sweep": (_I, [_P, _P, _I64, ct.POINTER(Wave), _P, _I64, ct.POINTER(SolverCfg), _P, _P,
                         _P]),
     "ccab_cgemm_update": (_I, [_P, _I, _I64, _I64, _I64, _P, _I64, _P, _I64, _P, _I64]),
+    "ccab_gmres_ir_batched": (_I, [_P, _I64, _I64, _P, _P, _I64, _I64, _P, _P,
+                                   ct.POINTER(SolverCfg), _P]),
+    "ccab_sweep_warm": (_I, [_P, _P, _I64, ct.POINTER(Wave), _P, _I64, ct.POINTER(SolverCfg), _P,
+                             _P, _P, _P]),
 }
 
 _lib = None

@@ -29,40 +29,50 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
                                                         int64_t lda, int64_t sA,
                                                         const c128* __restrict__ x, int64_t sx,
                                                         int64_t chunk, c128* __restrict__ part,
-                                                        int64_t nsplit) {
-    const int64_t b = blockIdx.z;
+                                                        int64_t nsplit,
+                                                        const int32_t* __restrict__ active) {
+    const int64_t z = blockIdx.z;
+    const int64_t b = active ? active[z] : z;
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     const int64_t j0 = int64_t(blockIdx.y) * chunk;
     const int64_t j1 = min(j0 + chunk, N);
     if (i >= M) return;
     const TA* a = A + b * sA + i;
     const c128* xb = x + b * sx;
-    double re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+    double re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
     int64_t j = j0;
-    for (; j + 1 < j1; j += 2) {
-        const c128 a0 = ld_c(a, j * lda), a1 = ld_c(a, (j + 1) * lda);
-        const c128 x0 = xb[j], x1 = xb[j + 1];
-        re0 += a0.re * x0.re - a0.im * x0.im;
-        im0 += a0.re * x0.im + a0.im * x0.re;
-        re1 += a1.re * x1.re - a1.im * x1.im;
-        im1 += a1.re * x1.im + a1.im * x1.re;
+    for (; j + 3 < j1; j += 4) {
+        c128 av[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            av[u] = ld_c(a, (j + u) * lda);
+            xv[u] = xb[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            re[u] += av[u].re * xv[u].re - av[u].im * xv[u].im;
+            im[u] += av[u].re * xv[u].im + av[u].im * xv[u].re;
+        }
     }
-    if (j < j1) {
+    for (; j < j1; ++j) {
         const c128 a0 = ld_c(a, j * lda);
         const c128 x0 = xb[j];
-        re0 += a0.re * x0.re - a0.im * x0.im;
-        im0 += a0.re * x0.im + a0.im * x0.re;
+        re[0] += a0.re * x0.re - a0.im * x0.im;
+        im[0] += a0.re * x0.im + a0.im * x0.re;
     }
-    part[(b * nsplit + blockIdx.y) * M + i] = c128{re0 + re1, im0 + im1};
+    part[(z * nsplit + blockIdx.y) * M + i] =
+        c128{(re[0] + re[1]) + (re[2] + re[3]), (im[0] + im[1]) + (im[2] + im[3])};
 }
 
 __global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int64_t nsplit,
-                                   c128 alpha, c128 beta, c128* y, int64_t sy) {
-    const int64_t b = blockIdx.y;
+                                   c128 alpha, c128 beta, c128* y, int64_t sy,
+                                   const int32_t* __restrict__ active) {
+    const int64_t z = blockIdx.y;
+    const int64_t b = active ? active[z] : z;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
     c128 s{0.0, 0.0};
-    for (int64_t p = 0; p < nsplit; ++p) s += part[(b * nsplit + p) * M + i];
+    for (int64_t p = 0; p < nsplit; ++p) s += part[(z * nsplit + p) * M + i];
     c128* yy = y + b * sy + i;
     const bool zero_beta = beta.re == 0.0 && beta.im == 0.0;
     *yy = alpha * s + (zero_beta ? c128{0.0, 0.0} : beta * *yy);
@@ -81,13 +91,15 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 }
 
 __global__ __launch_bounds__(256) void nrm2_kernel(int64_t n, const c128* __restrict__ v,
-                                                   int64_t stride, double* out) {
+                                                   int64_t stride, double* out,
+                                                   const int32_t* __restrict__ active) {
     __shared__ double red[4];
-    const c128* p = v + blockIdx.x * stride;
+    const int64_t sys = active ? active[blockIdx.x] : blockIdx.x;
+    const c128* p = v + sys * stride;
     double s = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 256) s += p[i].re * p[i].re + p[i].im * p[i].im;
     s = block_sum<256>(s, red);
-    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(s);
+    if (threadIdx.x == 0) out[sys] = sqrt(s);
 }
 
 template <typename TA>
@@ -126,36 +138,44 @@ __global__ void c64_to_c128_kernel(int64_t count, const c64* __restrict__ s, c12
 
 size_t gemv_workspace_bytes(int64_t n, int64_t batch) { return 0; }
 
-int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
-                 int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
-                 c128 beta, c128* d_y, int64_t stridey) {
-    if (M <= 0 || batch <= 0) return CCAB_OK;
+int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
+                const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
+                const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
+                int64_t stridey) {
+    if (M <= 0 || nact <= 0) return CCAB_OK;
     const int64_t rowblocks = (M + 255) / 256;
-    int64_t nsplit = (1024 + rowblocks * batch - 1) / (rowblocks * batch);
+    int64_t nsplit = (1024 + rowblocks * nact - 1) / (rowblocks * nact);
     nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
     const int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
     nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;
-    c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(batch) * nsplit * M * 16));
+    c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
-    dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)batch);
+    dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)nact);
     ProfScope ps(ctx, PROF_GEMV,
-                 double(M) * double(N) * double(batch) * (prec == CCAB_C128 ? 16.0 : 8.0));
+                 double(M) * double(N) * double(nact) * (prec == CCAB_C128 ? 16.0 : 8.0));
     if (N > 0) {
         if (prec == CCAB_C128)
             hipLaunchKernelGGL(gemv_part_kernel<c128>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit);
+                               part, nsplit, d_active);
         else
             hipLaunchKernelGGL(gemv_part_kernel<c64>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit);
+                               part, nsplit, d_active);
     } else {
         nsplit = 0;
     }
-    hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)rowblocks, (unsigned)batch), dim3(256),
-                       0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey);
+    hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)rowblocks, (unsigned)nact), dim3(256),
+                       0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
+}
+
+int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
+                 int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
+                 c128 beta, c128* d_y, int64_t stridey) {
+    return gemv_active(ctx, prec, M, N, batch, nullptr, d_A, lda, strideA, d_x, stridex, alpha,
+                       beta, d_y, stridey);
 }
 
 int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* d_A, int64_t lda,
@@ -165,10 +185,19 @@ int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* 
                         stridey);
 }
 
+int nrm2_active(ccab_ctx* ctx, int64_t n, int64_t nact, const int32_t* d_active,
+                const c128* d_v, int64_t stride, double* d_out) {
+    if (nact <= 0) return CCAB_OK;
+    hipLaunchKernelGGL(nrm2_kernel, dim3((unsigned)nact), dim3(256), 0, ctx->stream, n, d_v,
+                       stride, d_out, d_active);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
 int nrm2_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* d_v, int64_t stride,
                  double* d_out) {
     hipLaunchKernelGGL(nrm2_kernel, dim3((unsigned)batch), dim3(256), 0, ctx->stream, n, d_v,
-                       stride, d_out);
+                       stride, d_out, nullptr);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

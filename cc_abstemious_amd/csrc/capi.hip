@@ -1,6 +1,7 @@
 // extern "C" entry points of libccab (declared in include/ccab.h).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "ccab_internal.h"
@@ -11,6 +12,9 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
               int64_t strideA, c128* d_x, double ir_tol, int ir_max, ccab_solve_stats* st);
 int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const void* A,
                  int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc);
+int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c64* A64,
+                 int64_t lda, int64_t sA, c128* X, const c128* X0, int m, double inner_tol,
+                 double tol, int max_cycles, bool ref_mgs, ccab_solve_stats* st);
 
 // ---------------------------------------------------------------------------
 // context plumbing
@@ -244,6 +248,11 @@ int ccab_create(int device, ccab_ctx** out) {
     ctx->stream = ctx->own_stream;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (const char* gv = getenv("CCAB_GEMM_VARIANT")) {
+        const int v = atoi(gv);
+        ctx->gemm_variant[0] = ctx->gemm_variant[1] = v & 7;
+    }
+    if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
     *out = ctx;
     return CCAB_OK;
 }
@@ -268,6 +277,16 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
         ctx->prof_ms[cls] = 0.0;
         ctx->prof_work[cls] = 0.0;
     }
+    return CCAB_OK;
+}
+
+int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "bad precision");
+    CCAB_CHECK(gemm_variant >= 0 && gemm_variant < 8, "gemm variant must be 0..7");
+    CCAB_CHECK(lu_outer == 64 || lu_outer == 128, "lu_outer must be 64 or 128");
+    ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1] = gemm_variant;
+    ctx->lu_outer = lu_outer;
     return CCAB_OK;
 }
 
@@ -480,7 +499,7 @@ int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda, cca
 static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
                       const ccab_c128* d_rhs_inc_given, const double* d_cpts, int64_t n,
                       const ccab_solver_cfg* cfg, ccab_c128* d_phi, ccab_c128* d_vn,
-                      ccab_solve_stats* stats) {
+                      ccab_solve_stats* stats, const ccab_c128* d_x0 = nullptr) {
     CCAB_CHECK(ph && d_cpts && cfg && d_phi && d_vn && n > 0 && nfreq > 0, "invalid arguments");
     const int bc = ph[0].bc_type;
     for (int64_t f = 1; f < nfreq; ++f) {
@@ -511,6 +530,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     else if (solver == CCAB_SOLVER_DIRECT_MIXED) per = n2 * 24;
     else if (solver == CCAB_SOLVER_ITERATIVE)
         per = n2 * (cfg->gmres.matrix_prec == CCAB_C64 ? 8 : 16);
+    else if (solver == CCAB_SOLVER_ITERATIVE_MIXED) per = n2 * 24;
     else return ctx_fail(ctx, CCAB_EINVAL, "solver %d not available on the dense path", solver);
     size_t budget = size_t(96) << 30;
     size_t freemem = 0, total = 0;
@@ -527,11 +547,12 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     void* A = nullptr;
     c64* A64 = nullptr;
     if (solver == CCAB_SOLVER_DIRECT || solver == CCAB_SOLVER_DIRECT_MIXED ||
+        solver == CCAB_SOLVER_ITERATIVE_MIXED ||
         (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C128)) {
         A = ctx_buf(ctx, SLOT_SOLVE_A, size_t(chunk) * n2 * 16);
         if (!A) return CCAB_ENOMEM;
     }
-    if (solver == CCAB_SOLVER_DIRECT_MIXED ||
+    if (solver == CCAB_SOLVER_DIRECT_MIXED || solver == CCAB_SOLVER_ITERATIVE_MIXED ||
         (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C64)) {
         A64 = static_cast<c64*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(chunk) * n2 * 8));
         if (!A64) return CCAB_ENOMEM;
@@ -558,6 +579,12 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
                     stats[f0 + b].backward_error = -1.0;
                     stats[f0 + b].rel_residual = -1.0;
                 }
+        } else if (solver == CCAB_SOLVER_ITERATIVE_MIXED) {
+            const c128* x0 = d_x0 ? reinterpret_cast<const c128*>(d_x0 + f0 * n) : nullptr;
+            CCAB_TRY(gmres_ir_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
+                                  reinterpret_cast<c128*>(x), x0, cfg->gmres.restart,
+                                  cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
+                                  cfg->gmres.ref_mgs != 0, stats ? stats + f0 : nullptr));
         } else if (solver == CCAB_SOLVER_DIRECT_MIXED) {
             CCAB_TRY(mixed_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
                                reinterpret_cast<c128*>(x), cfg->ir_tolerance, cfg->ir_max_steps,
@@ -587,6 +614,31 @@ extern "C" int ccab_sweep(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, 
     CCAB_CHECK(w != nullptr, "null wave");
     CCAB_CHECK(w->kind == CCAB_WAVE_PLANE || w->kind == CCAB_WAVE_SPHERICAL, "bad wave kind");
     return sweep_impl(ctx, ph, nfreq, w, nullptr, d_cpts, n, cfg, d_phi, d_vn, stats);
+}
+
+extern "C" int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq,
+                               const ccab_wave* w, const double* d_cpts, int64_t n,
+                               const ccab_solver_cfg* cfg, const ccab_c128* d_x0,
+                               ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(w != nullptr && d_x0 != nullptr, "null wave or initial guess");
+    CCAB_CHECK(cfg && cfg->solver == CCAB_SOLVER_ITERATIVE_MIXED,
+               "a warm start needs the ITERATIVE_MIXED solver");
+    return sweep_impl(ctx, ph, nfreq, w, nullptr, d_cpts, n, cfg, d_phi, d_vn, stats, d_x0);
+}
+
+extern "C" int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch,
+                                     const ccab_c128* d_A128, const ccab_c64* d_A64, int64_t lda,
+                                     int64_t strideA, ccab_c128* d_x, const ccab_c128* d_x0,
+                                     const ccab_solver_cfg* cfg, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A64 && d_x && cfg && n > 0 && batch > 0 && lda >= n, "invalid arguments");
+    CCAB_CHECK(batch == 1 || strideA >= lda * n, "stride too small");
+    return gmres_ir_run(ctx, n, batch, reinterpret_cast<const c128*>(d_A128),
+                        reinterpret_cast<const c64*>(d_A64), lda, strideA,
+                        reinterpret_cast<c128*>(d_x), reinterpret_cast<const c128*>(d_x0),
+                        cfg->gmres.restart, cfg->ir_tolerance, cfg->gmres.tolerance,
+                        cfg->ir_max_steps, cfg->gmres.ref_mgs != 0, stats);
 }
 
 extern "C" int ccab_solve_surface(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,

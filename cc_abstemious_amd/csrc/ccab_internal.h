@@ -107,6 +107,11 @@ struct ccab_ctx {
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     int num_cus = 256;
+    // complex-GEMM variant per precision [c128, c64] (see lu.hip); set by
+    // ccab_set_tuning / ccab_autotune, initialised from CCAB_GEMM_VARIANT
+    int gemm_variant[2] = {3, 0};
+    // LU outer panel width (64 or 128; 128 halves the trailing-update traffic)
+    int lu_outer = 128;
     // in-process kernel timing (HIP events on ctx->stream), see prof_begin
     bool prof_on = false;
     struct ProfRec {
@@ -178,7 +183,8 @@ enum BufSlot {
     SLOT_SOA = 0, SLOT_PARAMS, SLOT_ROWSUM, SLOT_LU_LINV, SLOT_LU_UINV, SLOT_LU_PERM,
     SLOT_LU_INFO, SLOT_LU_Y, SLOT_GEMV_PART, SLOT_NORM, SLOT_MIXED_R, SLOT_MIXED_D,
     SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
-    SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_COUNT
+    SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
+    SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_COUNT
 };
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
 inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
@@ -239,6 +245,16 @@ int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* 
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);
 size_t gemv_workspace_bytes(int64_t n, int64_t batch);
+// the same over the systems listed in d_active (device int32[nact])
+int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
+                const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
+                const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
+                int64_t stridey);
+int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
+                 int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
+                 c128 beta, c128* d_y, int64_t stridey);
+int nrm2_active(ccab_ctx* ctx, int64_t n, int64_t nact, const int32_t* d_active,
+                const c128* d_v, int64_t stride, double* d_out);
 // out[b] = ||v_b||_2 (FP64), written to device
 int nrm2_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* d_v, int64_t stride,
                  double* d_out);

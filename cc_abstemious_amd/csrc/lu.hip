@@ -149,65 +149,287 @@ __global__ __launch_bounds__(kPanelThreads) void panel_kernel(Cx<T>* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the diagonal block: Linv (unit lower) and Uinv (upper), NB x NB
-// column-major, zero padded when nb < NB.  One wavefront-column per thread.
+// Inverses of the NB x NB diagonal blocks, column-major, zero padded when
+// nb < NB: Linv (unit lower, needed right after each panel for the U12
+// update) and Uinv (upper, only for the triangular solves, so all blocks are
+// inverted in one launch after the factorisation).  The block is staged in
+// LDS; lane c solves for column c with four partial sums per dot product.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool UPPER>
 __global__ __launch_bounds__(NB) void trinv_kernel(const Cx<T>* __restrict__ A, int64_t lda,
-                                                   int64_t strideA, int64_t k0, int nb,
-                                                   Cx<T>* __restrict__ Linv,
-                                                   Cx<T>* __restrict__ Uinv, int64_t inv_stride) {
+                                                   int64_t strideA, int64_t n, int64_t kb0,
+                                                   Cx<T>* __restrict__ Inv, int64_t inv_stride) {
     using C = Cx<T>;
     const int64_t b = blockIdx.x;
+    const int64_t kb = kb0 + blockIdx.y;
+    const int64_t k0 = kb * NB;
+    const int nb = (int)min<int64_t>(NB, n - k0);
     const C* a = A + b * strideA + k0 + k0 * lda;
-    C* li = Linv + b * inv_stride;
-    C* ui = Uinv + b * inv_stride;
-    __shared__ C blk[NB * NB];
+    C* out = Inv + b * inv_stride + kb * NB * NB;
+    __shared__ C blk[NB][NB + 1];  // blk[r][t] = A(k0 + r, k0 + t)
+    __shared__ C X[NB][NB + 1];    // X[r][c]
     const int c = threadIdx.x;
-    for (int t = 0; t < NB; ++t) {
-        blk[c + t * NB] = (c < nb && t < nb) ? a[c + t * lda] : C{T(0), T(0)};
+    const C zero{T(0), T(0)}, one{T(1), T(0)};
+    for (int t = 0; t < NB; ++t) blk[c][t] = (c < nb && t < nb) ? a[c + t * lda] : zero;
+    __syncthreads();
+    if (!UPPER) {
+        // column c of L^{-1}: x_c = 1, x_r = -sum_{t=c}^{r-1} L[r][t] x_t
+        for (int r = 0; r < NB; ++r) {
+            C x = zero;
+            if (c < nb && r < nb && r >= c) {
+                if (r == c) {
+                    x = one;
+                } else {
+                    C s0 = zero, s1 = zero, s2 = zero, s3 = zero;
+                    int t = c;
+                    for (; t + 3 < r; t += 4) {
+                        s0 += blk[r][t] * X[t][c];
+                        s1 += blk[r][t + 1] * X[t + 1][c];
+                        s2 += blk[r][t + 2] * X[t + 2][c];
+                        s3 += blk[r][t + 3] * X[t + 3][c];
+                    }
+                    for (; t < r; ++t) s0 += blk[r][t] * X[t][c];
+                    x = -((s0 + s1) + (s2 + s3));
+                }
+            }
+            X[r][c] = x;
+        }
+    } else {
+        // column c of U^{-1}: rows c down to 0
+        for (int r = NB - 1; r >= 0; --r) {
+            C y = zero;
+            if (c < nb && r <= c) {
+                C s0 = (r == c) ? one : zero, s1 = zero, s2 = zero, s3 = zero;
+                int t = r + 1;
+                for (; t + 3 <= c; t += 4) {
+                    s0 -= blk[r][t] * X[t][c];
+                    s1 -= blk[r][t + 1] * X[t + 1][c];
+                    s2 -= blk[r][t + 2] * X[t + 2][c];
+                    s3 -= blk[r][t + 3] * X[t + 3][c];
+                }
+                for (; t <= c; ++t) s0 -= blk[r][t] * X[t][c];
+                const C d = blk[r][r];
+                y = (d.re == T(0) && d.im == T(0)) ? zero : cdiv((s0 + s1) + (s2 + s3), d);
+            }
+            X[r][c] = y;
+        }
     }
     __syncthreads();
-    // Linv column c: solve L x = e_c (unit diagonal)
-    for (int r = 0; r < NB; ++r) {
-        C x{T(0), T(0)};
-        if (c < nb && r >= c && r < nb) {
-            if (r == c) {
-                x = C{T(1), T(0)};
-            } else {
-                C s{T(0), T(0)};
-                for (int t = c; t < r; ++t) s += blk[r + t * NB] * li[t + c * NB];
-                x = -s;
+    for (int e = c; e < NB * NB; e += NB) out[e] = X[e % NB][e / NB];
+}
+
+// ---------------------------------------------------------------------------
+// Slab factorisation: W columns c0..c0+W of rows c0..n-1 held in registers
+// (row r on lane (r-c0) % kSlabThreads, slot (r-c0) / kSlabThreads).  W
+// pivot steps (block arg-max of |re|+|im|, first index on ties, row exchange
+// through LDS, scale, rank-1 update), write back, then
+//   - the slab's row swaps on the other columns of the 64-wide panel,
+//   - U rows of the panel columns right of the slab: L_ss^{-1} A(c0:c0+W, .)
+// The rest of the panel is updated by the MFMA GEMM afterwards, so all
+// O(n NB^2) panel work runs on the matrix cores.
+// ---------------------------------------------------------------------------
+constexpr int kSlabThreads = 1024;
+
+template <typename T, int W, int R>
+__global__ __launch_bounds__(kSlabThreads) void slab_factor_kernel(Cx<T>* __restrict__ A, int64_t lda,
+                                                                   int64_t strideA, int64_t n,
+                                                                   int64_t k0, int nb, int c0o,
+                                                                   int32_t* __restrict__ ipiv,
+                                                                   int32_t* __restrict__ perm,
+                                                                   int32_t* __restrict__ info) {
+    using C = Cx<T>;
+    constexpr int NW = kSlabThreads / 64;
+    const int64_t b = blockIdx.x;
+    C* a = A + b * strideA;
+    int32_t* piv = ipiv + b * n;
+    int32_t* pm = perm + b * n;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const C zero{T(0), T(0)};
+    __shared__ C srow[2][W];
+    __shared__ C sL[W][W];
+    __shared__ T red_v[NW];
+    __shared__ int64_t red_i[NW];
+    __shared__ int64_t s_p;
+    __shared__ int s_zero;
+    __shared__ int64_t s_piv[W];
+
+    const int64_t c0 = k0 + c0o;
+    const int weff = min(W, nb - c0o);
+    C v[R][W];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int64_t row = c0 + tid + int64_t(q) * kSlabThreads;
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[q][w] = (row < n && w < weff) ? a[row + (c0 + w) * lda] : zero;
+    }
+#pragma unroll
+    for (int jj = 0; jj < W; ++jj) {
+        if (jj < weff) {
+            const int64_t dr = c0 + jj;
+            T best = T(-1);
+            int64_t bi = n;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int64_t row = c0 + tid + int64_t(q) * kSlabThreads;
+                if (row >= dr && row < n) {
+                    const T x = norm1(v[q][jj]);
+                    if (x > best) {
+                        best = x;
+                        bi = row;
+                    }
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const T ov = __shfl_xor(best, off, 64);
+                const int64_t oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) {
+                    best = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                red_v[wave] = best;
+                red_i[wave] = bi;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                T bv = red_v[0];
+                int64_t bx = red_i[0];
+                for (int w = 1; w < NW; ++w)
+                    if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bx)) {
+                        bv = red_v[w];
+                        bx = red_i[w];
+                    }
+                const bool zcol = !(bv > T(0));
+                s_zero = zcol ? 1 : 0;
+                const int64_t p = zcol ? dr : bx;
+                s_p = p;
+                s_piv[jj] = p;
+                piv[dr] = int32_t(p);
+                if (zcol) {
+                    info[b] = 1;
+                } else if (p != dr) {
+                    const int32_t t = pm[dr];
+                    pm[dr] = pm[p];
+                    pm[p] = t;
+                }
+            }
+            __syncthreads();
+            if (!s_zero) {
+                const int64_t p = s_p;
+                const int tp = int((p - c0) % kSlabThreads);
+                const int qp = int((p - c0) / kSlabThreads);
+                if (p != dr) {
+                    if (tid == tp) {
+#pragma unroll
+                        for (int q = 0; q < R; ++q)
+                            if (q == qp) {
+#pragma unroll
+                                for (int w = 0; w < W; ++w) srow[0][w] = v[q][w];
+                            }
+                    }
+                    if (tid == jj) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) srow[1][w] = v[0][w];
+                    }
+                    __syncthreads();
+                    if (tid == tp) {
+#pragma unroll
+                        for (int q = 0; q < R; ++q)
+                            if (q == qp) {
+#pragma unroll
+                                for (int w = 0; w < W; ++w) v[q][w] = srow[1][w];
+                            }
+                    }
+                    if (tid == jj) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) v[0][w] = srow[0][w];
+                    }
+                } else if (tid == jj) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) srow[0][w] = v[0][w];
+                }
+                __syncthreads();
+                C prow[W];
+#pragma unroll
+                for (int w = 0; w < W; ++w) prow[w] = srow[0][w];
+                const C inv = crecip(prow[jj]);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int64_t row = c0 + tid + int64_t(q) * kSlabThreads;
+                    if (row > dr && row < n) {
+                        const C l = v[q][jj] * inv;
+                        v[q][jj] = l;
+#pragma unroll
+                        for (int w = jj + 1; w < W; ++w) v[q][w] -= l * prow[w];
+                    }
+                }
+            }
+            __syncthreads();
+        } else if (tid == 0) {
+            s_piv[jj] = c0 + jj;
+        }
+    }
+    // write back; the top W x W of the slab (L_ss below its diagonal) to LDS
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int64_t row = c0 + tid + int64_t(q) * kSlabThreads;
+        if (row < n) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (w < weff) a[row + (c0 + w) * lda] = v[q][w];
+        }
+    }
+    if (tid < W) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) sL[tid][w] = v[0][w];
+    }
+    __syncthreads();
+    // the slab's swaps on the panel columns outside the slab (global, L2)
+    for (int cc = tid; cc < nb; cc += kSlabThreads) {
+        if (cc >= c0o && cc < c0o + weff) continue;
+        C* col = a + (k0 + cc) * lda;
+        for (int jj = 0; jj < weff; ++jj) {
+            const int64_t r = c0 + jj;
+            const int64_t p = s_piv[jj];
+            if (p != r) {
+                const C x = col[r];
+                col[r] = col[p];
+                col[p] = x;
             }
         }
-        li[r + c * NB] = x;
-    }
-    // Uinv column c: solve U y = e_c, rows c down to 0
-    for (int r = NB - 1; r >= 0; --r) {
-        C y{T(0), T(0)};
-        if (c < nb && r <= c) {
-            C s = (r == c) ? C{T(1), T(0)} : C{T(0), T(0)};
-            for (int t = r + 1; t <= c; ++t) s -= blk[r + t * NB] * ui[t + c * NB];
-            const C d = blk[r + r * NB];
-            y = (d.re == T(0) && d.im == T(0)) ? C{T(0), T(0)} : cdiv(s, d);
+        // U rows of the panel columns right of the slab: unit-L forward solve
+        if (cc >= c0o + weff) {
+            C u[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) u[w] = w < weff ? col[c0 + w] : zero;
+#pragma unroll
+            for (int w = 1; w < W; ++w)
+#pragma unroll
+                for (int t = 0; t < w; ++t) u[w] -= sL[w][t] * u[t];
+#pragma unroll
+            for (int w = 1; w < W; ++w)
+                if (w < weff) col[c0 + w] = u[w];
         }
-        ui[r + c * NB] = y;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Row swaps of one panel applied to every column outside the panel.
+// Row swaps of pivots r0..r0+np-1 (LAPACK order) applied to the columns in
+// [c_lo, c_hi) outside [x_lo, x_hi); one lane per column.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void laswp_kernel(Cx<T>* __restrict__ A, int64_t lda, int64_t strideA, int64_t n,
-                             int64_t k0, int nb, const int32_t* __restrict__ ipiv) {
+                             int64_t r0, int np, const int32_t* __restrict__ ipiv, int64_t c_lo,
+                             int64_t c_hi, int64_t x_lo, int64_t x_hi) {
     const int64_t b = blockIdx.y;
-    const int64_t col = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (col >= n || (col >= k0 && col < k0 + nb)) return;
+    const int64_t col = c_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (col >= c_hi || (col >= x_lo && col < x_hi)) return;
     Cx<T>* a = A + b * strideA + col * lda;
     const int32_t* piv = ipiv + b * n;
-    for (int j = 0; j < nb; ++j) {
-        const int64_t r = k0 + j;
+    for (int j = 0; j < np; ++j) {
+        const int64_t r = r0 + j;
         const int64_t p = piv[r];
         if (p != r) {
             const Cx<T> x = a[r];
@@ -217,85 +439,121 @@ __global__ void laswp_kernel(Cx<T>* __restrict__ A, int64_t lda, int64_t strideA
     }
 }
 
-// ---------------------------------------------------------------------------
-// Complex GEMM on MFMA.  MODE 0: C -= A B.  MODE 1: C = A B.
-// Workgroup tile 64 (complex rows) x 64 (cols), 4 wavefronts in 2x2, each
-// wave 32 complex rows (= 64 real rows, 4 MFMA tiles) x 32 cols (2 tiles).
-// K is walked in chunks of KC complex through LDS.  With MODE 1 and
-// M <= 64, C may alias B: a workgroup reads its whole B column block before
-// its first store.
-// ---------------------------------------------------------------------------
-constexpr int TM = 64;
 constexpr int TN = 64;
-constexpr int KC = 16;
+constexpr int kGroupM = 8;   // row tiles per group of the XCD-aware tile order
+constexpr int kXcd = 8;      // MI355X: workgroups go round-robin to 8 XCDs
+constexpr int KC = 16;       // complex k per LDS chunk
 
-template <typename T, int MODE>
+// Map the linear workgroup id to (system, row tile, column tile).
+// XCD = true: workgroup w runs on XCD w % 8; each XCD gets a contiguous range
+// of the tile sequence, and the sequence walks groups of kGroupM row tiles
+// column by column, so an XCD keeps a few A row-panels and B column-panels
+// in its own L2.  XCD = false: plain row-tile-fastest order.
+template <bool XCD>
+__device__ __forceinline__ bool tile_of(int64_t tm, int64_t tn, int64_t batch, int64_t& b,
+                                        int64_t& pm, int64_t& pn) {
+    const int64_t total = tm * tn * batch;
+    const int64_t w = blockIdx.x;
+    int64_t t = w;
+    if (XCD) {
+        const int64_t per = (total + kXcd - 1) / kXcd;
+        t = (w % kXcd) * per + w / kXcd;
+    }
+    if (t >= total) return false;
+    b = t / (tm * tn);
+    const int64_t r = t % (tm * tn);
+    if (XCD) {
+        const int64_t in_group = kGroupM * tn;
+        const int64_t g = r / in_group;
+        const int64_t first = g * kGroupM;
+        const int64_t gsz = min<int64_t>(tm - first, kGroupM);
+        pm = first + (r % in_group) % gsz;
+        pn = (r % in_group) / gsz;
+    } else {
+        pm = r % tm;
+        pn = r / tm;
+    }
+    return true;
+}
+
+// Complex GEMM on MFMA.  MODE 0: C -= A B.  MODE 1: C = A B.
+// Workgroup tile TM (complex rows) x 64 (cols), 4 wavefronts in 2x2; a wave
+// owns TM/2 complex rows (= TM real rows, TM/16 MFMA tiles) x 32 columns.
+// K is walked in chunks of KC complex through LDS.  EPI_LDS: the C tile goes
+// through LDS half by half so every global access is a contiguous column
+// segment; otherwise lanes update C directly.  With MODE 1 and M <= TM, C may
+// alias B: a workgroup reads its whole B column block before its first store.
+template <typename T, int MODE, int TM, bool XCD, bool EPI_LDS>
 __global__ __launch_bounds__(256) void cgemm_kernel(int64_t M, int64_t N, int64_t K,
                                                     const Cx<T>* __restrict__ A, int64_t lda,
                                                     int64_t sA, const Cx<T>* B, int64_t ldb,
                                                     int64_t sB, Cx<T>* C, int64_t ldc,
-                                                    int64_t sC) {
+                                                    int64_t sC, int64_t batch) {
     using Cxt = Cx<T>;
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
-    const int64_t bz = blockIdx.z;
-    const int64_t m0 = int64_t(blockIdx.x) * TM;
-    const int64_t n0 = int64_t(blockIdx.y) * TN;
+    constexpr int RT = TM / 16;  // MFMA row tiles per wave
+    const int64_t tm = (M + TM - 1) / TM, tn = (N + TN - 1) / TN;
+    int64_t bz, pm, pn;
+    if (!tile_of<XCD>(tm, tn, batch, bz, pm, pn)) return;
+    const int64_t m0 = pm * TM;
+    const int64_t n0 = pn * TN;
     const Cxt* a = A + bz * sA;
     const Cxt* bmat = B + bz * sB;
     Cxt* c = C + bz * sC;
 
-    __shared__ Cxt As[KC][TM + 1];  // [k][row]
-    __shared__ Cxt Bs[KC][TN + 1];  // [k][col]
+    constexpr size_t kA = sizeof(Cxt) * KC * (TM + 1);
+    constexpr size_t kB = sizeof(Cxt) * KC * (TN + 1);
+    constexpr size_t kCs = EPI_LDS ? sizeof(Cxt) * (TN / 2) * (TM + 1) : 0;
+    constexpr size_t kS = (kA + kB) > kCs ? (kA + kB) : kCs;
+    __shared__ __attribute__((aligned(16))) char smem[kS];
+    Cxt(*As)[TM + 1] = reinterpret_cast<Cxt(*)[TM + 1]>(smem);        // [k][row]
+    Cxt(*Bs)[TN + 1] = reinterpret_cast<Cxt(*)[TN + 1]>(smem + kA);   // [k][col]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int wr = wave & 1;   // 32-complex-row half
+    const int wr = wave & 1;   // row half
     const int wc = wave >> 1;  // 32-col half
     const int l16 = lane & 15;
     const int lq = lane >> 4;  // 0..3: real k index within the MFMA step
+    // MFMA row l16 -> real row of the tile.  The f32 instruction returns row
+    // 4*(lane/16) + v in register v, the f64 one row 4*v + lane/16; feeding
+    // the f64 A rows transposed within each 16 makes both precisions return
+    // real rows 4*lq + v, i.e. the re/im pairs of two complex rows.
+    const int mrow = sizeof(T) == 8 ? 4 * (l16 & 3) + (l16 >> 2) : l16;
 
-    acc_t acc[4][2];
+    acc_t acc[RT][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RT; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = MF::zero();
 
     for (int64_t kc0 = 0; kc0 < K; kc0 += KC) {
-        // A chunk: TM rows x KC cols (column-major source, rows contiguous)
         for (int e = tid; e < TM * KC; e += 256) {
             const int r = e % TM, kk = e / TM;
             const int64_t gr = m0 + r, gk = kc0 + kk;
             As[kk][r] = (gr < M && gk < K) ? a[gr + gk * lda] : Cxt{T(0), T(0)};
         }
-        // B chunk: KC rows x TN cols
         for (int e = tid; e < TN * KC; e += 256) {
             const int kk = e % KC, cc = e / KC;
             const int64_t gk = kc0 + kk, gc = n0 + cc;
             Bs[kk][cc] = (gk < K && gc < N) ? bmat[gk + gc * ldb] : Cxt{T(0), T(0)};
         }
         __syncthreads();
-        // 2*KC real k values, 4 per MFMA
 #pragma unroll
         for (int ks = 0; ks < 2 * KC; ks += 4) {
             const int rk = ks + lq;     // real k of this lane
             const int lk = rk >> 1;     // complex k
             const int tk = rk & 1;      // 0: multiplies Re(B), 1: multiplies Im(B)
-            T av[4], bv[2];
-            // MFMA row l16 -> real row of the tile.  The f32 instruction returns
-            // row 4*(lane/16) + v in register v; the f64 one returns row
-            // 4*v + lane/16, so for f64 the A rows are fed transposed within
-            // each group of 16, which makes both precisions hand back real rows
-            // 4*lq + v (= re/im pairs of two complex rows) in registers v.
-            const int mrow = sizeof(T) == 8 ? 4 * (l16 & 3) + (l16 >> 2) : l16;
+            T av[RT], bv[2];
 #pragma unroll
-            for (int tr = 0; tr < 4; ++tr) {
-                const int rrow = wr * 64 + tr * 16 + mrow;  // real row in tile
+            for (int tr = 0; tr < RT; ++tr) {
+                const int rrow = wr * TM + tr * 16 + mrow;
                 const Cxt z = As[lk][rrow >> 1];
-                const int s = rrow & 1;
+                const int sgn = rrow & 1;
                 // A'[2i+s][2l+t]: s=0: (Re, -Im), s=1: (Im, Re)
-                av[tr] = s == 0 ? (tk == 0 ? z.re : -z.im) : (tk == 0 ? z.im : z.re);
+                av[tr] = sgn == 0 ? (tk == 0 ? z.re : -z.im) : (tk == 0 ? z.im : z.re);
             }
 #pragma unroll
             for (int tc = 0; tc < 2; ++tc) {
@@ -303,33 +561,70 @@ __global__ __launch_bounds__(256) void cgemm_kernel(int64_t M, int64_t N, int64_
                 bv[tc] = tk == 0 ? z.re : z.im;
             }
 #pragma unroll
-            for (int tr = 0; tr < 4; ++tr)
+            for (int tr = 0; tr < RT; ++tr)
 #pragma unroll
                 for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = MF::mma(av[tr], bv[tc], acc[tr][tc]);
         }
         __syncthreads();
     }
-    // epilogue: lane holds real rows 4*lq + v of each 16-row tile
+    if constexpr (EPI_LDS) {
+        Cxt(*Cs)[TM + 1] = reinterpret_cast<Cxt(*)[TM + 1]>(smem);   // [col][row]
 #pragma unroll
-    for (int tr = 0; tr < 4; ++tr) {
+        for (int half = 0; half < 2; ++half) {
+            if (wc == half) {
 #pragma unroll
-        for (int tc = 0; tc < 2; ++tc) {
-            const int64_t col = n0 + wc * 32 + tc * 16 + l16;
+                for (int tr = 0; tr < RT; ++tr)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int64_t row = m0 + (wr * 64 + tr * 16) / 2 + 2 * lq + h;
+                    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int cr = (wr * TM + tr * 16) / 2 + 2 * lq + h;
+                            Cs[tc * 16 + l16][cr] = Cxt{acc[tr][tc][2 * h], acc[tr][tc][2 * h + 1]};
+                        }
+            }
+            __syncthreads();
+            for (int e = tid; e < TM * (TN / 2); e += 256) {
+                const int cr = e % TM, cc = e / TM;
+                const int64_t row = m0 + cr, col = n0 + half * (TN / 2) + cc;
                 if (row < M && col < N) {
-                    const Cxt v{acc[tr][tc][2 * h], acc[tr][tc][2 * h + 1]};
                     Cxt* dst = c + row + col * ldc;
-                    if (MODE == 0) {
-                        *dst = *dst - v;
-                    } else {
-                        *dst = v;
+                    if (MODE == 0) *dst = *dst - Cs[cc][cr];
+                    else *dst = Cs[cc][cr];
+                }
+            }
+            __syncthreads();
+        }
+    } else {
+#pragma unroll
+        for (int tr = 0; tr < RT; ++tr)
+#pragma unroll
+            for (int tc = 0; tc < 2; ++tc) {
+                const int64_t col = n0 + wc * 32 + tc * 16 + l16;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int64_t row = m0 + (wr * TM + tr * 16) / 2 + 2 * lq + h;
+                    if (row < M && col < N) {
+                        const Cxt v{acc[tr][tc][2 * h], acc[tr][tc][2 * h + 1]};
+                        Cxt* dst = c + row + col * ldc;
+                        if (MODE == 0) *dst = *dst - v;
+                        else *dst = v;
                     }
                 }
             }
-        }
     }
+}
+
+// Variant table: bit 0 = XCD-aware order, bit 1 = LDS epilogue, bit 2 = TM 128.
+constexpr int kGemmVariants = 8;
+
+template <typename T, int MODE, int V>
+void gemm_variant(ccab_ctx* ctx, int64_t grid, int64_t M, int64_t N, int64_t K, const Cx<T>* A,
+                  int64_t lda, int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C,
+                  int64_t ldc, int64_t sC, int64_t batch) {
+    constexpr int TMv = (V & 4) ? 128 : 64;
+    hipLaunchKernelGGL((cgemm_kernel<T, MODE, TMv, (V & 1) != 0, (V & 2) != 0>),
+                       dim3((unsigned)grid), dim3(256), 0, ctx->stream, M, N, K, A, lda, sA, B,
+                       ldb, sB, C, ldc, sC, batch);
 }
 
 template <typename T, int MODE>
@@ -337,9 +632,23 @@ int gemm_launch(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, 
                 int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C, int64_t ldc,
                 int64_t sC, int64_t batch) {
     if (M <= 0 || N <= 0 || K <= 0) return CCAB_OK;
-    dim3 grid((unsigned)((M + TM - 1) / TM), (unsigned)((N + TN - 1) / TN), (unsigned)batch);
-    hipLaunchKernelGGL((cgemm_kernel<T, MODE>), grid, dim3(256), 0, ctx->stream, M, N, K, A, lda,
-                       sA, B, ldb, sB, C, ldc, sC);
+    const int pi = sizeof(T) == 8 ? 0 : 1;
+    int v = ctx->gemm_variant[pi];
+    if (MODE == 1 || M <= 64) v &= ~4;  // in-place U12 needs one row tile
+    const int64_t tmv = (v & 4) ? 128 : 64;
+    const int64_t total = ((M + tmv - 1) / tmv) * ((N + TN - 1) / TN) * batch;
+    const int64_t grid = (v & 1) ? ((total + kXcd - 1) / kXcd) * kXcd : total;
+    CCAB_CHECK(grid < (int64_t(1) << 31), "GEMM grid too large");
+    switch (v) {
+    case 0: gemm_variant<T, MODE, 0>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 1: gemm_variant<T, MODE, 1>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 2: gemm_variant<T, MODE, 2>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 3: gemm_variant<T, MODE, 3>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 4: gemm_variant<T, MODE, 4>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 5: gemm_variant<T, MODE, 5>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    case 6: gemm_variant<T, MODE, 6>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    default: gemm_variant<T, MODE, 7>(ctx, grid, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch); break;
+    }
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
@@ -431,6 +740,67 @@ __global__ void perm_from_ipiv_kernel(const int32_t* __restrict__ ipiv, int64_t 
 
 inline int64_t nblocks_of(int64_t n) { return (n + NB - 1) / NB; }
 
+template <typename T, int W, int R>
+void slab_factor(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int64_t k0, int nb,
+                 int c0o, int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch) {
+    hipLaunchKernelGGL((slab_factor_kernel<T, W, R>), dim3((unsigned)batch), dim3(kSlabThreads), 0,
+                       ctx->stream, A, lda, sA, n, k0, nb, c0o, ipiv, perm, info);
+}
+
+template <typename T, int MODE>
+int gemm_launch(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, int64_t lda,
+                int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C, int64_t ldc,
+                int64_t sC, int64_t batch);
+
+// Right-looking over slabs of W columns: register pivoting of the slab, then
+// the MFMA GEMM update of the panel columns right of it.  Rows per lane
+// R = ceil((n - k0) / kSlabThreads) sets W so that the R x W register block
+// stays within the 128 VGPRs a 1024-lane workgroup allows.  Panels taller
+// than 16 rows per lane use the generic column-by-column kernel.
+template <typename T, int W, int R>
+int panel_slabs(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int64_t k0, int nb,
+                int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch) {
+    for (int c0o = 0; c0o < nb; c0o += W) {
+        slab_factor<T, W, R>(ctx, A, lda, sA, n, k0, nb, c0o, ipiv, perm, info, batch);
+        const int w = std::min(W, nb - c0o);
+        const int64_t c1 = k0 + c0o + w;         // first column right of the slab
+        const int64_t rest = k0 + nb - c1;       // panel columns still to update
+        if (rest > 0 && n - c1 > 0) {
+            CCAB_TRY((gemm_launch<T, 0>(ctx, n - c1, rest, w, A + c1 + (c1 - w) * lda, lda, sA,
+                                        A + (c1 - w) + c1 * lda, lda, sA, A + c1 + c1 * lda, lda,
+                                        sA, batch)));
+        }
+    }
+    return CCAB_OK;
+}
+
+template <typename T>
+int launch_panel(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int64_t k0, int nb,
+                 int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch) {
+    const int64_t m = n - k0;
+    const int64_t R = (m + kSlabThreads - 1) / kSlabThreads;
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int WA = F32 ? 8 : 4;   // R <= 4
+    constexpr int WB = F32 ? 4 : 2;   // R <= 8
+    constexpr int WC = F32 ? 2 : 1;   // R <= 16
+    switch (R) {
+    case 1: return panel_slabs<T, WA, 1>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 2: return panel_slabs<T, WA, 2>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 3: return panel_slabs<T, WA, 3>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 4: return panel_slabs<T, WA, 4>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 5: case 6: return panel_slabs<T, WB, 6>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 7: case 8: return panel_slabs<T, WB, 8>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 9: case 10: case 11: case 12:
+        return panel_slabs<T, WC, 12>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    case 13: case 14: case 15: case 16:
+        return panel_slabs<T, WC, 16>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch);
+    default:
+        hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0,
+                           ctx->stream, A, lda, sA, n, k0, nb, ipiv, perm, info);
+        return CCAB_OK;
+    }
+}
+
 template <typename T>
 int lu_aux(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>** linv, Cx<T>** uinv, int32_t** perm,
            int32_t** info) {
@@ -453,49 +823,82 @@ int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, 
     hipLaunchKernelGGL(init_perm_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
                        dim3(256), 0, ctx->stream, perm, n, info);
     const double B = double(batch);
-    for (int64_t kb = 0; kb < nblocks_of(n); ++kb) {
-        const int64_t k0 = kb * NB;
-        const int nb = (int)std::min<int64_t>(NB, n - k0);
-        double pflops = 0.0;
-        for (int j = 0; j < nb; ++j) pflops += 8.0 * double(n - k0 - j - 1) * double(nb - j - 1);
-        {
-            ProfScope ps(ctx, PROF_LU_PANEL, pflops * B);
-            hipLaunchKernelGGL(panel_kernel<T>, dim3((unsigned)batch), dim3(kPanelThreads), 0,
-                               ctx->stream, A, lda, strideA, n, k0, nb, ipiv, perm, info);
+    auto laswp = [&](int64_t r0, int np, int64_t c_lo, int64_t c_hi, int64_t x_lo, int64_t x_hi) {
+        if (c_hi <= c_lo) return;
+        ProfScope ps(ctx, PROF_LU_LASWP, 0.0);
+        hipLaunchKernelGGL(laswp_kernel<T>,
+                           dim3((unsigned)((c_hi - c_lo + 255) / 256), (unsigned)batch),
+                           dim3(256), 0, ctx->stream, A, lda, strideA, n, r0, np, ipiv, c_lo, c_hi,
+                           x_lo, x_hi);
+    };
+    // U rows j0..j0+nb of columns [c, c+w): in place, U = Linv(j0) A
+    auto trsm = [&](int64_t j0, int nb, int64_t c, int64_t w) -> int {
+        if (w <= 0) return CCAB_OK;
+        ProfScope ps(ctx, PROF_LU_TRSM, 8.0 * nb * double(w) * nb * B);
+        const int64_t kb = j0 / NB;
+        return gemm_launch<T, 1>(ctx, nb, w, nb, linv + kb * NB * NB, NB, inv_stride,
+                                 A + j0 + c * lda, lda, strideA, A + j0 + c * lda, lda, strideA,
+                                 batch);
+    };
+    // A[r:r+m, c:c+w] -= A[r:r+m, k:k+kk] A[k:k+kk, c:c+w]
+    auto update = [&](int64_t r, int64_t m, int64_t c, int64_t w, int64_t k, int64_t kk,
+                      int cls) -> int {
+        if (m <= 0 || w <= 0) return CCAB_OK;
+        ProfScope ps(ctx, cls, 8.0 * double(m) * double(w) * double(kk) * B);
+        return gemm_launch<T, 0>(ctx, m, w, kk, A + r + k * lda, lda, strideA, A + k + c * lda,
+                                 lda, strideA, A + r + c * lda, lda, strideA, batch);
+    };
+    // Outer panels of NBO = lu_outer columns (64 or 128) made of 64-wide inner
+    // panels; the trailing update then has K = NBO, which halves the C traffic
+    // of the O(n^3) part at NBO = 128.
+    const int NBO = ctx->lu_outer;
+    for (int64_t k0 = 0; k0 < n; k0 += NBO) {
+        const int64_t nbo = std::min<int64_t>(NBO, n - k0);
+        for (int64_t j0 = k0; j0 < k0 + nbo; j0 += NB) {
+            const int nb = (int)std::min<int64_t>(NB, k0 + nbo - j0);
+            const int64_t kb = j0 / NB;
+            double pflops = 0.0;
+            for (int j = 0; j < nb; ++j) pflops += 8.0 * double(n - j0 - j - 1) * double(nb - j - 1);
+            {
+                ProfScope ps(ctx, PROF_LU_PANEL, pflops * B);
+                CCAB_TRY(launch_panel<T>(ctx, A, lda, strideA, n, j0, nb, ipiv, perm, info, batch));
+            }
+            {
+                ProfScope ps(ctx, PROF_LU_TRINV, 0.0);
+                hipLaunchKernelGGL((trinv_kernel<T, false>), dim3((unsigned)batch, 1), dim3(NB),
+                                   0, ctx->stream, A, lda, strideA, n, kb, linv, inv_stride);
+            }
+            // this inner panel's swaps on the rest of the outer panel
+            laswp(j0, nb, k0, k0 + nbo, j0, j0 + nb);
+            const int64_t c1 = j0 + nb, w = k0 + nbo - c1;
+            if (w > 0) {
+                CCAB_TRY(trsm(j0, nb, c1, w));
+                CCAB_TRY(update(c1, n - c1, c1, w, j0, nb, PROF_LU_UPDATE));
+            }
         }
-        {
-            ProfScope ps(ctx, PROF_LU_TRINV, 0.0);
-            hipLaunchKernelGGL(trinv_kernel<T>, dim3((unsigned)batch), dim3(NB), 0, ctx->stream,
-                               A, lda, strideA, k0, nb, linv + kb * NB * NB, uinv + kb * NB * NB,
-                               inv_stride);
-        }
-        {
-            ProfScope ps(ctx, PROF_LU_LASWP, 0.0);
-            hipLaunchKernelGGL(laswp_kernel<T>, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
-                               dim3(256), 0, ctx->stream, A, lda, strideA, n, k0, nb, ipiv);
-        }
-        const int64_t rest = n - k0 - nb;
+        // all swaps of the outer panel on the columns outside it
+        laswp(k0, (int)nbo, 0, n, k0, k0 + nbo);
+        const int64_t rest = n - k0 - nbo;
         if (rest > 0) {
-            {
-                // U12 = Linv * A12 (in place: M = nb <= 64, one row tile)
-                ProfScope ps(ctx, PROF_LU_TRSM, 8.0 * nb * double(rest) * nb * B);
-                CCAB_TRY((gemm_launch<T, 1>(ctx, nb, rest, nb, linv + kb * NB * NB, NB,
-                                            inv_stride, A + k0 + (k0 + nb) * lda, lda, strideA,
-                                            A + k0 + (k0 + nb) * lda, lda, strideA, batch)));
+            const int64_t c1 = k0 + nbo;
+            for (int64_t j0 = k0; j0 < k0 + nbo; j0 += NB) {
+                const int nb = (int)std::min<int64_t>(NB, k0 + nbo - j0);
+                CCAB_TRY(trsm(j0, nb, c1, rest));
+                const int64_t r1 = j0 + nb;
+                if (r1 < k0 + nbo) CCAB_TRY(update(r1, k0 + nbo - r1, c1, rest, j0, nb, PROF_LU_TRSM));
             }
-            {
-                // A22 -= L21 * U12
-                ProfScope ps(ctx, PROF_LU_UPDATE, 8.0 * double(rest) * double(rest) * nb * B);
-                CCAB_TRY((gemm_launch<T, 0>(ctx, rest, rest, nb, A + (k0 + nb) + k0 * lda, lda,
-                                            strideA, A + k0 + (k0 + nb) * lda, lda, strideA,
-                                            A + (k0 + nb) + (k0 + nb) * lda, lda, strideA,
-                                            batch)));
-            }
+            CCAB_TRY(update(c1, rest, c1, rest, k0, nbo, PROF_LU_UPDATE));
         }
+    }
+    {
+        ProfScope ps(ctx, PROF_LU_TRINV, 0.0);
+        hipLaunchKernelGGL((trinv_kernel<T, true>), dim3((unsigned)batch, (unsigned)nblocks_of(n)),
+                           dim3(NB), 0, ctx->stream, A, lda, strideA, n, int64_t(0), uinv,
+                           inv_stride);
     }
     CCAB_HIP(hipGetLastError());
     ctx->lu_key = A;
-    ctx->lu_prec = sizeof(T) == 16 ? CCAB_C128 : CCAB_C64;
+    ctx->lu_prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
     ctx->lu_n = n;
     ctx->lu_batch = batch;
     ctx->lu_lda = lda;
@@ -519,7 +922,7 @@ int lu_solve_t(ccab_ctx* ctx, int64_t n, int64_t batch, const Cx<T>* LU, int64_t
     int32_t *perm, *info;
     const bool cached = ctx->lu_key == LU && ctx->lu_n == n && ctx->lu_batch == batch &&
                         ctx->lu_lda == lda && ctx->lu_stride == strideA &&
-                        ctx->lu_prec == (sizeof(T) == 16 ? CCAB_C128 : CCAB_C64);
+                        ctx->lu_prec == (sizeof(T) == 8 ? CCAB_C128 : CCAB_C64);
     CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
     const int64_t inv_stride = nblocks_of(n) * NB * NB;
     const int64_t nblk = nblocks_of(n);
@@ -527,15 +930,14 @@ int lu_solve_t(ccab_ctx* ctx, int64_t n, int64_t batch, const Cx<T>* LU, int64_t
         // factors from elsewhere: rebuild the diagonal inverses and permutation
         hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3((unsigned)((batch + 63) / 64)), dim3(64), 0,
                            ctx->stream, ipiv, n, perm, batch);
-        for (int64_t kb = 0; kb < nblk; ++kb) {
-            const int64_t k0 = kb * NB;
-            const int nb = (int)std::min<int64_t>(NB, n - k0);
-            hipLaunchKernelGGL(trinv_kernel<T>, dim3((unsigned)batch), dim3(NB), 0, ctx->stream,
-                               LU, lda, strideA, k0, nb, linv + kb * NB * NB, uinv + kb * NB * NB,
-                               inv_stride);
-        }
+        hipLaunchKernelGGL((trinv_kernel<T, false>), dim3((unsigned)batch, (unsigned)nblk),
+                           dim3(NB), 0, ctx->stream, LU, lda, strideA, n, int64_t(0), linv,
+                           inv_stride);
+        hipLaunchKernelGGL((trinv_kernel<T, true>), dim3((unsigned)batch, (unsigned)nblk),
+                           dim3(NB), 0, ctx->stream, LU, lda, strideA, n, int64_t(0), uinv,
+                           inv_stride);
         ctx->lu_key = LU;
-        ctx->lu_prec = sizeof(T) == 16 ? CCAB_C128 : CCAB_C64;
+        ctx->lu_prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
         ctx->lu_n = n;
         ctx->lu_batch = batch;
         ctx->lu_lda = lda;

@@ -7,6 +7,7 @@
 //    counterpart: it stands in for solve_lu in the frequency sweep).
 #include <cmath>
 #include <complex>
+#include <cstring>
 
 #include "ccab_internal.h"
 
@@ -47,9 +48,10 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
 // REF: coefficient sum_j conj(w_j) q_ij (nalgebra qk1.dotc(q_i), gmres.rs:168).
 template <bool REF>
 __global__ __launch_bounds__(kMgsThreads) void mgs_kernel(int64_t n, int64_t k, c128* Q, int64_t ldq,
-                                                          int64_t sQ, c128* h, int64_t sh) {
+                                                          int64_t sQ, c128* h, int64_t sh,
+                                                          const int32_t* __restrict__ active) {
     __shared__ double red[2 * kMgsThreads / 64];
-    const int64_t b = blockIdx.x;
+    const int64_t b = active ? active[blockIdx.x] : blockIdx.x;
     c128* q = Q + b * sQ;
     c128* w = q + (k + 1) * ldq;
     c128* hb = h + b * sh;
@@ -103,6 +105,23 @@ __global__ void add_c128_kernel(int64_t n, const c128* __restrict__ d, c128* x) 
 }
 
 inline dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// dst_b = src_b / nrm[b] for the active systems (q0 = r / ||r||)
+__global__ void scale_inv_norm_kernel(int64_t n, const c128* __restrict__ src, int64_t ss,
+                                      const double* __restrict__ nrm, c128* dst, int64_t sd,
+                                      const int32_t* __restrict__ active) {
+    const int64_t b = active[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[b * sd + i] = src[b * ss + i] * crecip(c128{nrm[b], 0.0});
+}
+
+// dst_b = src_b for the active systems
+__global__ void copy_active_kernel(int64_t n, const c128* __restrict__ src, c128* dst,
+                                   const int32_t* __restrict__ active) {
+    const int64_t b = active[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[b * n + i] = src[b * n + i];
+}
 
 using cd = std::complex<double>;
 
@@ -219,10 +238,10 @@ int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, 
             ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0);
             if (p.ref_mgs)
                 hipLaunchKernelGGL(mgs_kernel<true>, dim3(1), dim3(kMgsThreads), 0, s, n, k, Q, n,
-                                   0, h, 0);
+                                   0, h, 0, nullptr);
             else
                 hipLaunchKernelGGL(mgs_kernel<false>, dim3(1), dim3(kMgsThreads), 0, s, n, k, Q, n,
-                                   0, h, 0);
+                                   0, h, 0, nullptr);
             CCAB_HIP(hipMemcpyAsync(hp, h, 16 * (k + 2), hipMemcpyDeviceToHost, s));
             CCAB_HIP(hipStreamSynchronize(s));
             for (int64_t i = 0; i < k + 2; ++i) hk[i] = cd(hp[2 * i], hp[2 * i + 1]);
@@ -346,6 +365,196 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
             st[b].rel_residual = hp[b] / hp[batch + b];
             st[b].backward_error = -1.0;  // needs ||A||_F: not computed on this path
             st[b].converged = 1;
+        }
+    }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched GMRES with FP64 iterative refinement (no reference counterpart: it
+// replaces per-frequency LU/GMRES in the sweep).  Every cycle computes the
+// FP64 residual r = b - A x with the c128 matrix (or the c64 one when A128 is
+// null), then runs one GMRES(m) cycle on the c64 operator for A d = r (FP64
+// Krylov vectors, modified Gram-Schmidt in FP64) until ||r - A d|| <=
+// inner_tol ||r||, and updates x += d.  All systems advance together: one
+// batched c64 matvec and one batched MGS launch per Arnoldi step, the Givens
+// rotations of every system on the host; converged systems leave the active
+// set.  Stops when ||b - A x|| <= tol ||b|| (FP64).
+// ---------------------------------------------------------------------------
+int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c64* A64,
+                 int64_t lda, int64_t sA, c128* X, const c128* X0, int m, double inner_tol,
+                 double tol, int max_cycles, bool ref_mgs, ccab_solve_stats* st) {
+    if (m <= 0) m = 50;
+    m = (int)std::min<int64_t>(m, n);
+    if (inner_tol <= 0) inner_tol = 1e-4;
+    if (tol <= 0) tol = 1e-10;
+    if (max_cycles <= 0) max_cycles = 20;
+    hipStream_t s = ctx->stream;
+    const size_t nb = size_t(n) * B;
+    c128* Bv = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_B, nb * 16));
+    c128* R = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_R, nb * 16));
+    c128* Q = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_Q, nb * (m + 1) * 16));
+    c128* Hd = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_H, size_t(B) * (m + 2) * 16));
+    c128* Yd = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_Y, size_t(B) * (m + 1) * 16));
+    double* nrm = static_cast<double*>(ctx_buf(ctx, SLOT_GIR_NRM, size_t(B) * 8));
+    int32_t* act = static_cast<int32_t*>(ctx_buf(ctx, SLOT_GIR_ACT, size_t(B) * 4 * 2));
+    if (!Bv || !R || !Q || !Hd || !Yd || !nrm || !act) return CCAB_ENOMEM;
+    const int64_t sq = n * (m + 1);
+    const size_t pin_h = size_t(B) * (m + 2) * 16, pin_y = size_t(B) * (m + 1) * 16;
+    CCAB_TRY(pinned_reserve(ctx, pin_h + pin_y + size_t(B) * 16 + size_t(B) * 8));
+    char* pin = static_cast<char*>(ctx->pinned);
+    double* hH = reinterpret_cast<double*>(pin);
+    double* hY = reinterpret_cast<double*>(pin + pin_h);
+    int32_t* hAct = reinterpret_cast<int32_t*>(pin + pin_h + pin_y);
+    double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + size_t(B) * 8);
+    const c128 one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
+    const void* Ares = A128 ? static_cast<const void*>(A128) : static_cast<const void*>(A64);
+    const int pres = A128 ? CCAB_C128 : CCAB_C64;
+
+    hipLaunchKernelGGL(copy_c128_kernel, grid1(nb), dim3(256), 0, s, nb, X, Bv);
+    if (X0) hipLaunchKernelGGL(copy_c128_kernel, grid1(nb), dim3(256), 0, s, nb, X0, X);
+    else hipLaunchKernelGGL(fill_zero_kernel, grid1(nb), dim3(256), 0, s, nb, X);
+
+    std::vector<int32_t> active(B);
+    for (int64_t b = 0; b < B; ++b) active[b] = int32_t(b);
+    auto upload_active = [&](const std::vector<int32_t>& a, int32_t* dst) -> int {
+        std::memcpy(hAct, a.data(), a.size() * 4);
+        CCAB_HIP(hipMemcpyAsync(dst, hAct, a.size() * 4, hipMemcpyHostToDevice, s));
+        CCAB_HIP(hipStreamSynchronize(s));   // hAct is reused right away
+        return CCAB_OK;
+    };
+    auto read_norms = [&](const c128* V, int64_t stride, const std::vector<int32_t>& a,
+                          std::vector<double>& out) -> int {
+        CCAB_TRY(nrm2_active(ctx, n, (int64_t)a.size(), act, V, stride, nrm));
+        CCAB_HIP(hipMemcpyAsync(hN, nrm, size_t(B) * 8, hipMemcpyDeviceToHost, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+        for (int32_t b : a) out[b] = hN[b];
+        return CCAB_OK;
+    };
+    std::vector<double> bn(B), rn(B), rel(B, 1.0);
+    std::vector<int64_t> iters(B, 0), nmv(B, 0), cycles(B, 0);
+    std::vector<char> conv(B, 0);
+    CCAB_TRY(upload_active(active, act));
+    CCAB_TRY(read_norms(Bv, n, active, bn));
+    std::vector<cd> cs(size_t(B) * m), sn(size_t(B) * m), beta(size_t(B) * (m + 1)),
+        Hh(size_t(B) * (m + 1) * m);
+    std::vector<int> kdone(B);
+    for (int cyc = 0; cyc <= max_cycles; ++cyc) {
+        // FP64 residual of the active systems
+        CCAB_TRY(upload_active(active, act));
+        const unsigned na = (unsigned)active.size();
+        hipLaunchKernelGGL(copy_active_kernel, dim3((unsigned)((n + 255) / 256), na), dim3(256), 0,
+                           s, n, Bv, R, act);
+        if (cyc > 0 || X0) {
+            CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n));
+            for (int32_t b : active) nmv[b]++;
+        }
+        CCAB_TRY(read_norms(R, n, active, rn));
+        std::vector<int32_t> still;
+        for (int32_t b : active) {
+            rel[b] = bn[b] > 0 ? rn[b] / bn[b] : 0.0;
+            if (!std::isfinite(rel[b]))
+                return ctx_fail(ctx, CCAB_ENAN, "GMRES-IR residual is NaN in system %d", b);
+            if (rel[b] <= tol || rn[b] == 0.0) conv[b] = 1;
+            else still.push_back(b);
+        }
+        active.swap(still);
+        if (active.empty() || cyc == max_cycles) break;
+        CCAB_TRY(upload_active(active, act));
+        // one GMRES(m) cycle for A64 d = r on the active systems
+        hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
+                           (unsigned)active.size()), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
+        for (int32_t b : active) {
+            std::fill(cs.begin() + size_t(b) * m, cs.begin() + size_t(b + 1) * m, cd(0));
+            std::fill(sn.begin() + size_t(b) * m, sn.begin() + size_t(b + 1) * m, cd(0));
+            std::fill(beta.begin() + size_t(b) * (m + 1), beta.begin() + size_t(b + 1) * (m + 1), cd(0));
+            beta[size_t(b) * (m + 1)] = cd(rn[b], 0.0);
+            kdone[b] = -1;
+            cycles[b]++;
+        }
+        std::memset(hY, 0, pin_y);
+        std::vector<int32_t> inner = active;
+        for (int k = 0; k < m && !inner.empty(); ++k) {
+            CCAB_TRY(upload_active(inner, act));
+            const unsigned ni = (unsigned)inner.size();
+            CCAB_TRY(gemv_active(ctx, CCAB_C64, n, n, ni, act, A64, lda, sA, Q + int64_t(k) * n, sq,
+                                 one, zero, Q + int64_t(k + 1) * n, sq));
+            {
+                ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
+                if (ref_mgs)
+                    hipLaunchKernelGGL(mgs_kernel<true>, dim3(ni), dim3(kMgsThreads), 0, s, n, k, Q,
+                                       n, sq, Hd, m + 2, act);
+                else
+                    hipLaunchKernelGGL(mgs_kernel<false>, dim3(ni), dim3(kMgsThreads), 0, s, n, k,
+                                       Q, n, sq, Hd, m + 2, act);
+            }
+            CCAB_HIP(hipMemcpyAsync(hH, Hd, pin_h, hipMemcpyDeviceToHost, s));
+            CCAB_HIP(hipStreamSynchronize(s));
+            std::vector<int32_t> next;
+            for (int32_t b : inner) {
+                nmv[b]++;
+                iters[b]++;
+                cd* csb = cs.data() + size_t(b) * m;
+                cd* snb = sn.data() + size_t(b) * m;
+                cd* be = beta.data() + size_t(b) * (m + 1);
+                cd* H = Hh.data() + size_t(b) * (m + 1) * m;
+                std::vector<cd> hk(k + 2);
+                const double* hb = hH + size_t(b) * (m + 2) * 2;
+                for (int i = 0; i < k + 2; ++i) hk[i] = cd(hb[2 * i], hb[2 * i + 1]);
+                for (int i = 0; i < k; ++i) {
+                    const cd temp = csb[i] * hk[i] + snb[i] * hk[i + 1];
+                    hk[i + 1] = -std::conj(snb[i]) * hk[i] + csb[i] * hk[i + 1];
+                    hk[i] = temp;
+                }
+                const cd v1 = hk[k], v2 = hk[k + 1];
+                const cd signf = (v1 == cd(0.0, 0.0)) ? cd(1.0, 0.0) : v1 / std::abs(v1);
+                const double rr = std::sqrt(std::norm(v1) + std::norm(v2));
+                csb[k] = cd(std::abs(v1) / rr, 0.0);
+                snb[k] = signf * std::conj(v2) / rr;
+                hk[k] = csb[k] * hk[k] + snb[k] * hk[k + 1];
+                for (int i = 0; i <= k; ++i) H[i + size_t(k) * (m + 1)] = hk[i];
+                be[k + 1] = -std::conj(snb[k]) * be[k];
+                be[k] *= csb[k];
+                const double est = std::abs(be[k + 1]) / rn[b];
+                const bool done = est <= inner_tol || k == m - 1 ||
+                                  std::abs(be[k + 1]) <= 0.5 * tol * bn[b] || !std::isfinite(est);
+                if (done) {
+                    const int kk = k + 1;
+                    std::vector<cd> Hk(size_t(kk) * kk), yv(kk);
+                    for (int c = 0; c < kk; ++c)
+                        for (int q = 0; q < kk; ++q) Hk[q + size_t(c) * kk] = H[q + size_t(c) * (m + 1)];
+                    for (int i = 0; i < kk; ++i) yv[i] = be[i];
+                    host_lu_solve(kk, Hk, yv);
+                    double* yb = hY + size_t(b) * (m + 1) * 2;
+                    for (int i = 0; i < kk; ++i) {
+                        yb[2 * i] = yv[i].real();
+                        yb[2 * i + 1] = yv[i].imag();
+                    }
+                    kdone[b] = kk;
+                } else {
+                    next.push_back(b);
+                }
+            }
+            inner.swap(next);
+        }
+        // x += Q y for every system of this cycle (y zero beyond its own size)
+        CCAB_TRY(upload_active(active, act));
+        CCAB_HIP(hipMemcpyAsync(Yd, hY, pin_y, hipMemcpyHostToDevice, s));
+        CCAB_TRY(gemv_active(ctx, CCAB_C128, n, m, (int64_t)active.size(), act, Q, n, sq, Yd, m + 1,
+                             one, one, X, n));
+        CCAB_HIP(hipStreamSynchronize(s));
+    }
+    if (st) {
+        for (int64_t b = 0; b < B; ++b) {
+            st[b] = ccab_solve_stats{};
+            st[b].num_mv = nmv[b];
+            st[b].iterations = iters[b];
+            st[b].restarts = cycles[b];
+            st[b].ir_steps = cycles[b];
+            st[b].rel_residual = rel[b];
+            st[b].backward_error = -1.0;
+            st[b].converged = conv[b];
         }
     }
     CCAB_HIP(hipGetLastError());

@@ -42,8 +42,9 @@ def solve_for_surface(predata: PreData, rhs_inc: torch.Tensor, ctx=None):
     return phi, vn
 
 
-def sweep(predata: PreData, frequencies=None, ctx=None):
-    """Surface solution at every frequency: phi, vn of shape (nfreq, n)."""
+def sweep(predata: PreData, frequencies=None, x0: torch.Tensor | None = None, ctx=None):
+    """Surface solution at every frequency: phi, vn of shape (nfreq, n).
+    x0 (nfreq, n): initial guess of the solved unknown (ITERATIVE_MIXED only)."""
     ctx = ctx or capi.default_context(predata.device.index or 0)
     fs = predata.get_frequencies() if frequencies is None else list(frequencies)
     n = predata.get_num_eqn()
@@ -52,10 +53,36 @@ def sweep(predata: PreData, frequencies=None, ctx=None):
     stats = (capi.SolveStats * len(fs))()
     cfg = predata.solver.to_c()
     w = predata.incident_wave.to_c()
-    ctx.call("ccab_sweep", predata.physics_array(fs), len(fs), ct.byref(w), _ptr(predata.d_cpts),
-             n, ct.byref(cfg), _ptr(phi), _ptr(vn), stats)
+    if x0 is None:
+        ctx.call("ccab_sweep", predata.physics_array(fs), len(fs), ct.byref(w),
+                 _ptr(predata.d_cpts), n, ct.byref(cfg), _ptr(phi), _ptr(vn), stats)
+    else:
+        x0 = x0.to(torch.complex128).contiguous()
+        if x0.shape != (len(fs), n):
+            raise ValueError("x0 must have shape (nfreq, n)")
+        ctx.call("ccab_sweep_warm", predata.physics_array(fs), len(fs), ct.byref(w),
+                 _ptr(predata.d_cpts), n, ct.byref(cfg), _ptr(x0), _ptr(phi), _ptr(vn), stats)
     ctx.synchronize()
     return phi, vn, [s.as_dict() for s in stats]
+
+
+def gmres_ir_batched(A128: torch.Tensor | None, A64: torch.Tensor, x: torch.Tensor,
+                     x0: torch.Tensor | None = None, tol=1e-10, inner_tol=1e-4, restart=50,
+                     max_cycles=20, ref_mgs=False, ctx=None):
+    """Batched GMRES on the c64 operator with FP64 refinement against A128
+    (column-major systems, see _check_batch); x (batch, n): b -> x."""
+    ctx = ctx or capi.default_context(A64.device.index or 0)
+    nb, n = _check_batch(A64)
+    if A128 is not None:
+        _check_batch(A128)
+    cfg = capi.solver_cfg(capi.SOLVER_ITERATIVE_MIXED,
+                          capi.gmres_params(n, tol, restart, ref_mgs, capi.STOP_RELATIVE_RESIDUAL,
+                                            capi.C64), inner_tol, max_cycles)
+    stats = (capi.SolveStats * nb)()
+    ctx.call("ccab_gmres_ir_batched", n, nb, _ptr(A128), _ptr(A64), n, n * n, _ptr(x), _ptr(x0),
+             ct.byref(cfg), stats)
+    ctx.synchronize()
+    return x, [s.as_dict() for s in stats]
 
 
 def solve_lu(a: torch.Tensor, x: torch.Tensor, ctx=None):

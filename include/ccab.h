@@ -121,6 +121,12 @@ int ccab_profile_enable(ccab_ctx* ctx, int on);
 int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double* work,
                       int reset);
 
+/* LU tuning (no reference counterpart): the complex-GEMM variant for
+ * precision prec (bit 0 XCD-aware tile order, bit 1 LDS-staged epilogue,
+ * bit 2 128-row tiles) and the outer panel width (64 or 128) that sets the
+ * K of the trailing update.  ccab_autotune picks both by measurement. */
+int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer);
+
 /* ---- geometry (host) --------------------------------------------------- */
 /* One centroid collocation point per linear tri/quad element:
  * elements.rs:49-181 (TRIGP1 / QUADGP1, detj incl. the quad 0.25 factor at
@@ -194,6 +200,20 @@ int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda,
                      ccab_c128* d_x, const ccab_c128* d_x0, const ccab_gmres_params* p,
                      ccab_solve_stats* stats);
 
+/* Batched GMRES with FP64 iterative refinement (no reference counterpart;
+ * stands in for per-frequency solve_lu / GMRES::solve in the sweep).  Each
+ * cycle: FP64 residual r = b - A x with d_A128 (c64 d_A64 when d_A128 is
+ * NULL), one GMRES(m) cycle on the c64 operator for A d = r (FP64 Krylov
+ * vectors and Gram-Schmidt) to ||r - A d|| <= inner_tol ||r||, x += d; stop at
+ * ||b - A x|| <= tol ||b||.  Parameters from cfg: gmres.tolerance = tol (<=0 ->
+ * 1e-10), gmres.restart = m (0 -> 50), ir_tolerance = inner_tol (<=0 -> 1e-4),
+ * ir_max_steps = maximum cycles (<=0 -> 20), gmres.ref_mgs = Arnoldi
+ * coefficient.  d_x: b on entry, x on exit (n x batch); d_x0 optional warm start. */
+int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const ccab_c128* d_A128,
+                          const ccab_c64* d_A64, int64_t lda, int64_t strideA, ccab_c128* d_x,
+                          const ccab_c128* d_x0, const ccab_solver_cfg* cfg,
+                          ccab_solve_stats* stats);
+
 /* ---- the surface solve (device) ---------------------------------------------- */
 /* solve.rs:11-71 (solve_for_surface, dense branches): assembly + BC algebra +
  * LU / GMRES / mixed LU with FP64 refinement; outputs surface phi and vn. */
@@ -214,6 +234,15 @@ int ccab_sweep(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_
 int ccab_cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K,
                       const void* d_A, int64_t lda, const void* d_B, int64_t ldb, void* d_C,
                       int64_t ldc);
+
+/* ccab_sweep with solver CCAB_SOLVER_ITERATIVE_MIXED started from d_x0
+ * (n x nfreq, the solved unknown of solve.rs:36-69 per frequency), e.g. a
+ * learned initial guess (no reference counterpart: the reference always
+ * starts GMRES from zero, gmres.rs:38-39). */
+int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                    const double* d_cpts, int64_t n, const ccab_solver_cfg* cfg,
+                    const ccab_c128* d_x0, ccab_c128* d_phi, ccab_c128* d_vn,
+                    ccab_solve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -227,11 +227,14 @@ def _field_from(pd, phi, vn):
     (capi.BURTON_MILLER, capi.SOLVER_DIRECT, P_BM, 1e-12),
     (capi.CLASSICAL, capi.SOLVER_DIRECT_MIXED, P_CLASSICAL, 1e-10),
     (capi.BURTON_MILLER, capi.SOLVER_DIRECT_MIXED, P_BM, 1e-10),
+    (capi.CLASSICAL, capi.SOLVER_ITERATIVE_MIXED, P_CLASSICAL, 1e-9),
+    (capi.BURTON_MILLER, capi.SOLVER_ITERATIVE_MIXED, P_BM, 1e-9),
 ])
 def test_reference_fixtures_end_to_end(sphere, method, kind, expect, tol):
     """verification.rs:95-205 through the GPU surface solve."""
     _note(f"fixture method={method} solver={kind}")
-    pd = _predata(sphere, method, solver=Solver(kind=kind, tolerance=1e-5, max_iterations=1000))
+    tol = 1e-5 if kind == capi.SOLVER_ITERATIVE else 1e-12
+    pd = _predata(sphere, method, solver=Solver(kind=kind, tolerance=tol, max_iterations=1000))
     rhs_inc = incident_wave.get_incident_surface(pd)
     phi, vn = solve.solve_for_surface(pd, rhs_inc)
     p = _field_from(pd, phi.cpu().numpy(), vn.cpu().numpy())
@@ -278,3 +281,48 @@ def test_error_paths():
     with pytest.raises(capi.CcabError) as e:
         solve.lu_factor_batched(A)
     assert e.value.code == capi.ESINGULAR
+
+
+@pytest.mark.parametrize("n,prec", [(2900, capi.C128), (5200, capi.C128), (3000, capi.C64)])
+def test_lu_large_panels_vs_lapack(n, prec):
+    """Sizes that exercise every register-slab panel width (rows per lane 6 and
+    11) and the XCD tile order; checked against LAPACK zgesv (numpy)."""
+    _note(f"lu large n={n} prec={prec}")
+    rng = np.random.default_rng(7)
+    dt = torch.complex128 if prec == capi.C128 else torch.complex64
+    shift = 3.0 if prec == capi.C128 else 2.0 * math.sqrt(2 * n)   # keep c64 well conditioned
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + shift * np.eye(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A = torch.from_numpy(a.T.copy()).to(dt).cuda().unsqueeze(0).transpose(1, 2)
+    B = torch.from_numpy(b).to(dt).cuda().unsqueeze(0).contiguous()
+    ipiv = solve.lu_factor_batched(A)
+    solve.lu_solve_batched(A, ipiv, B)
+    ref = np.linalg.solve(a, b)
+    err = np.linalg.norm(B[0].cpu().numpy().astype(complex) - ref) / np.linalg.norm(ref)
+    assert err < (1e-10 if prec == capi.C128 else 1e-5), err
+
+
+@pytest.mark.parametrize("x0_kind", ["zero", "warm"])
+def test_sweep_gmres_ir_matches_fp64(x0_kind):
+    _note(f"sweep GMRES-IR vs c128 LU, icosphere level 2, BM, x0={x0_kind}")
+    m = mesh.icosphere(2)
+    ka = np.linspace(0.5, 4.0, 7)
+    fs = ka * 343.0 / (2 * math.pi)
+    pd = PreData(m, fs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                 (0, 0), IncidentWave(capi.WAVE_PLANE, (0.2, 0.3, 0.9), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_DIRECT))
+    phi_d, vn_d, _ = solve.sweep(pd)
+    pd.solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-12, restart=30,
+                       ir_tolerance=1e-4, ir_max_steps=30, ref_mgs=False)
+    x0 = None
+    if x0_kind == "warm":   # a perturbed solution as the initial guess
+        x0 = phi_d * (1 + 0.01 * torch.randn_like(phi_d.real))
+    phi_g, vn_g, st = solve.sweep(pd, x0=x0)
+    for k in range(len(fs)):
+        e = float(torch.linalg.vector_norm(phi_g[k] - phi_d[k]) / torch.linalg.vector_norm(phi_d[k]))
+        assert e < 1e-9, (k, e, st[k])
+        assert st[k]["converged"] == 1 and st[k]["rel_residual"] <= 1e-12
+    if x0_kind == "warm":
+        pd.solver.kind = capi.SOLVER_ITERATIVE_MIXED
+        _, _, st0 = solve.sweep(pd)
+        assert sum(s["iterations"] for s in st) < sum(s["iterations"] for s in st0)
