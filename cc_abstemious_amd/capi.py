@@ -81,11 +81,13 @@ class SolveStats(ct.Structure):
 class SolverCfg(ct.Structure):
     _fields_ = [
         ("solver", ct.c_int32),
-        ("pad_", ct.c_int32),
+        ("leaf_size", ct.c_int32),
         ("gmres", GmresParams),
         ("ir_tolerance", ct.c_double),
         ("ir_max_steps", ct.c_int32),
-        ("pad2_", ct.c_int32),
+        ("pad_", ct.c_int32),
+        ("aca_tolerance", ct.c_double),
+        ("eta", ct.c_double),
     ]
 
 
@@ -125,6 +127,13 @@ SIGNATURES = {
                                    ct.POINTER(SolverCfg), _P]),
     "ccab_sweep_warm": (_I, [_P, _P, _I64, ct.POINTER(Wave), _P, _I64, ct.POINTER(SolverCfg), _P,
                              _P, _P, _P]),
+    "ccab_hmatrix_build": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _I64, ct.c_double,
+                                ct.c_double, ct.POINTER(_P)]),
+    "ccab_hmatrix_destroy": (_I, [_P]),
+    "ccab_hmatrix_gemv": (_I, [_P, _P, _P, _P, _P, _P]),
+    "ccab_hmatrix_to_full": (_I, [_P, _P, _P, _I64]),
+    "ccab_hmatrix_stats": (_I, [_P, _P, _P]),
+    "ccab_gmres_hmatrix": (_I, [_P, _P, _P, _P, ct.POINTER(GmresParams), _P]),
 }
 
 _lib = None
@@ -182,12 +191,15 @@ def gmres_params(max_iterations=1000, tolerance=1e-5, restart=0, ref_mgs=True,
 
 
 def solver_cfg(solver=SOLVER_DIRECT, gmres: GmresParams | None = None, ir_tolerance=1e-12,
-               ir_max_steps=10) -> SolverCfg:
+               ir_max_steps=10, leaf_size=32, aca_tolerance=1e-4, eta=4.0) -> SolverCfg:
     c = SolverCfg()
     c.solver = int(solver)
     c.gmres = gmres if gmres is not None else gmres_params()
     c.ir_tolerance = float(ir_tolerance)
     c.ir_max_steps = int(ir_max_steps)
+    c.leaf_size = int(leaf_size)
+    c.aca_tolerance = float(aca_tolerance)
+    c.eta = float(eta)
     return c
 
 

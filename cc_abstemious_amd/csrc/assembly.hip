@@ -14,71 +14,12 @@
 // 128-byte lines.  The kernel is HBM-store bound: 8 B (c64) or 16 B (c128)
 // per stored entry and matrix.
 #include "ccab_internal.h"
+#include "greens.h"
 
 namespace {
 
 constexpr int kRows = 256;  // rows per workgroup (4 wavefronts)
 constexpr int kColChunk = 64;
-
-struct Soa {
-    const double *x, *y, *z, *nx, *ny, *nz, *area, *wt;
-};
-
-__device__ __forceinline__ Soa soa_of(const double* base, int64_t n) {
-    return Soa{base, base + n, base + 2 * n, base + 3 * n,
-               base + 4 * n, base + 5 * n, base + 6 * n, base + 7 * n};
-}
-
-__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
-__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
-__device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
-__device__ __forceinline__ float sqrt_t(float a) { return sqrtf(a); }
-
-// Off-diagonal kernel pair (g, h) for source point j (x, n_x) and collocation
-// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.
-template <typename T, bool BM>
-__device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx, T nxy, T nxz,
-                                       T nyx, T nyy, T nyz, Cx<T>& g, Cx<T>& h) {
-    const T rd = sqrt_t(rx * rx + ry * ry + rz * rz);
-    const T inv_r = T(1) / rd;
-    const T erx = rx * inv_r, ery = ry * inv_r, erz = rz * inv_r;
-    T s, c;
-    sincos_t(k * rd, &s, &c);
-    const T gs = inv_r * T(1.0 / (4.0 * kPi));
-    g = Cx<T>{c * gs, s * gs};
-    const Cx<T> f1{-inv_r, k};
-    const T rdoty = erx * nyx + ery * nyy + erz * nyz;
-    const Cx<T> gf1 = g * f1;
-    h = gf1 * (-rdoty);
-    if (BM) {
-        const T rdotx = erx * nxx + ery * nxy + erz * nxz;
-        const Cx<T> dg = gf1 * rdotx;
-        const T nn = nyx * nxx + nyy * nxy + nyz * nxz;
-        const T rr = rdotx * rdoty;
-        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)
-        const Cx<T> term{-f1.re * nn - (k * k * rd + T(3) * f1.re) * rr,
-                         -f1.im * nn - T(3) * f1.im * rr};
-        const Cx<T> dh = (g * inv_r) * term;
-        g += beta * dg;
-        h += beta * dh;
-    }
-}
-
-// Singular diagonal (influence_matrix.rs:42-50) incl. the global shifts of
-// get_dense_surface_matrices (influence_matrix.rs:78-79).  Always FP64.
-__device__ __forceinline__ void diag_entry(const SysParams& p, double area, double wt, c128& g,
-                                           c128& h) {
-    const double b = sqrt(area / kPi);
-    const double wa = wt * area;
-    g = c128{1.0 / (2.0 * kPi * b) * wa, 0.0};
-    h = c128{0.0, 0.0};
-    if (p.bm) {
-        const double s = -1.0 / (2.0 * kPi * (b * b * b));
-        h = c128{p.beta.re * s * wa, p.beta.im * s * wa};
-    }
-    g += p.gdiag;
-    h += p.hdiag;
-}
 
 template <typename T>
 __device__ __forceinline__ void store(void* base, int64_t off, c128 v) {

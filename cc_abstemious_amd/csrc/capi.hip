@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 #include "ccab_internal.h"
 
@@ -174,6 +175,76 @@ __global__ void scale_copy_kernel(int64_t n, const SysParams* __restrict__ param
 __global__ void copy_vec_kernel(int64_t n, const ccab_c128* __restrict__ src, ccab_c128* dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
+}
+
+__global__ void negate_kernel(int64_t n, ccab_c128* v) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = ccab_c128{-v[i].re, -v[i].im};
+}
+
+// host copy of the collocation points (the cluster tree is built on the host)
+int host_cpts(ccab_ctx* ctx, const double* d_cpts, int64_t n, std::vector<double>& h) {
+    h.resize(size_t(n) * 8);
+    CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts, size_t(n) * 64, hipMemcpyDeviceToHost, ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    return CCAB_OK;
+}
+
+struct HDel {
+    void operator()(ccab_hmatrix* h) const { hmatrix_free(h); }
+};
+using HPtr = std::unique_ptr<ccab_hmatrix, HDel>;
+
+// get_surface_hmatrix (solve.rs:74-150) for every frequency of a sweep; rhs
+// holds rhs_inc per frequency on entry and the solved unknown on exit.
+int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
+                       const double* soa, const double* d_cpts, int64_t n, int64_t nfreq,
+                       const ccab_solver_cfg* cfg, c128* rhs, const ccab_c128* d_x0,
+                       ccab_solve_stats* stats) {
+    std::vector<double> h;
+    CCAB_TRY(host_cpts(ctx, d_cpts, n, h));
+    const int64_t leaf = cfg->leaf_size > 0 ? cfg->leaf_size : 32;
+    const double tol = cfg->aca_tolerance > 0.0 ? cfg->aca_tolerance : 1e-4;
+    const double eta = cfg->eta > 0.0 ? cfg->eta : 4.0;
+    c128* tmp = static_cast<c128*>(ctx_buf(ctx, SLOT_HMAT_X, size_t(n) * 16));
+    if (!tmp) return CCAB_ENOMEM;
+    const dim3 g((unsigned)((n + 255) / 256));
+    const c128 one{1.0, 0.0}, mone{-1.0, 0.0};
+    for (int64_t f = 0; f < nfreq; ++f) {
+        const SysParams& p = hp[f];
+        c128* x = rhs + f * n;
+        ccab_c128* xc = reinterpret_cast<ccab_c128*>(x);
+        const bool zero_bc = p.bcv.re == 0.0 && p.bcv.im == 0.0;
+        if (bc == CCAB_BC_IMPEDANCE) {
+            hipLaunchKernelGGL(negate_kernel, g, dim3(256), 0, ctx->stream, n, xc);
+        } else if (zero_bc) {
+            // the empty H matrix still scales by beta (h_matrix.rs:219-223)
+            if (bc == CCAB_BC_NORMAL_VELOCITY)
+                hipLaunchKernelGGL(negate_kernel, g, dim3(256), 0, ctx->stream, n, xc);
+        } else {
+            ccab_hmatrix* hr_raw = nullptr;
+            CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_RHS, leaf, eta, tol, &hr_raw));
+            HPtr hr(hr_raw);
+            hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, p.bcv,
+                               reinterpret_cast<ccab_c128*>(tmp), 0);
+            // Pressure: rhs = H phi + rhs_inc; NormalVelocity: rhs = G vn - rhs_inc
+            CCAB_TRY(hmatrix_gemv(ctx, hr.get(), one, tmp,
+                                  bc == CCAB_BC_PRESSURE ? one : mone, x));
+            CCAB_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        ccab_hmatrix* hl_raw = nullptr;
+        CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_LHS, leaf, eta, tol, &hl_raw));
+        HPtr hl(hl_raw);
+        MatvecFn op = [&](const c128* xin, c128 a, c128 b, c128* yout) -> int {
+            return hmatrix_gemv(ctx, hl.get(), a, xin, b, yout);
+        };
+        ccab_solve_stats st{};
+        const c128* x0 = d_x0 ? reinterpret_cast<const c128*>(d_x0 + f * n) : nullptr;
+        CCAB_TRY(gmres_op_run(ctx, n, op, hmatrix_norm(hl.get()), x, x0, cfg->gmres, &st));
+        CCAB_HIP(hipStreamSynchronize(ctx->stream));
+        if (stats) stats[f] = st;
+    }
+    return CCAB_OK;
 }
 
 // SoA copy of the collocation points (8 n doubles) for the current call
@@ -523,6 +594,12 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
             CCAB_TRY(launch_incident(ctx, &hp[f], ph[f], *w, soa, n, rhs_c + f * n));
     }
     const int solver = cfg->solver;
+    if (solver == CCAB_SOLVER_HIERARCHICAL) {
+        CCAB_TRY(hierarchical_sweep(ctx, hp, bc, soa, d_cpts, n, nfreq, cfg, rhs, d_x0, stats));
+        CCAB_TRY(finish_surface(ctx, hp.data(), dp, bc, n, nfreq, rhs_c, d_phi, d_vn));
+        CCAB_HIP(hipGetLastError());
+        return CCAB_OK;
+    }
     const size_t n2 = size_t(n) * n;
     // process frequencies in chunks that fit a memory budget
     size_t per = 0;
@@ -622,8 +699,9 @@ extern "C" int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nf
                                ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats) {
     if (!ctx) return CCAB_EINVAL;
     CCAB_CHECK(w != nullptr && d_x0 != nullptr, "null wave or initial guess");
-    CCAB_CHECK(cfg && cfg->solver == CCAB_SOLVER_ITERATIVE_MIXED,
-               "a warm start needs the ITERATIVE_MIXED solver");
+    CCAB_CHECK(cfg && (cfg->solver == CCAB_SOLVER_ITERATIVE_MIXED ||
+                       cfg->solver == CCAB_SOLVER_HIERARCHICAL),
+               "a warm start needs the ITERATIVE_MIXED or HIERARCHICAL solver");
     return sweep_impl(ctx, ph, nfreq, w, nullptr, d_cpts, n, cfg, d_phi, d_vn, stats, d_x0);
 }
 
@@ -657,4 +735,67 @@ extern "C" int ccab_cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, 
     CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "bad precision");
     CCAB_CHECK(lda >= M && ldb >= K && ldc >= M, "bad leading dimensions");
     return cgemm_update(ctx, prec, M, N, K, d_A, lda, d_B, ldb, d_C, ldc);
+}
+
+// ---------------------------------------------------------------------------
+// hierarchical matrix
+// ---------------------------------------------------------------------------
+extern "C" int ccab_hmatrix_build(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                                  int64_t n, int side, int64_t leaf_size, double eta,
+                                  double aca_tolerance, ccab_hmatrix** out) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(ph && d_cpts && out && n > 0, "invalid arguments");
+    *out = nullptr;
+    CCAB_CHECK(side == CCAB_LHS || side == CCAB_RHS, "bad side %d", side);
+    CCAB_CHECK(leaf_size >= 1, "leaf_size must be positive");
+    CCAB_CHECK(eta > 0.0 && aca_tolerance > 0.0, "eta and aca_tolerance must be positive");
+    std::vector<SysParams> hp;
+    SysParams* dp;
+    CCAB_TRY(stage_params(ctx, ph, 1, hp, &dp));
+    double* soa;
+    CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
+    std::vector<double> h;
+    CCAB_TRY(host_cpts(ctx, d_cpts, n, h));
+    return hmatrix_build(ctx, hp[0], soa, h.data(), n, side, leaf_size, eta, aca_tolerance, out);
+}
+
+extern "C" int ccab_hmatrix_destroy(ccab_hmatrix* h) {
+    if (h) hmatrix_free(h);
+    return CCAB_OK;
+}
+
+extern "C" int ccab_hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* h, const double* alpha,
+                                 const ccab_c128* d_x, const double* beta, ccab_c128* d_y) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(h && alpha && beta && d_x && d_y, "invalid arguments");
+    CCAB_CHECK(hmatrix_device(h) == ctx->device, "H matrix lives on another device");
+    return hmatrix_gemv(ctx, h, c128{alpha[0], alpha[1]}, reinterpret_cast<const c128*>(d_x),
+                        c128{beta[0], beta[1]}, reinterpret_cast<c128*>(d_y));
+}
+
+extern "C" int ccab_hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_A,
+                                    int64_t lda) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(h && d_A && lda >= hmatrix_n(h), "invalid arguments");
+    CCAB_CHECK(hmatrix_device(h) == ctx->device, "H matrix lives on another device");
+    return hmatrix_to_full(ctx, h, reinterpret_cast<c128*>(d_A), lda);
+}
+
+extern "C" int ccab_hmatrix_stats(const ccab_hmatrix* h, int64_t* st, double* norm) {
+    if (!h || !st) return CCAB_EINVAL;
+    hmatrix_stats(h, st, norm);
+    return CCAB_OK;
+}
+
+extern "C" int ccab_gmres_hmatrix(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_x,
+                                  const ccab_c128* d_x0, const ccab_gmres_params* p,
+                                  ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(h && d_x && p, "invalid arguments");
+    CCAB_CHECK(hmatrix_device(h) == ctx->device, "H matrix lives on another device");
+    MatvecFn op = [&](const c128* xin, c128 a, c128 b, c128* yout) -> int {
+        return hmatrix_gemv(ctx, h, a, xin, b, yout);
+    };
+    return gmres_op_run(ctx, hmatrix_n(h), op, hmatrix_norm(h), reinterpret_cast<c128*>(d_x),
+                        reinterpret_cast<const c128*>(d_x0), *p, stats);
 }

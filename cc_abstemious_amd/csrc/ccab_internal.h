@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -184,7 +185,7 @@ enum BufSlot {
     SLOT_LU_INFO, SLOT_LU_Y, SLOT_GEMV_PART, SLOT_NORM, SLOT_MIXED_R, SLOT_MIXED_D,
     SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
-    SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_COUNT
+    SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_COUNT
 };
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
 inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
@@ -261,3 +262,21 @@ int nrm2_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* d_v, int64
 int frob_norm(ccab_ctx* ctx, int prec, int64_t n, const void* d_A, int64_t lda, double* d_out);
 int convert_c128_to_c64(ccab_ctx* ctx, int64_t count, const c128* src, c64* dst);
 int convert_c64_to_c128(ccab_ctx* ctx, int64_t count, const c64* src, c128* dst);
+
+// solver.hip
+// y = alpha A x + beta y for the operator GMRES iterates on
+using MatvecFn = std::function<int(const c128* x, c128 alpha, c128 beta, c128* y)>;
+int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, c128* d_x,
+                 const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st);
+
+// hmatrix.hip
+int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
+                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out);
+int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
+                 c128* y);
+int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda);
+void hmatrix_stats(const ccab_hmatrix* H, int64_t* st, double* norm);
+void hmatrix_free(ccab_hmatrix* H);
+int64_t hmatrix_n(const ccab_hmatrix* H);
+int hmatrix_device(const ccab_hmatrix* H);
+double hmatrix_norm(const ccab_hmatrix* H);

@@ -1,0 +1,1015 @@
+// Hierarchical matrix of the BEM operator on gfx950: cluster / block trees
+// (cluster.rs, block.rs; host C++), dense near-field blocks and batched ACA
+// low-rank blocks (aca.rs; one workgroup per admissible block), recompression
+// (QR + SVD per block, aca.rs:137-179), the H matrix-vector product
+// (h_matrix.rs:218-234) and GMRES over it (solve.rs:74-150).
+//
+// Block storage in HBM: row / column index lists back to back; dense blocks
+// column-major back to back (c128); low-rank blocks as U (nr x r) and V
+// (nc x r) with A ~ U V^T (non-conjugated, as the reference's ACA::gemv).
+// The matvec writes every block's contribution to a per-(block,row) slot and
+// a row-wise CSR reduction adds them in a fixed order (no atomics).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "ccab_internal.h"
+#include "greens.h"
+
+namespace {
+
+constexpr int RMAX = 64;        // rank capacity of a low-rank block (ACA terms before recompression)
+constexpr int kAcaThreads = 256;
+constexpr int kRngTable = 1024; // ChaCha12 words available to one block's draws
+
+struct HBlkDev {
+    int64_t r_off, nr, c_off, nc;  // into rows[] / cols[]
+    int64_t u_off, v_off;          // into U, V (capacity nr*RMAX, nc*RMAX)
+    int64_t d_off;                 // into D (dense blocks)
+    int64_t p_off;                 // into the matvec partial buffer
+    int64_t w_off;                 // ACA workspace (vectors), in c128 units
+    int64_t f_off;                 // ACA exclusion flags, in bytes
+    int32_t adm, rank;
+    int32_t status;                // ACA: 0 ok, 1 rank capacity reached, 2 rng exhausted
+    int32_t pad_;
+    double norm;                   // Frobenius norm estimate / dense Frobenius norm
+};
+
+// ---------------------------------------------------------------------------
+// rand 0.9 StdRng words (ChaCha12, seed_from_u64 via PCG32), host side: the
+// reference seeds every ACA block with 10 (aca.rs:64), so all blocks consume
+// the same word stream; it is tabulated once and read on the device.
+// ---------------------------------------------------------------------------
+inline uint32_t rotl32(uint32_t v, int c) { return (v << c) | (v >> (32 - c)); }
+
+void chacha12_words(uint64_t seed, uint32_t* out, int nwords) {
+    uint32_t key[8];
+    uint64_t st = seed;
+    for (int c = 0; c < 8; ++c) {
+        st = st * 6364136223846793005ull + 11634580027462260723ull;
+        const uint32_t xs = uint32_t(((st >> 18) ^ st) >> 27);
+        const uint32_t rot = uint32_t(st >> 59);
+        key[c] = (xs >> rot) | (xs << ((32 - rot) & 31));
+    }
+    for (int blk = 0; blk * 16 < nwords; ++blk) {
+        uint32_t s0[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                           key[0], key[1], key[2], key[3], key[4], key[5], key[6], key[7],
+                           uint32_t(blk), 0u, 0u, 0u};
+        uint32_t x[16];
+        std::memcpy(x, s0, sizeof(x));
+        auto qr = [&](int a, int b, int c, int d) {
+            x[a] += x[b]; x[d] ^= x[a]; x[d] = rotl32(x[d], 16);
+            x[c] += x[d]; x[b] ^= x[c]; x[b] = rotl32(x[b], 12);
+            x[a] += x[b]; x[d] ^= x[a]; x[d] = rotl32(x[d], 8);
+            x[c] += x[d]; x[b] ^= x[c]; x[b] = rotl32(x[b], 7);
+        };
+        for (int r = 0; r < 12; r += 2) {
+            qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+            qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+        }
+        for (int i = 0; i < 16 && blk * 16 + i < nwords; ++i) out[blk * 16 + i] = x[i] + s0[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host cluster tree (cluster.rs:35-110) and block tree (block.rs:38-96)
+// ---------------------------------------------------------------------------
+struct HNode {
+    double lo[3], hi[3], diam;
+    std::vector<int64_t> idx;
+    int son[2] = {-1, -1};
+};
+
+int build_cluster(std::vector<HNode>& T, const double* xyz, int64_t stride,
+                  std::vector<int64_t>&& idx, int64_t leaf) {
+    const int id = (int)T.size();
+    T.emplace_back();
+    {
+        HNode& c = T.back();
+        c.idx = std::move(idx);
+        for (int d = 0; d < 3; ++d) {
+            c.lo[d] = INFINITY;
+            c.hi[d] = -INFINITY;
+        }
+        for (int64_t t : c.idx)
+            for (int d = 0; d < 3; ++d) {
+                c.lo[d] = std::fmin(xyz[stride * t + d], c.lo[d]);
+                c.hi[d] = std::fmax(xyz[stride * t + d], c.hi[d]);
+            }
+        double s = 0.0;
+        for (int d = 0; d < 3; ++d) s += (c.hi[d] - c.lo[d]) * (c.hi[d] - c.lo[d]);
+        c.diam = std::sqrt(s);
+    }
+    if ((int64_t)T[id].idx.size() <= leaf) return id;
+    int jmax = 0;
+    double diff = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double e = T[id].hi[d] - T[id].lo[d];
+        if (e > diff) {
+            jmax = d;
+            diff = e;
+        }
+    }
+    const double gamma = 0.5 * (T[id].lo[jmax] + T[id].hi[jmax]);
+    std::vector<int64_t> a, b;
+    for (int64_t t : T[id].idx) (xyz[stride * t + jmax] <= gamma ? a : b).push_back(t);
+    if (a.empty() || b.empty()) return id;  // coincident points cannot be split
+    const int s0 = build_cluster(T, xyz, stride, std::move(a), leaf);
+    const int s1 = build_cluster(T, xyz, stride, std::move(b), leaf);
+    T[id].son[0] = s0;
+    T[id].son[1] = s1;
+    return id;
+}
+
+double cluster_dist(const HNode& a, const HNode& b) {
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double t1 = std::fmax(0.0, a.lo[d] - b.hi[d]);
+        const double t2 = std::fmax(0.0, b.lo[d] - a.hi[d]);
+        s += t1 * t1;
+        s += t2 * t2;
+    }
+    return std::sqrt(s);
+}
+
+struct HostBlock {
+    int r, c;
+    bool adm;
+};
+
+void build_blocks(const std::vector<HNode>& T, int r, int c, double eta,
+                  std::vector<HostBlock>& out) {
+    const bool adm = std::fmin(T[r].diam, T[c].diam) <= eta * cluster_dist(T[r], T[c]);
+    const bool rl = T[r].son[0] < 0, cl = T[c].son[0] < 0;
+    if (!(adm || rl || cl)) {
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) build_blocks(T, T[r].son[a], T[c].son[b], eta, out);
+    } else {
+        out.push_back({r, c, adm});
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double cabs(c128 z) { return hypot(z.re, z.im); }
+
+// dense near-field blocks: D[i + j nr] = entry(rows[i], cols[j])
+__global__ void dense_block_kernel(SysParams p, const double* __restrict__ soa_base, int64_t n,
+                                   const HBlkDev* __restrict__ blks,
+                                   const int32_t* __restrict__ list, const int64_t* __restrict__ rows,
+                                   const int64_t* __restrict__ cols, c128 fa, c128 fb,
+                                   c128* __restrict__ D) {
+    const HBlkDev bk = blks[list[blockIdx.x]];
+    const Soa s = soa_of(soa_base, n);
+    const int64_t cnt = bk.nr * bk.nc;
+    for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cnt;
+         e += int64_t(blockDim.x) * gridDim.y) {
+        const int64_t i = e % bk.nr, j = e / bk.nr;
+        D[bk.d_off + e] = side_entry(p, s, rows[bk.r_off + i], cols[bk.c_off + j], fa, fb);
+    }
+}
+
+// Frobenius norm of each dense block (h_matrix.rs:189-196 sums them)
+__global__ void dense_norm_kernel(HBlkDev* blks, const int32_t* __restrict__ list,
+                                  const c128* __restrict__ D) {
+    __shared__ double red[4];
+    HBlkDev& bk = blks[list[blockIdx.x]];
+    const int64_t cnt = bk.nr * bk.nc;
+    double s = 0.0;
+    for (int64_t e = threadIdx.x; e < cnt; e += 256) s += norm_sqr(D[bk.d_off + e]);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) bk.norm = sqrt(red[0] + red[1] + red[2] + red[3]);
+}
+
+// --- ACA (aca.rs:57-136), one workgroup of kAcaThreads per admissible block ---
+struct AcaCtx {
+    SysParams p;
+    Soa s;
+    const int64_t* rows;
+    const int64_t* cols;
+    int64_t nr, nc;
+    c128* U;  // nr x RMAX
+    c128* V;  // nc x RMAX
+    c128 fa, fb;
+};
+
+// r[j] = A(rows[i], cols[j]) - sum_k U[i,k] V[j,k]   (residual row)
+__device__ void aca_row(const AcaCtx& a, int64_t i, int rank, c128* r) {
+    const int64_t gi = a.rows[i];
+    for (int64_t j = threadIdx.x; j < a.nc; j += kAcaThreads) {
+        c128 v = side_entry(a.p, a.s, gi, a.cols[j], a.fa, a.fb);
+        for (int k = 0; k < rank; ++k) v -= a.U[i + k * a.nr] * a.V[j + k * a.nc];
+        r[j] = v;
+    }
+}
+
+// r[i] = A(rows[i], cols[j]) - sum_k V[j,k] U[i,k]   (residual column)
+__device__ void aca_col(const AcaCtx& a, int64_t j, int rank, c128* r) {
+    const int64_t gj = a.cols[j];
+    for (int64_t i = threadIdx.x; i < a.nr; i += kAcaThreads) {
+        c128 v = side_entry(a.p, a.s, a.rows[i], gj, a.fa, a.fb);
+        for (int k = 0; k < rank; ++k) v -= a.V[j + k * a.nc] * a.U[i + k * a.nr];
+        r[i] = v;
+    }
+}
+
+// index of the max |r| outside the excluded set, first index on ties
+// (aca.rs:256-268); len if all excluded
+__device__ int64_t argmax_excl(const c128* r, int64_t len, const uint8_t* excl, double* red_v,
+                               int64_t* red_i) {
+    double best = -1.0;
+    int64_t bi = len;
+    for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) {
+        if (excl[t]) continue;
+        const double v = cabs(r[t]);
+        if (best < v) {
+            best = v;
+            bi = t;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off, 64);
+        const int64_t oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) {
+            best = ov;
+            bi = oi;
+        }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red_v[threadIdx.x >> 6] = best;
+        red_i[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    double bv = red_v[0];
+    int64_t bx = red_i[0];
+    for (int w = 1; w < kAcaThreads / 64; ++w)
+        if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bx)) {
+            bv = red_v[w];
+            bx = red_i[w];
+        }
+    return bx;
+}
+
+// sum over the block of f(t) for t < len (complex, all threads get the sum)
+template <typename F>
+__device__ c128 block_csum(int64_t len, F f, double* red) {
+    double re = 0.0, im = 0.0;
+    for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) {
+        const c128 v = f(t);
+        re += v.re;
+        im += v.im;
+    }
+    re = wave_sum(re);
+    im = wave_sum(im);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[2 * (threadIdx.x >> 6)] = re;
+        red[2 * (threadIdx.x >> 6) + 1] = im;
+    }
+    __syncthreads();
+    c128 s{0.0, 0.0};
+    for (int w = 0; w < kAcaThreads / 64; ++w) s += c128{red[2 * w], red[2 * w + 1]};
+    return s;
+}
+
+// Uniform::new_inclusive(0, hi).sample(rng) for usize in 32-bit mode
+// (rand 0.9 UniformUsize): Lemire's widening multiply with rejection.
+__device__ int64_t rng_uniform(const uint32_t* words, int* pos, uint32_t range, bool* ok) {
+    const uint32_t thresh = (uint32_t)(0u - range) % range;
+    while (*pos < kRngTable) {
+        const uint64_t w = uint64_t(words[(*pos)++]) * range;
+        if (uint32_t(w) >= thresh) return int64_t(w >> 32);
+    }
+    *ok = false;
+    return 0;
+}
+
+__global__ __launch_bounds__(kAcaThreads) void aca_kernel(SysParams p, const double* __restrict__ soa_base,
+                                                          int64_t n, HBlkDev* blks,
+                                                          const int32_t* __restrict__ list,
+                                                          const int64_t* __restrict__ rows,
+                                                          const int64_t* __restrict__ cols,
+                                                          c128 fa, c128 fb, c128* U, c128* V,
+                                                          c128* W, uint8_t* F,
+                                                          const uint32_t* __restrict__ rng,
+                                                          double tol) {
+    __shared__ double red_v[kAcaThreads / 64];
+    __shared__ int64_t red_i[kAcaThreads / 64];
+    __shared__ double red[2 * kAcaThreads / 64];
+    __shared__ double dots[RMAX][4][kAcaThreads / 64];
+    __shared__ int64_t s_ref[2];
+    __shared__ int s_status;
+    HBlkDev& bk = blks[list[blockIdx.x]];
+    AcaCtx a{p, soa_of(soa_base, n), rows + bk.r_off, cols + bk.c_off, bk.nr, bk.nc,
+             U + bk.u_off, V + bk.v_off, fa, fb};
+    const int64_t nr = bk.nr, nc = bk.nc;
+    c128* riref = W + bk.w_off;       // nc
+    c128* rjref = riref + nc;         // nr
+    c128* ristar = rjref + nr;        // nc
+    c128* rjstar = ristar + nc;       // nr
+    uint8_t* exI = F + bk.f_off;      // nr
+    uint8_t* exJ = exI + nr;          // nc
+    for (int64_t t = threadIdx.x; t < nr + nc; t += kAcaThreads) exI[t] = 0;
+    const int64_t nmin = nr < nc ? nr : nc;
+    if (threadIdx.x == 0) {
+        int pos = 0;
+        bool ok = true;
+        s_ref[0] = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+        s_ref[1] = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+        s_status = pos | (ok ? 0 : (1 << 30));
+    }
+    __syncthreads();
+    int64_t iref = s_ref[0], jref = s_ref[1];
+    int pos = s_status & 0xffff;
+    int status = (s_status >> 30) ? 2 : 0;
+    aca_row(a, iref, 0, riref);
+    aca_col(a, jref, 0, rjref);
+    __syncthreads();
+    double total = 0.0;
+    int rank = 0;
+    for (int64_t k = 0; k < nmin; ++k) {
+        if (rank == RMAX) {
+            status = 1;
+            break;
+        }
+        // pivots (aca.rs:83-101); an index equal to the length means every
+        // candidate was NaN, which stops the block (stored dense)
+        int64_t jstar = argmax_excl(riref, nc, exJ, red_v, red_i);
+        int64_t istar = argmax_excl(rjref, nr, exI, red_v, red_i);
+        if (jstar >= nc || istar >= nr) {
+            status = 3;
+            break;
+        }
+        if (cabs(rjref[istar]) > cabs(riref[jstar])) {
+            aca_row(a, istar, rank, ristar);
+            __syncthreads();
+            jstar = argmax_excl(ristar, nc, exJ, red_v, red_i);
+            if (jstar >= nc) {
+                status = 3;
+                break;
+            }
+            aca_col(a, jstar, rank, rjstar);
+        } else {
+            aca_col(a, jstar, rank, rjstar);
+            __syncthreads();
+            istar = argmax_excl(rjstar, nr, exI, red_v, red_i);
+            if (istar >= nr) {
+                status = 3;
+                break;
+            }
+            aca_row(a, istar, rank, ristar);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            exI[istar] = 1;
+            exJ[jstar] = 1;
+        }
+        // u = rjstar, v = ristar / ristar[jstar]
+        const c128 factor = crecip(ristar[jstar]);
+        for (int64_t t = threadIdx.x; t < nc; t += kAcaThreads) a.V[t + rank * nc] = ristar[t] * factor;
+        for (int64_t t = threadIdx.x; t < nr; t += kAcaThreads) a.U[t + rank * nr] = rjstar[t];
+        __syncthreads();
+        // norm estimate (aca.rs:224-241): |u|^2 |v|^2 + sum_i 2 |u_i.u| |v_i.v|
+        const c128* uk = a.U + rank * nr;
+        const c128* vk = a.V + rank * nc;
+        const c128 un = block_csum(nr, [&](int64_t t) { return c128{norm_sqr(uk[t]), 0.0}; }, red);
+        const c128 vn = block_csum(nc, [&](int64_t t) { return c128{norm_sqr(vk[t]), 0.0}; }, red);
+        const double nk2 = un.re * vn.re;
+        total += nk2;
+        // cross terms 2 |u_i . u_k| |v_i . v_k| for the previous terms: wave
+        // partials of all pairs, one barrier, the same ordered sum in every thread
+        for (int i = 0; i < rank; ++i) {
+            double ur = 0, ui = 0, vr = 0, vi = 0;
+            const c128* u0 = a.U + i * nr;
+            const c128* v0 = a.V + i * nc;
+            for (int64_t t = threadIdx.x; t < nr; t += kAcaThreads) {
+                const c128 z = u0[t] * uk[t];
+                ur += z.re;
+                ui += z.im;
+            }
+            for (int64_t t = threadIdx.x; t < nc; t += kAcaThreads) {
+                const c128 z = v0[t] * vk[t];
+                vr += z.re;
+                vi += z.im;
+            }
+            ur = wave_sum(ur);
+            ui = wave_sum(ui);
+            vr = wave_sum(vr);
+            vi = wave_sum(vi);
+            if ((threadIdx.x & 63) == 0) {
+                const int w = threadIdx.x >> 6;
+                dots[i][0][w] = ur;
+                dots[i][1][w] = ui;
+                dots[i][2][w] = vr;
+                dots[i][3][w] = vi;
+            }
+        }
+        __syncthreads();
+        for (int i = 0; i < rank; ++i) {
+            double d[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < 4; ++q)
+                for (int w = 0; w < kAcaThreads / 64; ++w) d[q] += dots[i][q][w];
+            total += 2.0 * hypot(d[0], d[1]) * hypot(d[2], d[3]);
+        }
+        ++rank;
+        const double step = sqrt(nk2) / sqrt(total);
+        if (step < tol / 50.0) break;
+        // next reference row / column (aca.rs:116-132)
+        if (iref == istar) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                bool ok = true;
+                int64_t ii = iref;
+                if (exI[ii]) {
+                    ii = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+                    for (int64_t t = 0; t < nmin - 1; ++t) {
+                        ii = (ii + 1) % nmin;
+                        if (!exI[ii]) break;
+                    }
+                }
+                s_ref[0] = ii;
+                s_status = pos | (ok ? 0 : (1 << 30));
+            }
+            __syncthreads();
+            iref = s_ref[0];
+            pos = s_status & 0xffff;
+            if (s_status >> 30) status = 2;
+            aca_row(a, iref, rank, riref);
+        } else {
+            const c128 ui = a.U[iref + (rank - 1) * nr];
+            for (int64_t t = threadIdx.x; t < nc; t += kAcaThreads) riref[t] -= ui * a.V[t + (rank - 1) * nc];
+        }
+        if (jref == jstar) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                bool ok = true;
+                int64_t jj = jref;
+                if (exJ[jj]) {
+                    jj = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+                    for (int64_t t = 0; t < nmin - 1; ++t) {
+                        jj = (jj + 1) % nmin;
+                        if (!exJ[jj]) break;
+                    }
+                }
+                s_ref[1] = jj;
+                s_status = pos | (ok ? 0 : (1 << 30));
+            }
+            __syncthreads();
+            jref = s_ref[1];
+            pos = s_status & 0xffff;
+            if (s_status >> 30) status = 2;
+            aca_col(a, jref, rank, rjref);
+        } else {
+            const c128 vj = a.V[jref + (rank - 1) * nc];
+            for (int64_t t = threadIdx.x; t < nr; t += kAcaThreads) rjref[t] -= vj * a.U[t + (rank - 1) * nr];
+        }
+        __syncthreads();
+        if (status) break;
+    }
+    if (threadIdx.x == 0) {
+        bk.rank = rank;
+        bk.status = status;
+        bk.norm = sqrt(total);
+    }
+}
+
+// --- recompression (aca.rs:137-179): QR(U), QR(V) by CGS2, SVD of R_u R_v^T
+//     by one-sided Jacobi (one wavefront, LDS), truncation, new U and V -------
+__global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
+                                                                 const int32_t* __restrict__ list,
+                                                                 c128* U, c128* V, c128* U2,
+                                                                 c128* V2, c128* Rws, double tol) {
+    __shared__ c128 Wm[RMAX][RMAX + 1];  // [row][col]
+    __shared__ c128 Vs[RMAX][RMAX + 1];
+    __shared__ double red[2 * kAcaThreads / 64];
+    __shared__ double sv[RMAX];
+    __shared__ int ord[RMAX];
+    __shared__ double s_tmp[RMAX][2][kAcaThreads / 64];
+    HBlkDev& bk = blks[list[blockIdx.x]];
+    if (bk.status != 0) return;
+    const int r = bk.rank;
+    const int64_t nr = bk.nr, nc = bk.nc;
+    c128* Ru = Rws + int64_t(blockIdx.x) * 2 * RMAX * RMAX;  // [c][i] = R[i][c]
+    c128* Rv = Ru + RMAX * RMAX;
+    for (int side = 0; side < 2; ++side) {
+        c128* Q = side == 0 ? U + bk.u_off : V + bk.v_off;
+        const int64_t len = side == 0 ? nr : nc;
+        c128* R = side == 0 ? Ru : Rv;
+        for (int e = threadIdx.x; e < RMAX * RMAX; e += kAcaThreads) R[e] = c128{0.0, 0.0};
+        __syncthreads();
+        for (int c = 0; c < r; ++c) {
+            c128* qc = Q + c * len;
+            for (int pass = 0; pass < 2; ++pass) {
+                // h_i = q_i^H q_c for all i < c (wave partials, one barrier)
+                for (int i = 0; i < c; ++i) {
+                    const c128* qi = Q + i * len;
+                    double hr = 0, hi = 0;
+                    for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) {
+                        const c128 z = conj(qi[t]) * qc[t];
+                        hr += z.re;
+                        hi += z.im;
+                    }
+                    hr = wave_sum(hr);
+                    hi = wave_sum(hi);
+                    if ((threadIdx.x & 63) == 0) {
+                        s_tmp[i][0][threadIdx.x >> 6] = hr;
+                        s_tmp[i][1][threadIdx.x >> 6] = hi;
+                    }
+                }
+                __syncthreads();
+                for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) {
+                    c128 v = qc[t];
+                    for (int i = 0; i < c; ++i) {
+                        c128 h{0.0, 0.0};
+                        for (int w = 0; w < kAcaThreads / 64; ++w)
+                            h += c128{s_tmp[i][0][w], s_tmp[i][1][w]};
+                        v -= h * Q[i * len + t];
+                    }
+                    qc[t] = v;
+                }
+                if (threadIdx.x < c) {
+                    c128 h{0.0, 0.0};
+                    for (int w = 0; w < kAcaThreads / 64; ++w)
+                        h += c128{s_tmp[threadIdx.x][0][w], s_tmp[threadIdx.x][1][w]};
+                    R[c * RMAX + threadIdx.x] += h;
+                }
+                __syncthreads();
+            }
+            double nn = 0.0;
+            for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) nn += norm_sqr(qc[t]);
+            nn = wave_sum(nn);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nn;
+            __syncthreads();
+            const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
+            const double inv = rn > 1e-300 ? 1.0 / rn : 0.0;
+            for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) qc[t] = qc[t] * inv;
+            if (threadIdx.x == 0) R[c * RMAX + c] = c128{rn, 0.0};
+            __syncthreads();
+        }
+    }
+    // W = R_u R_v^T (r x r)
+    for (int e = threadIdx.x; e < r * r; e += kAcaThreads) {
+        const int i = e % r, c = e / r;
+        c128 sum{0.0, 0.0};
+        for (int t = 0; t < r; ++t) sum += Ru[t * RMAX + i] * Rv[t * RMAX + c];
+        Wm[i][c] = sum;
+        Vs[i][c] = c128{i == c ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    // one-sided Jacobi on the columns of Wm (wave 0, lane = row)
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        for (int sweep = 0; sweep < 60; ++sweep) {
+            double off = 0.0;
+            for (int p = 0; p < r - 1; ++p)
+                for (int q = p + 1; q < r; ++q) {
+                    const c128 wp = lane < r ? Wm[lane][p] : c128{0, 0};
+                    const c128 wq = lane < r ? Wm[lane][q] : c128{0, 0};
+                    const double app = wave_sum(norm_sqr(wp));
+                    const double aqq = wave_sum(norm_sqr(wq));
+                    const c128 z = conj(wp) * wq;
+                    const c128 apq{wave_sum(z.re), wave_sum(z.im)};
+                    const double mag = hypot(apq.re, apq.im);
+                    if (mag == 0.0 || mag <= 1e-15 * sqrt(app * aqq)) continue;
+                    off = fmax(off, mag / sqrt(app * aqq));
+                    const c128 ph{apq.re / mag, apq.im / mag};
+                    const double tau = (aqq - app) / (2.0 * mag);
+                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    if (lane < r) {
+                        Wm[lane][p] = wp * cs - (conj(ph) * wq) * sn;
+                        Wm[lane][q] = (ph * wp) * sn + wq * cs;
+                        const c128 vp = Vs[lane][p], vq = Vs[lane][q];
+                        Vs[lane][p] = vp * cs - (conj(ph) * vq) * sn;
+                        Vs[lane][q] = (ph * vp) * sn + vq * cs;
+                    }
+                }
+            if (off < 1e-15) break;
+        }
+        // singular values = column norms of the rotated W
+        if (lane < r) {
+            double sq = 0.0;
+            for (int i = 0; i < r; ++i) sq += norm_sqr(Wm[i][lane]);
+            sv[lane] = sqrt(sq);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // descending order, stable
+        for (int c = 0; c < r; ++c) ord[c] = c;
+        for (int x = 1; x < r; ++x) {
+            const int t = ord[x];
+            int y = x - 1;
+            while (y >= 0 && sv[ord[y]] < sv[t]) {
+                ord[y + 1] = ord[y];
+                --y;
+            }
+            ord[y + 1] = t;
+        }
+    }
+    __syncthreads();
+    // truncation: first sigma_i < tol * sum_{<=i} sigma, keep >= 1
+    int index = 0;
+    {
+        double sum = 0.0;
+        for (int t = 0; t < r; ++t) {
+            sum += sv[ord[t]];
+            if (sv[ord[t]] < tol * sum) break;
+            ++index;
+        }
+        index = index + 1 < r ? index + 1 : r;
+        if (index < 1) index = 1;
+    }
+    // with W Y = X Sigma (Y = the accumulated rotations): U2[:, t] = Q_u (W Y)[:, o_t]
+    // = sigma Q_u x, V2[:, t] = Q_v conj(Y[:, o_t]), so U2 V2^T = Q_u W Q_v^T truncated
+    const c128* Qu = U + bk.u_off;
+    const c128* Qv = V + bk.v_off;
+    for (int t = 0; t < index; ++t) {
+        const int o = ord[t];
+        for (int64_t i = threadIdx.x; i < nr; i += kAcaThreads) {
+            c128 acc{0.0, 0.0};
+            for (int c = 0; c < r; ++c) acc += Qu[i + c * nr] * Wm[c][o];
+            U2[bk.u_off + i + t * nr] = acc;
+        }
+        for (int64_t j = threadIdx.x; j < nc; j += kAcaThreads) {
+            c128 acc{0.0, 0.0};
+            for (int c = 0; c < r; ++c) acc += Qv[j + c * nc] * conj(Vs[c][o]);
+            V2[bk.v_off + j + t * nc] = acc;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) bk.rank = index;
+}
+
+// norm estimate of the recompressed factors ab initio (aca.rs:212-222)
+__global__ __launch_bounds__(kAcaThreads) void aca_norm_kernel(HBlkDev* blks,
+                                                               const int32_t* __restrict__ list,
+                                                               const c128* __restrict__ U,
+                                                               const c128* __restrict__ V) {
+    __shared__ double red[2 * kAcaThreads / 64];
+    HBlkDev& bk = blks[list[blockIdx.x]];
+    if (bk.status != 0) return;  // stored dense instead
+    const int r = bk.rank;
+    const int64_t nr = bk.nr, nc = bk.nc;
+    const c128* Ub = U + bk.u_off;
+    const c128* Vb = V + bk.v_off;
+    double tot = 0.0;
+    for (int i = 0; i < r; ++i) {
+        const c128 un = block_csum(nr, [&](int64_t t) { return c128{norm_sqr(Ub[t + i * nr]), 0.0}; }, red);
+        const c128 vn = block_csum(nc, [&](int64_t t) { return c128{norm_sqr(Vb[t + i * nc]), 0.0}; }, red);
+        tot += un.re * vn.re;
+        for (int j = 0; j < i; ++j) {
+            const c128 du = block_csum(nr, [&](int64_t t) { return Ub[t + i * nr] * Ub[t + j * nr]; }, red);
+            const c128 dv = block_csum(nc, [&](int64_t t) { return Vb[t + i * nc] * Vb[t + j * nc]; }, red);
+            tot += 2.0 * hypot(du.re, du.im) * hypot(dv.re, dv.im);
+        }
+    }
+    if (threadIdx.x == 0) bk.norm = sqrt(tot);
+}
+
+// --- matvec ------------------------------------------------------------------
+// part[p_off + i] = alpha * (D x_cols)[i]  (dense blocks)
+__global__ void hmv_dense_kernel(const HBlkDev* __restrict__ blks, const int32_t* __restrict__ list,
+                                 const int64_t* __restrict__ cols, const c128* __restrict__ D,
+                                 const c128* __restrict__ x, c128 alpha, c128* part) {
+    const HBlkDev bk = blks[list[blockIdx.x]];
+    for (int64_t i = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; i < bk.nr;
+         i += int64_t(blockDim.x) * gridDim.y) {
+        c128 s{0.0, 0.0};
+        for (int64_t j = 0; j < bk.nc; ++j) s += D[bk.d_off + i + j * bk.nr] * x[cols[bk.c_off + j]];
+        part[bk.p_off + i] = alpha * s;
+    }
+}
+
+// part[p_off + i] = alpha * (U (V^T x_cols))[i]  (low-rank blocks)
+__global__ __launch_bounds__(256) void hmv_lowrank_kernel(const HBlkDev* __restrict__ blks,
+                                                          const int32_t* __restrict__ list,
+                                                          const int64_t* __restrict__ cols,
+                                                          const c128* __restrict__ U,
+                                                          const c128* __restrict__ V,
+                                                          const c128* __restrict__ x, c128 alpha,
+                                                          c128* part) {
+    __shared__ c128 t[RMAX];
+    __shared__ double red[2 * 4 * RMAX];
+    const HBlkDev bk = blks[list[blockIdx.x]];
+    const int r = bk.rank;
+    for (int k = 0; k < r; ++k) {
+        double re = 0, im = 0;
+        for (int64_t j = threadIdx.x; j < bk.nc; j += 256) {
+            const c128 z = V[bk.v_off + j + k * bk.nc] * x[cols[bk.c_off + j]];
+            re += z.re;
+            im += z.im;
+        }
+        re = wave_sum(re);
+        im = wave_sum(im);
+        if ((threadIdx.x & 63) == 0) {
+            red[(k * 4 + (threadIdx.x >> 6)) * 2] = re;
+            red[(k * 4 + (threadIdx.x >> 6)) * 2 + 1] = im;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < r) {
+        c128 s{0.0, 0.0};
+        for (int w = 0; w < 4; ++w) s += c128{red[(threadIdx.x * 4 + w) * 2], red[(threadIdx.x * 4 + w) * 2 + 1]};
+        t[threadIdx.x] = alpha * s;
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < bk.nr; i += 256) {
+        c128 s{0.0, 0.0};
+        for (int k = 0; k < r; ++k) s += U[bk.u_off + i + k * bk.nr] * t[k];
+        part[bk.p_off + i] = s;
+    }
+}
+
+// y[i] = beta y[i] + sum of the contributions of row i (fixed order)
+__global__ void hmv_reduce_kernel(int64_t n, const int64_t* __restrict__ rowptr,
+                                  const int64_t* __restrict__ ent, const c128* __restrict__ part,
+                                  c128 beta, c128* y) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    c128 s{0.0, 0.0};
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) s += part[ent[e]];
+    // BLAS convention: beta = 0 does not read y (which may be uninitialised)
+    if (beta.re == 0.0 && beta.im == 0.0) y[i] = s;
+    else if (beta.re == 1.0 && beta.im == 0.0) y[i] = y[i] + s;
+    else y[i] = beta * y[i] + s;
+}
+
+// A[rows[i] + cols[j] lda] = block entry (the blocks partition the matrix,
+// so every entry is written once; h_matrix.rs:235-247)
+__global__ void hmat_full_kernel(const HBlkDev* __restrict__ blks, const int64_t* __restrict__ rows,
+                                 const int64_t* __restrict__ cols, const c128* __restrict__ U,
+                                 const c128* __restrict__ V, const c128* __restrict__ D, c128* A,
+                                 int64_t lda) {
+    const HBlkDev bk = blks[blockIdx.x];
+    const int64_t cnt = bk.nr * bk.nc;
+    for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cnt;
+         e += int64_t(blockDim.x) * gridDim.y) {
+        const int64_t i = e % bk.nr, j = e / bk.nr;
+        c128 v{0.0, 0.0};
+        if (bk.adm) {
+            for (int k = 0; k < bk.rank; ++k) v += U[bk.u_off + i + k * bk.nr] * V[bk.v_off + j + k * bk.nc];
+        } else {
+            v = D[bk.d_off + e];
+        }
+        A[rows[bk.r_off + i] + cols[bk.c_off + j] * lda] = v;
+    }
+}
+
+template <typename T>
+T* dalloc(std::vector<void*>& owned, size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    owned.push_back(p);
+    return static_cast<T*>(p);
+}
+
+}  // namespace
+
+struct ccab_hmatrix {
+    int64_t n = 0;
+    int device = 0;
+    std::vector<void*> owned;  // device allocations
+    std::vector<HBlkDev> blk;  // host copy (ranks, norms after build)
+    HBlkDev* d_blk = nullptr;
+    int64_t* d_rows = nullptr;
+    int64_t* d_cols = nullptr;
+    c128* d_U = nullptr;
+    c128* d_V = nullptr;
+    c128* d_D = nullptr;
+    c128* d_part = nullptr;
+    int64_t* d_rowptr = nullptr;
+    int64_t* d_ent = nullptr;
+    int32_t* d_dense_list = nullptr;
+    int32_t* d_lr_list = nullptr;
+    int64_t n_dense = 0, n_lr = 0;
+    double norm = 0.0;
+    int64_t aca_fallbacks = 0;
+    ~ccab_hmatrix() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+};
+
+int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
+                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out) {
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<ccab_hmatrix> owner(new ccab_hmatrix());
+    ccab_hmatrix* H = owner.get();
+    H->n = n;
+    H->device = ctx->device;
+    auto fail = [&](int code, const char* msg) { return ctx_fail(ctx, code, "%s", msg); };
+    // cluster and block trees on the host (index order as cluster.rs)
+    std::vector<HNode> T;
+    {
+        std::vector<int64_t> all(n);
+        for (int64_t i = 0; i < n; ++i) all[i] = i;
+        build_cluster(T, h_cpts, 8, std::move(all), leaf);
+    }
+    std::vector<HostBlock> bl;
+    build_blocks(T, 0, 0, eta, bl);
+    const int64_t nb = (int64_t)bl.size();
+    std::vector<int64_t> rows, cols;
+    H->blk.resize(nb);
+    int64_t u_off = 0, v_off = 0, d_off = 0, p_off = 0, w_off = 0, f_off = 0;
+    std::vector<int32_t> dl, ll;
+    for (int64_t b = 0; b < nb; ++b) {
+        HBlkDev& d = H->blk[b];
+        std::memset(&d, 0, sizeof(d));
+        const auto& R = T[bl[b].r].idx;
+        const auto& C = T[bl[b].c].idx;
+        d.r_off = (int64_t)rows.size();
+        d.nr = (int64_t)R.size();
+        d.c_off = (int64_t)cols.size();
+        d.nc = (int64_t)C.size();
+        rows.insert(rows.end(), R.begin(), R.end());
+        cols.insert(cols.end(), C.begin(), C.end());
+        d.adm = bl[b].adm ? 1 : 0;
+        d.p_off = p_off;
+        p_off += d.nr;
+        if (d.adm) {
+            // the ACA takes at most min(nr, nc) terms
+            const int64_t cap = std::min<int64_t>(std::min(d.nr, d.nc), RMAX);
+            d.u_off = u_off;
+            d.v_off = v_off;
+            u_off += d.nr * cap;
+            v_off += d.nc * cap;
+            d.w_off = w_off;
+            w_off += 2 * (d.nr + d.nc);
+            d.f_off = f_off;
+            f_off += d.nr + d.nc;
+            ll.push_back((int32_t)b);
+        } else {
+            d.d_off = d_off;
+            d_off += d.nr * d.nc;
+            dl.push_back((int32_t)b);
+        }
+    }
+    H->d_rows = dalloc<int64_t>(H->owned, rows.size());
+    H->d_cols = dalloc<int64_t>(H->owned, cols.size());
+    H->d_blk = dalloc<HBlkDev>(H->owned, nb);
+    H->d_U = dalloc<c128>(H->owned, u_off);
+    H->d_V = dalloc<c128>(H->owned, v_off);
+    c128* U2 = dalloc<c128>(H->owned, u_off);
+    c128* V2 = dalloc<c128>(H->owned, v_off);
+    c128* Wws = dalloc<c128>(H->owned, w_off);
+    uint8_t* Fws = dalloc<uint8_t>(H->owned, f_off);
+    c128* Rws = dalloc<c128>(H->owned, std::max<size_t>(ll.size(), 1) * 2 * RMAX * RMAX);
+    uint32_t* d_rng = dalloc<uint32_t>(H->owned, kRngTable);
+    // fallbacks move blocks from the low-rank to the dense list: size both for every block
+    H->d_dense_list = dalloc<int32_t>(H->owned, nb + 1);
+    H->d_lr_list = dalloc<int32_t>(H->owned, nb + 1);
+    if (!H->d_rows || !H->d_cols || !H->d_blk || !H->d_U || !H->d_V || !U2 || !V2 || !Wws ||
+        !Fws || !Rws || !d_rng || !H->d_dense_list || !H->d_lr_list)
+        return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    std::vector<uint32_t> words(kRngTable);
+    chacha12_words(10, words.data(), kRngTable);
+    CCAB_HIP(hipMemcpy(H->d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_cols, cols.data(), cols.size() * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(d_rng, words.data(), kRngTable * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_lr_list, ll.data(), ll.size() * 4, hipMemcpyHostToDevice));
+    const c128 fa = side == CCAB_LHS ? p.lhs_a : p.rhs_a;
+    const c128 fb = side == CCAB_LHS ? p.lhs_b : p.rhs_b;
+    // batched ACA + recompression of the admissible blocks
+    if (!ll.empty()) {
+        hipLaunchKernelGGL(aca_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, p, d_soa,
+                           n, H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, fa, fb, H->d_U,
+                           H->d_V, Wws, Fws, d_rng, tol);
+        hipLaunchKernelGGL(recompress_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s,
+                           H->d_blk, H->d_lr_list, H->d_U, H->d_V, U2, V2, Rws, tol);
+        std::swap(H->d_U, U2);
+        std::swap(H->d_V, V2);
+        hipLaunchKernelGGL(aca_norm_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s,
+                           H->d_blk, H->d_lr_list, H->d_U, H->d_V);
+        CCAB_HIP(hipGetLastError());
+        CCAB_HIP(hipMemcpyAsync(H->blk.data(), H->d_blk, nb * sizeof(HBlkDev), hipMemcpyDeviceToHost, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+    }
+    // blocks whose ACA hit the rank capacity become dense
+    std::vector<int32_t> ll2;
+    for (int32_t b : ll) {
+        HBlkDev& d = H->blk[b];
+        if (d.status != 0) {
+            d.adm = 0;
+            d.d_off = d_off;
+            d_off += d.nr * d.nc;
+            dl.push_back(b);
+            H->aca_fallbacks++;
+        } else {
+            ll2.push_back(b);
+        }
+    }
+    std::sort(dl.begin(), dl.end());
+    H->n_dense = (int64_t)dl.size();
+    H->n_lr = (int64_t)ll2.size();
+    H->d_D = dalloc<c128>(H->owned, d_off);
+    H->d_part = dalloc<c128>(H->owned, p_off);
+    if (!H->d_D || !H->d_part) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
+    if (!dl.empty()) {
+        hipLaunchKernelGGL(dense_block_kernel, dim3((unsigned)dl.size(), 4), dim3(256), 0, s, p,
+                           d_soa, n, H->d_blk, H->d_dense_list, H->d_rows, H->d_cols, fa, fb, H->d_D);
+        hipLaunchKernelGGL(dense_norm_kernel, dim3((unsigned)dl.size()), dim3(256), 0, s, H->d_blk,
+                           H->d_dense_list, H->d_D);
+    }
+    CCAB_HIP(hipGetLastError());
+    CCAB_HIP(hipMemcpyAsync(H->blk.data(), H->d_blk, nb * sizeof(HBlkDev), hipMemcpyDeviceToHost, s));
+    CCAB_HIP(hipStreamSynchronize(s));
+    // norm = sum of dense Frobenius norms, then the low-rank estimates (h_matrix.rs:189-196)
+    double nd = 0.0, nl = 0.0;
+    for (int32_t b : dl) nd += H->blk[b].norm;
+    for (int32_t b : ll2) nl += H->blk[b].norm;
+    H->norm = nd + nl;
+    // matvec plan: row CSR over the per-(block,row) contribution slots
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int64_t b = 0; b < nb; ++b)
+        for (int64_t i = 0; i < H->blk[b].nr; ++i) cnt[rows[H->blk[b].r_off + i] + 1]++;
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> ent(p_off), fillp(cnt.begin(), cnt.end() - 1);
+    // dense contributions first, then low-rank, each in block order (h_matrix.rs:228-233)
+    for (int pass = 0; pass < 2; ++pass)
+        for (int64_t b = 0; b < nb; ++b) {
+            const HBlkDev& d = H->blk[b];
+            if ((pass == 0) != (d.adm == 0)) continue;
+            for (int64_t i = 0; i < d.nr; ++i) ent[fillp[rows[d.r_off + i]]++] = d.p_off + i;
+        }
+    H->d_rowptr = dalloc<int64_t>(H->owned, n + 1);
+    H->d_ent = dalloc<int64_t>(H->owned, p_off);
+    if (!H->d_rowptr || !H->d_ent) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), p_off * 8, hipMemcpyHostToDevice));
+    // release the build workspace
+    for (void* w : {static_cast<void*>(U2), static_cast<void*>(V2), static_cast<void*>(Wws),
+                    static_cast<void*>(Fws), static_cast<void*>(Rws)}) {
+        auto it = std::find(H->owned.begin(), H->owned.end(), w);
+        if (it != H->owned.end()) {
+            (void)hipFree(*it);
+            H->owned.erase(it);
+        }
+    }
+    *out = owner.release();
+    return CCAB_OK;
+}
+
+int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
+                 c128* y) {
+    hipStream_t s = ctx->stream;
+    ProfScope ps(ctx, PROF_GEMV, 0.0);
+    if (H->n_dense)
+        hipLaunchKernelGGL(hmv_dense_kernel, dim3((unsigned)H->n_dense, 1), dim3(256), 0, s,
+                           H->d_blk, H->d_dense_list, H->d_cols, H->d_D, x, alpha, H->d_part);
+    if (H->n_lr)
+        hipLaunchKernelGGL(hmv_lowrank_kernel, dim3((unsigned)H->n_lr), dim3(256), 0, s, H->d_blk,
+                           H->d_lr_list, H->d_cols, H->d_U, H->d_V, x, alpha, H->d_part);
+    hipLaunchKernelGGL(hmv_reduce_kernel, dim3((unsigned)((H->n + 255) / 256)), dim3(256), 0, s,
+                       H->n, H->d_rowptr, H->d_ent, H->d_part, beta, y);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+void hmatrix_stats(const ccab_hmatrix* H, int64_t* st, double* norm) {
+    int64_t adm = 0, iad = 0, storage = 0, rsum = 0;
+    for (const auto& d : H->blk) {
+        if (d.adm) {
+            adm++;
+            storage += int64_t(d.rank) * (d.nr + d.nc);
+            rsum += d.rank;
+        } else {
+            iad++;
+            storage += d.nr * d.nc;
+        }
+    }
+    st[0] = adm;
+    st[1] = iad;
+    st[2] = storage;
+    st[3] = rsum;
+    st[4] = H->aca_fallbacks;
+    if (norm) *norm = H->norm;
+}
+
+int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda) {
+    const int64_t nb = (int64_t)H->blk.size();
+    if (nb)
+        hipLaunchKernelGGL(hmat_full_kernel, dim3((unsigned)nb, 4), dim3(256), 0, ctx->stream,
+                           H->d_blk, H->d_rows, H->d_cols, H->d_U, H->d_V, H->d_D, A, lda);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+void hmatrix_free(ccab_hmatrix* H) { delete H; }
+int hmatrix_device(const ccab_hmatrix* H) { return H->device; }
+
+int64_t hmatrix_n(const ccab_hmatrix* H) { return H->n; }
+double hmatrix_norm(const ccab_hmatrix* H) { return H->norm; }

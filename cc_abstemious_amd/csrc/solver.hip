@@ -161,8 +161,11 @@ void host_lu_solve(int64_t k, std::vector<cd>& A, std::vector<cd>& x) {
 
 }  // namespace
 
-int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, c128* d_x,
-              const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st) {
+// GMRES over any operator y = alpha A x + beta y (gmres.rs:35-202); anorm is
+// the norm the backward error uses (dense: Frobenius; H matrix: the sum of
+// its block norms, h_matrix.rs:189-196).
+int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, c128* d_x,
+                 const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st) {
     int64_t max_it = p.max_iterations <= 0 ? 1000 : p.max_iterations;
     max_it = std::min<int64_t>(max_it, n);
     const double thresh = p.tolerance <= 0.0 ? 1.0e-5 : p.tolerance;
@@ -193,18 +196,14 @@ int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, 
     int64_t num_mv = 0, iters = 0, restarts = 0;
     auto matvec = [&](const c128* xin, c128 alpha, c128 beta, c128* yout) -> int {
         ++num_mv;
-        return gemv_general(ctx, prec, n, n, 1, d_A, lda, 0, xin, 0, alpha, beta, yout, 0);
+        return apply(xin, alpha, beta, yout);
     };
     hipLaunchKernelGGL(copy_c128_kernel, grid1(n), dim3(256), 0, s, n, d_x, b);
     if (d_x0)
         hipLaunchKernelGGL(copy_c128_kernel, grid1(n), dim3(256), 0, s, n, d_x0, d_x);
     else
         hipLaunchKernelGGL(fill_zero_kernel, grid1(n), dim3(256), 0, s, n, d_x);
-    double anorm = 0.0, bnorm = 0.0;
-    CCAB_TRY(frob_norm(ctx, prec, n, d_A, lda, nrm + 1));
-    CCAB_HIP(hipMemcpyAsync(hp, nrm + 1, 8, hipMemcpyDeviceToHost, s));
-    CCAB_HIP(hipStreamSynchronize(s));
-    anorm = hp[0];
+    double bnorm = 0.0;
     CCAB_TRY(read_norm(b, &bnorm));
     const bool rel = p.stop_rule == CCAB_STOP_RELATIVE_RESIDUAL;
     auto err_of = [&](double rn, double xn) { return rel ? rn / bnorm : rn / (anorm * xn + bnorm); };
@@ -301,6 +300,22 @@ int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, 
     }
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
+}
+
+int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, c128* d_x,
+              const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st) {
+    double* nrm = static_cast<double*>(ctx_buf(ctx, SLOT_GMRES_NRM, 64));
+    if (!nrm) return CCAB_ENOMEM;
+    CCAB_TRY(pinned_reserve(ctx, 64));
+    double* hp = static_cast<double*>(ctx->pinned);
+    CCAB_TRY(frob_norm(ctx, prec, n, d_A, lda, nrm));
+    CCAB_HIP(hipMemcpyAsync(hp, nrm, 8, hipMemcpyDeviceToHost, ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    const double anorm = hp[0];
+    MatvecFn dense = [&](const c128* xin, c128 alpha, c128 beta, c128* yout) -> int {
+        return gemv_general(ctx, prec, n, n, 1, d_A, lda, 0, xin, 0, alpha, beta, yout, 0);
+    };
+    return gmres_op_run(ctx, n, dense, anorm, d_x, d_x0, p, st);
 }
 
 // Mixed precision: x = (LU_c64)^{-1} b, then x += (LU_c64)^{-1} (b - A128 x)

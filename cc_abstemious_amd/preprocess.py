@@ -60,11 +60,15 @@ class Solver:
     matrix_prec: int = capi.C128
     ir_tolerance: float = 1e-12
     ir_max_steps: int = 10
+    leaf_size: int = 32              # hierarchical: solve.rs:91,117
+    aca_tolerance: float = 1e-4      # hierarchical: solve.rs:92,118
+    eta: float = 4.0                 # hierarchical: h_matrix.rs:136
 
     def to_c(self) -> capi.SolverCfg:
         g = capi.gmres_params(self.max_iterations, self.tolerance, self.restart, self.ref_mgs,
                               self.stop_rule, self.matrix_prec)
-        return capi.solver_cfg(self.kind, g, self.ir_tolerance, self.ir_max_steps)
+        return capi.solver_cfg(self.kind, g, self.ir_tolerance, self.ir_max_steps,
+                               self.leaf_size, self.aca_tolerance, self.eta)
 
 
 class PreData:

@@ -2,7 +2,8 @@
 
 solve_for_surface(predata, rhs_inc) -> (phi, vn)      solve.rs:11-71
 solve_lu(a, x)                                        solve.rs:166-171
-GMRES(max_it, thresh); .a = matrix; .solve(x)         gmres.rs:22-59
+GMRES(max_it, thresh); .a = matrix or .hmatrix = HMatrix; .solve(x)
+                                                      gmres.rs:22-59
 sweep(predata) -> (phi, vn, stats)                    analysis.rs:86-111 (surface part)
 """
 from __future__ import annotations
@@ -158,13 +159,26 @@ class GMRES:
         self.ref_mgs = ref_mgs
         self.stop_rule = stop_rule
         self.a = None
+        self.hmatrix = None          # gmres.rs:18 (an HMatrix operator instead of .a)
         self.num_mv = 0
         self.stats = None
 
     def solve(self, x: torch.Tensor, x0: torch.Tensor | None = None, ctx=None):
         """x: b on entry, solution on exit (complex128, contiguous)."""
+        if self.hmatrix is not None:
+            h = self.hmatrix
+            ctx = ctx or h.ctx
+            p = capi.gmres_params(self.max_it, self.thresh, self.restart, self.ref_mgs,
+                                  self.stop_rule, capi.C128)
+            st = capi.SolveStats()
+            ctx.call("ccab_gmres_hmatrix", h._h, _ptr(x), _ptr(x0) if x0 is not None else None,
+                     ct.byref(p), ct.byref(st))
+            ctx.synchronize()
+            self.stats = st.as_dict()
+            self.num_mv = st.num_mv
+            return x
         if self.a is None:
-            raise ValueError("GMRES needs an operator: set .a")
+            raise ValueError("GMRES needs an operator: set .a or .hmatrix")
         a = self.a
         ctx = ctx or capi.default_context(a.device.index or 0)
         lda = _colmajor_ok(a)

@@ -36,6 +36,7 @@ extern "C" {
 typedef struct { double re, im; } ccab_c128;
 typedef struct { float re, im; } ccab_c64;
 typedef struct ccab_ctx ccab_ctx;
+typedef struct ccab_hmatrix ccab_hmatrix;
 
 /* input_data.rs:407-449 */
 enum { CCAB_CLASSICAL = 0, CCAB_BURTON_MILLER = 1 };
@@ -96,11 +97,13 @@ typedef struct {
 
 typedef struct {
     int32_t solver;          /* CCAB_SOLVER_* */
-    int32_t pad_;
-    ccab_gmres_params gmres; /* ITERATIVE(_MIXED) */
+    int32_t leaf_size;       /* HIERARCHICAL: cluster leaf cardinality (<=0 -> 32, solve.rs:91,117) */
+    ccab_gmres_params gmres; /* ITERATIVE(_MIXED), HIERARCHICAL */
     double ir_tolerance;     /* mixed: stop when ||dx||/||x|| < this (<=0 -> 1e-12) */
     int32_t ir_max_steps;    /* mixed: <=0 -> 10 */
-    int32_t pad2_;
+    int32_t pad_;
+    double aca_tolerance;    /* HIERARCHICAL: ACA tolerance (<=0 -> 1e-4, solve.rs:92,118) */
+    double eta;              /* HIERARCHICAL: admissibility (<=0 -> 4.0, h_matrix.rs:136) */
 } ccab_solver_cfg;
 
 /* ---- context ---------------------------------------------------------- */
@@ -214,9 +217,47 @@ int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const ccab_c1
                           const ccab_c128* d_x0, const ccab_solver_cfg* cfg,
                           ccab_solve_stats* stats);
 
+/* ---- hierarchical matrix (device) ---------------------------------------------- */
+/* src/solve/h_matrix.rs:127-148 (HMatrix::new_from) of the side matrix
+ * a H + b G (side CCAB_LHS / CCAB_RHS, the entries of
+ * get_surface_row_or_column): cluster tree split at the bounding-box midpoint
+ * of the longest side down to leaf_size points (src/solve/h_matrix/cluster.rs:19-110),
+ * block tree with admissibility min(diam) <= eta * dist
+ * (src/solve/h_matrix/block.rs:18-96), inadmissible blocks dense, admissible
+ * blocks by ACA with partial pivoting and SVD recompression at aca_tolerance
+ * (src/solve/aca.rs:57-179), all blocks of one kind in one batched launch.
+ * An admissible block whose ACA would need more than 64 terms is stored dense
+ * (counted in stats[4]).  The handle keeps its device memory until
+ * ccab_hmatrix_destroy and is used on the device of the building context. */
+int ccab_hmatrix_build(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts, int64_t n,
+                       int side, int64_t leaf_size, double eta, double aca_tolerance,
+                       ccab_hmatrix** out);
+int ccab_hmatrix_destroy(ccab_hmatrix* h);
+/* src/solve/h_matrix.rs:218-234 (HMatrix::gemv): d_y = alpha H d_x + beta d_y,
+ * alpha and beta complex {re, im} (beta = 0: d_y is not read, as in BLAS);
+ * block contributions are summed per row in a fixed order (dense blocks, then
+ * low-rank blocks, in block-tree order). */
+int ccab_hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* h, const double* alpha,
+                      const ccab_c128* d_x, const double* beta, ccab_c128* d_y);
+/* src/solve/h_matrix.rs:236-247 (to_full): the n x n matrix the H matrix stands for */
+int ccab_hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_A, int64_t lda);
+/* src/solve/h_matrix.rs:189-216 (update_norm, print_stats): st[0] admissible
+ * blocks, st[1] inadmissible blocks, st[2] stored entries, st[3] sum of the
+ * ranks, st[4] admissible blocks stored dense; *norm = sum of the dense
+ * Frobenius norms and the ACA norm estimates (the ||A|| of the GMRES stop). */
+int ccab_hmatrix_stats(const ccab_hmatrix* h, int64_t* st, double* norm);
+/* src/solve/gmres.rs:35-202 with the H matrix as operator (src/solve.rs:119-125).
+ * d_x: b on entry, x on exit; d_x0 initial guess or NULL. */
+int ccab_gmres_hmatrix(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_x,
+                       const ccab_c128* d_x0, const ccab_gmres_params* p,
+                       ccab_solve_stats* stats);
+
 /* ---- the surface solve (device) ---------------------------------------------- */
 /* solve.rs:11-71 (solve_for_surface, dense branches): assembly + BC algebra +
- * LU / GMRES / mixed LU with FP64 refinement; outputs surface phi and vn. */
+ * LU / GMRES / mixed LU with FP64 refinement; outputs surface phi and vn.
+ * CCAB_SOLVER_HIERARCHICAL follows get_surface_hmatrix (solve.rs:74-150):
+ * RHS-side H matrix (when the BC value is nonzero) for the right-hand side,
+ * LHS-side H matrix and GMRES over it. */
 int ccab_solve_surface(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts, int64_t n,
                        const ccab_c128* d_rhs_inc, const ccab_solver_cfg* cfg,
                        ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats);

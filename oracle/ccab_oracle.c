@@ -500,10 +500,13 @@ int orc_gmres_op(int64_t n, orc_matvec_fn mv, void* user, double anorm, orc_c128
     int64_t max_restarts = max_it / m;
     int64_t num_mv = 0, iters = 0, restarts = 0;
     double b_norm = vnorm(b, n);
-    cplx* r = (cplx*)malloc(sizeof(cplx) * n);
-    cplx* tmp = (cplx*)malloc(sizeof(cplx) * n);
+    /* the operator may compute y = A x + 0 * y (h_matrix.rs:219-223 scales b
+     * by beta): start every output vector finite, as the reference's
+     * DVector allocations are */
+    cplx* r = (cplx*)calloc(n, sizeof(cplx));
+    cplx* tmp = (cplx*)calloc(n, sizeof(cplx));
     cplx* q = (cplx*)malloc(sizeof(cplx) * n * (m + 1));
-    cplx* qk1 = (cplx*)malloc(sizeof(cplx) * n);
+    cplx* qk1 = (cplx*)calloc(n, sizeof(cplx));
     cplx* sn = (cplx*)malloc(sizeof(cplx) * m);
     cplx* cs = (cplx*)malloc(sizeof(cplx) * m);
     cplx* bet = (cplx*)malloc(sizeof(cplx) * (m + 1));

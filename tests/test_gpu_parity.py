@@ -233,8 +233,8 @@ def _field_from(pd, phi, vn):
 def test_reference_fixtures_end_to_end(sphere, method, kind, expect, tol):
     """verification.rs:95-205 through the GPU surface solve."""
     _note(f"fixture method={method} solver={kind}")
-    tol = 1e-5 if kind == capi.SOLVER_ITERATIVE else 1e-12
-    pd = _predata(sphere, method, solver=Solver(kind=kind, tolerance=tol, max_iterations=1000))
+    gtol = 1e-5 if kind == capi.SOLVER_ITERATIVE else 1e-12
+    pd = _predata(sphere, method, solver=Solver(kind=kind, tolerance=gtol, max_iterations=1000))
     rhs_inc = incident_wave.get_incident_surface(pd)
     phi, vn = solve.solve_for_surface(pd, rhs_inc)
     p = _field_from(pd, phi.cpu().numpy(), vn.cpu().numpy())

@@ -37,7 +37,7 @@ def test_struct_layouts():
     assert ct.sizeof(capi.Wave) == 48
     assert ct.sizeof(capi.GmresParams) == 32
     assert ct.sizeof(capi.SolveStats) == 56
-    assert ct.sizeof(capi.SolverCfg) == 56
+    assert ct.sizeof(capi.SolverCfg) == 72
 
 
 @pytest.mark.parametrize("fname,body", [("sphere.vtk", 3), ("refined_sphere.vtk", 3),
