@@ -91,6 +91,18 @@ class SolverCfg(ct.Structure):
     ]
 
 
+class SurrogateCfg(ct.Structure):
+    _fields_ = [
+        ("width", ct.c_int32),
+        ("seed", ct.c_int32),
+        ("lr", ct.c_double),
+        ("beta1", ct.c_double),
+        ("beta2", ct.c_double),
+        ("eps", ct.c_double),
+        ("weight_decay", ct.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every exported symbol of include/ccab.h
 _P = ct.c_void_p
 _I64 = ct.c_int64
@@ -134,6 +146,20 @@ SIGNATURES = {
     "ccab_hmatrix_to_full": (_I, [_P, _P, _P, _I64]),
     "ccab_hmatrix_stats": (_I, [_P, _P, _P]),
     "ccab_gmres_hmatrix": (_I, [_P, _P, _P, _P, ct.POINTER(GmresParams), _P]),
+    "ccab_field": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
+    "ccab_dense_field_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _P, _I64, _P, _P, _I64]),
+    "ccab_incident_field": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
+    "ccab_surrogate_create": (_I, [_P, ct.POINTER(SurrogateCfg), ct.POINTER(_P)]),
+    "ccab_surrogate_destroy": (_I, [_P]),
+    "ccab_surrogate_num_params": (_I64, [_P]),
+    "ccab_surrogate_get_params": (_I, [_P, _P, _P]),
+    "ccab_surrogate_set_params": (_I, [_P, _P, _P]),
+    "ccab_surrogate_set_lr": (_I, [_P, ct.c_double]),
+    "ccab_surrogate_features": (_I, [_P, _P, _P, _I64, _P, _I64, _P, _P, _P]),
+    "ccab_surrogate_loss_grad": (_I, [_P, _P, _P, _P, _I64, _P, _P]),
+    "ccab_surrogate_train_step": (_I, [_P, _P, _P, _P, _P, _I64, _P]),
+    "ccab_surrogate_forward": (_I, [_P, _P, _P, _I64, _P]),
+    "ccab_surrogate_predict": (_I, [_P, _P, _P, _P, _I64, _P, _I64, _P]),
 }
 
 _lib = None
@@ -229,7 +255,7 @@ class Context:
         self.check(self._L.ccab_set_stream(self.handle, ct.c_void_p(stream_ptr or 0)))
 
     PROFILE_CLASSES = ["assembly", "lu_panel", "lu_trsm", "lu_update", "lu_laswp", "lu_trinv",
-                       "trsv", "gemv", "mgs"]
+                       "trsv", "gemv", "mgs", "field", "hmatrix_gemv", "surrogate"]
 
     def profile(self, on: bool = True):
         self.check(self._L.ccab_profile_enable(self.handle, int(on)))

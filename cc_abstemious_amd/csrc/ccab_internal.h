@@ -91,14 +91,25 @@ static_assert(sizeof(c64) == sizeof(ccab_c64), "layout");
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
+// Persistent per-slot device buffers.  A slot is owned by one routine; the
+// buffer only grows (growing synchronises the stream first).  Returns
+// nullptr and records the error on failure.
+enum BufSlot {
+    SLOT_SOA = 0, SLOT_PARAMS, SLOT_ROWSUM, SLOT_LU_LINV, SLOT_LU_UINV, SLOT_LU_PERM,
+    SLOT_LU_INFO, SLOT_LU_Y, SLOT_GEMV_PART, SLOT_NORM, SLOT_MIXED_R, SLOT_MIXED_D,
+    SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
+    SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
+    SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
+    SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_COUNT
+};
 struct ccab_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t own_stream = nullptr;
     std::string err;
     // grow-only device buffers, one per slot (see enum BufSlot)
-    void* buf[32] = {};
-    size_t buf_bytes[32] = {};
+    void* buf[SLOT_COUNT] = {};
+    size_t buf_bytes[SLOT_COUNT] = {};
     // identity of the factors whose diagonal-block inverses and row
     // permutation are held in the LU slots
     const void* lu_key = nullptr;
@@ -139,7 +150,10 @@ enum ProfClass {
     PROF_TRSV = 6,       // bytes of L/U read
     PROF_GEMV = 7,       // bytes of matrix read
     PROF_MGS = 8,
-    PROF_NCLASS = 9
+    PROF_FIELD = 9,      // (field point, surface point) pairs
+    PROF_HMAT = 10,      // bytes of H-matrix blocks read
+    PROF_SURROGATE = 11, // flops of the network
+    PROF_NCLASS = 12
 };
 void prof_begin(ccab_ctx* ctx, hipEvent_t* ev);
 void prof_end(ccab_ctx* ctx, hipEvent_t ev, int cls, double work);
@@ -177,16 +191,6 @@ int ctx_fail(ccab_ctx* ctx, int code, const char* fmt, ...);
         if (rc_ != CCAB_OK) return rc_;                                                 \
     } while (0)
 
-// Persistent per-slot device buffers.  A slot is owned by one routine; the
-// buffer only grows (growing synchronises the stream first).  Returns
-// nullptr and records the error on failure.
-enum BufSlot {
-    SLOT_SOA = 0, SLOT_PARAMS, SLOT_ROWSUM, SLOT_LU_LINV, SLOT_LU_UINV, SLOT_LU_PERM,
-    SLOT_LU_INFO, SLOT_LU_Y, SLOT_GEMV_PART, SLOT_NORM, SLOT_MIXED_R, SLOT_MIXED_D,
-    SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
-    SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
-    SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_COUNT
-};
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
 inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
 int pinned_reserve(ccab_ctx* ctx, size_t bytes);

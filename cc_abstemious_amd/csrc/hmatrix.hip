@@ -794,6 +794,7 @@ struct ccab_hmatrix {
     int32_t* d_lr_list = nullptr;
     int64_t n_dense = 0, n_lr = 0;
     double norm = 0.0;
+    double mv_bytes = 0.0;  // block data one matvec reads
     int64_t aca_fallbacks = 0;
     ~ccab_hmatrix() {
         for (void* p : owned) (void)hipFree(p);
@@ -932,6 +933,8 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     for (int32_t b : dl) nd += H->blk[b].norm;
     for (int32_t b : ll2) nl += H->blk[b].norm;
     H->norm = nd + nl;
+    for (const auto& d : H->blk)
+        H->mv_bytes += 16.0 * (d.adm ? double(d.rank) * double(d.nr + d.nc) : double(d.nr) * double(d.nc));
     // matvec plan: row CSR over the per-(block,row) contribution slots
     std::vector<int64_t> cnt(n + 1, 0);
     for (int64_t b = 0; b < nb; ++b)
@@ -966,7 +969,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
 int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
                  c128* y) {
     hipStream_t s = ctx->stream;
-    ProfScope ps(ctx, PROF_GEMV, 0.0);
+    ProfScope ps(ctx, PROF_HMAT, H->mv_bytes);
     if (H->n_dense)
         hipLaunchKernelGGL(hmv_dense_kernel, dim3((unsigned)H->n_dense, 1), dim3(256), 0, s,
                            H->d_blk, H->d_dense_list, H->d_cols, H->d_D, x, alpha, H->d_part);

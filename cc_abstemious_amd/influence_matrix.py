@@ -1,6 +1,7 @@
 """Influence matrices on the GPU -- mirror of src/influence_matrix.rs.
 
 get_dense_surface_matrices   influence_matrix.rs:64-98
+get_dense_field_matrices     influence_matrix.rs:101-127
 get_surface_row_or_column    influence_matrix.rs:170-231
 assemble_system              fused assembly + BC algebra of solve.rs:24-71
 """
@@ -32,6 +33,20 @@ def get_dense_surface_matrices(predata: PreData, prec: int = capi.C128, ctx=None
              _ptr(G), n)
     ctx.synchronize()
     return H, G
+
+
+def get_dense_field_matrices(predata: PreData, ctx=None):
+    """Returns (M, L) as nfp x n column-major complex128 cuda tensors."""
+    ctx = ctx or capi.default_context(predata.device.index or 0)
+    if predata.d_fps is None or predata.d_fps.shape[0] == 0:
+        raise ValueError("no field points: call predata.set_field_points")
+    n, nfp = predata.get_num_eqn(), predata.d_fps.shape[0]
+    M = colmajor(nfp, n, torch.complex128, predata.device)
+    L = colmajor(nfp, n, torch.complex128, predata.device)
+    ctx.call("ccab_dense_field_matrices", ct.byref(predata.physics()), _ptr(predata.d_cpts), n,
+             _ptr(predata.d_fps), nfp, _ptr(M), _ptr(L), nfp)
+    ctx.synchronize()
+    return M, L
 
 
 def get_surface_row_or_column(predata: PreData, i, j, side: int = LHS, ctx=None) -> torch.Tensor:

@@ -91,6 +91,17 @@ class PreData:
         self.ifreq = 0
         self.device = torch.device("cuda", device)
         self.d_cpts = torch.from_numpy(self.cpts).to(self.device)
+        self.field_points = np.zeros((0, 3))   # preprocess.rs get_field_points
+        self.d_fps = None
+        self.output_total = True               # OutputType::Total (solve.rs:176)
+        self.output_pressure = True            # OutputField::Pressure (postprocess.rs:72)
+
+    def set_field_points(self, fps):
+        self.field_points = np.ascontiguousarray(np.asarray(fps, dtype=np.float64).reshape(-1, 3))
+        self.d_fps = torch.from_numpy(self.field_points).to(self.device)
+
+    def get_field_points(self):
+        return self.field_points
 
     # preprocess.rs:263-304
     def set_frequency_index(self, index: int):

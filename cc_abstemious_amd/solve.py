@@ -5,6 +5,9 @@ solve_lu(a, x)                                        solve.rs:166-171
 GMRES(max_it, thresh); .a = matrix or .hmatrix = HMatrix; .solve(x)
                                                       gmres.rs:22-59
 sweep(predata) -> (phi, vn, stats)                    analysis.rs:86-111 (surface part)
+get_field(predata, m, l, phi, vn, phi_inc_fp)         solve.rs:173-185
+field(predata, phi, vn) -> (scattered, incident)      analysis.rs:100-104 + postprocess.rs:70-88,
+                                                      fused, batched over frequencies
 """
 from __future__ import annotations
 
@@ -192,3 +195,33 @@ class GMRES:
         self.stats = st.as_dict()
         self.num_mv = st.num_mv
         return x
+
+
+def get_field(predata: PreData, m: torch.Tensor, l: torch.Tensor, phi: torch.Tensor,
+              vn: torch.Tensor, phi_inc_fp: torch.Tensor) -> torch.Tensor:
+    """phi_fp = (total ? phi_inc_fp : 0) - M phi + L vn (solve.rs:173-185), from
+    explicit M, L (see influence_matrix.get_dense_field_matrices)."""
+    out = phi_inc_fp.clone() if predata.output_total else torch.zeros_like(phi_inc_fp)
+    return out - m @ phi + l @ vn
+
+
+def field(predata: PreData, phi: torch.Tensor, vn: torch.Tensor, frequencies=None, ctx=None):
+    """Field results for every frequency at once without storing M and L:
+    (scattered, incident), each (nfreq, nfp); scattered holds the total field
+    when predata.output_total, and both are pressures when
+    predata.output_pressure (analysis.rs:100-111, postprocess.rs:70-88)."""
+    ctx = ctx or capi.default_context(predata.device.index or 0)
+    fs = predata.get_frequencies() if frequencies is None else list(frequencies)
+    if predata.d_fps is None or predata.d_fps.shape[0] == 0:
+        raise ValueError("no field points: call predata.set_field_points")
+    n, nfp = predata.get_num_eqn(), predata.d_fps.shape[0]
+    phi = phi.to(torch.complex128).reshape(len(fs), n).contiguous()
+    vn = vn.to(torch.complex128).reshape(len(fs), n).contiguous()
+    out = torch.empty((len(fs), nfp), dtype=torch.complex128, device=predata.device)
+    inc = torch.empty((len(fs), nfp), dtype=torch.complex128, device=predata.device)
+    w = predata.incident_wave.to_c()
+    ctx.call("ccab_field", predata.physics_array(fs), len(fs), ct.byref(w), _ptr(predata.d_cpts),
+             n, _ptr(predata.d_fps), nfp, _ptr(phi), _ptr(vn), int(predata.output_total),
+             int(predata.output_pressure), _ptr(out), _ptr(inc))
+    ctx.synchronize()
+    return out, inc

@@ -119,7 +119,9 @@ int ccab_synchronize(ccab_ctx* ctx);
  * reference counterpart; the reference only logs).  Classes: 0 assembly
  * (work = bytes stored), 1 LU panel, 2 LU triangular update, 3 LU trailing
  * complex GEMM (work = flops), 4 row swaps, 5 diagonal-block inverses,
- * 6 triangular solves, 7 GEMV (work = matrix bytes), 8 Gram-Schmidt. */
+ * 6 triangular solves, 7 GEMV (work = matrix bytes), 8 Gram-Schmidt,
+ * 9 field evaluation (work = point pairs), 10 H-matrix GEMV (work = block
+ * bytes), 11 surrogate network (work = flops). */
 int ccab_profile_enable(ccab_ctx* ctx, int on);
 int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double* work,
                       int reset);
@@ -217,6 +219,31 @@ int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const ccab_c1
                           const ccab_c128* d_x0, const ccab_solver_cfg* cfg,
                           ccab_solve_stats* stats);
 
+/* ---- field points (device) --------------------------------------------------- */
+/* influence_matrix.rs:101-127 (get_dense_field_matrices) fused with
+ * solve.rs:173-185 (get_field), incident_wave.rs:66-100 (get_incident_field)
+ * and postprocess.rs:70-88 (convert to pressure), for nfreq frequencies of one
+ * mesh; M and L are never stored.  d_fps: 3 doubles per field point; d_phi,
+ * d_vn: n x nfreq surface solution; d_out[f*nfp + p] =
+ *   s_f ((want_total ? phi_inc : 0) - sum_j M_pj phi_fj + sum_j L_pj vn_fj),
+ * s_f = i omega_f rho if to_pressure else 1 (the reference's sign);
+ * d_inc (optional, nfp x nfreq) = s_f phi_inc.  w may be NULL when neither
+ * want_total nor d_inc asks for the incident field. */
+int ccab_field(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+               const double* d_cpts, int64_t n, const double* d_fps, int64_t nfp,
+               const ccab_c128* d_phi, const ccab_c128* d_vn, int want_total, int to_pressure,
+               ccab_c128* d_out, ccab_c128* d_inc);
+/* incident_wave.rs:66-100 (get_incident_field): the incident potential at
+ * nfp field points (d_out[nfp]) */
+int ccab_incident_field(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
+                        const double* d_fps, int64_t nfp, ccab_c128* d_out);
+/* influence_matrix.rs:101-127 (get_dense_field_matrices) -> (M, L), nfp x n
+ * column-major c128 with leading dimension ld (either pointer may be NULL);
+ * for callers that keep the reference's separate get_field step. */
+int ccab_dense_field_matrices(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                              int64_t n, const double* d_fps, int64_t nfp, ccab_c128* d_M,
+                              ccab_c128* d_L, int64_t ld);
+
 /* ---- hierarchical matrix (device) ---------------------------------------------- */
 /* src/solve/h_matrix.rs:127-148 (HMatrix::new_from) of the side matrix
  * a H + b G (side CCAB_LHS / CCAB_RHS, the entries of
@@ -284,6 +311,56 @@ int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
                     const double* d_cpts, int64_t n, const ccab_solver_cfg* cfg,
                     const ccab_c128* d_x0, ccab_c128* d_phi, ccab_c128* d_vn,
                     ccab_solve_stats* stats);
+
+/* ---- learned warm start (device; no reference counterpart) ------------------- */
+/* A pointwise neural operator: per collocation point, 32 features of the
+ * geometry (position about the body centroid, normal), the plane wave
+ * (direction) and ka -> 4 linear layers (32 -> W -> W -> W -> 2, SiLU) ->
+ * psi = phi / phi_inc, phi_inc = A/(i omega rho) e^{i k d.x}.  The initial
+ * guess is x0 = phi_inc psi for the rigid / normal-velocity unknown phi
+ * (solve.rs:44-52).  BF16 GEMMs on MFMA with FP32 accumulation, FP32 Adam.
+ * Parameters (FP32, flat): W1[W][32] b1[W] W2[W][W] b2[W] W3[W][W] b3[W]
+ * W4[2][W] b4[2], weights row-major (out x in).  Features: 32 bf16 per point
+ * (uint16 storage), point p = s*n + i for system s. */
+typedef struct ccab_surrogate ccab_surrogate;
+typedef struct {
+    int32_t width;         /* hidden width: multiple of 64 (<=0 -> 256) */
+    int32_t seed;          /* parameter initialisation */
+    double lr;             /* Adam step (<=0 -> 1e-3) */
+    double beta1, beta2;   /* <=0 -> 0.9, 0.999 */
+    double eps;            /* <=0 -> 1e-8 */
+    double weight_decay;   /* decoupled, weights only */
+} ccab_surrogate_cfg;
+
+int ccab_surrogate_create(ccab_ctx* ctx, const ccab_surrogate_cfg* cfg, ccab_surrogate** out);
+int ccab_surrogate_destroy(ccab_surrogate* s);
+int64_t ccab_surrogate_num_params(const ccab_surrogate* s);
+int ccab_surrogate_get_params(ccab_ctx* ctx, const ccab_surrogate* s, float* h_params);
+/* also resets the Adam moments and step count */
+int ccab_surrogate_set_params(ccab_ctx* ctx, ccab_surrogate* s, const float* h_params);
+int ccab_surrogate_set_lr(ccab_surrogate* s, double lr);
+/* features of nsys plane-wave systems on one mesh (ph[s], w[s] per system)
+ * into d_feat (nsys*n x 32 bf16); with d_phi (n x nsys, the solved surface
+ * potential) also the targets psi into d_target (nsys*n x 2 float). */
+int ccab_surrogate_features(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
+                            int64_t nsys, const double* d_cpts, int64_t n,
+                            const ccab_c128* d_phi, uint16_t* d_feat, float* d_target);
+/* mean over P points of |out - target|^2 and its gradient (no update) */
+int ccab_surrogate_loss_grad(ccab_ctx* ctx, ccab_surrogate* s, const uint16_t* d_feat,
+                             const float* d_target, int64_t P, double* loss, float* h_grad);
+/* one Adam step on the rows d_index[0..batch) of (d_feat, d_target);
+ * *loss (optional, synchronises) = the batch loss before the step */
+int ccab_surrogate_train_step(ccab_ctx* ctx, ccab_surrogate* s, const uint16_t* d_feat,
+                              const float* d_target, const int64_t* d_index, int64_t batch,
+                              double* loss);
+/* network output (P x 2 float) for P feature rows */
+int ccab_surrogate_forward(ccab_ctx* ctx, ccab_surrogate* s, const uint16_t* d_feat, int64_t P,
+                           float* d_out);
+/* initial guesses x0 (n x nsys) for nsys plane-wave systems on one mesh, for
+ * ccab_sweep_warm / ccab_gmres_ir_batched */
+int ccab_surrogate_predict(ccab_ctx* ctx, ccab_surrogate* s, const ccab_physics* ph,
+                           const ccab_wave* w, int64_t nsys, const double* d_cpts, int64_t n,
+                           ccab_c128* d_x0);
 
 #ifdef __cplusplus
 }

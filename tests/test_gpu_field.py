@@ -1,0 +1,114 @@
+"""GPU parity of the field-point evaluation (influence_matrix.rs:101-127,
+solve.rs:173-185, incident_wave.rs:66-100, postprocess.rs:70-88) against the
+oracle and the reference's field fixtures (verification.rs)."""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from cc_abstemious_amd import capi, incident_wave, influence_matrix, mesh, solve
+from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+P_CLASSICAL = -3.9380091293979903e-05 + 4.7605686656319906e-04j   # verification.rs:305-307
+P_BM = 1.2125591274707629e-04 + 4.814465829556038e-04j            # verification.rs:390-392
+
+
+def _note(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def _oph(pd, f):
+    return O.physics(f, pd.sound_speed, pd.mass_density, pd.method, pd.problem_type, pd.bc_type,
+                     pd.bc_value)
+
+
+def _ow(w):
+    return O.wave(w.kind, tuple(w.vec), tuple(w.amplitude))
+
+
+@pytest.mark.parametrize("method,expect", [(capi.CLASSICAL, P_CLASSICAL),
+                                           (capi.BURTON_MILLER, P_BM)])
+def test_field_fixture(method, expect):
+    """verification.rs:95-205: pressure at (10, 0, 0) after the LU solve."""
+    _note(f"field fixture method={method}")
+    m = mesh.read_vtk(os.path.join(GOLD, "sphere.vtk"), 3)
+    pd = PreData(m, [100.0], 1500.0, 1000.0, method, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                 (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, (1.0, 0.0, 0.0), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_DIRECT))
+    pd.set_field_points([[10.0, 0.0, 0.0]])
+    pd.output_total = False   # the fixtures evaluate the scattered field (verification.rs)
+    rhs = incident_wave.get_incident_surface(pd)
+    phi, vn = solve.solve_for_surface(pd, rhs)
+    sc, inc = solve.field(pd, phi, vn)
+    p = complex(sc[0, 0].cpu())
+    assert abs(p - expect) / abs(expect) < 1e-12
+
+
+@pytest.mark.parametrize("bc,bcv,kind,total", [
+    (capi.BC_NORMAL_VELOCITY, (1.0, 0.0), capi.WAVE_PLANE, True),
+    (capi.BC_PRESSURE, (0.5, -0.3), capi.WAVE_SPHERICAL, True),
+    (capi.BC_IMPEDANCE, (415.03, 83.006), capi.WAVE_PLANE, False),
+])
+def test_field_vs_oracle_batched(bc, bcv, kind, total):
+    m = mesh.icosphere(3)
+    ka = np.array([0.7, 2.0, 4.5])
+    freqs = ka * 343.0 / (2 * math.pi)
+    vec = (0.2, 0.3, -0.1) if kind == capi.WAVE_SPHERICAL else (0.0, 0.6, 0.8)
+    pd = PreData(m, freqs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR, bc, bcv,
+                 IncidentWave(kind, vec, (1.0, 0.25)), Solver(kind=capi.SOLVER_DIRECT))
+    rng = np.random.default_rng(7)
+    fps = rng.standard_normal((100, 3))
+    fps = 5.0 * fps / np.linalg.norm(fps, axis=1, keepdims=True)
+    pd.set_field_points(fps)
+    pd.output_total = total
+    _note(f"field vs oracle bc={bc} kind={kind} total={total} n={pd.get_num_eqn()} nfp=100 nf=3")
+    phi, vn, _ = solve.sweep(pd)
+    sc, inc = solve.field(pd, phi, vn)
+    for f in range(len(freqs)):
+        ph = _oph(pd, freqs[f])
+        w = _ow(pd.incident_wave)
+        inc_o = O.incident_field(ph, w, fps)
+        ref = O.field(ph, pd.cpts, fps, phi[f].cpu().numpy(), vn[f].cpu().numpy(), inc_o,
+                      want_total=total, to_pressure=True)
+        got = sc[f].cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+        omega_rho = 2 * math.pi * freqs[f] * 1.21
+        np.testing.assert_allclose(inc[f].cpu().numpy(), 1j * omega_rho * inc_o, rtol=1e-13,
+                                   atol=1e-16)
+
+
+def test_field_matrices_and_get_field():
+    """Explicit M, L (get_dense_field_matrices) and get_field agree with the
+    fused evaluation and the oracle's matrices."""
+    m = mesh.icosphere(2)
+    pd = PreData(m, [150.0], 343.0, 1.21, capi.CLASSICAL, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                 (1.0, 0.0), IncidentWave(capi.WAVE_PLANE, (1.0, 0.0, 0.0), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_DIRECT))
+    fps = np.array([[3.0, 0.0, 0.0], [0.0, -2.5, 1.0], [1.5, 1.5, 1.5], [0.0, 0.0, -4.0]])
+    pd.set_field_points(fps)
+    _note("field matrices")
+    M, L = influence_matrix.get_dense_field_matrices(pd)
+    ph = _oph(pd, 150.0)
+    n, nfp = pd.get_num_eqn(), fps.shape[0]
+    Mo = np.zeros((nfp, n), np.complex128, order="F")
+    Lo = np.zeros((nfp, n), np.complex128, order="F")
+    O.lib().orc_dense_field_matrices(ph, O._p(np.ascontiguousarray(pd.cpts)), n, O._p(fps), nfp,
+                                     O._p(Mo), O._p(Lo))
+    np.testing.assert_allclose(M.cpu().numpy(), Mo, rtol=1e-13, atol=1e-18)
+    np.testing.assert_allclose(L.cpu().numpy(), Lo, rtol=1e-13, atol=1e-18)
+    rhs = incident_wave.get_incident_surface(pd)
+    phi, vn = solve.solve_for_surface(pd, rhs)
+    inc = incident_wave.get_incident_field(pd)
+    np.testing.assert_allclose(inc.cpu().numpy(), O.incident_field(ph, _ow(pd.incident_wave), fps),
+                               rtol=1e-13)
+    pd.output_pressure = False
+    explicit = solve.get_field(pd, M, L, phi, vn, inc)
+    fused, _ = solve.field(pd, phi.reshape(1, -1), vn.reshape(1, -1))
+    np.testing.assert_allclose(fused[0].cpu().numpy(), explicit.cpu().numpy(), rtol=1e-12)

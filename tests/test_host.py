@@ -38,6 +38,7 @@ def test_struct_layouts():
     assert ct.sizeof(capi.GmresParams) == 32
     assert ct.sizeof(capi.SolveStats) == 56
     assert ct.sizeof(capi.SolverCfg) == 72
+    assert ct.sizeof(capi.SurrogateCfg) == 48
 
 
 @pytest.mark.parametrize("fname,body", [("sphere.vtk", 3), ("refined_sphere.vtk", 3),
