@@ -123,6 +123,54 @@ __global__ void copy_active_kernel(int64_t n, const c128* __restrict__ src, c128
     if (i < n) dst[b * n + i] = src[b * n + i];
 }
 
+// Warm start judged by the residual: with u = A x0 = b - r (r the FP64
+// residual of x0), alpha = (u^H b) / (u^H u) minimises ||b - alpha A x0||,
+// so the scaled guess is never worse than the zero start (alpha = 0 when u
+// is orthogonal to b).  Then x0 <- alpha x0, r <- b - alpha u.  One
+// workgroup per system, fixed-order reduction.
+__global__ __launch_bounds__(256) void warm_scale_kernel(int64_t n, const c128* __restrict__ Bv,
+                                                         c128* R, c128* X,
+                                                         const int32_t* __restrict__ active) {
+    __shared__ double red[3][4];
+    __shared__ c128 s_alpha;
+    const int64_t b = active[blockIdx.x];
+    const c128* bb = Bv + b * n;
+    c128* rr = R + b * n;
+    c128* xx = X + b * n;
+    double ure = 0.0, uim = 0.0, uu = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const c128 u = bb[i] - rr[i];
+        const c128 d = conj(u) * bb[i];
+        ure += d.re;
+        uim += d.im;
+        uu += norm_sqr(u);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ure += __shfl_xor(ure, off, 64);
+        uim += __shfl_xor(uim, off, 64);
+        uu += __shfl_xor(uu, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = ure;
+        red[1][threadIdx.x >> 6] = uim;
+        red[2][threadIdx.x >> 6] = uu;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const double c = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        const double d = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+        s_alpha = d > 0.0 ? c128{a / d, c / d} : c128{0.0, 0.0};
+    }
+    __syncthreads();
+    const c128 al = s_alpha;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const c128 u = bb[i] - rr[i];
+        xx[i] = al * xx[i];
+        rr[i] = bb[i] - al * u;
+    }
+}
+
 using cd = std::complex<double>;
 
 // Host complex LU solve (partial pivoting on |re|+|im|) for the small
@@ -465,6 +513,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c6
             CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n));
             for (int32_t b : active) nmv[b]++;
         }
+        if (cyc == 0 && X0)
+            hipLaunchKernelGGL(warm_scale_kernel, dim3(na), dim3(256), 0, s, n, Bv, R, X, act);
         CCAB_TRY(read_norms(R, n, active, rn));
         std::vector<int32_t> still;
         for (int32_t b : active) {

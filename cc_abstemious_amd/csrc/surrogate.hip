@@ -56,6 +56,7 @@ struct SurSys {
     c128 amp;
     double c0[3];
     double a;
+    double mean_area;
 };
 
 // ---------------------------------------------------------------------------
@@ -107,8 +108,11 @@ __global__ void feat_kernel(const SurSys* __restrict__ sys, int64_t nsys, const 
     f[23] = cos(q.k * (rl - dx));
     f[24] = sin(q.k * (rl - dx));
     f[25] = 1.0 / (1.0 + ka);
-    for (int t = 26; t < kFeat; ++t) f[t] = 0.0;
+    f[26] = soa[6 * n + i] / q.mean_area - 1.0;
+#pragma unroll
+    for (int t = 27; t < kFeat; ++t) f[t] = 0.0;
     uint16_t* o = feat + p * kFeat;
+#pragma unroll
     for (int t = 0; t < kFeat; ++t) o[t] = f2bf(float(f[t]));
     if (phi && target) {
         // psi = phi / (amp e^{i k d.x}), x absolute as in the incident wave
@@ -120,6 +124,22 @@ __global__ void feat_kernel(const SurSys* __restrict__ sys, int64_t nsys, const 
         target[2 * p] = float(psi.re);
         target[2 * p + 1] = float(psi.im);
     }
+}
+
+// targets of system s divided by their RMS over the n points: the network
+// learns the shape of psi; the solver fits amplitude and phase of the guess
+// to the residual
+__global__ __launch_bounds__(256) void normalize_targets_kernel(int64_t n, float* target) {
+    __shared__ double red[4];
+    float* t = target + int64_t(blockIdx.x) * n * 2;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < 2 * n; i += 256) s += double(t[i]) * double(t[i]);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    const double tot = red[0] + red[1] + red[2] + red[3];
+    const float inv = tot > 0.0 ? float(1.0 / sqrt(tot / double(n))) : 0.0f;
+    for (int64_t i = threadIdx.x; i < 2 * n; i += 256) t[i] *= inv;
 }
 
 // x0 = amp e^{i k d.x} (out0 + i out1)
@@ -620,6 +640,10 @@ int stage_sys(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w, int64_t
         for (int d = 0; d < 3; ++d) a2 += (h[8 * i + d] - c0[d]) * (h[8 * i + d] - c0[d]);
     const double a = std::sqrt(a2 / double(n));
     CCAB_CHECK(a > 0.0, "degenerate mesh (zero radius)");
+    double area = 0.0;
+    for (int64_t i = 0; i < n; ++i) area += h[8 * i + 6];
+    area /= double(n);
+    CCAB_CHECK(area > 0.0, "degenerate mesh (zero area)");
     std::vector<SurSys> hs(nsys);
     for (int64_t s = 0; s < nsys; ++s) {
         CCAB_TRY(check_physics(ctx, ph[s]));
@@ -637,6 +661,7 @@ int stage_sys(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w, int64_t
         q.amp = cdiv(c128{w[s].amplitude[0], w[s].amplitude[1]},
                      c128{0.0, p.omega * ph[s].mass_density});
         q.a = a;
+        q.mean_area = area;
     }
     *d_sys = static_cast<SurSys*>(ctx_buf(ctx, SLOT_SUR_SYS, sizeof(SurSys) * nsys));
     if (!*d_sys) return CCAB_ENOMEM;
@@ -743,6 +768,9 @@ int ccab_surrogate_features(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wa
     CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts, n, soa));
     hipLaunchKernelGGL(feat_kernel, grid1d(nsys * n), dim3(256), 0, ctx->stream, sys, nsys, soa, n,
                        d_phi, d_feat, d_target);
+    if (d_target)
+        hipLaunchKernelGGL(normalize_targets_kernel, dim3((unsigned)nsys), dim3(256), 0, ctx->stream,
+                           n, d_target);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

@@ -213,7 +213,9 @@ int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda,
  * ||b - A x|| <= tol ||b||.  Parameters from cfg: gmres.tolerance = tol (<=0 ->
  * 1e-10), gmres.restart = m (0 -> 50), ir_tolerance = inner_tol (<=0 -> 1e-4),
  * ir_max_steps = maximum cycles (<=0 -> 20), gmres.ref_mgs = Arnoldi
- * coefficient.  d_x: b on entry, x on exit (n x batch); d_x0 optional warm start. */
+ * coefficient.  d_x: b on entry, x on exit (n x batch); d_x0 optional warm start,
+ * first scaled by the complex alpha minimising the FP64 residual
+ * ||b - alpha A x0|| (so a poor guess is never worse than the zero start). */
 int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch, const ccab_c128* d_A128,
                           const ccab_c64* d_A64, int64_t lda, int64_t strideA, ccab_c128* d_x,
                           const ccab_c128* d_x0, const ccab_solver_cfg* cfg,
@@ -316,9 +318,11 @@ int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
 /* A pointwise neural operator: per collocation point, 32 features of the
  * geometry (position about the body centroid, normal), the plane wave
  * (direction) and ka -> 4 linear layers (32 -> W -> W -> W -> 2, SiLU) ->
- * psi = phi / phi_inc, phi_inc = A/(i omega rho) e^{i k d.x}.  The initial
- * guess is x0 = phi_inc psi for the rigid / normal-velocity unknown phi
- * (solve.rs:44-52).  BF16 GEMMs on MFMA with FP32 accumulation, FP32 Adam.
+ * psi = phi / phi_inc, phi_inc = A/(i omega rho) e^{i k d.x}, normalised to
+ * unit RMS per system (the shape of the solution).  The initial guess is
+ * x0 = phi_inc psi for the rigid / normal-velocity unknown phi
+ * (solve.rs:44-52); ccab_gmres_ir_batched fits its complex scale to the
+ * FP64 residual.  BF16 GEMMs on MFMA with FP32 accumulation, FP32 Adam.
  * Parameters (FP32, flat): W1[W][32] b1[W] W2[W][W] b2[W] W3[W][W] b3[W]
  * W4[2][W] b4[2], weights row-major (out x in).  Features: 32 bf16 per point
  * (uint16 storage), point p = s*n + i for system s. */
@@ -341,7 +345,8 @@ int ccab_surrogate_set_params(ccab_ctx* ctx, ccab_surrogate* s, const float* h_p
 int ccab_surrogate_set_lr(ccab_surrogate* s, double lr);
 /* features of nsys plane-wave systems on one mesh (ph[s], w[s] per system)
  * into d_feat (nsys*n x 32 bf16); with d_phi (n x nsys, the solved surface
- * potential) also the targets psi into d_target (nsys*n x 2 float). */
+ * potential) also the targets psi / rms_s(psi) into d_target (nsys*n x 2
+ * float). */
 int ccab_surrogate_features(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
                             int64_t nsys, const double* d_cpts, int64_t n,
                             const ccab_c128* d_phi, uint16_t* d_feat, float* d_target);

@@ -3,7 +3,8 @@
 The surrogate has no counterpart in the reference (its GMRES starts from
 zero, gmres.rs:38-39); this module is the mathematical definition the HIP
 kernels are checked against:
-  * features of a collocation point under a plane wave (32 per point),
+  * features of a collocation point under a plane wave (32 per point) and
+    the training targets (psi = phi / phi_inc at unit RMS per system),
   * the network 32 -> W -> W -> W -> 2 with SiLU, its MSE loss and the
     analytic gradient (pinned by central finite differences in the tests),
   * one Adam step (decoupled weight decay on weights only).
@@ -85,6 +86,7 @@ def features(cpts, sp):
     f[:, 23] = np.cos(k * (rl - dx))
     f[:, 24] = np.sin(k * (rl - dx))
     f[:, 25] = 1.0 / (1.0 + ka)
+    f[:, 26] = cpts[:, 6] / cpts[:, 6].mean() - 1.0
     return f
 
 
@@ -92,9 +94,12 @@ def incident(cpts, sp):
     return sp["amp"] * np.exp(1j * sp["k"] * (cpts[:, 0:3] @ sp["d"]))
 
 
-def targets(cpts, sp, phi):
-    """psi = phi / phi_inc as (n, 2) (re, im)."""
+def targets(cpts, sp, phi, normalize=True):
+    """psi = phi / phi_inc as (n, 2) (re, im), divided by its RMS over the
+    system's points when normalize."""
     psi = np.asarray(phi) / incident(cpts, sp)
+    if normalize:
+        psi = psi / np.sqrt(np.mean(np.abs(psi) ** 2))
     return np.stack([psi.real, psi.imag], axis=1)
 
 

@@ -90,3 +90,11 @@ def test_features_geometry_identities():
     # the target of the incident field itself is psi = 1
     t = S.targets(cp, sp, S.incident(cp, sp))
     np.testing.assert_allclose(t, np.tile([1.0, 0.0], (cp.shape[0], 1)), atol=1e-14)
+    # normalised targets: unit RMS and invariant to the solution's scale
+    phi = S.incident(cp, sp) * (1.0 + 0.3 * np.cos(3 * f[:, 12]))
+    t1, t2 = S.targets(cp, sp, phi), S.targets(cp, sp, (2.0 - 1.5j) * phi)
+    assert abs(np.mean(np.sum(t1 ** 2, axis=1)) - 1.0) < 1e-12
+    np.testing.assert_allclose(np.abs(t1[:, 0] + 1j * t1[:, 1]), np.abs(t2[:, 0] + 1j * t2[:, 1]),
+                               rtol=1e-12)
+    # area feature: relative element area
+    np.testing.assert_allclose(f[:, 26], cp[:, 6] / cp[:, 6].mean() - 1.0, atol=1e-15)
