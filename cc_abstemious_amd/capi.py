@@ -117,6 +117,7 @@ SIGNATURES = {
     "ccab_profile_enable": (_I, [_P, _I]),
     "ccab_profile_read": (_I, [_P, _I, _P, _P, _P, _I]),
     "ccab_set_tuning": (_I, [_P, _I, _I, _I]),
+    "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_incident_surface": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
     "ccab_dense_surface_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _P, _P, _I64]),
@@ -247,6 +248,18 @@ class Context:
 
     def call(self, name: str, *args):
         self.check(getattr(self._L, name)(self.handle, *args))
+
+    def autotune(self, prec: int, n: int, batch: int = 1):
+        """Pick the fastest LU GEMM variant and outer panel width by timing;
+        returns (variant, outer, {(outer, variant): ms})."""
+        v, o = ct.c_int(), ct.c_int()
+        tab = (ct.c_double * 16)()
+        self.call("ccab_autotune", int(prec), int(n), int(batch), ct.byref(v), ct.byref(o), tab)
+        times = {(64 if i < 8 else 128, i % 8): tab[i] for i in range(16)}
+        return v.value, o.value, times
+
+    def set_tuning(self, prec: int, variant: int, outer: int):
+        self.call("ccab_set_tuning", int(prec), int(variant), int(outer))
 
     def synchronize(self):
         self.check(self._L.ccab_synchronize(self.handle))

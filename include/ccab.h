@@ -129,8 +129,15 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
 /* LU tuning (no reference counterpart): the complex-GEMM variant for
  * precision prec (bit 0 XCD-aware tile order, bit 1 LDS-staged epilogue,
  * bit 2 128-row tiles) and the outer panel width (64 or 128) that sets the
- * K of the trailing update.  ccab_autotune picks both by measurement. */
+ * K of the trailing update. */
 int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer);
+/* The autotuner: times the batched LU of `batch` synthetic n x n systems in
+ * precision prec for every GEMM variant (0..7) and outer width (64, 128),
+ * keeps the fastest in the context (as ccab_set_tuning) and returns it;
+ * ms_table (optional, 16 doubles) receives the time of variant v at width 64
+ * (index v) and 128 (index 8 + v).  Factors do not depend on the variant. */
+int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, int* best_variant,
+                  int* best_outer, double* ms_table);
 
 /* ---- geometry (host) --------------------------------------------------- */
 /* One centroid collocation point per linear tri/quad element:

@@ -1,0 +1,54 @@
+"""The LU autotuner: it returns the fastest measured configuration, and the
+factors do not depend on the GEMM variant it picks (results invariant to
+tile choice); the outer panel width changes only the rounding."""
+import numpy as np
+import pytest
+import torch
+
+from cc_abstemious_amd import capi, solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("prec", [capi.C64, capi.C128])
+def test_autotune_picks_fastest(prec):
+    _note(f"autotune prec={prec} n=1536 batch=2")
+    ctx = capi.Context()
+    v, o, times = ctx.autotune(prec, 1536, 2)
+    assert 0 <= v < 8 and o in (64, 128)
+    assert all(np.isfinite(t) and t > 0 for t in times.values())
+    assert times[(o, v)] == min(times.values())
+    print({k: round(t, 3) for k, t in times.items()})
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", [capi.C64, capi.C128])
+def test_factors_invariant_to_gemm_variant(prec):
+    n, batch = 700, 2
+    _note(f"LU factors vs GEMM variant prec={prec} n={n}")
+    dt = torch.complex64 if prec == capi.C64 else torch.complex128
+    g = torch.Generator(device="cpu").manual_seed(5)
+    A0 = (torch.randn(batch, n, n, dtype=torch.complex128, generator=g)
+          + n ** 0.5 * torch.eye(n, dtype=torch.complex128)).to(dt).cuda()
+    ctx = capi.Context()
+    ref = None
+    for outer in (128, 64):
+        for v in range(8):
+            ctx.set_tuning(prec, v, outer)
+            A = A0.clone().transpose(1, 2).contiguous().transpose(1, 2)  # column-major per system
+            piv = solve.lu_factor_batched(A, ctx=ctx)
+            LU = A
+            if ref is None:
+                ref = (LU.clone(), piv.clone())
+                continue
+            assert torch.equal(piv, ref[1]), (outer, v)
+            if outer == 128:
+                assert torch.equal(LU, ref[0]), (outer, v)
+            else:
+                e = float((LU - ref[0]).abs().max() / ref[0].abs().max())
+                assert e < (1e-5 if prec == capi.C64 else 1e-13), (outer, v, e)
+    ctx.close()
