@@ -17,7 +17,7 @@ EXTERIOR, INTERIOR = 0, 1
 BC_PRESSURE, BC_NORMAL_VELOCITY, BC_IMPEDANCE = 0, 1, 2
 WAVE_PLANE, WAVE_SPHERICAL = 0, 1
 LHS, RHS = 0, 1
-C128, C64 = 0, 1
+C128, C64, C32 = 0, 1, 2
 SOLVER_DIRECT, SOLVER_ITERATIVE, SOLVER_HIERARCHICAL = 0, 1, 2
 SOLVER_DIRECT_MIXED, SOLVER_ITERATIVE_MIXED = 3, 4
 STOP_BACKWARD_ERROR, STOP_RELATIVE_RESIDUAL = 0, 1

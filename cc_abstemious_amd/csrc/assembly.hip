@@ -21,17 +21,26 @@ namespace {
 constexpr int kRows = 256;  // rows per workgroup (4 wavefronts)
 constexpr int kColChunk = 64;
 
+// FP16 with saturation at the largest finite half
+__device__ __forceinline__ uint16_t to_half_sat(double v) {
+    const float f = fminf(fmaxf(float(v), -65504.0f), 65504.0f);
+    return __half_as_ushort(__float2half_rn(f));
+}
+
 template <typename T>
-__device__ __forceinline__ void store(void* base, int64_t off, c128 v) {
+__device__ __forceinline__ void store(void* base, int64_t off, c128 v, double h_scale = 1.0) {
     if constexpr (sizeof(T) == 16) {
         reinterpret_cast<c128*>(base)[off] = v;
-    } else {
+    } else if constexpr (sizeof(T) == 8) {
         reinterpret_cast<c64*>(base)[off] = c64{static_cast<float>(v.re), static_cast<float>(v.im)};
+    } else {
+        reinterpret_cast<c32*>(base)[off] = c32{to_half_sat(v.re * h_scale), to_half_sat(v.im * h_scale)};
     }
 }
 
 // MODE ASM_HG: out0 = H, out1 = G (either may be null), element type OutT.
-// MODE ASM_LHS: out0 = LHS c128 (or null), out1 = LHS c64 (or null).
+// MODE ASM_LHS: out0 = LHS c128 (or null), out1 = LHS as OutT (c64, or c32
+// scaled by p.h_scale; or null).
 template <typename T, bool BM, int MODE, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __restrict__ params,
                                                          const double* __restrict__ soa_base,
@@ -73,7 +82,7 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
         } else {
             const c128 L = p.lhs_a * H + p.lhs_b * G;
             if (out0) store<c128>(out0, off, L);
-            if (out1) store<c64>(out1, off, L);
+            if (out1) store<OutT>(out1, off, L, p.h_scale);
             if (p.rowsum_of == 1) {
                 rs_re += G.re;
                 rs_im += G.im;
@@ -234,7 +243,7 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         const double e = prec_compute == PREC_F64 ? 16.0 : 8.0;
         bytes_per_entry = (out0 ? e : 0.0) + (out1 ? e : 0.0);
     } else {
-        bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? 8.0 : 0.0);
+        bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? (prec_compute == PREC_F64_H ? 4.0 : 8.0) : 0.0);
     }
     ProfScope ps(ctx, PROF_ASSEMBLY, double(n) * double(n) * double(batch) * bytes_per_entry);
     if (mode == ASM_HG) {
@@ -250,9 +259,12 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
             CCAB_CHECK(out0 == nullptr, "FP32 assembly writes only the c64 LHS");
             if (bm) launch_one<float, true, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
             else launch_one<float, false, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+        } else if (prec_compute == PREC_F64_H) {
+            if (bm) launch_one<double, true, ASM_LHS, c32>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+            else launch_one<double, false, ASM_LHS, c32>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
         } else {
-            if (bm) launch_one<double, true, ASM_LHS, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
-            else launch_one<double, false, ASM_LHS, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+            if (bm) launch_one<double, true, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
+            else launch_one<double, false, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
         }
     }
     CCAB_HIP(hipGetLastError());

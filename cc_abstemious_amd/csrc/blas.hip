@@ -8,6 +8,8 @@
 // of 256 lanes owns 256 rows; lanes walk a chunk of columns whose x[j] is
 // wave-uniform.  Column chunks are spread over grid.y so that even a single
 // system fills the 256 CUs; partial sums are combined in fixed order.
+#include <cstdint>
+
 #include "ccab_internal.h"
 
 namespace {
@@ -22,6 +24,12 @@ template <>
 __device__ __forceinline__ c128 ld_c<c64>(const c64* p, int64_t off) {
     const c64 v = p[off];
     return c128{double(v.re), double(v.im)};
+}
+template <>
+__device__ __forceinline__ c128 ld_c<c32>(const c32* p, int64_t off) {
+    const c32 v = p[off];
+    return c128{double(__half2float(__ushort_as_half(v.re))),
+                double(__half2float(__ushort_as_half(v.im)))};
 }
 
 template <typename TA>
@@ -64,15 +72,109 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
         c128{(re[0] + re[1]) + (re[2] + re[3]), (im[0] + im[1]) + (im[2] + im[3])};
 }
 
+// R consecutive rows per lane with one 16-byte load per column (R = 2 for
+// c64, 4 for c32); needs lda, sA and the row start multiples of R
+template <typename TA, int R>
+struct Vec16;
+template <>
+struct Vec16<c64, 2> {
+    using type = float4;
+};
+template <>
+struct Vec16<c32, 4> {
+    using type = uint4;
+};
+
+template <int R>
+__device__ __forceinline__ void unpack16(const float4& v, c128* a) {
+    a[0] = c128{double(v.x), double(v.y)};
+    a[1] = c128{double(v.z), double(v.w)};
+}
+__device__ __forceinline__ double hf(uint32_t bits16) {
+    return double(__half2float(__ushort_as_half(uint16_t(bits16))));
+}
+template <int R>
+__device__ __forceinline__ void unpack16(const uint4& v, c128* a) {
+    a[0] = c128{hf(v.x & 0xffffu), hf(v.x >> 16)};
+    a[1] = c128{hf(v.y & 0xffffu), hf(v.y >> 16)};
+    a[2] = c128{hf(v.z & 0xffffu), hf(v.z >> 16)};
+    a[3] = c128{hf(v.w & 0xffffu), hf(v.w >> 16)};
+}
+
+template <typename TA, int R>
+__global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N, const TA* __restrict__ A,
+                                                            int64_t lda, int64_t sA,
+                                                            const c128* __restrict__ x, int64_t sx,
+                                                            int64_t chunk, c128* __restrict__ part,
+                                                            int64_t nsplit,
+                                                            const int32_t* __restrict__ active) {
+    using V = typename Vec16<TA, R>::type;
+    const int64_t z = blockIdx.z;
+    const int64_t b = active ? active[z] : z;
+    const int64_t i0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) * R;
+    const int64_t j0 = int64_t(blockIdx.y) * chunk;
+    const int64_t j1 = min(j0 + chunk, N);
+    if (i0 >= M) return;
+    const TA* a = A + b * sA + i0;
+    const c128* xb = x + b * sx;
+    double re[R], im[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) re[r] = im[r] = 0.0;
+    if (i0 + R <= M) {
+        int64_t j = j0;
+        for (; j + 3 < j1; j += 4) {
+            V v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const V*>(a + (j + u) * lda);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                c128 av[R];
+                unpack16<R>(v[u], av);
+                const c128 xv = xb[j + u];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    re[r] += av[r].re * xv.re - av[r].im * xv.im;
+                    im[r] += av[r].re * xv.im + av[r].im * xv.re;
+                }
+            }
+        }
+        for (; j < j1; ++j) {
+            c128 av[R];
+            unpack16<R>(*reinterpret_cast<const V*>(a + j * lda), av);
+            const c128 xv = xb[j];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                re[r] += av[r].re * xv.re - av[r].im * xv.im;
+                im[r] += av[r].re * xv.im + av[r].im * xv.re;
+            }
+        }
+    } else {  // last rows of the matrix
+        for (int64_t j = j0; j < j1; ++j) {
+            const c128 xv = xb[j];
+            for (int r = 0; r < R && i0 + r < M; ++r) {
+                const c128 av = ld_c(a + r, j * lda);
+                re[r] += av.re * xv.re - av.im * xv.im;
+                im[r] += av.re * xv.im + av.im * xv.re;
+            }
+        }
+    }
+    c128* o = part + (z * nsplit + blockIdx.y) * M + i0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (i0 + r < M) o[r] = c128{re[r], im[r]};
+}
+
 __global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int64_t nsplit,
                                    c128 alpha, c128 beta, c128* y, int64_t sy,
-                                   const int32_t* __restrict__ active) {
+                                   const int32_t* __restrict__ active,
+                                   const double* __restrict__ inv_scale) {
     const int64_t z = blockIdx.y;
     const int64_t b = active ? active[z] : z;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
     c128 s{0.0, 0.0};
     for (int64_t p = 0; p < nsplit; ++p) s += part[(z * nsplit + p) * M + i];
+    if (inv_scale) s = s * inv_scale[b];
     c128* yy = y + b * sy + i;
     const bool zero_beta = beta.re == 0.0 && beta.im == 0.0;
     *yy = alpha * s + (zero_beta ? c128{0.0, 0.0} : beta * *yy);
@@ -141,9 +243,16 @@ size_t gemv_workspace_bytes(int64_t n, int64_t batch) { return 0; }
 int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
-                int64_t stridey) {
+                int64_t stridey, const double* d_inv_scale) {
     if (M <= 0 || nact <= 0) return CCAB_OK;
-    const int64_t rowblocks = (M + 255) / 256;
+    if (prec == CCAB_C32 && !d_inv_scale) return ctx_fail(ctx, CCAB_EINVAL, "c32 GEMV needs scales");
+    // rows per lane: one 16-byte load per column where the layout allows it
+    int R = prec == CCAB_C32 ? 4 : prec == CCAB_C64 ? 2 : 1;
+    if (lda % R != 0 || strideA % R != 0 ||
+        (reinterpret_cast<uintptr_t>(d_A) % 16) != 0)
+        R = 1;
+    const int64_t rowblocks = (M + 256 * R - 1) / (256 * R);
+    const int64_t finish_blocks = (M + 255) / 256;
     int64_t nsplit = (1024 + rowblocks * nact - 1) / (rowblocks * nact);
     nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
     const int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
@@ -152,11 +261,24 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     if (!part) return CCAB_ENOMEM;
     dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)nact);
     ProfScope ps(ctx, PROF_GEMV,
-                 double(M) * double(N) * double(nact) * (prec == CCAB_C128 ? 16.0 : 8.0));
+                 double(M) * double(N) * double(nact) *
+                     (prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
     if (N > 0) {
         if (prec == CCAB_C128)
             hipLaunchKernelGGL(gemv_part_kernel<c128>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active);
+        else if (prec == CCAB_C32 && R == 4)
+            hipLaunchKernelGGL((gemv_part_vec_kernel<c32, 4>), grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c32*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active);
+        else if (prec == CCAB_C64 && R == 2)
+            hipLaunchKernelGGL((gemv_part_vec_kernel<c64, 2>), grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active);
+        else if (prec == CCAB_C32)
+            hipLaunchKernelGGL(gemv_part_kernel<c32>, grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c32*>(d_A), lda, strideA, d_x, stridex, chunk,
                                part, nsplit, d_active);
         else
             hipLaunchKernelGGL(gemv_part_kernel<c64>, grid, dim3(256), 0, ctx->stream, M, N,
@@ -165,8 +287,9 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     } else {
         nsplit = 0;
     }
-    hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)rowblocks, (unsigned)nact), dim3(256),
-                       0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active);
+    hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)finish_blocks, (unsigned)nact), dim3(256),
+                       0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active,
+                       prec == CCAB_C32 ? d_inv_scale : nullptr);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

@@ -6,6 +6,7 @@
 #include <memory>
 
 #include "ccab_internal.h"
+#include "greens.h"
 
 int gmres_run(ccab_ctx* ctx, int64_t n, int prec, const void* d_A, int64_t lda, c128* d_x,
               const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st);
@@ -13,9 +14,10 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
               int64_t strideA, c128* d_x, double ir_tol, int ir_max, ccab_solve_stats* st);
 int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const void* A,
                  int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc);
-int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c64* A64,
-                 int64_t lda, int64_t sA, c128* X, const c128* X0, int m, double inner_tol,
-                 double tol, int max_cycles, bool ref_mgs, ccab_solve_stats* st);
+int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
+                 int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
+                 const c128* X0, int m, double inner_tol, double tol, int max_cycles,
+                 bool ref_mgs, ccab_solve_stats* st);
 
 // ---------------------------------------------------------------------------
 // context plumbing
@@ -137,6 +139,7 @@ SysParams make_params(const ccab_physics& ph) {
         p.rhs_scale = {0, 0};
         p.rhs_inc_sign = -1.0;
     }
+    p.h_scale = 1.0;
     return p;
 }
 
@@ -607,7 +610,8 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     else if (solver == CCAB_SOLVER_DIRECT_MIXED) per = n2 * 24;
     else if (solver == CCAB_SOLVER_ITERATIVE)
         per = n2 * (cfg->gmres.matrix_prec == CCAB_C64 ? 8 : 16);
-    else if (solver == CCAB_SOLVER_ITERATIVE_MIXED) per = n2 * 24;
+    else if (solver == CCAB_SOLVER_ITERATIVE_MIXED)
+        per = n2 * (cfg->gmres.matrix_prec == CCAB_C32 ? 20 : 24);
     else return ctx_fail(ctx, CCAB_EINVAL, "solver %d not available on the dense path", solver);
     size_t budget = size_t(96) << 30;
     size_t freemem = 0, total = 0;
@@ -629,10 +633,38 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         A = ctx_buf(ctx, SLOT_SOLVE_A, size_t(chunk) * n2 * 16);
         if (!A) return CCAB_ENOMEM;
     }
+    const bool half_op = solver == CCAB_SOLVER_ITERATIVE_MIXED && cfg->gmres.matrix_prec == CCAB_C32;
     if (solver == CCAB_SOLVER_DIRECT_MIXED || solver == CCAB_SOLVER_ITERATIVE_MIXED ||
         (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C64)) {
-        A64 = static_cast<c64*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(chunk) * n2 * 8));
+        // c64 copy, or the scaled c32 copy for the half-precision GMRES operator
+        A64 = static_cast<c64*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(chunk) * n2 * (half_op ? 4 : 8)));
         if (!A64) return CCAB_ENOMEM;
+    }
+    double* inv_scale = nullptr;
+    if (half_op) {
+        // FP16 scale per system: the largest diagonal entry (which bounds the
+        // near-singular entries of the row) maps to about 2^12, 16x below the
+        // largest finite half
+        std::vector<double> h(size_t(n) * 8);
+        CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts, size_t(n) * 64, hipMemcpyDeviceToHost, ctx->stream));
+        CCAB_HIP(hipStreamSynchronize(ctx->stream));
+        std::vector<double> isc(nfreq);
+        for (int64_t f = 0; f < nfreq; ++f) {
+            double mx = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                c128 G, H;
+                diag_entry(hp[f], h[8 * i + 6], h[8 * i + 7], G, H);
+                const c128 L = hp[f].lhs_a * H + hp[f].lhs_b * G;
+                mx = std::max(mx, std::sqrt(norm_sqr(L)));
+            }
+            CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
+            hp[f].h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
+            isc[f] = 1.0 / hp[f].h_scale;
+        }
+        CCAB_HIP(hipMemcpy(dp, hp.data(), sizeof(SysParams) * nfreq, hipMemcpyHostToDevice));
+        inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nfreq) * 8));
+        if (!inv_scale) return CCAB_ENOMEM;
+        CCAB_HIP(hipMemcpy(inv_scale, isc.data(), size_t(nfreq) * 8, hipMemcpyHostToDevice));
     }
     int32_t* piv = nullptr;
     if (solver == CCAB_SOLVER_DIRECT) {
@@ -642,8 +674,9 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     for (int64_t f0 = 0; f0 < nfreq; f0 += chunk) {
         const int64_t nb = std::min(chunk, nfreq - f0);
         int64_t nch = 0;
-        CCAB_TRY(launch_assemble(ctx, dp + f0, hp.data() + f0, nb, soa, n, PREC_F64, ASM_LHS, A,
-                                 A64, n, int64_t(n2), part, &nch));
+        CCAB_TRY(launch_assemble(ctx, dp + f0, hp.data() + f0, nb, soa, n,
+                                 half_op ? PREC_F64_H : PREC_F64, ASM_LHS, A, A64, n, int64_t(n2),
+                                 part, &nch));
         ccab_c128* x = rhs_c + f0 * n;
         CCAB_TRY(launch_rhs_finish(ctx, dp + f0, nb, n, part, nch, x, x));
         if (solver == CCAB_SOLVER_DIRECT) {
@@ -658,8 +691,9 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
                 }
         } else if (solver == CCAB_SOLVER_ITERATIVE_MIXED) {
             const c128* x0 = d_x0 ? reinterpret_cast<const c128*>(d_x0 + f0 * n) : nullptr;
-            CCAB_TRY(gmres_ir_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
-                                  reinterpret_cast<c128*>(x), x0, cfg->gmres.restart,
+            CCAB_TRY(gmres_ir_run(ctx, n, nb, static_cast<const c128*>(A), A64,
+                                  half_op ? CCAB_C32 : CCAB_C64, half_op ? inv_scale + f0 : nullptr,
+                                  n, int64_t(n2), reinterpret_cast<c128*>(x), x0, cfg->gmres.restart,
                                   cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
                                   cfg->gmres.ref_mgs != 0, stats ? stats + f0 : nullptr));
         } else if (solver == CCAB_SOLVER_DIRECT_MIXED) {
@@ -712,8 +746,8 @@ extern "C" int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch,
     if (!ctx) return CCAB_EINVAL;
     CCAB_CHECK(d_A64 && d_x && cfg && n > 0 && batch > 0 && lda >= n, "invalid arguments");
     CCAB_CHECK(batch == 1 || strideA >= lda * n, "stride too small");
-    return gmres_ir_run(ctx, n, batch, reinterpret_cast<const c128*>(d_A128),
-                        reinterpret_cast<const c64*>(d_A64), lda, strideA,
+    return gmres_ir_run(ctx, n, batch, reinterpret_cast<const c128*>(d_A128), d_A64, CCAB_C64,
+                        nullptr, lda, strideA,
                         reinterpret_cast<c128*>(d_x), reinterpret_cast<const c128*>(d_x0),
                         cfg->gmres.restart, cfg->ir_tolerance, cfg->gmres.tolerance,
                         cfg->ir_max_steps, cfg->gmres.ref_mgs != 0, stats);

@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP translation units of libccab.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 
 #include <cstdarg>
 #include <cstdint>
@@ -20,6 +21,11 @@ struct Cx {
 };
 using c128 = Cx<double>;
 using c64 = Cx<float>;
+// complex FP16 storage (IEEE half bits), scaled per system: the GMRES-IR
+// operator when half the c64 traffic is wanted
+struct c32 {
+    uint16_t re, im;
+};
 
 template <typename T>
 __host__ __device__ __forceinline__ Cx<T> cx(T r, T i) {
@@ -100,7 +106,7 @@ enum BufSlot {
     SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
-    SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_COUNT
+    SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_COUNT
 };
 struct ccab_ctx {
     int device = 0;
@@ -210,6 +216,7 @@ struct SysParams {
     int32_t bm;
     int32_t rowsum_of;   // 0 none, 1 sum of G, 2 sum of H
     double rhs_inc_sign; // sign applied to rhs_inc
+    double h_scale;      // factor of the FP16 (c32) copy of the LHS (power of two)
 };
 SysParams make_params(const ccab_physics& ph);
 int check_physics(ccab_ctx* ctx, const ccab_physics& ph);
@@ -235,7 +242,9 @@ int launch_rowcol(ccab_ctx* ctx, const SysParams& p, const double* d_soa, int64_
                   ccab_c128* d_out);
 
 enum { ASM_HG = 0, ASM_LHS = 1 };
-enum { PREC_F64 = 0, PREC_F32 = 1, PREC_MIXED = 2 };
+// assembly arithmetic / second-output type: FP64 (+ c64 copy), FP32 (c64),
+// FP64 with a scaled FP16 (c32) copy of the LHS
+enum { PREC_F64 = 0, PREC_F32 = 1, PREC_MIXED = 2, PREC_F64_H = 3 };
 
 // lu.hip
 int lu_factor(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, void* d_A, int64_t lda,
@@ -251,10 +260,12 @@ int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* 
                  c128 beta, c128* d_y, int64_t stridey);
 size_t gemv_workspace_bytes(int64_t n, int64_t batch);
 // the same over the systems listed in d_active (device int32[nact])
+// prec CCAB_C32: d_inv_scale[b] multiplies system b's product (its matrix is
+// stored scaled by 1 / d_inv_scale[b])
 int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
-                int64_t stridey);
+                int64_t stridey, const double* d_inv_scale = nullptr);
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

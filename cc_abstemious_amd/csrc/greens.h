@@ -50,8 +50,8 @@ __device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx,
 
 // Singular diagonal (influence_matrix.rs:42-50) incl. the global shifts of
 // get_dense_surface_matrices (influence_matrix.rs:78-79).  Always FP64.
-__device__ __forceinline__ void diag_entry(const SysParams& p, double area, double wt, c128& g,
-                                           c128& h) {
+__host__ __device__ __forceinline__ void diag_entry(const SysParams& p, double area, double wt,
+                                                    c128& g, c128& h) {
     const double b = sqrt(area / kPi);
     const double wa = wt * area;
     g = c128{1.0 / (2.0 * kPi * b) * wa, 0.0};

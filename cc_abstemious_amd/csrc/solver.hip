@@ -445,9 +445,12 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 // rotations of every system on the host; converged systems leave the active
 // set.  Stops when ||b - A x|| <= tol ||b|| (FP64).
 // ---------------------------------------------------------------------------
-int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c64* A64,
-                 int64_t lda, int64_t sA, c128* X, const c128* X0, int m, double inner_tol,
-                 double tol, int max_cycles, bool ref_mgs, ccab_solve_stats* st) {
+int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
+                 int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
+                 const c128* X0, int m, double inner_tol, double tol, int max_cycles,
+                 bool ref_mgs, ccab_solve_stats* st) {
+    if (op_prec == CCAB_C32 && (!A128 || !d_inv_scale))
+        return ctx_fail(ctx, CCAB_EINVAL, "the FP16 operator needs the c128 matrix and scales");
     if (m <= 0) m = 50;
     m = (int)std::min<int64_t>(m, n);
     if (inner_tol <= 0) inner_tol = 1e-4;
@@ -472,7 +475,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c6
     int32_t* hAct = reinterpret_cast<int32_t*>(pin + pin_h + pin_y);
     double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + size_t(B) * 8);
     const c128 one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
-    const void* Ares = A128 ? static_cast<const void*>(A128) : static_cast<const void*>(A64);
+    const void* Ares = A128 ? static_cast<const void*>(A128) : Aop;
     const int pres = A128 ? CCAB_C128 : CCAB_C64;
 
     hipLaunchKernelGGL(copy_c128_kernel, grid1(nb), dim3(256), 0, s, nb, X, Bv);
@@ -543,8 +546,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const c6
         for (int k = 0; k < m && !inner.empty(); ++k) {
             CCAB_TRY(upload_active(inner, act));
             const unsigned ni = (unsigned)inner.size();
-            CCAB_TRY(gemv_active(ctx, CCAB_C64, n, n, ni, act, A64, lda, sA, Q + int64_t(k) * n, sq,
-                                 one, zero, Q + int64_t(k + 1) * n, sq));
+            CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq,
+                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale));
             {
                 ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
                 if (ref_mgs)

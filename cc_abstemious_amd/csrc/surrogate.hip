@@ -863,6 +863,8 @@ int ccab_surrogate_predict(ccab_ctx* ctx, ccab_surrogate* s, const ccab_physics*
     if (!soa) return CCAB_ENOMEM;
     CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts, n, soa));
     hipStream_t st = ctx->stream;
+    const double W = s->W;
+    ProfScope ps(ctx, PROF_SURROGATE, 2.0 * double(P) * (kFeat * W + 2.0 * W * W + 2.0 * W));
     hipLaunchKernelGGL(feat_kernel, grid1d(P), dim3(256), 0, st, sys, nsys, soa, n, nullptr, s->X,
                        nullptr);
     CCAB_TRY(sur_forward(ctx, s, P, nullptr, 0.0f, loss_blocks(P)));

@@ -45,8 +45,9 @@ enum { CCAB_BC_PRESSURE = 0, CCAB_BC_NORMAL_VELOCITY = 1, CCAB_BC_IMPEDANCE = 2 
 enum { CCAB_WAVE_PLANE = 0, CCAB_WAVE_SPHERICAL = 1 };
 /* influence_matrix.rs:164-167 */
 enum { CCAB_LHS = 0, CCAB_RHS = 1 };
-/* storage / arithmetic precision of a matrix */
-enum { CCAB_C128 = 0, CCAB_C64 = 1 };
+/* storage / arithmetic precision of a matrix; CCAB_C32 (complex FP16, scaled
+ * per system) is available as the GMRES operator of the ITERATIVE_MIXED sweep */
+enum { CCAB_C128 = 0, CCAB_C64 = 1, CCAB_C32 = 2 };
 /* solver kinds, input_data.rs:419-424 (+ the mixed-precision LU) */
 enum { CCAB_SOLVER_DIRECT = 0, CCAB_SOLVER_ITERATIVE = 1, CCAB_SOLVER_HIERARCHICAL = 2,
        CCAB_SOLVER_DIRECT_MIXED = 3, CCAB_SOLVER_ITERATIVE_MIXED = 4 };

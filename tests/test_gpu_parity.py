@@ -326,3 +326,26 @@ def test_sweep_gmres_ir_matches_fp64(x0_kind):
         pd.solver.kind = capi.SOLVER_ITERATIVE_MIXED
         _, _, st0 = solve.sweep(pd)
         assert sum(s["iterations"] for s in st) < sum(s["iterations"] for s in st0)
+
+
+
+@pytest.mark.parametrize("level", [2, 3])
+def test_sweep_gmres_ir_fp16_operator_matches_fp64(level):
+    """The complex-FP16 (scaled) GMRES operator inside FP64 refinement reaches
+    the FP64 LU solution; only the inner solve sees FP16 entries."""
+    _note(f"sweep GMRES-IR with c32 operator vs c128 LU, icosphere level {level}, BM")
+    m = mesh.icosphere(level)
+    ka = np.linspace(0.5, 8.0, 6)
+    fs = ka * 343.0 / (2 * math.pi)
+    pd = PreData(m, fs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                 (0, 0), IncidentWave(capi.WAVE_PLANE, (0.2, 0.3, 0.9), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_DIRECT))
+    phi_d, vn_d, _ = solve.sweep(pd)
+    pd.solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-10, restart=50,
+                       ir_tolerance=1e-3, ir_max_steps=30, ref_mgs=False, matrix_prec=capi.C32)
+    phi_h, _, st = solve.sweep(pd)
+    for k in range(len(fs)):
+        e = float(torch.linalg.vector_norm(phi_h[k] - phi_d[k]) / torch.linalg.vector_norm(phi_d[k]))
+        assert e < 1e-8, (k, e, st[k])
+        assert st[k]["converged"] == 1 and st[k]["rel_residual"] <= 1e-10
+    print("iterations", [s["iterations"] for s in st], "cycles", [s["ir_steps"] for s in st])
