@@ -268,7 +268,7 @@ class Context:
         self.check(self._L.ccab_set_stream(self.handle, ct.c_void_p(stream_ptr or 0)))
 
     PROFILE_CLASSES = ["assembly", "lu_panel", "lu_trsm", "lu_update", "lu_laswp", "lu_trinv",
-                       "trsv", "gemv", "mgs", "field", "hmatrix_gemv", "surrogate"]
+                       "trsv", "gemv", "mgs", "field", "hmatrix_gemv", "surrogate", "gemv_c128"]
 
     def profile(self, on: bool = True):
         self.check(self._L.ccab_profile_enable(self.handle, int(on)))

@@ -260,7 +260,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
     dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)nact);
-    ProfScope ps(ctx, PROF_GEMV,
+    ProfScope ps(ctx, prec == CCAB_C128 ? PROF_GEMV128 : PROF_GEMV,
                  double(M) * double(N) * double(nact) *
                      (prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
     if (N > 0) {

@@ -159,7 +159,8 @@ enum ProfClass {
     PROF_FIELD = 9,      // (field point, surface point) pairs
     PROF_HMAT = 10,      // bytes of H-matrix blocks read
     PROF_SURROGATE = 11, // flops of the network
-    PROF_NCLASS = 12
+    PROF_GEMV128 = 12,   // bytes of c128 matrix read (FP64 residuals)
+    PROF_NCLASS = 13
 };
 void prof_begin(ccab_ctx* ctx, hipEvent_t* ev);
 void prof_end(ccab_ctx* ctx, hipEvent_t ev, int cls, double work);

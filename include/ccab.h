@@ -120,9 +120,10 @@ int ccab_synchronize(ccab_ctx* ctx);
  * reference counterpart; the reference only logs).  Classes: 0 assembly
  * (work = bytes stored), 1 LU panel, 2 LU triangular update, 3 LU trailing
  * complex GEMM (work = flops), 4 row swaps, 5 diagonal-block inverses,
- * 6 triangular solves, 7 GEMV (work = matrix bytes), 8 Gram-Schmidt,
- * 9 field evaluation (work = point pairs), 10 H-matrix GEMV (work = block
- * bytes), 11 surrogate network (work = flops). */
+ * 6 triangular solves, 7 GEMV over c64 / c32 matrices (work = matrix
+ * bytes), 8 Gram-Schmidt, 9 field evaluation (work = point pairs),
+ * 10 H-matrix GEMV (work = block bytes), 11 surrogate network (work =
+ * flops), 12 GEMV over c128 matrices (work = matrix bytes). */
 int ccab_profile_enable(ccab_ctx* ctx, int on);
 int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double* work,
                       int reset);
