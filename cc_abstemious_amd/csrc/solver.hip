@@ -467,13 +467,14 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     int32_t* act = static_cast<int32_t*>(ctx_buf(ctx, SLOT_GIR_ACT, size_t(B) * 4 * 2));
     if (!Bv || !R || !Q || !Hd || !Yd || !nrm || !act) return CCAB_ENOMEM;
     const int64_t sq = n * (m + 1);
-    const size_t pin_h = size_t(B) * (m + 2) * 16, pin_y = size_t(B) * (m + 1) * 16;
-    CCAB_TRY(pinned_reserve(ctx, pin_h + pin_y + size_t(B) * 16 + size_t(B) * 8));
+    const size_t pin_h = ws_align(size_t(B) * (m + 2) * 16), pin_y = ws_align(size_t(B) * (m + 1) * 16);
+    const size_t pin_a = ws_align(size_t(B) * 4);
+    CCAB_TRY(pinned_reserve(ctx, pin_h + pin_y + pin_a + size_t(B) * 8));
     char* pin = static_cast<char*>(ctx->pinned);
     double* hH = reinterpret_cast<double*>(pin);
     double* hY = reinterpret_cast<double*>(pin + pin_h);
     int32_t* hAct = reinterpret_cast<int32_t*>(pin + pin_h + pin_y);
-    double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + size_t(B) * 8);
+    double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + pin_a);
     const c128 one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
     const void* Ares = A128 ? static_cast<const void*>(A128) : Aop;
     const int pres = A128 ? CCAB_C128 : CCAB_C64;
@@ -484,10 +485,15 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
 
     std::vector<int32_t> active(B);
     for (int64_t b = 0; b < B; ++b) active[b] = int32_t(b);
+    // the device copy of the active list changes only when a system
+    // finishes: skip the upload (and its synchronisation) otherwise
+    std::vector<int32_t> uploaded;
     auto upload_active = [&](const std::vector<int32_t>& a, int32_t* dst) -> int {
+        if (a == uploaded) return CCAB_OK;
         std::memcpy(hAct, a.data(), a.size() * 4);
         CCAB_HIP(hipMemcpyAsync(dst, hAct, a.size() * 4, hipMemcpyHostToDevice, s));
         CCAB_HIP(hipStreamSynchronize(s));   // hAct is reused right away
+        uploaded = a;
         return CCAB_OK;
     };
     auto read_norms = [&](const c128* V, int64_t stride, const std::vector<int32_t>& a,
