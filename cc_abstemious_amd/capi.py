@@ -118,6 +118,8 @@ SIGNATURES = {
     "ccab_profile_read": (_I, [_P, _I, _P, _P, _P, _I]),
     "ccab_set_tuning": (_I, [_P, _I, _I, _I]),
     "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
+    "ccab_set_graphs": (_I, [_P, _I]),
+    "ccab_stream_wait": (_I, [_P, _P]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_incident_surface": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
     "ccab_dense_surface_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _P, _P, _I64]),
@@ -230,6 +232,15 @@ def solver_cfg(solver=SOLVER_DIRECT, gmres: GmresParams | None = None, ir_tolera
     return c
 
 
+def _torch_stream():
+    """HIP stream handle of torch's current stream (None = legacy default)."""
+    import torch
+    if not torch.cuda.is_initialized():
+        return None
+    h = torch.cuda.current_stream().cuda_stream
+    return ct.c_void_p(h) if h else None
+
+
 class Context:
     """One device context (HIP stream + workspace) of libccab."""
 
@@ -247,6 +258,9 @@ class Context:
             raise CcabError(rc, self._L.ccab_last_error(self.handle).decode())
 
     def call(self, name: str, *args):
+        # inputs produced by torch are queued on torch's current stream: order
+        # the library's stream after them
+        self.check(self._L.ccab_stream_wait(self.handle, _torch_stream()))
         self.check(getattr(self._L, name)(self.handle, *args))
 
     def autotune(self, prec: int, n: int, batch: int = 1):
@@ -257,6 +271,10 @@ class Context:
         self.call("ccab_autotune", int(prec), int(n), int(batch), ct.byref(v), ct.byref(o), tab)
         times = {(64 if i < 8 else 128, i % 8): tab[i] for i in range(16)}
         return v.value, o.value, times
+
+    def set_graphs(self, on: bool):
+        """Replay captured HIP graphs of the LU factorisation."""
+        self.call("ccab_set_graphs", int(bool(on)))
 
     def set_tuning(self, prec: int, variant: int, outer: int):
         self.call("ccab_set_tuning", int(prec), int(variant), int(outer))

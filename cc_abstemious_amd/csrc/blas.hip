@@ -253,7 +253,8 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
         R = 1;
     const int64_t rowblocks = (M + 256 * R - 1) / (256 * R);
     const int64_t finish_blocks = (M + 255) / 256;
-    int64_t nsplit = (1024 + rowblocks * nact - 1) / (rowblocks * nact);
+    const int64_t target = ctx->gemv_wg_target;
+    int64_t nsplit = (target + rowblocks * nact - 1) / (rowblocks * nact);
     nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
     const int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
     nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;

@@ -32,12 +32,32 @@ int ctx_fail(ccab_ctx* ctx, int code, const char* fmt, ...) {
     return code;
 }
 
+hipGraphExec_t graph_find(ccab_ctx* ctx, const GraphKey& k) {
+    for (auto& g : ctx->graphs)
+        if (g.first == k) return g.second;
+    return nullptr;
+}
+
+void graph_store(ccab_ctx* ctx, const GraphKey& k, hipGraphExec_t e) {
+    if (ctx->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(ctx->graphs.front().second);
+        ctx->graphs.erase(ctx->graphs.begin());
+    }
+    ctx->graphs.emplace_back(k, e);
+}
+
+void graph_clear(ccab_ctx* ctx) {
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    ctx->graphs.clear();
+}
+
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes) {
     if (slot < 0 || slot >= SLOT_COUNT) return nullptr;
     bytes = ws_align(std::max<size_t>(bytes, 256));
     if (ctx->buf_bytes[slot] >= bytes) return ctx->buf[slot];
     if (ctx->buf[slot]) {
         (void)hipStreamSynchronize(ctx->stream);
+        graph_clear(ctx);  // captured graphs may point into this buffer
         (void)hipFree(ctx->buf[slot]);
         ctx->buf[slot] = nullptr;
         ctx->buf_bytes[slot] = 0;
@@ -327,6 +347,9 @@ int ccab_create(int device, ccab_ctx** out) {
         ctx->gemm_variant[0] = ctx->gemm_variant[1] = v & 7;
     }
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
+    if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
+    if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
+        ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
     *out = ctx;
     return CCAB_OK;
 }
@@ -364,10 +387,30 @@ int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer) {
     return CCAB_OK;
 }
 
+int ccab_stream_wait(ccab_ctx* ctx, void* hip_stream) {
+    if (!ctx) return CCAB_EINVAL;
+    hipStream_t other = static_cast<hipStream_t>(hip_stream);
+    if (other == ctx->stream) return CCAB_OK;
+    if (!ctx->wait_event) CCAB_HIP(hipEventCreateWithFlags(&ctx->wait_event, hipEventDisableTiming));
+    CCAB_HIP(hipEventRecord(ctx->wait_event, other));
+    CCAB_HIP(hipStreamWaitEvent(ctx->stream, ctx->wait_event, 0));
+    return CCAB_OK;
+}
+
+int ccab_set_graphs(ccab_ctx* ctx, int on) {
+    if (!ctx) return CCAB_EINVAL;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->use_graphs = on != 0;
+    if (!ctx->use_graphs) graph_clear(ctx);
+    return CCAB_OK;
+}
+
 int ccab_destroy(ccab_ctx* ctx) {
     if (!ctx) return CCAB_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    graph_clear(ctx);
+    if (ctx->wait_event) (void)hipEventDestroy(ctx->wait_event);
     prof_resolve(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int s = 0; s < SLOT_COUNT; ++s)

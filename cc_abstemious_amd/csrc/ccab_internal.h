@@ -108,6 +108,20 @@ enum BufSlot {
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_COUNT
 };
+// identity of a captured launch sequence (shape, buffers, tuning)
+struct GraphKey {
+    int prec;
+    int64_t n, batch, lda, stride;
+    const void *a, *ipiv, *linv, *uinv, *perm, *info;
+    int outer, variant;
+    bool operator==(const GraphKey& o) const {
+        return prec == o.prec && n == o.n && batch == o.batch && lda == o.lda &&
+               stride == o.stride && a == o.a && ipiv == o.ipiv && linv == o.linv &&
+               uinv == o.uinv && perm == o.perm && info == o.info && outer == o.outer &&
+               variant == o.variant;
+    }
+};
+
 struct ccab_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -130,6 +144,12 @@ struct ccab_ctx {
     int gemm_variant[2] = {3, 0};
     // LU outer panel width (64 or 128; 128 halves the trailing-update traffic)
     int lu_outer = 128;
+    // GEMV: workgroups per launch the column split aims for
+    int gemv_wg_target = 1024;
+    hipEvent_t wait_event = nullptr;  // ccab_stream_wait
+    // replay captured HIP graphs of fixed launch sequences (the LU)
+    bool use_graphs = false;
+    std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
     // in-process kernel timing (HIP events on ctx->stream), see prof_begin
     bool prof_on = false;
     struct ProfRec {
@@ -199,6 +219,10 @@ int ctx_fail(ccab_ctx* ctx, int code, const char* fmt, ...);
     } while (0)
 
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
+// captured graphs: lookup (nullptr if absent) and insert (keeps the last 8)
+hipGraphExec_t graph_find(ccab_ctx* ctx, const GraphKey& k);
+void graph_store(ccab_ctx* ctx, const GraphKey& k, hipGraphExec_t e);
+void graph_clear(ccab_ctx* ctx);
 inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
 int pinned_reserve(ccab_ctx* ctx, size_t bytes);
 

@@ -813,12 +813,12 @@ int lu_aux(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>** linv, Cx<T>** uinv, 
     return CCAB_OK;
 }
 
+// The whole factorisation as a fixed launch sequence (no allocation, no host
+// synchronisation): what a HIP graph captures.
 template <typename T>
-int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, int64_t strideA,
-                int32_t* ipiv) {
-    Cx<T>*linv, *uinv;
-    int32_t *perm, *info;
-    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
+int lu_factor_launch(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda,
+                     int64_t strideA, int32_t* ipiv, Cx<T>* linv, Cx<T>* uinv, int32_t* perm,
+                     int32_t* info) {
     const int64_t inv_stride = nblocks_of(n) * NB * NB;
     hipLaunchKernelGGL(init_perm_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
                        dim3(256), 0, ctx->stream, perm, n, info);
@@ -897,6 +897,42 @@ int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, 
                            inv_stride);
     }
     CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+template <typename T>
+int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, int64_t strideA,
+                int32_t* ipiv) {
+    Cx<T>*linv, *uinv;
+    int32_t *perm, *info;
+    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
+    const int prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
+    if (ctx->use_graphs && !ctx->prof_on && ctx->stream) {
+        // replay a captured factorisation for this shape, these buffers and
+        // this tuning; capture it on first use
+        GraphKey key{prec, n, batch, lda, strideA, A, ipiv, linv, uinv, perm, info,
+                     ctx->lu_outer, ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1]};
+        hipGraphExec_t exec = graph_find(ctx, key);
+        if (!exec) {
+            CCAB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            const int rc = lu_factor_launch<T>(ctx, n, batch, A, lda, strideA, ipiv, linv, uinv,
+                                               perm, info);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+            if (rc != CCAB_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            CCAB_HIP(ec);
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            CCAB_HIP(ei);
+            graph_store(ctx, key, exec);
+        }
+        CCAB_HIP(hipGraphLaunch(exec, ctx->stream));
+    } else {
+        CCAB_TRY(lu_factor_launch<T>(ctx, n, batch, A, lda, strideA, ipiv, linv, uinv, perm, info));
+    }
     ctx->lu_key = A;
     ctx->lu_prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
     ctx->lu_n = n;

@@ -115,6 +115,10 @@ const char* ccab_last_error(const ccab_ctx* ctx);
 /* Run on the caller's hipStream_t (NULL = the context's own stream). */
 int ccab_set_stream(ccab_ctx* ctx, void* hip_stream);
 int ccab_synchronize(ccab_ctx* ctx);
+/* Order the context's stream after all work queued so far on hip_stream
+ * (NULL = the legacy default stream): an event recorded there, waited on by
+ * the context's stream.  For callers that produce inputs on another stream. */
+int ccab_stream_wait(ccab_ctx* ctx, void* hip_stream);
 
 /* In-process kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only logs).  Classes: 0 assembly
@@ -133,6 +137,13 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
  * bit 2 128-row tiles) and the outer panel width (64 or 128) that sets the
  * K of the trailing update. */
 int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer);
+/* HIP graphs (no reference counterpart): with on != 0 the blocked LU is
+ * captured once per (shape, buffers, tuning) and replayed on later calls
+ * (the 400-600 launches of a factorisation become one graph launch); off by
+ * default, also set by the environment variable CCAB_GRAPHS=1.  Disabled
+ * while ccab_profile_enable is on.  Captured graphs are dropped when a
+ * workspace buffer is reallocated. */
+int ccab_set_graphs(ccab_ctx* ctx, int on);
 /* The autotuner: times the batched LU of `batch` synthetic n x n systems in
  * precision prec for every GEMM variant (0..7) and outer width (64, 128),
  * keeps the fastest in the context (as ccab_set_tuning) and returns it;

@@ -52,3 +52,31 @@ def test_factors_invariant_to_gemm_variant(prec):
                 e = float((LU - ref[0]).abs().max() / ref[0].abs().max())
                 assert e < (1e-5 if prec == capi.C64 else 1e-13), (outer, v, e)
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [capi.C64, capi.C128])
+def test_lu_graph_replay_matches_eager(prec):
+    """Captured-and-replayed LU (HIP graph) equals the eager launches, over
+    several replays with eager work and synchronisations in between."""
+    n, batch = 700, 3
+    _note(f"LU graph replay prec={prec} n={n} batch={batch}")
+    dt = torch.complex64 if prec == capi.C64 else torch.complex128
+    g = torch.Generator(device="cpu").manual_seed(11)
+    A0 = (torch.randn(batch, n, n, dtype=torch.complex128, generator=g)
+          + n ** 0.5 * torch.eye(n, dtype=torch.complex128)).to(dt).cuda()
+    ctx = capi.Context()
+    work = A0.clone().transpose(1, 2).contiguous().transpose(1, 2)
+    piv_e = solve.lu_factor_batched(work, ctx=ctx)
+    LU_e = work.clone()
+    ctx.set_graphs(True)
+    for rep in range(4):
+        work.copy_(A0.transpose(1, 2).contiguous().transpose(1, 2))
+        piv = solve.lu_factor_batched(work, ctx=ctx)      # rep 0 captures, then replays
+        assert torch.equal(piv, piv_e), rep
+        assert torch.equal(work, LU_e), rep
+        # unrelated eager work between replays
+        b = torch.randn(batch, n, dtype=torch.complex128, generator=g).cuda().to(dt)
+        solve.lu_solve_batched(work, piv, b.clone(), ctx=ctx)
+        torch.cuda.synchronize()
+    ctx.set_graphs(False)
+    ctx.close()
