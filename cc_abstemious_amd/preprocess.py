@@ -1,6 +1,6 @@
 """Problem data of one analysis -- the host-side mirror of PreData
-(src/preprocess.rs:254-332) and of the UserInput fields the hot path reads
-(src/preprocess/input_data.rs:471-484).
+(src/preprocess.rs:24-107) and of preprocess() (preprocess.rs:110-124), which
+builds it from a UserInput (src/preprocess/input_data.rs:79-91).
 
 Collocation points live in HBM as the (n, 8) float64 record x y z nx ny nz
 area wt; the frequency is implicit state selected with set_frequency_index,
@@ -14,13 +14,15 @@ import math
 import numpy as np
 import torch
 
-from . import capi
-from .mesh import Mesh
+from . import capi, input_data
+from .mesh import Mesh, read_vtk
 
 
 def linspace(start: float, end: float, npts: int) -> list[float]:
-    """tools.rs:1-10 (end point exact)."""
-    dx = (end - start) / (npts - 1)
+    """tools.rs:1-10 (end point exact; one point gives [end])."""
+    if npts < 1:
+        raise ValueError("linspace needs at least one point")
+    dx = (end - start) / (npts - 1) if npts > 1 else 0.0
     x = [start] * npts
     for i in range(1, npts - 1):
         x[i] = start + dx * i
@@ -50,7 +52,7 @@ class IncidentWave:
 
 @dataclasses.dataclass
 class Solver:
-    """Solver of input_data.rs:419-424 plus the mixed-precision variants."""
+    """Solver of input_data.rs:27-32 plus the mixed-precision variants."""
     kind: int = capi.SOLVER_DIRECT
     tolerance: float = 1e-5
     max_iterations: int = 1000
@@ -95,6 +97,7 @@ class PreData:
         self.d_fps = None
         self.output_total = True               # OutputType::Total (solve.rs:176)
         self.output_pressure = True            # OutputField::Pressure (postprocess.rs:72)
+        self.output_file = ""                  # Output::file (input_data.rs:73)
 
     def set_field_points(self, fps):
         self.field_points = np.ascontiguousarray(np.asarray(fps, dtype=np.float64).reshape(-1, 3))
@@ -103,7 +106,7 @@ class PreData:
     def get_field_points(self):
         return self.field_points
 
-    # preprocess.rs:263-304
+    # preprocess.rs:35-45
     def set_frequency_index(self, index: int):
         self.ifreq = int(index)
 
@@ -122,6 +125,52 @@ class PreData:
     def get_num_eqn(self) -> int:
         return int(self.cpts.shape[0])
 
+    # the remaining getters of preprocess.rs:43-107
+    def get_sound_speed(self) -> float:
+        return self.sound_speed
+
+    def get_mass_density(self) -> float:
+        return self.mass_density
+
+    def get_problem_type(self) -> int:
+        return self.problem_type
+
+    def get_method_type(self) -> int:
+        return self.method
+
+    def get_hdiag(self) -> complex:
+        """-1/2 on the H diagonal for exterior problems (preprocess.rs:61-67)."""
+        return complex(-0.5 if self.problem_type == capi.EXTERIOR else 0.0, 0.0)
+
+    def get_gdiag(self) -> complex:
+        """beta/2 = i/(2k) on the G diagonal for Burton-Miller (preprocess.rs:69-75)."""
+        return 0.5j / self.get_wavenumber() if self.method == capi.BURTON_MILLER else 0j
+
+    def get_solver(self) -> Solver:
+        return self.solver
+
+    def get_incident_wave(self) -> IncidentWave:
+        return self.incident_wave
+
+    def get_surface_bc(self):
+        """(bc_type, value) of the surface boundary condition."""
+        return self.bc_type, self.bc_value
+
+    def get_mesh(self) -> Mesh:
+        return self.mesh
+
+    def get_cpts(self) -> np.ndarray:
+        return self.cpts
+
+    def get_output_filename(self) -> str:
+        return self.output_file
+
+    def get_output_field(self) -> str:
+        return "Pressure" if self.output_pressure else "VelocityPotential"
+
+    def get_output_type(self) -> str:
+        return "Total" if self.output_total else "Scattered"
+
     def physics(self, frequency: float | None = None) -> capi.Physics:
         f = self.get_frequency() if frequency is None else frequency
         return capi.physics(f, self.sound_speed, self.mass_density, self.method,
@@ -133,6 +182,45 @@ class PreData:
         for i, f in enumerate(fs):
             arr[i] = self.physics(f)
         return arr
+
+
+_METHOD = {"Classical": capi.CLASSICAL, "BurtonMiller": capi.BURTON_MILLER}
+_PROBLEM = {"Exterior": capi.EXTERIOR, "Interior": capi.INTERIOR}
+_BC = {"Pressure": capi.BC_PRESSURE, "NormalVelocity": capi.BC_NORMAL_VELOCITY,
+       "Impedance": capi.BC_IMPEDANCE}
+_WAVE = {"PlaneWave": capi.WAVE_PLANE, "SphericalWave": capi.WAVE_SPHERICAL}
+_SOLVER = {"Direct": capi.SOLVER_DIRECT, "Iterative": capi.SOLVER_ITERATIVE,
+           "Hierarchical": capi.SOLVER_HIERARCHICAL}
+
+
+def solver_from_input(s: input_data.SolverInput) -> Solver:
+    """Solver enum of input_data.rs:27-32 as a solver configuration: Direct is
+    the FP64 LU, Iterative and Hierarchical keep the reference's GMRES (restart
+    ceil(sqrt(max_iterations)), backward-error stop, Arnoldi coefficient of
+    gmres.rs:168) and, for Hierarchical, leaf 32 and ACA tolerance 1e-4
+    (solve.rs:91-92)."""
+    if s.kind == "Direct":
+        return Solver(kind=capi.SOLVER_DIRECT)
+    return Solver(kind=_SOLVER[s.kind], tolerance=s.tolerance, max_iterations=s.max_iterations)
+
+
+def preprocess(user_input: input_data.UserInput, device: int = 0) -> PreData:
+    """preprocess.rs:110-124: read the mesh body, expand the frequency list and
+    compute the collocation points of the analysis described by user_input."""
+    u = user_input
+    m = read_vtk(u.mesh_file, u.body_index)
+    pd = PreData(m, input_data.frequency_list(u.frequency), u.sound_speed, u.mass_density,
+                 _METHOD[u.method_type], _PROBLEM[u.problem_type], _BC[u.surface_bc.bc_type],
+                 u.surface_bc.value,
+                 IncidentWave(_WAVE[u.incident_wave.kind], tuple(u.incident_wave.vec),
+                              tuple(u.incident_wave.amplitude)),
+                 solver_from_input(u.solver), device=device)
+    pd.output_total = u.output.o_type == "Total"
+    pd.output_pressure = u.output.field == "Pressure"
+    pd.output_file = u.output.file
+    pd.set_field_points(np.asarray(u.output.field_points, dtype=np.float64).reshape(-1, 3))
+    pd.input = u
+    return pd
 
 
 def colmajor(n: int, m: int, dtype, device) -> torch.Tensor:

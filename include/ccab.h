@@ -38,7 +38,7 @@ typedef struct { float re, im; } ccab_c64;
 typedef struct ccab_ctx ccab_ctx;
 typedef struct ccab_hmatrix ccab_hmatrix;
 
-/* input_data.rs:407-449 */
+/* input_data.rs:15-25 (ProblemType, MethodType), :34-51 (IncidentWaveInput, BCType) */
 enum { CCAB_CLASSICAL = 0, CCAB_BURTON_MILLER = 1 };
 enum { CCAB_EXTERIOR = 0, CCAB_INTERIOR = 1 };
 enum { CCAB_BC_PRESSURE = 0, CCAB_BC_NORMAL_VELOCITY = 1, CCAB_BC_IMPEDANCE = 2 };
@@ -48,7 +48,7 @@ enum { CCAB_LHS = 0, CCAB_RHS = 1 };
 /* storage / arithmetic precision of a matrix; CCAB_C32 (complex FP16, scaled
  * per system) is available as the GMRES operator of the ITERATIVE_MIXED sweep */
 enum { CCAB_C128 = 0, CCAB_C64 = 1, CCAB_C32 = 2 };
-/* solver kinds, input_data.rs:419-424 (+ the mixed-precision LU) */
+/* solver kinds, input_data.rs:27-32 (+ the mixed-precision LU) */
 enum { CCAB_SOLVER_DIRECT = 0, CCAB_SOLVER_ITERATIVE = 1, CCAB_SOLVER_HIERARCHICAL = 2,
        CCAB_SOLVER_DIRECT_MIXED = 3, CCAB_SOLVER_ITERATIVE_MIXED = 4 };
 /* GMRES stopping rule */
@@ -56,7 +56,7 @@ enum { CCAB_STOP_BACKWARD_ERROR = 0, /* gmres.rs:142-144 */
        CCAB_STOP_RELATIVE_RESIDUAL = 1 };
 
 /* The physics of one frequency (UserInput fields + PreData frequency,
- * input_data.rs:471-484, preprocess.rs:263-304). */
+ * input_data.rs:79-91, preprocess.rs:35-45). */
 typedef struct {
     double frequency;     /* Hz; k = 2*pi*f / c */
     double sound_speed;
@@ -68,7 +68,7 @@ typedef struct {
     double bc_value[2];   /* SurfaceBoundaryCondition.value */
 } ccab_physics;
 
-/* IncidentWaveInput (input_data.rs:426-436) */
+/* IncidentWaveInput (input_data.rs:34-44) */
 typedef struct {
     int32_t kind;          /* CCAB_WAVE_PLANE / CCAB_WAVE_SPHERICAL */
     int32_t pad_;
@@ -79,7 +79,7 @@ typedef struct {
 typedef struct {
     int64_t max_iterations;  /* <= 0 -> 1000, capped at n (gmres.rs:41-48) */
     double tolerance;        /* <= 0 -> 1e-5 */
-    int32_t restart;         /* 0 -> ceil(sqrt(max_iterations)) (gmres.rs:276) */
+    int32_t restart;         /* 0 -> ceil(sqrt(max_iterations)) (gmres.rs:45-46) */
     int32_t ref_mgs;         /* 1 -> Arnoldi coefficient qk1.dotc(q_i) (gmres.rs:168) */
     int32_t stop_rule;       /* CCAB_STOP_* */
     int32_t matrix_prec;     /* CCAB_C128 or CCAB_C64 operator */

@@ -1,0 +1,167 @@
+"""The analysis driver on the GPU (analysis.rs:57-117): the reference's own
+end-to-end tests run through Analysis exactly as verification.rs builds them,
+a frequency sweep against the oracle's per-frequency loop, and the reference's
+input file src/tests/input_1.json (n = 33546) run unchanged through the command
+line against the closed-form rigid-sphere field."""
+import copy
+import csv
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import analytic
+from cc_abstemious_amd import input_data as I
+from cc_abstemious_amd.__main__ import main
+from cc_abstemious_amd.analysis import Analysis, AnalysisState
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _note(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture
+def ref_tree(tmp_path, monkeypatch):
+    """A working directory laid out like the reference's checkout, so inputs
+    that name ./src/tests/<mesh>.vtk run unchanged."""
+    d = tmp_path / "src" / "tests"
+    d.mkdir(parents=True)
+    for name in ("sphere.vtk", "sphere_refined_2.vtk", "input_1.json"):
+        os.symlink(os.path.join(GOLD, name), d / name)
+    monkeypatch.chdir(tmp_path)
+    return tmp_path
+
+
+def default_input() -> I.UserInput:
+    """verification.rs:8-33."""
+    return I.UserInput(
+        mesh_file="./src/tests/sphere.vtk", body_index=3,
+        frequency=I.FrequencyInput(kind="List", values=[100.0]),
+        sound_speed=1.0, mass_density=1.0, problem_type="Exterior", method_type="Classical",
+        solver=I.SolverInput(kind="Direct"),
+        incident_wave=I.IncidentWaveInput(kind="SphericalWave", vec=(0.0, 0.0, 0.0),
+                                          amplitude=(0.0, 0.0)),
+        surface_bc=I.SurfaceBoundaryCondition(bc_type="NormalVelocity", value=(0.0, 0.0)),
+        output=I.Output(o_type="Scattered", field="Pressure", file="", field_points=[]))
+
+
+def _water_plane_wave(u: I.UserInput):
+    u.frequency = I.FrequencyInput(kind="List", values=[100.0])
+    u.sound_speed, u.mass_density = 1500.0, 1000.0
+    u.incident_wave = I.IncidentWaveInput(kind="PlaneWave", vec=(1.0, 0.0, 0.0),
+                                          amplitude=(1.0, 0.0))
+    u.output.field_points.append((10.0 * math.cos(0.0), 10.0 * math.sin(0.0), 0.0))
+    return u
+
+
+# verification.rs:95-205: (method, solver, expected scattered pressure at
+# (10, 0, 0), absolute tolerance of the reference's assert_relative_eq)
+CASES = {
+    "direct": ("Classical", I.SolverInput(kind="Direct"),
+               -0.000039380091293979903 + 0.00047605686656319906j, 1e-10),
+    "iterative": ("Classical", I.SolverInput(kind="Iterative", tolerance=1e-5, max_iterations=1000),
+                  -0.000039380091293979903 + 0.00047605686656319906j, 1e-8),
+    "hmatrix": ("Classical", I.SolverInput(kind="Hierarchical", tolerance=1e-5,
+                                           max_iterations=1000),
+                -3.9380308176032198e-05 + 0.00047605563272935873j, 1e-10),
+    "burton_miller": ("BurtonMiller", I.SolverInput(kind="Direct"),
+                      0.00012125591274707629 + 0.0004814465829556038j, 1e-10),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_verification_cases_through_analysis(ref_tree, case):
+    method, solver, expect, eps = CASES[case]
+    _note(f"analysis: verification.rs rigid_sphere_plane_wave ({case})")
+    u = _water_plane_wave(default_input())
+    u.method_type = method
+    u.solver = solver
+    a = Analysis()
+    a.set_input(u)
+    assert a.analysis_state == AnalysisState.POST_INPUT
+    a.run()
+    assert a.analysis_state == AnalysisState.POST_SOLVE
+    fp = a.get_result()
+    assert len(fp) == 1 and fp[0].frequency == 100.0
+    p = complex(fp[0].scattered[0])
+    assert abs(p.real - expect.real) <= eps and abs(p.imag - expect.imag) <= eps
+    if case == "direct":
+        assert abs(abs(p) - 0.0004776828778512934) <= 1e-10   # verification.rs:116
+    if case == "burton_miller":
+        assert abs(abs(p) - 0.0004964814282687255) <= 1e-10   # verification.rs:201
+
+
+def test_sweep_vs_oracle_per_frequency(ref_tree):
+    """The batched sweep behind Analysis.run against the reference's loop of
+    analysis.rs:86-111 run by the oracle one frequency at a time: BM,
+    LinearSpaced 10-1000 Hz x 20, total field and velocity potential at a
+    ring of points, then the per-point CSV (verification.rs:66-93)."""
+    u = _water_plane_wave(default_input())
+    u.method_type = "BurtonMiller"
+    u.frequency = I.FrequencyInput(kind="LinearSpaced", start=10.0, end=1000.0, number=20)
+    u.output.o_type = "Total"
+    u.output.field = "VelocityPotential"
+    u.output.file = "./src/tests/rigid_sphere_plane_wave_bem.csv"
+    u.output.field_points = [(10.0 * math.cos(t), 10.0 * math.sin(t), 0.5)
+                             for t in np.linspace(0, 2 * math.pi, 12, endpoint=False)]
+    _note("analysis: BM sweep 20 frequencies vs oracle loop")
+    a = Analysis()
+    a.set_input(copy.deepcopy(u))
+    a.run()
+    fp = a.get_result()
+    fs = I.frequency_list(u.frequency)
+    assert [r.frequency for r in fp] == fs
+    cpts = O.mesh_cpts_from_vtk(os.path.join(GOLD, "sphere.vtk"), 3)
+    fps = np.array(u.output.field_points)
+    w = O.wave(O.WAVE_PLANE, (1.0, 0.0, 0.0), (1.0, 0.0))
+    for r, f in zip(fp, fs):
+        ph = O.physics(f, 1500.0, 1000.0, O.BURTON_MILLER, O.EXTERIOR, O.BC_NORMAL_VELOCITY,
+                       (0.0, 0.0))
+        rhs = O.incident_surface(ph, w, cpts)
+        phi, vn, _ = O.solve_surface_dense(ph, cpts, rhs)
+        inc = O.incident_field(ph, w, fps)
+        ref = O.field(ph, cpts, fps, phi, vn, inc, want_total=True, to_pressure=False)
+        assert np.max(np.abs(r.scattered - ref)) <= 1e-11 * np.max(np.abs(ref))
+        np.testing.assert_allclose(r.incident, inc, rtol=1e-13, atol=1e-18)
+    assert a.write_results_at_point(3) is None
+    with open(u.output.file) as fh:
+        rows = list(csv.reader(fh))
+    assert rows[0] == ["freq", "vpo_re", "vpo_im", "inc_re", "inc_im"]
+    assert len(rows) == 21
+    for row, r in zip(rows[1:], fp):
+        assert float(row[0]) == r.frequency
+        assert complex(float(row[1]), float(row[2])) == complex(r.scattered[3])
+        assert complex(float(row[3]), float(row[4])) == complex(r.incident[3])
+
+
+def test_input_1_json_command_line(ref_tree):
+    """src/tests/input_1.json unchanged (n = 33546 triangles of a sphere of
+    radius 0.5, classical, FP64 LU, ka = 0.209) through the command line of
+    src/main.rs: the CSV holds the scattered pressure at (0, 1, 1), within
+    discretisation error of the rigid-sphere series (the reference's
+    verification note).  Centroid collocation on flat triangles converges
+    about linearly in the element size: 324 elements are 4% off the series at
+    (10, 0, 0), these 33546 (10x smaller elements) 0.42% at (0, 1, 1)."""
+    _note("analysis: input_1.json n=33546 direct LU via the command line")
+    assert main(["./src/tests/input_1.json"]) == 0
+    assert os.path.exists("input_1.log")
+    out = "./src/tests/rigid_sphere_plane_wave_bem.csv"
+    with open(out) as fh:
+        rows = list(csv.reader(fh))
+    assert rows[0] == ["index", "x", "y", "z", "pre_re", "pre_im", "inc_re", "inc_im"]
+    assert rows[1][:4] == ["0", "0", "1", "1"]
+    p = complex(float(rows[1][4]), float(rows[1][5]))
+    k = 2 * math.pi * 100.0 / 1500.0
+    ref = complex(analytic.rigid_sphere_scattered_at([[0.0, 1.0, 1.0]], 0.5, k, (0.0, 1.0, 0.0))[0])
+    rel = abs(p - ref) / abs(ref)
+    print(f"input_1.json: p_s = {p:.6e}; series {ref:.6e}; rel diff {rel:.2e}")
+    assert rel < 6e-3
+    pinc = complex(float(rows[1][6]), float(rows[1][7]))
+    assert abs(pinc - np.exp(1j * k * 1.0)) < 1e-12        # p_inc = e^{ik y}

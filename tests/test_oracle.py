@@ -123,9 +123,9 @@ def test_cluster_diameter():
 
 def test_block_tree_admissibility():
     g = _grid(25)
-    # block.rs:243-263: children[1][1][1] admissible
+    # block.rs:104-123: children[1][1][1] admissible
     assert O.block_tree_admissible_at(g, 32, 4.0, [1, 1, 1]) == 1
-    # block.rs:264-285: flattened list[5] admissible
+    # block.rs:125-145: flattened list[5] admissible
     blocks, _ = O.build_blocks(g, 32, 4.0)
     assert blocks[5][2] is True
     # every (row, col) pair is covered exactly once
@@ -182,7 +182,7 @@ def test_hmatrix_to_full_fixture():
 
 
 def test_hmatrix_mult_fixture():
-    # h_matrix.rs:323-337: 20x20 grid matvec
+    # h_matrix.rs:310-325: 20x20 grid matvec
     pts, A = _hyperbolic(20)
     _, b = O.generate_random_ab(400, 400, 10)
     H = O.HMatrix(400, lambda r, c: A[np.ix_(r, c)].reshape(-1), pts, 20, 1e-4, 4.0)
