@@ -117,6 +117,7 @@ SIGNATURES = {
     "ccab_profile_enable": (_I, [_P, _I]),
     "ccab_profile_read": (_I, [_P, _I, _P, _P, _P, _I]),
     "ccab_set_tuning": (_I, [_P, _I, _I, _I]),
+    "ccab_set_lu_coop": (_I, [_P, _I64]),
     "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
     "ccab_set_graphs": (_I, [_P, _I]),
     "ccab_stream_wait": (_I, [_P, _P]),
@@ -278,6 +279,11 @@ class Context:
 
     def set_tuning(self, prec: int, variant: int, outer: int):
         self.call("ccab_set_tuning", int(prec), int(variant), int(outer))
+
+    def set_lu_coop(self, min_rows: int):
+        """Panels of at least min_rows rows use the multi-workgroup panel
+        factorisation when the batch is small (0: never)."""
+        self.call("ccab_set_lu_coop", int(min_rows))
 
     def synchronize(self):
         self.check(self._L.ccab_synchronize(self.handle))

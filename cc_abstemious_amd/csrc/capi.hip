@@ -347,6 +347,7 @@ int ccab_create(int device, ccab_ctx** out) {
         ctx->gemm_variant[0] = ctx->gemm_variant[1] = v & 7;
     }
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
+    if (const char* lc = getenv("CCAB_LU_COOP_ROWS")) ctx->lu_coop_rows = std::max(0, atoi(lc));
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
@@ -384,6 +385,13 @@ int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer) {
     CCAB_CHECK(lu_outer == 64 || lu_outer == 128, "lu_outer must be 64 or 128");
     ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1] = gemm_variant;
     ctx->lu_outer = lu_outer;
+    return CCAB_OK;
+}
+
+int ccab_set_lu_coop(ccab_ctx* ctx, int64_t min_rows) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(min_rows >= 0, "min_rows must be >= 0");
+    ctx->lu_coop_rows = min_rows;
     return CCAB_OK;
 }
 

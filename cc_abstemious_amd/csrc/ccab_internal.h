@@ -106,19 +106,21 @@ enum BufSlot {
     SLOT_MIXED_NORMS, SLOT_MIXED_B, SLOT_GMRES, SLOT_SOLVE_A, SLOT_SOLVE_A64, SLOT_SOLVE_RHS,
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
-    SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_COUNT
+    SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
+    SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
     int prec;
     int64_t n, batch, lda, stride;
-    const void *a, *ipiv, *linv, *uinv, *perm, *info;
+    const void *a, *ipiv, *linv, *uinv, *perm, *info, *xbuf;
     int outer, variant;
+    int64_t coop_rows;
     bool operator==(const GraphKey& o) const {
         return prec == o.prec && n == o.n && batch == o.batch && lda == o.lda &&
                stride == o.stride && a == o.a && ipiv == o.ipiv && linv == o.linv &&
-               uinv == o.uinv && perm == o.perm && info == o.info && outer == o.outer &&
-               variant == o.variant;
+               uinv == o.uinv && perm == o.perm && info == o.info && xbuf == o.xbuf &&
+               outer == o.outer && variant == o.variant && coop_rows == o.coop_rows;
     }
 };
 
@@ -144,6 +146,9 @@ struct ccab_ctx {
     int gemm_variant[2] = {3, 0};
     // LU outer panel width (64 or 128; 128 halves the trailing-update traffic)
     int lu_outer = 128;
+    // panels of at least this many rows are factored by many workgroups per
+    // system when the batch is small (0: never); ccab_set_lu_coop
+    int64_t lu_coop_rows = 2048;
     // GEMV: workgroups per launch the column split aims for
     int gemv_wg_target = 1024;
     hipEvent_t wait_event = nullptr;  // ccab_stream_wait

@@ -416,6 +416,378 @@ __global__ __launch_bounds__(kSlabThreads) void slab_factor_kernel(Cx<T>* __rest
 }
 
 // ---------------------------------------------------------------------------
+// The same slab factorisation spread over G workgroups of one system, for
+// small batches whose tall panels would otherwise run on one CU each.
+// Workgroup g holds rows c0 + g*RW .. c0 + (g+1)*RW - 1 (RW = 256*RQ) of the
+// slab in registers.  Per column, every workgroup publishes its best pivot
+// candidate (largest norm1, first row on ties) together with that row, and
+// the owner of the diagonal row publishes the diagonal row; after one grid
+// barrier all workgroups choose the same pivot by the same rule, the owners
+// of the diagonal and pivot rows exchange them, and every workgroup scales
+// and updates its own rows.  The pivot rule is the one of panel_kernel and
+// slab_factor_kernel; the slab is wider (16 c128 / 32 c64 columns against
+// 1-8), so more of the panel's updates are rank-1 steps in registers rather
+// than MFMA GEMMs, and the factors agree with those paths to rounding.
+//
+// Exchange memory per system (64-bit words): a 32-word header (word 0: the
+// barrier counter, word 8: the abort flag), then two parities (column jj
+// uses jj & 1, so a workgroup still reading column jj never sees column jj+1
+// overwrite it) of: G candidates (value, row), G candidate rows of W
+// entries, the diagonal row.  Exchange data moves with agent-scope atomic
+// loads and stores; the barrier counter is released and acquired at agent
+// scope.  A barrier whose wait exceeds spin_limit sleeps sets the abort flag
+// and info = -2 instead of hanging; every later barrier then passes at once,
+// so the kernel always exits.
+// ---------------------------------------------------------------------------
+constexpr int kCoopThreads = 256;
+constexpr int kCoopHeaderWords = 32;
+// polls (each an agent-scope load plus a ~128-cycle sleep) a barrier may
+// wait, a few seconds, before it gives up
+constexpr unsigned kCoopSpinLimit = 1u << 21;
+
+template <typename T>
+struct CoopWords;
+template <>
+struct CoopWords<double> {
+    static constexpr int N = 2;  // 64-bit words per complex entry
+    static __device__ __forceinline__ void put(uint64_t* p, c128 z) {
+        __hip_atomic_store(p, uint64_t(__double_as_longlong(z.re)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(p + 1, uint64_t(__double_as_longlong(z.im)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ __forceinline__ c128 get(uint64_t* p) {
+        const uint64_t a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t b = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return c128{__longlong_as_double((long long)a), __longlong_as_double((long long)b)};
+    }
+};
+template <>
+struct CoopWords<float> {
+    static constexpr int N = 1;
+    static __device__ __forceinline__ void put(uint64_t* p, c64 z) {
+        const uint64_t w = uint64_t(__float_as_uint(z.re)) | (uint64_t(__float_as_uint(z.im)) << 32);
+        __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ __forceinline__ c64 get(uint64_t* p) {
+        const uint64_t w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return c64{__uint_as_float(uint32_t(w)), __uint_as_float(uint32_t(w >> 32))};
+    }
+};
+
+// words of one parity block and of one system's exchange area
+inline int64_t coop_parity_words(int G, int W, int wpc) {
+    const int64_t w = 2 * int64_t(G) + (int64_t(G) + 1) * W * wpc;
+    return (w + 31) / 32 * 32;
+}
+inline int64_t coop_system_words(int G, int W, int wpc) {
+    return kCoopHeaderWords + 2 * coop_parity_words(G, W, wpc);
+}
+
+__global__ void coop_reset_kernel(uint64_t* xbuf, int64_t sys_words, int64_t batch) {
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    xbuf[b * sys_words] = 0;      // barrier counter
+    xbuf[b * sys_words + 8] = 0;  // abort flag
+}
+
+// Grid barrier of one system: waits until the counter reaches target (all G
+// workgroups have arrived at this barrier).  Returns false, uniformly in the
+// workgroup, when the system's abort flag is set (by this or another
+// workgroup's timed-out wait).
+__device__ __forceinline__ bool coop_barrier(uint64_t* hdr, uint64_t target, unsigned spin_limit,
+                                             int* s_ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(hdr, uint64_t(1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(hdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(hdr + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                break;
+            }
+            if (++spins > spin_limit) {
+                __hip_atomic_store(hdr + 8, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+template <typename T, int W, int RQ>
+__global__ __launch_bounds__(kCoopThreads) void coop_slab_kernel(
+    Cx<T>* __restrict__ A, int64_t lda, int64_t strideA, int64_t n, int64_t k0, int nb, int c0o,
+    int32_t* __restrict__ ipiv, int32_t* __restrict__ perm, int32_t* __restrict__ info,
+    uint64_t* __restrict__ xbuf, int64_t sys_words, int64_t par_words, unsigned spin_limit) {
+    using C = Cx<T>;
+    using X = CoopWords<T>;
+    constexpr int NW = kCoopThreads / 64;
+    constexpr int64_t RW = int64_t(kCoopThreads) * RQ;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int64_t b = blockIdx.y;
+    C* a = A + b * strideA;
+    int32_t* piv = ipiv + b * n;
+    int32_t* pm = perm + b * n;
+    uint64_t* hdr = xbuf + b * sys_words;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const C zero{T(0), T(0)};
+    __shared__ C prow_s[W];
+    __shared__ T red_v[NW];
+    __shared__ int64_t red_i[NW];
+    __shared__ int64_t s_p;
+    __shared__ int s_zero;
+    __shared__ int s_ok;
+
+    const int64_t c0 = k0 + c0o;
+    const int weff = min(W, nb - c0o);
+    const int64_t rbase = c0 + int64_t(g) * RW;
+    C v[RQ][W];
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+        const int64_t row = rbase + tid + int64_t(q) * kCoopThreads;
+#pragma unroll
+        for (int w = 0; w < W; ++w) v[q][w] = (row < n && w < weff) ? a[row + (c0 + w) * lda] : zero;
+    }
+    // The column loop is not unrolled (its body holds a grid barrier); column
+    // jj of the register slab is reached by selects over the unrolled w.
+    int jj = 0;
+#pragma unroll 1
+    for (; jj < weff; ++jj) {
+        const int64_t dr = c0 + jj;
+        uint64_t* par = hdr + kCoopHeaderWords + (jj & 1) * par_words;
+        uint64_t* cand = par;                       // 2 words per workgroup
+        uint64_t* rows = par + 2 * int64_t(G);      // W entries per workgroup
+        uint64_t* drow = rows + int64_t(G) * W * X::N;
+        // --- this workgroup's candidate
+        T best = T(-1);
+        int64_t bi = n;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) {
+            const int64_t row = rbase + tid + int64_t(q) * kCoopThreads;
+            C vj = zero;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (w == jj) vj = v[q][w];
+            if (row >= dr && row < n) {
+                const T x = norm1(vj);
+                if (x > best) {
+                    best = x;
+                    bi = row;
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const T ov = __shfl_xor(best, off, 64);
+            const int64_t oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            red_v[wave] = best;
+            red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            T bv = red_v[0];
+            int64_t bx = red_i[0];
+            for (int w = 1; w < NW; ++w)
+                if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bx)) {
+                    bv = red_v[w];
+                    bx = red_i[w];
+                }
+            __hip_atomic_store(cand + 2 * g, uint64_t(__double_as_longlong(double(bv))),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(cand + 2 * g + 1, uint64_t(bx), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s_p = bx;
+        }
+        __syncthreads();
+        {
+            // the candidate row, and the diagonal row, from their owners
+            const int64_t wr = s_p;
+            const int64_t ow = wr - rbase, od = dr - rbase;
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                if (wr < n && ow == tid + int64_t(q) * kCoopThreads) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) X::put(rows + (int64_t(g) * W + w) * X::N, v[q][w]);
+                }
+                if (od == tid + int64_t(q) * kCoopThreads) {
+#pragma unroll
+                    for (int w = 0; w < W; ++w) X::put(drow + int64_t(w) * X::N, v[q][w]);
+                }
+            }
+        }
+        if (!coop_barrier(hdr, uint64_t(jj + 1) * uint64_t(G), spin_limit, &s_ok)) break;
+        // --- the global pivot: every workgroup reduces the same candidates
+        if (wave == 0) {
+            double bv = -1.0;
+            int64_t bx = n;
+            for (int t = lane; t < G; t += 64) {
+                const double x = __longlong_as_double((long long)__hip_atomic_load(
+                    cand + 2 * t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                const int64_t i = int64_t(__hip_atomic_load(cand + 2 * t + 1, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT));
+                if (x > bv || (x == bv && i < bx)) {
+                    bv = x;
+                    bx = i;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(bv, off, 64);
+                const int64_t oi = __shfl_xor(bx, off, 64);
+                if (ov > bv || (ov == bv && oi < bx)) {
+                    bv = ov;
+                    bx = oi;
+                }
+            }
+            if (lane == 0) {
+                const bool zcol = !(bv > 0.0);
+                s_zero = zcol ? 1 : 0;
+                s_p = zcol ? dr : bx;
+            }
+        }
+        __syncthreads();
+        const int64_t p = s_p;
+        if (p < dr || p >= n) break;  // only after a failed barrier: never index with it
+        if (!s_zero) {
+            const int gw = int((p - c0) / RW);
+            if (tid < W) prow_s[tid] = X::get(rows + (int64_t(gw) * W + tid) * X::N);
+            __syncthreads();
+            if (p != dr) {
+                const int64_t op = p - rbase, od = dr - rbase;
+#pragma unroll
+                for (int q = 0; q < RQ; ++q) {
+                    if (od == tid + int64_t(q) * kCoopThreads) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) v[q][w] = prow_s[w];
+                    }
+                    if (op == tid + int64_t(q) * kCoopThreads) {
+#pragma unroll
+                        for (int w = 0; w < W; ++w) v[q][w] = X::get(drow + int64_t(w) * X::N);
+                    }
+                }
+            }
+            // scale column jj below the diagonal, rank-1 update of the columns
+            // right of it (the pivot row stays in LDS: broadcast reads)
+            C pj = zero;
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (w == jj) pj = prow_s[w];
+            const C inv = crecip(pj);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) {
+                const int64_t row = rbase + tid + int64_t(q) * kCoopThreads;
+                if (row > dr && row < n) {
+                    C vj = zero;
+#pragma unroll
+                    for (int w = 0; w < W; ++w)
+                        if (w == jj) vj = v[q][w];
+                    const C l = vj * inv;
+#pragma unroll
+                    for (int w = 0; w < W; ++w) {
+                        if (w == jj) v[q][w] = l;
+                        else if (w > jj) v[q][w] -= l * prow_s[w];
+                    }
+                }
+            }
+        }
+        if (g == 0 && tid == 0) {
+            piv[dr] = int32_t(p);
+            if (s_zero) {
+                info[b] = 1;
+            } else if (p != dr) {
+                const int32_t t = pm[dr];
+                pm[dr] = pm[p];
+                pm[p] = t;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && __hip_atomic_load(hdr + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        info[b] = -2;
+    // after a failed barrier: identity pivots for the columns not done, so
+    // every later kernel that reads ipiv stays in range (the result is
+    // reported as an error)
+    if (g == 0 && tid == 0)
+        for (int j = jj; j < weff; ++j) piv[c0 + j] = int32_t(c0 + j);
+    // write back this workgroup's rows of the slab
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+        const int64_t row = rbase + tid + int64_t(q) * kCoopThreads;
+        if (row < n) {
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (w < weff) a[row + (c0 + w) * lda] = v[q][w];
+        }
+    }
+}
+
+// After coop_slab_kernel: the slab's swaps on the other panel columns and
+// the U rows right of the slab (L_ss^{-1} A(c0:c0+W, .)), from the pivots and
+// the slab's top W x W block in memory; one lane per panel column, the same
+// operations in the same order as the tail of slab_factor_kernel.
+template <typename T, int W>
+__global__ __launch_bounds__(64) void coop_finish_kernel(Cx<T>* __restrict__ A, int64_t lda,
+                                                         int64_t strideA, int64_t n, int64_t k0,
+                                                         int nb, int c0o,
+                                                         const int32_t* __restrict__ ipiv) {
+    using C = Cx<T>;
+    const int64_t b = blockIdx.x;
+    C* a = A + b * strideA;
+    const int32_t* piv = ipiv + b * n;
+    const int tid = threadIdx.x;
+    const C zero{T(0), T(0)};
+    const int64_t c0 = k0 + c0o;
+    const int weff = min(W, nb - c0o);
+    __shared__ C sL[W][W];
+    __shared__ int64_t s_piv[W];
+    __shared__ C su[W][64];
+    for (int e = tid; e < W * W; e += 64) {
+        const int r = e % W, c = e / W;
+        sL[r][c] = (r < weff && c < weff) ? a[(c0 + r) + (c0 + c) * lda] : zero;
+    }
+    if (tid < W) {
+        const int64_t pv = tid < weff ? int64_t(piv[c0 + tid]) : c0 + tid;
+        s_piv[tid] = (pv >= c0 + tid && pv < n) ? pv : c0 + tid;
+    }
+    __syncthreads();
+    for (int cc = tid; cc < nb; cc += 64) {
+        if (cc >= c0o && cc < c0o + weff) continue;
+        C* col = a + (k0 + cc) * lda;
+        for (int jj = 0; jj < weff; ++jj) {
+            const int64_t r = c0 + jj;
+            const int64_t p = s_piv[jj];
+            if (p != r) {
+                const C x = col[r];
+                col[r] = col[p];
+                col[p] = x;
+            }
+        }
+        if (cc >= c0o + weff) {
+            // unit-lower forward solve, column by column: entry w still sees
+            // its subtractions in the order t = 0, 1, .., w-1
+            for (int w = 0; w < weff; ++w) su[w][tid] = col[c0 + w];
+            for (int t = 0; t < weff; ++t) {
+                const C ut = su[t][tid];
+                for (int w = t + 1; w < weff; ++w) su[w][tid] -= sL[w][t] * ut;
+            }
+            for (int w = 1; w < weff; ++w) col[c0 + w] = su[w][tid];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Row swaps of pivots r0..r0+np-1 (LAPACK order) applied to the columns in
 // [c_lo, c_hi) outside [x_lo, x_hi); one lane per column.
 // ---------------------------------------------------------------------------
@@ -774,10 +1146,58 @@ int panel_slabs(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int
     return CCAB_OK;
 }
 
+// Width of the slab a coop_slab_kernel holds in registers: 16 complex
+// entries per row and lane for c128, 32 for c64.
+template <typename T, int RQ>
+constexpr int coop_width() { return (sizeof(T) == 8 ? 16 : 32) / RQ; }
+
+// Exchange words per system: sized for the largest grid (ctx->num_cus).
+inline int64_t coop_words_per_system(const ccab_ctx* ctx) {
+    return coop_system_words(std::max(ctx->num_cus, 1), 32, 2);
+}
+
+template <typename T, int RQ>
+int coop_panel_slabs(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int64_t k0,
+                     int nb, int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch,
+                     uint64_t* xbuf) {
+    constexpr int W = coop_width<T, RQ>();
+    constexpr int64_t RW = int64_t(kCoopThreads) * RQ;
+    const int64_t sys_words = coop_words_per_system(ctx);
+    for (int c0o = 0; c0o < nb; c0o += W) {
+        const int64_t c0 = k0 + c0o;
+        const int G = int((n - c0 + RW - 1) / RW);
+        const int64_t par_words = coop_parity_words(G, W, CoopWords<T>::N);
+        hipLaunchKernelGGL(coop_reset_kernel, dim3((unsigned)((batch + 63) / 64)), dim3(64), 0,
+                           ctx->stream, xbuf, sys_words, batch);
+        hipLaunchKernelGGL((coop_slab_kernel<T, W, RQ>), dim3((unsigned)G, (unsigned)batch),
+                           dim3(kCoopThreads), 0, ctx->stream, A, lda, sA, n, k0, nb, c0o, ipiv,
+                           perm, info, xbuf, sys_words, par_words, kCoopSpinLimit);
+        hipLaunchKernelGGL((coop_finish_kernel<T, W>), dim3((unsigned)batch), dim3(64), 0,
+                           ctx->stream, A, lda, sA, n, k0, nb, c0o, ipiv);
+        const int w = std::min(W, nb - c0o);
+        const int64_t c1 = c0 + w;
+        const int64_t rest = k0 + nb - c1;
+        if (rest > 0 && n - c1 > 0) {
+            CCAB_TRY((gemm_launch<T, 0>(ctx, n - c1, rest, w, A + c1 + (c1 - w) * lda, lda, sA,
+                                        A + (c1 - w) + c1 * lda, lda, sA, A + c1 + c1 * lda, lda,
+                                        sA, batch)));
+        }
+    }
+    return CCAB_OK;
+}
+
 template <typename T>
 int launch_panel(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, int64_t k0, int nb,
-                 int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch) {
+                 int32_t* ipiv, int32_t* perm, int32_t* info, int64_t batch, uint64_t* xbuf) {
     const int64_t m = n - k0;
+    // small batches of tall panels: spread each system's slab over many CUs
+    // (a grid of at most num_cus workgroups, so all of them are resident)
+    if (xbuf && ctx->lu_coop_rows > 0 && m >= ctx->lu_coop_rows) {
+        if ((m + kCoopThreads - 1) / kCoopThreads * batch <= ctx->num_cus)
+            return coop_panel_slabs<T, 1>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch, xbuf);
+        if ((m + 2 * kCoopThreads - 1) / (2 * kCoopThreads) * batch <= ctx->num_cus)
+            return coop_panel_slabs<T, 2>(ctx, A, lda, sA, n, k0, nb, ipiv, perm, info, batch, xbuf);
+    }
     const int64_t R = (m + kSlabThreads - 1) / kSlabThreads;
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int WA = F32 ? 8 : 4;   // R <= 4
@@ -803,13 +1223,22 @@ int launch_panel(ccab_ctx* ctx, Cx<T>* A, int64_t lda, int64_t sA, int64_t n, in
 
 template <typename T>
 int lu_aux(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>** linv, Cx<T>** uinv, int32_t** perm,
-           int32_t** info) {
+           int32_t** info, uint64_t** xbuf = nullptr) {
     const size_t inv = size_t(batch) * nblocks_of(n) * NB * NB * sizeof(Cx<T>);
     *linv = static_cast<Cx<T>*>(ctx_buf(ctx, SLOT_LU_LINV, inv));
     *uinv = static_cast<Cx<T>*>(ctx_buf(ctx, SLOT_LU_UINV, inv));
     *perm = static_cast<int32_t*>(ctx_buf(ctx, SLOT_LU_PERM, size_t(batch) * n * 4));
     *info = static_cast<int32_t*>(ctx_buf(ctx, SLOT_LU_INFO, size_t(batch) * 4));
     if (!*linv || !*uinv || !*perm || !*info) return CCAB_ENOMEM;
+    if (xbuf) {
+        // exchange memory of the multi-workgroup panel, when it can be used
+        *xbuf = nullptr;
+        if (ctx->lu_coop_rows > 0 && n >= ctx->lu_coop_rows && batch <= ctx->num_cus) {
+            *xbuf = static_cast<uint64_t*>(
+                ctx_buf(ctx, SLOT_LU_COOP, size_t(batch) * coop_words_per_system(ctx) * 8));
+            if (!*xbuf) return CCAB_ENOMEM;
+        }
+    }
     return CCAB_OK;
 }
 
@@ -818,7 +1247,7 @@ int lu_aux(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>** linv, Cx<T>** uinv, 
 template <typename T>
 int lu_factor_launch(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda,
                      int64_t strideA, int32_t* ipiv, Cx<T>* linv, Cx<T>* uinv, int32_t* perm,
-                     int32_t* info) {
+                     int32_t* info, uint64_t* xbuf) {
     const int64_t inv_stride = nblocks_of(n) * NB * NB;
     hipLaunchKernelGGL(init_perm_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
                        dim3(256), 0, ctx->stream, perm, n, info);
@@ -861,7 +1290,8 @@ int lu_factor_launch(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t 
             for (int j = 0; j < nb; ++j) pflops += 8.0 * double(n - j0 - j - 1) * double(nb - j - 1);
             {
                 ProfScope ps(ctx, PROF_LU_PANEL, pflops * B);
-                CCAB_TRY(launch_panel<T>(ctx, A, lda, strideA, n, j0, nb, ipiv, perm, info, batch));
+                CCAB_TRY(launch_panel<T>(ctx, A, lda, strideA, n, j0, nb, ipiv, perm, info, batch,
+                                         xbuf));
             }
             {
                 ProfScope ps(ctx, PROF_LU_TRINV, 0.0);
@@ -905,18 +1335,20 @@ int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, 
                 int32_t* ipiv) {
     Cx<T>*linv, *uinv;
     int32_t *perm, *info;
-    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info));
+    uint64_t* xbuf;
+    CCAB_TRY(lu_aux<T>(ctx, n, batch, &linv, &uinv, &perm, &info, &xbuf));
     const int prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
     if (ctx->use_graphs && !ctx->prof_on && ctx->stream) {
         // replay a captured factorisation for this shape, these buffers and
         // this tuning; capture it on first use
-        GraphKey key{prec, n, batch, lda, strideA, A, ipiv, linv, uinv, perm, info,
-                     ctx->lu_outer, ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1]};
+        GraphKey key{prec, n, batch, lda, strideA, A, ipiv, linv, uinv, perm, info, xbuf,
+                     ctx->lu_outer, ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1],
+                     ctx->lu_coop_rows};
         hipGraphExec_t exec = graph_find(ctx, key);
         if (!exec) {
             CCAB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             const int rc = lu_factor_launch<T>(ctx, n, batch, A, lda, strideA, ipiv, linv, uinv,
-                                               perm, info);
+                                               perm, info, xbuf);
             hipGraph_t graph = nullptr;
             const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
             if (rc != CCAB_OK) {
@@ -931,7 +1363,8 @@ int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, 
         }
         CCAB_HIP(hipGraphLaunch(exec, ctx->stream));
     } else {
-        CCAB_TRY(lu_factor_launch<T>(ctx, n, batch, A, lda, strideA, ipiv, linv, uinv, perm, info));
+        CCAB_TRY(lu_factor_launch<T>(ctx, n, batch, A, lda, strideA, ipiv, linv, uinv, perm, info,
+                                     xbuf));
     }
     ctx->lu_key = A;
     ctx->lu_prec = sizeof(T) == 8 ? CCAB_C128 : CCAB_C64;
@@ -945,8 +1378,12 @@ int lu_factor_t(ccab_ctx* ctx, int64_t n, int64_t batch, Cx<T>* A, int64_t lda, 
                             ctx->stream));
     CCAB_HIP(hipStreamSynchronize(ctx->stream));
     const int32_t* hi = static_cast<const int32_t*>(ctx->pinned);
-    for (int64_t b = 0; b < batch; ++b)
+    for (int64_t b = 0; b < batch; ++b) {
+        if (hi[b] == -2)
+            return ctx_fail(ctx, CCAB_EHIP, "LU panel grid barrier timed out (system %lld)",
+                            (long long)b);
         if (hi[b]) return ctx_fail(ctx, CCAB_ESINGULAR, "zero pivot in system %lld", (long long)b);
+    }
     return CCAB_OK;
 }
 

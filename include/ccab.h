@@ -137,6 +137,12 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
  * bit 2 128-row tiles) and the outer panel width (64 or 128) that sets the
  * K of the trailing update. */
 int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer);
+/* LU panel placement (no reference counterpart): panels of at least
+ * min_rows rows are factored by up to one workgroup per CU per system
+ * (grid-synchronised per column) when the batch leaves CUs idle; 0 keeps
+ * one workgroup per system.  Default 2048 (environment CCAB_LU_COOP_ROWS).
+ * The pivot rule is the same either way; the factors agree to rounding. */
+int ccab_set_lu_coop(ccab_ctx* ctx, int64_t min_rows);
 /* HIP graphs (no reference counterpart): with on != 0 the blocked LU is
  * captured once per (shape, buffers, tuning) and replayed on later calls
  * (the 400-600 launches of a factorisation become one graph launch); off by
