@@ -143,6 +143,7 @@ SIGNATURES = {
                                    ct.POINTER(SolverCfg), _P]),
     "ccab_sweep_warm": (_I, [_P, _P, _I64, ct.POINTER(Wave), _P, _I64, ct.POINTER(SolverCfg), _P,
                              _P, _P, _P]),
+    "ccab_solve_bodies": (_I, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ccab_hmatrix_build": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _I64, ct.c_double,
                                 ct.c_double, ct.POINTER(_P)]),
     "ccab_hmatrix_destroy": (_I, [_P]),

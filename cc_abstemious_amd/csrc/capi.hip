@@ -17,7 +17,7 @@ int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
-                 bool ref_mgs, ccab_solve_stats* st);
+                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg = nullptr);
 
 // ---------------------------------------------------------------------------
 // context plumbing
@@ -270,6 +270,15 @@ int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
     return CCAB_OK;
 }
 
+// device bytes the matrices of one chunk of systems may take
+size_t mem_budget(ccab_ctx* ctx) {
+    if (ctx->mem_budget) return ctx->mem_budget;
+    size_t budget = size_t(96) << 30;
+    size_t freemem = 0, total = 0;
+    if (hipMemGetInfo(&freemem, &total) == hipSuccess) budget = std::min(budget, freemem / 10 * 7);
+    return budget;
+}
+
 // SoA copy of the collocation points (8 n doubles) for the current call
 int stage_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double** soa) {
     *soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(n) * 64));
@@ -348,6 +357,8 @@ int ccab_create(int device, ccab_ctx** out) {
     }
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
     if (const char* lc = getenv("CCAB_LU_COOP_ROWS")) ctx->lu_coop_rows = std::max(0, atoi(lc));
+    if (const char* mb = getenv("CCAB_MEM_BUDGET_MB"))
+        ctx->mem_budget = size_t(std::max(0L, atol(mb))) << 20;
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
@@ -664,9 +675,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     else if (solver == CCAB_SOLVER_ITERATIVE_MIXED)
         per = n2 * (cfg->gmres.matrix_prec == CCAB_C32 ? 20 : 24);
     else return ctx_fail(ctx, CCAB_EINVAL, "solver %d not available on the dense path", solver);
-    size_t budget = size_t(96) << 30;
-    size_t freemem = 0, total = 0;
-    if (hipMemGetInfo(&freemem, &total) == hipSuccess) budget = std::min(budget, freemem / 10 * 7);
+    const size_t budget = mem_budget(ctx);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nfreq, int64_t(budget / per)));
     bool need_rowsum = false;
     for (auto& p : hp) need_rowsum |= p.rowsum_of != 0;
@@ -788,6 +797,172 @@ extern "C" int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nf
                        cfg->solver == CCAB_SOLVER_HIERARCHICAL),
                "a warm start needs the ITERATIVE_MIXED or HIERARCHICAL solver");
     return sweep_impl(ctx, ph, nfreq, w, nullptr, d_cpts, n, cfg, d_phi, d_vn, stats, d_x0);
+}
+
+// Bodies of uneven size (one system per body, each with its own mesh,
+// frequency and incident wave) solved together.  ITERATIVE_MIXED: per body
+// the incident RHS, the fused assembly (FP64 arithmetic; c128 matrix and the
+// c64 or scaled-FP16 operator copy) and the BC algebra, packed back to back
+// in HBM; then one GMRES-IR over all bodies whose matvecs and Arnoldi steps
+// are single batched launches (ragged GEMV; vectors padded to the largest
+// body with zeros).  Bodies are taken in chunks that fit the memory budget.
+// Other solvers run body by body.
+static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const double* d_cpts,
+                       const ccab_physics* ph, const ccab_wave* w, const ccab_solver_cfg* cfg,
+                       const ccab_c128* d_x0, ccab_c128* d_phi, ccab_c128* d_vn,
+                       ccab_solve_stats* stats) {
+    CCAB_CHECK(nbody > 0 && nh && d_cpts && ph && w && cfg && d_phi && d_vn, "invalid arguments");
+    std::vector<int64_t> coff(nbody + 1, 0);
+    for (int64_t b = 0; b < nbody; ++b) {
+        CCAB_CHECK(nh[b] > 0 && nh[b] < (int64_t(1) << 31), "invalid body size");
+        CCAB_CHECK(w[b].kind == CCAB_WAVE_PLANE || w[b].kind == CCAB_WAVE_SPHERICAL, "bad wave kind");
+        CCAB_CHECK(ph[b].bc_type == ph[0].bc_type, "bodies share one BC type");
+        coff[b + 1] = coff[b] + nh[b];
+    }
+    if (cfg->solver != CCAB_SOLVER_ITERATIVE_MIXED) {
+        CCAB_CHECK(!d_x0 || cfg->solver == CCAB_SOLVER_HIERARCHICAL,
+                   "a warm start needs the ITERATIVE_MIXED or HIERARCHICAL solver");
+        for (int64_t b = 0; b < nbody; ++b)
+            CCAB_TRY(sweep_impl(ctx, ph + b, 1, w + b, nullptr, d_cpts + 8 * coff[b], nh[b], cfg,
+                                d_phi + coff[b], d_vn + coff[b], stats ? stats + b : nullptr,
+                                d_x0 ? d_x0 + coff[b] : nullptr));
+        return CCAB_OK;
+    }
+    const int bc = ph[0].bc_type;
+    const bool half_op = cfg->gmres.matrix_prec == CCAB_C32;
+    const size_t esz_op = half_op ? 4 : 8;
+    std::vector<SysParams> hp;
+    SysParams* dp;
+    CCAB_TRY(stage_params(ctx, ph, nbody, hp, &dp));
+    // chunks of bodies within the memory budget
+    const size_t budget = mem_budget(ctx);
+    for (int64_t b0 = 0; b0 < nbody;) {
+        int64_t b1 = b0;
+        size_t bytes = 0;
+        while (b1 < nbody) {
+            const size_t add = size_t(nh[b1]) * nh[b1] * (16 + esz_op);
+            if (b1 > b0 && bytes + add > budget) break;
+            bytes += add;
+            ++b1;
+        }
+        const int64_t nc = b1 - b0;
+        int64_t nmax = 0, nmin = INT64_MAX;
+        std::vector<int64_t> hn(nc), haoff(nc);
+        int64_t entries = 0;
+        for (int64_t c = 0; c < nc; ++c) {
+            hn[c] = nh[b0 + c];
+            haoff[c] = entries;
+            entries += hn[c] * hn[c];
+            nmax = std::max(nmax, hn[c]);
+            nmin = std::min(nmin, hn[c]);
+        }
+        int align = 4;
+        for (int64_t c = 0; c < nc; ++c)
+            while (align > 1 && (hn[c] % align != 0 || haoff[c] % align != 0)) align /= 2;
+        const int64_t npts = coff[b1] - coff[b0];
+        c128* A128 = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_A, size_t(entries) * 16));
+        char* Aop = static_cast<char*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(entries) * esz_op));
+        c128* X = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_RHS, size_t(nc) * nmax * 16));
+        double* soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(npts) * 64));
+        int64_t* dr = static_cast<int64_t*>(ctx_buf(ctx, SLOT_RAGGED, size_t(nc) * 16));
+        if (!A128 || !Aop || !X || !soa || !dr) return CCAB_ENOMEM;
+        c128* X0 = nullptr;
+        if (d_x0) {
+            X0 = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_X, size_t(nc) * nmax * 16));
+            if (!X0) return CCAB_ENOMEM;
+        }
+        bool need_rowsum = false;
+        for (int64_t c = 0; c < nc; ++c) need_rowsum |= hp[b0 + c].rowsum_of != 0;
+        double* part = nullptr;
+        if (need_rowsum) {
+            part = static_cast<double*>(
+                ctx_buf(ctx, SLOT_ROWSUM, size_t((nmax + 63) / 64) * nmax * 16));
+            if (!part) return CCAB_ENOMEM;
+        }
+        double* inv_scale = nullptr;
+        if (half_op) {
+            // FP16 scale per body, as in the sweep: the largest diagonal entry
+            // maps to about 2^12
+            std::vector<double> h(size_t(npts) * 8);
+            CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts + 8 * coff[b0], size_t(npts) * 64,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+            CCAB_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<double> isc(nc);
+            for (int64_t c = 0; c < nc; ++c) {
+                SysParams& p = hp[b0 + c];
+                const double* hb = h.data() + 8 * (coff[b0 + c] - coff[b0]);
+                double mx = 0.0;
+                for (int64_t i = 0; i < hn[c]; ++i) {
+                    c128 G, H;
+                    diag_entry(p, hb[8 * i + 6], hb[8 * i + 7], G, H);
+                    const c128 L = p.lhs_a * H + p.lhs_b * G;
+                    mx = std::max(mx, std::sqrt(norm_sqr(L)));
+                }
+                CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
+                p.h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
+                isc[c] = 1.0 / p.h_scale;
+            }
+            CCAB_HIP(hipMemcpy(dp + b0, hp.data() + b0, sizeof(SysParams) * nc, hipMemcpyHostToDevice));
+            inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nc) * 8));
+            if (!inv_scale) return CCAB_ENOMEM;
+            CCAB_HIP(hipMemcpy(inv_scale, isc.data(), size_t(nc) * 8, hipMemcpyHostToDevice));
+        }
+        {
+            std::vector<int64_t> up(2 * nc);
+            std::copy(hn.begin(), hn.end(), up.begin());
+            std::copy(haoff.begin(), haoff.end(), up.begin() + nc);
+            CCAB_HIP(hipMemcpy(dr, up.data(), size_t(nc) * 16, hipMemcpyHostToDevice));
+        }
+        // padded right-hand sides (zero beyond each body's size) and guesses
+        hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nc * nmax + 255) / 256)), dim3(256), 0,
+                           ctx->stream, nc * nmax, c128{0.0, 0.0}, reinterpret_cast<ccab_c128*>(X), 0);
+        if (X0)
+            hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nc * nmax + 255) / 256)), dim3(256), 0,
+                               ctx->stream, nc * nmax, c128{0.0, 0.0},
+                               reinterpret_cast<ccab_c128*>(X0), 0);
+        for (int64_t c = 0; c < nc; ++c) {
+            const int64_t b = b0 + c, n = hn[c];
+            double* soa_b = soa + 8 * (coff[b] - coff[b0]);
+            ccab_c128* xb = reinterpret_cast<ccab_c128*>(X + c * nmax);
+            CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts + 8 * coff[b], n, soa_b));
+            CCAB_TRY(launch_incident(ctx, &hp[b], ph[b], w[b], soa_b, n, xb));
+            int64_t nch = 0;
+            CCAB_TRY(launch_assemble(ctx, dp + b, hp.data() + b, 1, soa_b, n,
+                                     half_op ? PREC_F64_H : PREC_F64, ASM_LHS, A128 + haoff[c],
+                                     Aop + size_t(haoff[c]) * esz_op, n, n * n, part, &nch));
+            CCAB_TRY(launch_rhs_finish(ctx, dp + b, 1, n, part, nch, xb, xb));
+            if (X0)
+                CCAB_HIP(hipMemcpyAsync(X0 + c * nmax, d_x0 + coff[b], size_t(n) * 16,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        Ragged rg;
+        rg.n = dr;
+        rg.aoff = dr + nc;
+        rg.align = align;
+        rg.n_min = nmin;
+        rg.hn = hn.data();
+        CCAB_TRY(gmres_ir_run(ctx, nmax, nc, A128, Aop, half_op ? CCAB_C32 : CCAB_C64,
+                              half_op ? inv_scale : nullptr, nmax, 0, X, X0, cfg->gmres.restart,
+                              cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
+                              cfg->gmres.ref_mgs != 0, stats ? stats + b0 : nullptr, &rg));
+        for (int64_t c = 0; c < nc; ++c) {
+            const int64_t b = b0 + c;
+            CCAB_TRY(finish_surface(ctx, hp.data() + b, dp + b, bc, hn[c], 1,
+                                    reinterpret_cast<const ccab_c128*>(X + c * nmax),
+                                    d_phi + coff[b], d_vn + coff[b]));
+        }
+        b0 = b1;
+    }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+extern "C" int ccab_solve_bodies(ccab_ctx* ctx, int64_t nbody, const int64_t* n,
+                                 const double* d_cpts, const ccab_physics* ph, const ccab_wave* w,
+                                 const ccab_solver_cfg* cfg, const ccab_c128* d_x0,
+                                 ccab_c128* d_phi, ccab_c128* d_vn, ccab_solve_stats* stats) {
+    if (!ctx) return CCAB_EINVAL;
+    return bodies_impl(ctx, nbody, n, d_cpts, ph, w, cfg, d_x0, d_phi, d_vn, stats);
 }
 
 extern "C" int ccab_gmres_ir_batched(ccab_ctx* ctx, int64_t n, int64_t batch,

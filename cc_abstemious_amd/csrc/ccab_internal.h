@@ -107,7 +107,7 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_COUNT
+    SLOT_RAGGED, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -151,6 +151,9 @@ struct ccab_ctx {
     int64_t lu_coop_rows = 2048;
     // GEMV: workgroups per launch the column split aims for
     int gemv_wg_target = 1024;
+    // device bytes a sweep or a batch of bodies may hold at once (0: 70% of
+    // the free memory, at most 96 GiB); environment CCAB_MEM_BUDGET_MB
+    size_t mem_budget = 0;
     hipEvent_t wait_event = nullptr;  // ccab_stream_wait
     // replay captured HIP graphs of fixed launch sequences (the LU)
     bool use_graphs = false;
@@ -290,12 +293,29 @@ int gemv_batched(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, const void* 
                  c128 beta, c128* d_y, int64_t stridey);
 size_t gemv_workspace_bytes(int64_t n, int64_t batch);
 // the same over the systems listed in d_active (device int32[nact])
+// Systems of uneven size in one batch: system b is n[b] x n[b], column-major
+// with leading dimension n[b], at element offset aoff[b] of the packed
+// matrix array (device arrays); its vectors keep the uniform stride of the
+// largest system, zero beyond n[b].  align: the largest of 4, 2, 1 that
+// divides every n[b] and aoff[b] (it selects the 16-byte-load GEMV).
+struct Ragged {
+    const int64_t* n = nullptr;
+    const int64_t* aoff = nullptr;
+    int align = 1;
+    int64_t n_min = 0;
+    const int64_t* hn = nullptr;  // host copy of n
+    double elems = 0.0;           // matrix entries one product reads (profiler work)
+};
+
 // prec CCAB_C32: d_inv_scale[b] multiplies system b's product (its matrix is
-// stored scaled by 1 / d_inv_scale[b])
+// stored scaled by 1 / d_inv_scale[b]).  rg (square products only): the
+// matrices are ragged (lda and strideA unused); rows of y beyond n[b] get
+// beta * y.
 int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
-                int64_t stridey, const double* d_inv_scale = nullptr);
+                int64_t stridey, const double* d_inv_scale = nullptr,
+                const Ragged* rg = nullptr);
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

@@ -448,11 +448,12 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
-                 bool ref_mgs, ccab_solve_stats* st) {
+                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg) {
     if (op_prec == CCAB_C32 && (!A128 || !d_inv_scale))
         return ctx_fail(ctx, CCAB_EINVAL, "the FP16 operator needs the c128 matrix and scales");
     if (m <= 0) m = 50;
-    m = (int)std::min<int64_t>(m, n);
+    // a Krylov space cannot exceed the smallest system
+    m = (int)std::min<int64_t>(m, rg ? rg->n_min : n);
     if (inner_tol <= 0) inner_tol = 1e-4;
     if (tol <= 0) tol = 1e-10;
     if (max_cycles <= 0) max_cycles = 20;
@@ -504,6 +505,13 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         for (int32_t b : a) out[b] = hN[b];
         return CCAB_OK;
     };
+    // matrix entries a product over a set of systems reads (profiler work)
+    auto set_elems = [&](const std::vector<int32_t>& a) {
+        if (!rg) return;
+        double e = 0.0;
+        for (int32_t b : a) e += double(rg->hn[b]) * double(rg->hn[b]);
+        rg->elems = e;
+    };
     std::vector<double> bn(B), rn(B), rel(B, 1.0);
     std::vector<int64_t> iters(B, 0), nmv(B, 0), cycles(B, 0);
     std::vector<char> conv(B, 0);
@@ -519,7 +527,9 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         hipLaunchKernelGGL(copy_active_kernel, dim3((unsigned)((n + 255) / 256), na), dim3(256), 0,
                            s, n, Bv, R, act);
         if (cyc > 0 || X0) {
-            CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n));
+            set_elems(active);
+            CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n,
+                                 nullptr, rg));
             for (int32_t b : active) nmv[b]++;
         }
         if (cyc == 0 && X0)
@@ -552,8 +562,9 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         for (int k = 0; k < m && !inner.empty(); ++k) {
             CCAB_TRY(upload_active(inner, act));
             const unsigned ni = (unsigned)inner.size();
+            set_elems(inner);
             CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq,
-                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale));
+                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg));
             {
                 ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
                 if (ref_mgs)

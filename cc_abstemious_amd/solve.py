@@ -5,6 +5,7 @@ solve_lu(a, x)                                        solve.rs:166-171
 GMRES(max_it, thresh); .a = matrix or .hmatrix = HMatrix; .solve(x)
                                                       gmres.rs:22-59
 sweep(predata) -> (phi, vn, stats)                    analysis.rs:86-111 (surface part)
+solve_bodies([predata, ...]) -> (phis, vns, stats)    solve.rs:11-71 over uneven bodies
 get_field(predata, m, l, phi, vn, phi_inc_fp)         solve.rs:173-185
 field(predata, phi, vn) -> (scattered, incident)      analysis.rs:100-104 + postprocess.rs:70-88,
                                                       fused, batched over frequencies
@@ -68,6 +69,42 @@ def sweep(predata: PreData, frequencies=None, x0: torch.Tensor | None = None, ct
                  _ptr(predata.d_cpts), n, ct.byref(cfg), _ptr(x0), _ptr(phi), _ptr(vn), stats)
     ctx.synchronize()
     return phi, vn, [s.as_dict() for s in stats]
+
+
+def solve_bodies(bodies, x0=None, solver=None, ctx=None):
+    """solve_for_surface (solve.rs:11-71) for several bodies of uneven size in
+    one batched call: bodies is a list of PreData, each solved at its current
+    frequency with its own incident wave (one BC type for all); solver
+    defaults to the first body's.  x0: optional list of initial guesses (one
+    (n_b,) tensor per body; ITERATIVE_MIXED or HIERARCHICAL).  Returns the
+    lists phi, vn (one (n_b,) tensor per body) and the per-body stats."""
+    if not bodies:
+        raise ValueError("no bodies")
+    ctx = ctx or capi.default_context(bodies[0].device.index or 0)
+    nb = len(bodies)
+    ns = [pd.get_num_eqn() for pd in bodies]
+    n_arr = (ct.c_int64 * nb)(*ns)
+    cpts = torch.cat([pd.d_cpts for pd in bodies]).contiguous()
+    ph = (capi.Physics * nb)()
+    w = (capi.Wave * nb)()
+    for i, pd in enumerate(bodies):
+        ph[i] = pd.physics()
+        w[i] = pd.incident_wave.to_c()
+    cfg = (solver or bodies[0].solver).to_c()
+    total = sum(ns)
+    dev = bodies[0].device
+    phi = torch.empty(total, dtype=torch.complex128, device=dev)
+    vn = torch.empty(total, dtype=torch.complex128, device=dev)
+    x0p = None
+    if x0 is not None:
+        if len(x0) != nb or any(x.numel() != n for x, n in zip(x0, ns)):
+            raise ValueError("x0 must hold one (n_b,) guess per body")
+        x0p = torch.cat([x.reshape(-1).to(dev) for x in x0]).to(torch.complex128).contiguous()
+    stats = (capi.SolveStats * nb)()
+    ctx.call("ccab_solve_bodies", nb, n_arr, _ptr(cpts), ph, w, ct.byref(cfg), _ptr(x0p),
+             _ptr(phi), _ptr(vn), stats)
+    ctx.synchronize()
+    return list(phi.split(ns)), list(vn.split(ns)), [s.as_dict() for s in stats]
 
 
 def gmres_ir_batched(A128: torch.Tensor | None, A64: torch.Tensor, x: torch.Tensor,

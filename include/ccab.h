@@ -339,6 +339,22 @@ int ccab_sweep_warm(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
                     const double* d_cpts, int64_t n, const ccab_solver_cfg* cfg,
                     const ccab_c128* d_x0, ccab_c128* d_phi, ccab_c128* d_vn,
                     ccab_solve_stats* stats);
+/* solve_for_surface (solve.rs:11-71, with the incident RHS of
+ * incident_wave.rs:7-63) for nbody bodies of uneven size in one call: body b
+ * has n[b] (host array) collocation points, packed back to back in d_cpts
+ * ((sum n) x 8), its own physics ph[b] (frequency; one BC type for all) and
+ * incident wave w[b].  d_x0 (optional, packed like the outputs; needs the
+ * ITERATIVE_MIXED or HIERARCHICAL solver) holds initial guesses; d_phi and
+ * d_vn receive sum(n) values each, stats[b] per body.  With the
+ * ITERATIVE_MIXED solver all bodies share one GMRES-IR: its matvecs (one
+ * ragged batched GEMV over matrices packed back to back) and Arnoldi steps
+ * are single launches over every body; other solvers run body by body.
+ * Batches of bodies are formed the way the reference would loop over
+ * analyses; the batch itself has no reference counterpart. */
+int ccab_solve_bodies(ccab_ctx* ctx, int64_t nbody, const int64_t* n, const double* d_cpts,
+                      const ccab_physics* ph, const ccab_wave* w, const ccab_solver_cfg* cfg,
+                      const ccab_c128* d_x0, ccab_c128* d_phi, ccab_c128* d_vn,
+                      ccab_solve_stats* stats);
 
 /* ---- learned warm start (device; no reference counterpart) ------------------- */
 /* A pointwise neural operator: per collocation point, 32 features of the
