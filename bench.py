@@ -12,7 +12,9 @@ resident in HBM.  Also timed: the same sweep warm-started by the learned
 initial guess (network trained beforehand, untimed, on solver solutions for
 other directions and frequencies; inference inside the timed sweep), the
 c64-operator sweep, the plain FP64 dense LU sweep (c128, FP64 MFMA; the
-accuracy reference, autotuned) and the c64-LU + FP64-IR sweep.
+accuracy reference, autotuned) and the c64-LU + FP64-IR sweep; and, as its
+own line ("bodies"), 256 perturbed-sphere bodies of 1280-5120 unknowns solved
+in one call (ragged batched GMRES-IR) against the same bodies body by body.
 
 Prints ONE JSON line.  value = sweep solves/s (frequencies solved per second).
 """
@@ -54,6 +56,8 @@ def parse():
     ap.add_argument("--calibrate", action="store_true",
                     help="after the runs, one read-only (sum) and one write-only (fill) kernel "
                          "over 1 GiB, to scale hardware traffic counters")
+    ap.add_argument("--bodies", type=int, default=256,
+                    help="uneven bodies of the batched-bodies line (0 skips it)")
     ap.add_argument("--train-dirs", type=int, default=12)
     ap.add_argument("--train-steps", type=int, default=6000)
     return ap.parse_args()
@@ -83,6 +87,71 @@ def cpu_baseline(cpts, ph_args, lu_n):
                        f"n={n} H,G on {threads} threads ({t_asm:.2f} s) + oracle unblocked "
                        f"partial-pivot LU timed at n={lu_n} and scaled by (n/{lu_n})^3 "
                        f"({t_lu:.1f} s)")}
+
+
+def bodies_block(args, ctx):
+    """Uneven bodies in one call: spheres radially perturbed by real spherical
+    harmonics up to degree 8 (coefficients N(0, 0.03^2)), geodesic meshes of
+    20 nu^2 triangles, nu uniform in 8..16 (1280-5120 unknowns), rigid BC,
+    Burton-Miller, one plane wave per body (seeded direction), ka uniform in
+    [1, 6]; GMRES-IR on the complex-FP16 operator to relative residual 1e-7,
+    zero start.  Baselines on the first 16 bodies: the same solver body by
+    body and the FP64 LU body by body."""
+    import torch
+    from cc_abstemious_amd import capi, mesh, solve
+    from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+    rng = np.random.default_rng(2024)
+    solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=30,
+                    ir_tolerance=1e-3, ir_max_steps=20, ref_mgs=False,
+                    stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C32)
+    bodies = []
+    for _ in range(args.bodies):
+        nu = int(rng.integers(8, 17))
+        cf = rng.standard_normal(81) * 0.03
+        cf[0] = 0.0
+        d = rng.standard_normal(3)
+        d /= np.linalg.norm(d)
+        ka = rng.uniform(1.0, 6.0)
+        bodies.append(PreData(mesh.perturbed_sphere(nu, cf), [ka * 343.0 / (2 * math.pi)], 343.0,
+                              1.21, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                              (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)),
+                              solver))
+    ns = [pd.get_num_eqn() for pd in bodies]
+    solve.solve_bodies(bodies[:8], ctx=ctx)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    phis, _, st = solve.solve_bodies(bodies, ctx=ctx)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    nsub = min(16, len(bodies))
+    t1 = time.perf_counter()
+    for pd in bodies[:nsub]:
+        solve.sweep(pd, ctx=ctx)
+    torch.cuda.synchronize()
+    one_rate = nsub / (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    lu, _, _ = solve.solve_bodies(bodies[:nsub], solver=Solver(kind=capi.SOLVER_DIRECT), ctx=ctx)
+    torch.cuda.synchronize()
+    lu_rate = nsub / (time.perf_counter() - t1)
+    err = max(float(torch.linalg.vector_norm(phis[i] - lu[i]) / torch.linalg.vector_norm(lu[i]))
+              for i in range(nsub))
+    its = [s_["iterations"] for s_ in st]
+    gemv = prof.get("gemv", {"count": 0, "ms": float("nan"), "work": 0.0})
+    return {"workload": (f"{len(bodies)} uneven bodies (perturbed spheres, {min(ns)}-{max(ns)} "
+                         "unknowns), Burton-Miller rigid, plane wave, ka in [1, 6], GMRES-IR "
+                         "(complex-FP16 operator) to relative residual 1e-7, zero start"),
+            "solves_per_s": len(bodies) / dt, "seconds": dt, "sum_n2": int(sum(n * n for n in ns)),
+            "all_converged": all(bool(s_["converged"]) for s_ in st),
+            "gmres_iters": {"mean": float(np.mean(its)), "max": int(max(its))},
+            "body_by_body_solves_per_s": one_rate, "fp64_lu_body_by_body_solves_per_s": lu_rate,
+            "speedup_vs_fp64_lu": (len(bodies) / dt) / lu_rate,
+            "rel_err_vs_fp64_lu_max": err, "baseline_bodies": nsub,
+            "ragged_gemv_gbs": (gemv["work"] / (gemv["ms"] * 1e-3) / 1e9) if gemv["count"] else None,
+            "kernel_ms": {k: round(v["ms"], 3) for k, v in prof.items()}}
 
 
 def main():
@@ -292,6 +361,7 @@ def main():
                      "traffic_ratio_measured": GEMV_C32_TRAFFIC_RATIO},
         "roofline_lu": lu_roof,
     }
+    result["bodies"] = bodies_block(args, ctx) if args.bodies > 0 else None
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pd.cpts, (freqs[-1], c, rho, 1, 0, 1, (0.0, 0.0)),
                                               args.cpu_lu_n)
