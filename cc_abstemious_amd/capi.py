@@ -118,6 +118,7 @@ SIGNATURES = {
     "ccab_profile_read": (_I, [_P, _I, _P, _P, _P, _I]),
     "ccab_set_tuning": (_I, [_P, _I, _I, _I]),
     "ccab_set_lu_coop": (_I, [_P, _I64]),
+    "ccab_stream_join": (_I, [_P, _P]),
     "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
     "ccab_set_graphs": (_I, [_P, _I]),
     "ccab_stream_wait": (_I, [_P, _P]),
@@ -165,6 +166,7 @@ SIGNATURES = {
     "ccab_surrogate_train_step": (_I, [_P, _P, _P, _P, _P, _I64, _P]),
     "ccab_surrogate_forward": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_surrogate_predict": (_I, [_P, _P, _P, _P, _I64, _P, _I64, _P]),
+    "ccab_surrogate_predict_bodies": (_I, [_P, _P, _I64, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -262,8 +264,11 @@ class Context:
     def call(self, name: str, *args):
         # inputs produced by torch are queued on torch's current stream: order
         # the library's stream after them
-        self.check(self._L.ccab_stream_wait(self.handle, _torch_stream()))
+        ts = _torch_stream()
+        self.check(self._L.ccab_stream_wait(self.handle, ts))
         self.check(getattr(self._L, name)(self.handle, *args))
+        # and torch's stream after the library's: results are read there next
+        self.check(self._L.ccab_stream_join(self.handle, ts))
 
     def autotune(self, prec: int, n: int, batch: int = 1):
         """Pick the fastest LU GEMM variant and outer panel width by timing;

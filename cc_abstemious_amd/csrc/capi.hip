@@ -416,6 +416,16 @@ int ccab_stream_wait(ccab_ctx* ctx, void* hip_stream) {
     return CCAB_OK;
 }
 
+int ccab_stream_join(ccab_ctx* ctx, void* hip_stream) {
+    if (!ctx) return CCAB_EINVAL;
+    hipStream_t other = static_cast<hipStream_t>(hip_stream);
+    if (other == ctx->stream) return CCAB_OK;
+    if (!ctx->join_event) CCAB_HIP(hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming));
+    CCAB_HIP(hipEventRecord(ctx->join_event, ctx->stream));
+    CCAB_HIP(hipStreamWaitEvent(other, ctx->join_event, 0));
+    return CCAB_OK;
+}
+
 int ccab_set_graphs(ccab_ctx* ctx, int on) {
     if (!ctx) return CCAB_EINVAL;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -430,6 +440,7 @@ int ccab_destroy(ccab_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     graph_clear(ctx);
     if (ctx->wait_event) (void)hipEventDestroy(ctx->wait_event);
+    if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     prof_resolve(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int s = 0; s < SLOT_COUNT; ++s)

@@ -155,6 +155,7 @@ struct ccab_ctx {
     // the free memory, at most 96 GiB); environment CCAB_MEM_BUDGET_MB
     size_t mem_budget = 0;
     hipEvent_t wait_event = nullptr;  // ccab_stream_wait
+    hipEvent_t join_event = nullptr;  // ccab_stream_join
     // replay captured HIP graphs of fixed launch sequences (the LU)
     bool use_graphs = false;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;

@@ -62,15 +62,9 @@ struct SurSys {
 // ---------------------------------------------------------------------------
 // features (and targets psi = phi / phi_inc)
 // ---------------------------------------------------------------------------
-__global__ void feat_kernel(const SurSys* __restrict__ sys, int64_t nsys, const double* __restrict__ soa,
-                            int64_t n, const ccab_c128* __restrict__ phi, uint16_t* feat,
-                            float* target) {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= nsys * n) return;
-    const int64_t s = p / n, i = p - s * n;
-    const SurSys q = sys[s];
-    const double x = soa[i], y = soa[n + i], z = soa[2 * n + i];
-    const double nx = soa[3 * n + i], ny = soa[4 * n + i], nz = soa[5 * n + i];
+// the 32 BF16 features of one collocation point (position, normal, area)
+__device__ inline void feat_point(const SurSys& q, double x, double y, double z, double nx,
+                                  double ny, double nz, double area, uint16_t* o) {
     const double rx = x - q.c0[0], ry = y - q.c0[1], rz = z - q.c0[2];
     const double rl = sqrt(rx * rx + ry * ry + rz * rz);
     const double inv = rl > 0.0 ? 1.0 / rl : 0.0;
@@ -108,12 +102,32 @@ __global__ void feat_kernel(const SurSys* __restrict__ sys, int64_t nsys, const 
     f[23] = cos(q.k * (rl - dx));
     f[24] = sin(q.k * (rl - dx));
     f[25] = 1.0 / (1.0 + ka);
-    f[26] = soa[6 * n + i] / q.mean_area - 1.0;
+    f[26] = area / q.mean_area - 1.0;
 #pragma unroll
     for (int t = 27; t < kFeat; ++t) f[t] = 0.0;
-    uint16_t* o = feat + p * kFeat;
 #pragma unroll
     for (int t = 0; t < kFeat; ++t) o[t] = f2bf(float(f[t]));
+}
+
+// x0 at a point x: amp e^{i k d.x} (out0 + i out1)
+__device__ inline ccab_c128 guess_point(const SurSys& q, double x, double y, double z,
+                                        const float* out2) {
+    double sn, cs;
+    sincos(q.k * (q.d[0] * x + q.d[1] * y + q.d[2] * z), &sn, &cs);
+    const c128 v = (q.amp * c128{cs, sn}) * c128{double(out2[0]), double(out2[1])};
+    return ccab_c128{v.re, v.im};
+}
+
+__global__ void feat_kernel(const SurSys* __restrict__ sys, int64_t nsys, const double* __restrict__ soa,
+                            int64_t n, const ccab_c128* __restrict__ phi, uint16_t* feat,
+                            float* target) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= nsys * n) return;
+    const int64_t s = p / n, i = p - s * n;
+    const SurSys q = sys[s];
+    const double x = soa[i], y = soa[n + i], z = soa[2 * n + i];
+    feat_point(q, x, y, z, soa[3 * n + i], soa[4 * n + i], soa[5 * n + i], soa[6 * n + i],
+               feat + p * kFeat);
     if (phi && target) {
         // psi = phi / (amp e^{i k d.x}), x absolute as in the incident wave
         double sn, cs;
@@ -149,11 +163,88 @@ __global__ void predict_finish_kernel(const SurSys* __restrict__ sys, int64_t ns
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= nsys * n) return;
     const int64_t s = p / n, i = p - s * n;
-    const SurSys q = sys[s];
-    double sn, cs;
-    sincos(q.k * (q.d[0] * soa[i] + q.d[1] * soa[n + i] + q.d[2] * soa[2 * n + i]), &sn, &cs);
-    const c128 v = (q.amp * c128{cs, sn}) * c128{double(out[2 * p]), double(out[2 * p + 1])};
-    x0[p] = ccab_c128{v.re, v.im};
+    x0[p] = guess_point(sys[s], soa[i], soa[n + i], soa[2 * n + i], out + 2 * p);
+}
+
+// body geometry on the device, one workgroup per body: centroid, RMS
+// radius and mean element area from the packed records (x y z nx ny nz area
+// wt), the same sums in the same order for every call
+// (n_all > 0: every system is on the one mesh of points 0 .. n_all-1)
+__global__ __launch_bounds__(256) void body_geometry_kernel(SurSys* sys,
+                                                            const int64_t* __restrict__ coff,
+                                                            int64_t n_all,
+                                                            const double* __restrict__ cpts) {
+    __shared__ double red[4][4];
+    __shared__ double c0s[3];
+    const int64_t b = blockIdx.x;
+    const int64_t p0 = n_all > 0 ? 0 : coff[b], n = n_all > 0 ? n_all : coff[b + 1] - coff[b];
+    const double* r = cpts + 8 * p0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        acc[0] += r[8 * i];
+        acc[1] += r[8 * i + 1];
+        acc[2] += r[8 * i + 2];
+        acc[3] += r[8 * i + 6];
+    }
+    for (int t = 0; t < 4; ++t) {
+        for (int off = 32; off > 0; off >>= 1) acc[t] += __shfl_xor(acc[t], off, 64);
+        if (lane == 0) red[t][wave] = acc[t];
+    }
+    __syncthreads();
+    if (threadIdx.x < 3)
+        c0s[threadIdx.x] = (red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                            red[threadIdx.x][3]) / double(n);
+    const double area = (red[3][0] + red[3][1] + red[3][2] + red[3][3]) / double(n);
+    __syncthreads();
+    double a2 = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double dx = r[8 * i] - c0s[0], dy = r[8 * i + 1] - c0s[1], dz = r[8 * i + 2] - c0s[2];
+        a2 += dx * dx + dy * dy + dz * dz;
+    }
+    for (int off = 32; off > 0; off >>= 1) a2 += __shfl_xor(a2, off, 64);
+    __syncthreads();
+    if (lane == 0) red[0][wave] = a2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        SurSys& q = sys[b];
+        q.c0[0] = c0s[0];
+        q.c0[1] = c0s[1];
+        q.c0[2] = c0s[2];
+        q.a = sqrt((red[0][0] + red[0][1] + red[0][2] + red[0][3]) / double(n));
+        q.mean_area = area;
+    }
+}
+
+// bodies of uneven size, points packed back to back: body s owns points
+// coff[s] .. coff[s+1]-1 (records x y z nx ny nz area wt)
+__device__ inline int64_t body_of(const int64_t* __restrict__ coff, int64_t nbody, int64_t p) {
+    int64_t lo = 0, hi = nbody - 1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) / 2;
+        if (coff[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void feat_bodies_kernel(const SurSys* __restrict__ sys, const int64_t* __restrict__ coff,
+                                   int64_t nbody, const double* __restrict__ cpts, uint16_t* feat) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= coff[nbody]) return;
+    const double* r = cpts + 8 * p;
+    feat_point(sys[body_of(coff, nbody, p)], r[0], r[1], r[2], r[3], r[4], r[5], r[6],
+               feat + p * kFeat);
+}
+
+__global__ void predict_finish_bodies_kernel(const SurSys* __restrict__ sys,
+                                             const int64_t* __restrict__ coff, int64_t nbody,
+                                             const double* __restrict__ cpts,
+                                             const float* __restrict__ out, ccab_c128* x0) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= coff[nbody]) return;
+    const double* r = cpts + 8 * p;
+    x0[p] = guess_point(sys[body_of(coff, nbody, p)], r[0], r[1], r[2], out + 2 * p);
 }
 
 __global__ void gather_kernel(const uint16_t* __restrict__ feat, const float* __restrict__ target,
@@ -625,49 +716,46 @@ int sur_backward(ccab_ctx* ctx, ccab_surrogate* S, int64_t P) {
 
 int loss_blocks(int64_t P) { return (int)std::min<int64_t>(4096, (P + 3) / 4); }
 
-// per-system constants on the device
-int stage_sys(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w, int64_t nsys,
-              const double* d_cpts, int64_t n, SurSys** d_sys) {
-    std::vector<double> h(size_t(n) * 8);
-    CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts, size_t(n) * 64, hipMemcpyDeviceToHost, ctx->stream));
-    CCAB_HIP(hipStreamSynchronize(ctx->stream));
-    double c0[3] = {0, 0, 0};
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) c0[d] += h[8 * i + d];
-    for (double& c : c0) c /= double(n);
-    double a2 = 0.0;
-    for (int64_t i = 0; i < n; ++i)
-        for (int d = 0; d < 3; ++d) a2 += (h[8 * i + d] - c0[d]) * (h[8 * i + d] - c0[d]);
-    const double a = std::sqrt(a2 / double(n));
-    CCAB_CHECK(a > 0.0, "degenerate mesh (zero radius)");
-    double area = 0.0;
-    for (int64_t i = 0; i < n; ++i) area += h[8 * i + 6];
-    area /= double(n);
-    CCAB_CHECK(area > 0.0, "degenerate mesh (zero area)");
-    std::vector<SurSys> hs(nsys);
-    for (int64_t s = 0; s < nsys; ++s) {
-        CCAB_TRY(check_physics(ctx, ph[s]));
-        CCAB_CHECK(w[s].kind == CCAB_WAVE_PLANE, "the surrogate models plane-wave incidence");
-        const SysParams p = make_params(ph[s]);
-        double d[3] = {w[s].vec[0], w[s].vec[1], w[s].vec[2]};
-        const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        CCAB_CHECK(nn > 0.0, "incident wave vector has zero length");
-        SurSys& q = hs[s];
-        q.k = p.k;
-        for (int t = 0; t < 3; ++t) {
-            q.d[t] = d[t] / nn;
-            q.c0[t] = c0[t];
-        }
-        q.amp = cdiv(c128{w[s].amplitude[0], w[s].amplitude[1]},
-                     c128{0.0, p.omega * ph[s].mass_density});
-        q.a = a;
-        q.mean_area = area;
+int make_sys(ccab_ctx* ctx, const ccab_physics& ph, const ccab_wave& w, const double c0[3],
+             double a, double area, SurSys& q) {
+    CCAB_TRY(check_physics(ctx, ph));
+    CCAB_CHECK(w.kind == CCAB_WAVE_PLANE, "the surrogate models plane-wave incidence");
+    const SysParams p = make_params(ph);
+    double d[3] = {w.vec[0], w.vec[1], w.vec[2]};
+    const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    CCAB_CHECK(nn > 0.0, "incident wave vector has zero length");
+    q.k = p.k;
+    for (int t = 0; t < 3; ++t) {
+        q.d[t] = d[t] / nn;
+        q.c0[t] = c0[t];
     }
-    *d_sys = static_cast<SurSys*>(ctx_buf(ctx, SLOT_SUR_SYS, sizeof(SurSys) * nsys));
+    q.amp = cdiv(c128{w.amplitude[0], w.amplitude[1]}, c128{0.0, p.omega * ph.mass_density});
+    q.a = a;
+    q.mean_area = area;
+    return CCAB_OK;
+}
+
+int upload_sys(ccab_ctx* ctx, const std::vector<SurSys>& hs, SurSys** d_sys) {
+    *d_sys = static_cast<SurSys*>(ctx_buf(ctx, SLOT_SUR_SYS, sizeof(SurSys) * hs.size()));
     if (!*d_sys) return CCAB_ENOMEM;
-    CCAB_HIP(hipMemcpyAsync(*d_sys, hs.data(), sizeof(SurSys) * nsys, hipMemcpyHostToDevice,
+    CCAB_HIP(hipMemcpyAsync(*d_sys, hs.data(), sizeof(SurSys) * hs.size(), hipMemcpyHostToDevice,
                             ctx->stream));
     CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    return CCAB_OK;
+}
+
+// per-system constants on the device (nsys systems on one mesh): wave and
+// physics from the host, the mesh's centroid, RMS radius and mean element
+// area by body_geometry_kernel
+int stage_sys(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w, int64_t nsys,
+              const double* d_cpts, int64_t n, SurSys** d_sys) {
+    const double none[3] = {0.0, 0.0, 0.0};
+    std::vector<SurSys> hs(nsys);
+    for (int64_t s = 0; s < nsys; ++s) CCAB_TRY(make_sys(ctx, ph[s], w[s], none, 1.0, 1.0, hs[s]));
+    CCAB_TRY(upload_sys(ctx, hs, d_sys));
+    hipLaunchKernelGGL(body_geometry_kernel, dim3((unsigned)nsys), dim3(256), 0, ctx->stream,
+                       *d_sys, static_cast<const int64_t*>(nullptr), n, d_cpts);
+    CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
 
@@ -846,6 +934,42 @@ int ccab_surrogate_forward(ccab_ctx* ctx, ccab_surrogate* s, const uint16_t* d_f
     CCAB_HIP(hipMemcpyAsync(s->X, d_feat, size_t(P) * kFeat * 2, hipMemcpyDeviceToDevice, ctx->stream));
     CCAB_TRY(sur_forward(ctx, s, P, nullptr, 0.0f, loss_blocks(P)));
     CCAB_HIP(hipMemcpyAsync(d_out, s->out, size_t(P) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    return CCAB_OK;
+}
+
+int ccab_surrogate_predict_bodies(ccab_ctx* ctx, ccab_surrogate* s, int64_t nbody,
+                                  const int64_t* n, const double* d_cpts, const ccab_physics* ph,
+                                  const ccab_wave* w, ccab_c128* d_x0) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(s && nbody > 0 && n && d_cpts && ph && w && d_x0, "invalid arguments");
+    CCAB_CHECK(s->device == ctx->device, "surrogate lives on another device");
+    std::vector<int64_t> coff(nbody + 1, 0);
+    for (int64_t b = 0; b < nbody; ++b) {
+        CCAB_CHECK(n[b] > 0, "invalid body size");
+        coff[b + 1] = coff[b] + n[b];
+    }
+    const int64_t P = coff[nbody];
+    CCAB_TRY(sur_reserve(ctx, s, P));
+    // wave and physics on the host; body geometry on the device
+    const double none[3] = {0.0, 0.0, 0.0};
+    std::vector<SurSys> hs(nbody);
+    for (int64_t b = 0; b < nbody; ++b) CCAB_TRY(make_sys(ctx, ph[b], w[b], none, 1.0, 1.0, hs[b]));
+    SurSys* sys;
+    CCAB_TRY(upload_sys(ctx, hs, &sys));
+    int64_t* dco = static_cast<int64_t*>(ctx_buf(ctx, SLOT_RAGGED, size_t(nbody + 1) * 8));
+    if (!dco) return CCAB_ENOMEM;
+    CCAB_HIP(hipMemcpy(dco, coff.data(), size_t(nbody + 1) * 8, hipMemcpyHostToDevice));
+    hipStream_t st = ctx->stream;
+    const double W = s->W;
+    ProfScope ps(ctx, PROF_SURROGATE, 2.0 * double(P) * (kFeat * W + 2.0 * W * W + 2.0 * W));
+    hipLaunchKernelGGL(body_geometry_kernel, dim3((unsigned)nbody), dim3(256), 0, st, sys, dco,
+                       int64_t(0), d_cpts);
+    hipLaunchKernelGGL(feat_bodies_kernel, grid1d(P), dim3(256), 0, st, sys, dco, nbody, d_cpts,
+                       s->X);
+    CCAB_TRY(sur_forward(ctx, s, P, nullptr, 0.0f, loss_blocks(P)));
+    hipLaunchKernelGGL(predict_finish_bodies_kernel, grid1d(P), dim3(256), 0, st, sys, dco, nbody,
+                       d_cpts, s->out, d_x0);
+    CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
 

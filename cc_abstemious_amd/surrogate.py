@@ -167,6 +167,22 @@ class Surrogate:
         self.ctx.synchronize()
         return x0
 
+    def predict_bodies(self, bodies) -> list:
+        """Initial guesses for bodies of uneven size (list of PreData, each at
+        its current frequency with its own plane wave) in one pass; one (n_b,)
+        tensor per body, as solve.solve_bodies takes them."""
+        nb = len(bodies)
+        ns = [pd.get_num_eqn() for pd in bodies]
+        n_arr = (ct.c_int64 * nb)(*ns)
+        cpts = torch.cat([pd.d_cpts for pd in bodies]).contiguous()
+        ph = (capi.Physics * nb)(*[pd.physics() for pd in bodies])
+        warr = (capi.Wave * nb)(*[pd.incident_wave.to_c() for pd in bodies])
+        x0 = torch.empty(sum(ns), dtype=torch.complex128, device=self.device)
+        self.ctx.call("ccab_surrogate_predict_bodies", self._h, nb, n_arr, _ptr(cpts), ph, warr,
+                      _ptr(x0))
+        self.ctx.synchronize()   # x0 is read on torch's stream next
+        return list(x0.split(ns))
+
     def close(self):
         if getattr(self, "_h", None):
             capi.lib().ccab_surrogate_destroy(self._h)

@@ -119,6 +119,11 @@ int ccab_synchronize(ccab_ctx* ctx);
  * (NULL = the legacy default stream): an event recorded there, waited on by
  * the context's stream.  For callers that produce inputs on another stream. */
 int ccab_stream_wait(ccab_ctx* ctx, void* hip_stream);
+/* The reverse order: work enqueued on hip_stream after this call waits for
+ * everything enqueued on the context's stream so far (device-side, no host
+ * synchronisation); with ccab_stream_wait before a call, a caller on another
+ * stream needs no ccab_synchronize to read the results. */
+int ccab_stream_join(ccab_ctx* ctx, void* hip_stream);
 
 /* In-process kernel timing with HIP events on the context's stream (no
  * reference counterpart; the reference only logs).  Classes: 0 assembly
@@ -408,6 +413,13 @@ int ccab_surrogate_forward(ccab_ctx* ctx, ccab_surrogate* s, const uint16_t* d_f
 int ccab_surrogate_predict(ccab_ctx* ctx, ccab_surrogate* s, const ccab_physics* ph,
                            const ccab_wave* w, int64_t nsys, const double* d_cpts, int64_t n,
                            ccab_c128* d_x0);
+/* initial guesses for bodies of uneven size in one pass (features, network
+ * and guesses over all packed points), laid out as ccab_solve_bodies takes
+ * them: body b has n[b] (host) points packed in d_cpts, physics ph[b] and
+ * plane wave w[b]; d_x0 receives sum(n) values */
+int ccab_surrogate_predict_bodies(ccab_ctx* ctx, ccab_surrogate* s, int64_t nbody,
+                                  const int64_t* n, const double* d_cpts,
+                                  const ccab_physics* ph, const ccab_wave* w, ccab_c128* d_x0);
 
 #ifdef __cplusplus
 }
