@@ -7,7 +7,9 @@ with ka linearly spaced in [0.5, 8], c = 343 m/s, rho = 1.21 kg/m^3.
 One step = the whole sweep: batched fused assembly (FP64 arithmetic; c128 LHS
 and a per-system scaled complex-FP16 copy) + batched GMRES on the FP16
 operator with FP64 iterative refinement against the c128 matrix, to an FP64
-relative residual of 1e-10, from the zero start; inputs (collocation points)
+relative residual of 1e-7 (the metric's accuracy bound is 1e-6 relative error
+vs the FP64 LU; the measured error is reported, and the same sweep is also
+timed at 1e-10 as "strict"), from the zero start; inputs (collocation points)
 resident in HBM.  Also timed: the same sweep warm-started by the learned
 initial guess (network trained beforehand, untimed, on solver solutions for
 other directions and frequencies; inference inside the timed sweep), the
@@ -49,8 +51,12 @@ def parse():
     ap.add_argument("--nfreq", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--tol", type=float, default=1e-10,
-                    help="FP64 relative residual at which a frequency counts as solved")
+    ap.add_argument("--tol", type=float, default=1e-7,
+                    help="FP64 relative residual at which a frequency counts as solved (the "
+                         "metric asks for <= 1e-6 relative error vs the FP64 LU; 1e-7 leaves "
+                         "a 10x margin, checked in rel_err_vs_fp64_lu)")
+    ap.add_argument("--strict-tol", type=float, default=1e-10,
+                    help="tolerance of the additionally timed strict sweep (0 skips it)")
     ap.add_argument("--cpu-lu-n", type=int, default=2048)
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
@@ -307,6 +313,15 @@ def main():
     (phi_c64, _, st_c64), c64_rate = timed(Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol,
                                                   restart=50, ir_tolerance=1e-5, ir_max_steps=20,
                                                   ref_mgs=False))
+    strict = None
+    if args.strict_tol > 0:
+        (phi_s, _, st_s), strict_rate = timed(Solver(kind=capi.SOLVER_ITERATIVE_MIXED,
+                                                     tolerance=args.strict_tol, restart=30,
+                                                     ir_tolerance=1e-3, ir_max_steps=30,
+                                                     ref_mgs=False, matrix_prec=capi.C32))
+        strict = {"tol": args.strict_tol, "solves_per_s": strict_rate,
+                  "gmres_iters": {"mean": float(np.mean([x["iterations"] for x in st_s])),
+                                  "ir_cycles_max": int(max(x["ir_steps"] for x in st_s))}}
     pd.solver = headline
 
     tuning = {}
@@ -336,6 +351,9 @@ def main():
     if not args.skip_fp64:
         (phi64, _, _), fp64_rate, _ = timed_sweep(capi.SOLVER_DIRECT)
         rel_err = relerr(phi, phi64)
+        if strict:
+            strict["rel_err_vs_fp64_lu"] = relerr(phi_s, phi64)
+            strict["speedup_vs_fp64_lu"] = strict["solves_per_s"] / fp64_rate
         (phim, _, stm), lu_rate, lu_prof = timed_sweep(capi.SOLVER_DIRECT_MIXED)
         lu_err = relerr(phim, phi64)
 
@@ -371,9 +389,10 @@ def main():
         "config": {"workload": (f"rigid-sphere plane-wave scattering, icosphere {n} triangles, "
                                 f"Burton-Miller, {args.nfreq}-frequency sweep ka in [0.5, 8]; "
                                 "batched GMRES on a complex-FP16 operator with FP64 iterative "
-                                "refinement, zero start"),
+                                f"refinement to FP64 relative residual {args.tol:g}, zero start"),
                    "n": n, "nfreq": args.nfreq, "solver": "iterative_mixed"},
         "rel_err_vs_fp64_lu": rel_err,
+        "strict": strict,
         "warm_start": sur_info,
         "warm_start_rel_err_vs_fp64_lu": (relerr(phi_w, phi64)
                                           if (phi_w is not None and not args.skip_fp64) else None),
