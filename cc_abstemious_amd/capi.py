@@ -21,6 +21,9 @@ C128, C64, C32 = 0, 1, 2
 SOLVER_DIRECT, SOLVER_ITERATIVE, SOLVER_HIERARCHICAL = 0, 1, 2
 SOLVER_DIRECT_MIXED, SOLVER_ITERATIVE_MIXED = 3, 4
 STOP_BACKWARD_ERROR, STOP_RELATIVE_RESIDUAL = 0, 1
+# physics corrections (opt-in; 0 = the reference's numerics), include/ccab.h
+FIX_BM_DH, FIX_FIELD_SIGN, FIX_IMPEDANCE_VN, FIX_SPHERICAL_ORIGIN, FIX_QUAD_AREA = 1, 2, 4, 8, 16
+FIX_ALL = 31
 
 
 class CcabError(RuntimeError):
@@ -37,7 +40,7 @@ class Physics(ct.Structure):
         ("method", ct.c_int32),
         ("problem_type", ct.c_int32),
         ("bc_type", ct.c_int32),
-        ("pad_", ct.c_int32),
+        ("corrections", ct.c_int32),
         ("bc_value", ct.c_double * 2),
     ]
 
@@ -123,6 +126,7 @@ SIGNATURES = {
     "ccab_set_graphs": (_I, [_P, _I]),
     "ccab_stream_wait": (_I, [_P, _P]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
+    "ccab_collocation_points_ex": (_I, [_P, _P, _P, _I64, _I, _P]),
     "ccab_incident_surface": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
     "ccab_dense_surface_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _P, _P, _I64]),
     "ccab_assemble_system": (_I, [_P, _P, _I64, _P, _I64, _P, _P, _I64, _I64, _P, _P]),
@@ -190,8 +194,10 @@ def lib():
 
 
 def physics(frequency, sound_speed=343.0, mass_density=1.21, method=CLASSICAL,
-            problem_type=EXTERIOR, bc_type=BC_NORMAL_VELOCITY, bc_value=(0.0, 0.0)) -> Physics:
+            problem_type=EXTERIOR, bc_type=BC_NORMAL_VELOCITY, bc_value=(0.0, 0.0),
+            corrections=0) -> Physics:
     ph = Physics()
+    ph.corrections = int(corrections)
     ph.frequency = float(frequency)
     ph.sound_speed = float(sound_speed)
     ph.mass_density = float(mass_density)

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
     const T nyx = T(s.nx[ic]), nyy = T(s.ny[ic]), nyz = T(s.nz[ic]);
     const T k = T(p.k);
     const Cx<T> beta{T(p.beta.re), T(p.beta.im)};
+    const T dh_sign = T(p.dh_sign);
     const int64_t sys_off = b * stride;
     double rs_re = 0.0, rs_im = 0.0;
     for (int64_t j = j0; j < j1; ++j) {
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
         } else {
             Cx<T> g, h;
             greens<T, BM>(k, beta, T(s.x[j] - yx), T(s.y[j] - yy), T(s.z[j] - yz), T(s.nx[j]),
-                          T(s.ny[j]), T(s.nz[j]), nyx, nyy, nyz, g, h);
+                          T(s.ny[j]), T(s.nz[j]), nyx, nyy, nyz, g, h, dh_sign);
             G = c128{double(g.re) * wa, double(g.im) * wa};
             H = c128{double(h.re) * wa, double(h.im) * wa};
         }
@@ -198,7 +199,7 @@ __global__ void rowcol_kernel(SysParams p, const double* __restrict__ soa_base, 
     } else {
         c128 g, h;
         greens<double, BM>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
-                           s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);
+                           s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h, p.dh_sign);
         const double wa = s.wt[j] * s.area[j];
         G = g * wa;
         H = h * wa;
@@ -286,7 +287,10 @@ int launch_incident(ccab_ctx* ctx, const SysParams* hp, const ccab_physics& ph,
     double d[3] = {w.vec[0], w.vec[1], w.vec[2]};
     const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     if (!(nn > 0.0)) return ctx_fail(ctx, CCAB_ENAN, "incident wave vector has zero length");
-    for (double& v : d) v /= nn;  // normalize(); spherical origin moves to the unit sphere
+    // normalize(); the spherical origin moves to the unit sphere unless
+    // CCAB_FIX_SPHERICAL_ORIGIN keeps it where it is
+    if (w.kind == CCAB_WAVE_PLANE || !(ph.corrections & CCAB_FIX_SPHERICAL_ORIGIN))
+        for (double& v : d) v /= nn;
     const c128 pamp{w.amplitude[0], w.amplitude[1]};
     const c128 amp = cdiv(pamp, c128{0.0, hp->omega * ph.mass_density});
     hipLaunchKernelGGL(incident_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, *hp,

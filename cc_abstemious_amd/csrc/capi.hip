@@ -160,6 +160,10 @@ SysParams make_params(const ccab_physics& ph) {
         p.rhs_inc_sign = -1.0;
     }
     p.h_scale = 1.0;
+    p.corrections = ph.corrections;
+    p.dh_sign = (ph.corrections & CCAB_FIX_BM_DH) ? 1.0 : -1.0;
+    p.vn_factor = (ph.corrections & CCAB_FIX_IMPEDANCE_VN) ? c128{-p.lhs_b.re, -p.lhs_b.im}
+                                                           : p.lhs_b;
     return p;
 }
 
@@ -171,6 +175,7 @@ int check_physics(ccab_ctx* ctx, const ccab_physics& ph) {
     CCAB_CHECK(ph.problem_type == CCAB_EXTERIOR || ph.problem_type == CCAB_INTERIOR,
                "bad problem type");
     CCAB_CHECK(ph.bc_type >= CCAB_BC_PRESSURE && ph.bc_type <= CCAB_BC_IMPEDANCE, "bad bc type");
+    CCAB_CHECK((ph.corrections & ~31) == 0, "unknown correction bits");
     if (ph.bc_type == CCAB_BC_IMPEDANCE)
         CCAB_CHECK(ph.bc_value[0] != 0.0 || ph.bc_value[1] != 0.0, "impedance must be non-zero");
     return CCAB_OK;
@@ -185,11 +190,12 @@ __global__ void fill_kernel(int64_t n, c128 v, ccab_c128* dst, int64_t stride) {
 
 __global__ void scale_copy_kernel(int64_t n, const SysParams* __restrict__ params,
                                   const ccab_c128* __restrict__ src, ccab_c128* dst) {
-    // vn = f * phi for the impedance BC (solve.rs:67)
+    // vn = f * phi for the impedance BC (solve.rs:67), or its negation with
+    // CCAB_FIX_IMPEDANCE_VN
     const int64_t b = blockIdx.y;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const c128 f = params[b].lhs_b;
+    const c128 f = params[b].vn_factor;
     const ccab_c128 s = src[b * n + i];
     const c128 v = f * c128{s.re, s.im};
     dst[b * n + i] = ccab_c128{v.re, v.im};
@@ -466,9 +472,10 @@ int ccab_synchronize(ccab_ctx* ctx) {
 }
 
 // elements.rs:49-181 + preprocess.rs:126-153 (host)
-int ccab_collocation_points(const double* nodes, const int64_t* conn, const int32_t* nverts,
-                            int64_t nelem, double* cpts) {
+int ccab_collocation_points_ex(const double* nodes, const int64_t* conn, const int32_t* nverts,
+                               int64_t nelem, int corrections, double* cpts) {
     if (!nodes || !conn || !nverts || !cpts || nelem < 0) return CCAB_EINVAL;
+    const bool quad_area = (corrections & CCAB_FIX_QUAD_AREA) != 0;
     for (int64_t e = 0; e < nelem; ++e) {
         const int64_t* cn = conn + 4 * e;
         double* out = cpts + 8 * e;
@@ -506,8 +513,10 @@ int ccab_collocation_points(const double* nodes, const int64_t* conn, const int3
             nu[1] = dxi[2] * deta[0] - dxi[0] * deta[2];
             nu[2] = dxi[0] * deta[1] - dxi[1] * deta[0];
             const double nn = std::sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
-            out[6] = 0.25 * nn;
-            out[7] = 4.0;
+            // CCAB_FIX_QUAD_AREA: the one-point Gauss area 4 |dxi x deta| with
+            // weight 1 (the reference's wt * area is a quarter of it)
+            out[6] = quad_area ? 4.0 * nn : 0.25 * nn;
+            out[7] = quad_area ? 1.0 : 4.0;
             for (int d = 0; d < 3; ++d) out[3 + d] = nu[d] / nn;
         } else {
             return CCAB_EINVAL;
@@ -518,6 +527,11 @@ int ccab_collocation_points(const double* nodes, const int64_t* conn, const int3
         out[2] = x[2];
     }
     return CCAB_OK;
+}
+
+int ccab_collocation_points(const double* nodes, const int64_t* conn, const int32_t* nverts,
+                            int64_t nelem, double* cpts) {
+    return ccab_collocation_points_ex(nodes, conn, nverts, nelem, 0, cpts);
 }
 
 int ccab_incident_surface(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,

@@ -251,6 +251,10 @@ struct SysParams {
     int32_t rowsum_of;   // 0 none, 1 sum of G, 2 sum of H
     double rhs_inc_sign; // sign applied to rhs_inc
     double h_scale;      // factor of the FP16 (c32) copy of the LHS (power of two)
+    double dh_sign;      // sign of the (k^2 r + 3 f1)(e.n_x)(e.n_y) term of dh
+    c128 vn_factor;      // impedance BC: vn = vn_factor * phi
+    int32_t corrections; // CCAB_FIX_* bits
+    int32_t pad_;
 };
 SysParams make_params(const ccab_physics& ph);
 int check_physics(ccab_ctx* ctx, const ccab_physics& ph);

@@ -19,9 +19,10 @@ namespace {
 
 struct FieldSys {
     double k;
-    c128 scale;  // i omega rho (pressure) or 1
-    c128 amp;    // incident amplitude A / (i omega rho)
-    double d[3]; // plane: unit direction; spherical: source on the unit sphere
+    c128 scale;   // i omega rho (pressure) or 1
+    c128 amp;     // incident amplitude A / (i omega rho)
+    double d[3];  // plane: unit direction; spherical: the source point
+    double msign; // +1: -M phi + L vn (solve.rs:180-181); -1 with CCAB_FIX_FIELD_SIGN
 };
 
 constexpr int kFieldWave = 64;
@@ -80,8 +81,9 @@ __global__ void field_finish_kernel(const FieldSys* __restrict__ sys, int kind,
             inc = (q.amp * c128{cs, sn}) * (1.0 / (4.0 * kPi * r));
         }
     }
-    c128 acc = want_total ? inc : c128{0.0, 0.0};
-    for (int c = 0; c < nchunk; ++c) acc += part[(f * nchunk + c) * nfp + p];
+    c128 sc{0.0, 0.0};
+    for (int c = 0; c < nchunk; ++c) sc += part[(f * nchunk + c) * nfp + p];
+    const c128 acc = (want_total ? inc : c128{0.0, 0.0}) + sc * q.msign;
     const c128 o = q.scale * acc;
     out[f * nfp + p] = ccab_c128{o.re, o.im};
     if (inc_out) {
@@ -138,8 +140,11 @@ int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
             std::sqrt(w->vec[0] * w->vec[0] + w->vec[1] * w->vec[1] + w->vec[2] * w->vec[2]);
         if (need_inc && !(nn > 0.0))
             return ctx_fail(ctx, CCAB_ENAN, "incident wave vector has zero length");
-        // normalize(); a spherical source moves to the unit sphere (incident_wave.rs)
-        for (int t = 0; t < 3; ++t) d[t] = nn > 0.0 ? w->vec[t] / nn : 0.0;
+        // normalize(); a spherical source moves to the unit sphere
+        // (incident_wave.rs:91) unless CCAB_FIX_SPHERICAL_ORIGIN
+        const bool keep = w->kind == CCAB_WAVE_SPHERICAL &&
+                          (ph[0].corrections & CCAB_FIX_SPHERICAL_ORIGIN);
+        for (int t = 0; t < 3; ++t) d[t] = keep ? w->vec[t] : (nn > 0.0 ? w->vec[t] / nn : 0.0);
     }
     std::vector<FieldSys> hs(nfreq);
     for (int64_t f = 0; f < nfreq; ++f) {
@@ -151,6 +156,7 @@ int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
         q.scale = to_pressure ? c128{0.0, wr} : c128{1.0, 0.0};
         q.amp = w ? cdiv(c128{w->amplitude[0], w->amplitude[1]}, c128{0.0, wr}) : c128{0.0, 0.0};
         for (int t = 0; t < 3; ++t) q.d[t] = d[t];
+        q.msign = (ph[f].corrections & CCAB_FIX_FIELD_SIGN) ? -1.0 : 1.0;
     }
     *dsys = static_cast<FieldSys*>(ctx_buf(ctx, SLOT_FIELD_SYS, sizeof(FieldSys) * nfreq));
     if (!*dsys) return CCAB_ENOMEM;

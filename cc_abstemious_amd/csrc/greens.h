@@ -19,10 +19,12 @@ __device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
 __device__ __forceinline__ float sqrt_t(float a) { return sqrtf(a); }
 
 // Off-diagonal kernel pair (g, h) for source point j (x, n_x) and collocation
-// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.
+// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.  dh_sign: -1 as
+// the reference (influence_matrix.rs:30), +1 for d^2 G / dn_x dn_y.
 template <typename T, bool BM>
 __device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx, T nxy, T nxz,
-                                       T nyx, T nyy, T nyz, Cx<T>& g, Cx<T>& h) {
+                                       T nyx, T nyy, T nyz, Cx<T>& g, Cx<T>& h,
+                                       T dh_sign = T(-1)) {
     const T rd = sqrt_t(rx * rx + ry * ry + rz * rz);
     const T inv_r = T(1) / rd;
     const T erx = rx * inv_r, ery = ry * inv_r, erz = rz * inv_r;
@@ -38,10 +40,10 @@ __device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx,
         const T rdotx = erx * nxx + ery * nxy + erz * nxz;
         const Cx<T> dg = gf1 * rdotx;
         const T nn = nyx * nxx + nyy * nxy + nyz * nxz;
-        const T rr = rdotx * rdoty;
-        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)
-        const Cx<T> term{-f1.re * nn - (k * k * rd + T(3) * f1.re) * rr,
-                         -f1.im * nn - T(3) * f1.im * rr};
+        const T rr = rdotx * rdoty * dh_sign;
+        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)  (dh_sign = -1)
+        const Cx<T> term{-f1.re * nn + (k * k * rd + T(3) * f1.re) * rr,
+                         -f1.im * nn + T(3) * f1.im * rr};
         const Cx<T> dh = (g * inv_r) * term;
         g += beta * dg;
         h += beta * dh;
@@ -76,7 +78,8 @@ __device__ __forceinline__ c128 side_entry(const SysParams& p, const Soa& s, int
         c128 g, h;
         if (p.bm)
             greens<double, true>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
-                                 s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);
+                                 s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h,
+                                 p.dh_sign);
         else
             greens<double, false>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
                                   s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);

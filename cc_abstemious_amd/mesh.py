@@ -34,14 +34,15 @@ class Mesh:
     def num_elements(self) -> int:
         return int(self.conn.shape[0])
 
-    def collocation_points(self) -> np.ndarray:
+    def collocation_points(self, corrections: int = 0) -> np.ndarray:
         """(n, 8) records x y z nx ny nz area wt, one per element
-        (elements.rs:172-181, preprocess.rs:126-153), computed by libccab."""
+        (elements.rs:172-181, preprocess.rs:126-153), computed by libccab;
+        corrections & capi.FIX_QUAD_AREA gives quads their full area."""
         out = np.zeros((self.num_elements, 8), dtype=np.float64)
-        rc = capi.lib().ccab_collocation_points(
+        rc = capi.lib().ccab_collocation_points_ex(
             self.nodes.ctypes.data_as(ct.c_void_p), self.conn.ctypes.data_as(ct.c_void_p),
             self.nverts.ctypes.data_as(ct.c_void_p), self.num_elements,
-            out.ctypes.data_as(ct.c_void_p))
+            int(corrections) & capi.FIX_QUAD_AREA, out.ctypes.data_as(ct.c_void_p))
         if rc != capi.OK:
             raise capi.CcabError(rc, "invalid or degenerate element")
         return out

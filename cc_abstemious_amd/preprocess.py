@@ -78,9 +78,11 @@ class PreData:
                  method=capi.CLASSICAL, problem_type=capi.EXTERIOR,
                  bc_type=capi.BC_NORMAL_VELOCITY, bc_value=(0.0, 0.0),
                  incident_wave: IncidentWave | None = None, solver: Solver | None = None,
-                 device: int = 0):
+                 device: int = 0, corrections: int = 0):
+        # corrections: capi.FIX_* bits (0 = the reference's numerics)
+        self.corrections = int(corrections)
         self.mesh = mesh
-        self.cpts = mesh.collocation_points()
+        self.cpts = mesh.collocation_points(self.corrections)
         self.frequency_list = list(frequencies)
         self.sound_speed = float(sound_speed)
         self.mass_density = float(mass_density)
@@ -174,7 +176,7 @@ class PreData:
     def physics(self, frequency: float | None = None) -> capi.Physics:
         f = self.get_frequency() if frequency is None else frequency
         return capi.physics(f, self.sound_speed, self.mass_density, self.method,
-                            self.problem_type, self.bc_type, self.bc_value)
+                            self.problem_type, self.bc_type, self.bc_value, self.corrections)
 
     def physics_array(self, frequencies=None):
         fs = self.frequency_list if frequencies is None else list(frequencies)

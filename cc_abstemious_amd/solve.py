@@ -237,8 +237,11 @@ class GMRES:
 def get_field(predata: PreData, m: torch.Tensor, l: torch.Tensor, phi: torch.Tensor,
               vn: torch.Tensor, phi_inc_fp: torch.Tensor) -> torch.Tensor:
     """phi_fp = (total ? phi_inc_fp : 0) - M phi + L vn (solve.rs:173-185), from
-    explicit M, L (see influence_matrix.get_dense_field_matrices)."""
+    explicit M, L (see influence_matrix.get_dense_field_matrices); the
+    scattered part changes sign with capi.FIX_FIELD_SIGN."""
     out = phi_inc_fp.clone() if predata.output_total else torch.zeros_like(phi_inc_fp)
+    if getattr(predata, "corrections", 0) & capi.FIX_FIELD_SIGN:
+        return out + m @ phi - l @ vn
     return out - m @ phi + l @ vn
 
 

@@ -55,6 +55,26 @@ enum { CCAB_SOLVER_DIRECT = 0, CCAB_SOLVER_ITERATIVE = 1, CCAB_SOLVER_HIERARCHIC
 enum { CCAB_STOP_BACKWARD_ERROR = 0, /* gmres.rs:142-144 */
        CCAB_STOP_RELATIVE_RESIDUAL = 1 };
 
+/* Physics corrections (no reference counterpart; opt-in, 0 = the reference's
+ * numerics bit for bit).  Each flag replaces one reference formula by the
+ * textbook one:
+ *   CCAB_FIX_BM_DH            Burton-Miller hypersingular kernel: the sign of
+ *                             the (k^2 r + 3 f1)(e.n_x)(e.n_y) term
+ *                             (influence_matrix.rs:30), so that H = dG/dn_y
+ *   CCAB_FIX_FIELD_SIGN       field points: phi = phi_inc + M phi - L vn
+ *                             (solve.rs:180-181 has the opposite sign)
+ *   CCAB_FIX_IMPEDANCE_VN     impedance BC: vn = +i omega rho phi / Z, the
+ *                             value the solved system implies (solve.rs:67,146
+ *                             return -i omega rho phi / Z)
+ *   CCAB_FIX_SPHERICAL_ORIGIN spherical wave: the source stays at the given
+ *                             origin (incident_wave.rs:44,91 move it to the
+ *                             unit sphere)
+ *   CCAB_FIX_QUAD_AREA        ccab_collocation_points_ex: quad area = the
+ *                             element's area with weight 1 (elements.rs:168
+ *                             takes a quarter with weight 4) */
+enum { CCAB_FIX_BM_DH = 1, CCAB_FIX_FIELD_SIGN = 2, CCAB_FIX_IMPEDANCE_VN = 4,
+       CCAB_FIX_SPHERICAL_ORIGIN = 8, CCAB_FIX_QUAD_AREA = 16 };
+
 /* The physics of one frequency (UserInput fields + PreData frequency,
  * input_data.rs:79-91, preprocess.rs:35-45). */
 typedef struct {
@@ -64,7 +84,7 @@ typedef struct {
     int32_t method;       /* CCAB_CLASSICAL / CCAB_BURTON_MILLER */
     int32_t problem_type; /* CCAB_EXTERIOR / CCAB_INTERIOR */
     int32_t bc_type;      /* CCAB_BC_* */
-    int32_t pad_;
+    int32_t corrections;  /* CCAB_FIX_* bits; 0 for the reference's numerics */
     double bc_value[2];   /* SurfaceBoundaryCondition.value */
 } ccab_physics;
 
@@ -170,6 +190,9 @@ int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, int* best_v
  * (-1 padded), nverts 3 or 4.  h_cpts: 8*nelem doubles. */
 int ccab_collocation_points(const double* h_nodes, const int64_t* h_conn,
                             const int32_t* h_nverts, int64_t nelem, double* h_cpts);
+/* ccab_collocation_points with CCAB_FIX_QUAD_AREA (or 0) in corrections */
+int ccab_collocation_points_ex(const double* nodes, const int64_t* conn, const int32_t* nverts,
+                               int64_t nelem, int corrections, double* cpts);
 
 /* ---- incident wave (device) ---------------------------------------------- */
 /* incident_wave.rs:7-63 (get_incident_surface): d_rhs[n] */
