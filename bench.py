@@ -55,6 +55,9 @@ def parse():
                     help="FP64 relative residual at which a frequency counts as solved (the "
                          "metric asks for <= 1e-6 relative error vs the FP64 LU; 1e-7 leaves "
                          "a 10x margin, checked in rel_err_vs_fp64_lu)")
+    ap.add_argument("--inner-tol", type=float, default=1e-3,
+                    help="relative tolerance of each inner GMRES cycle (FP16 operator)")
+    ap.add_argument("--restart", type=int, default=30, help="inner GMRES restart length")
     ap.add_argument("--strict-tol", type=float, default=1e-10,
                     help="tolerance of the additionally timed strict sweep (0 skips it)")
     ap.add_argument("--cpu-lu-n", type=int, default=2048)
@@ -231,8 +234,9 @@ def main():
                  (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)), None)
     n = pd.get_num_eqn()
     ctx = capi.default_context()
-    headline = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol, restart=30,
-                      ir_tolerance=1e-3, ir_max_steps=30, ref_mgs=False, matrix_prec=capi.C32)
+    headline = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol, restart=args.restart,
+                      ir_tolerance=args.inner_tol, ir_max_steps=30, ref_mgs=False,
+                      matrix_prec=capi.C32)
     pd.solver = headline
 
     for _ in range(args.warmup):
@@ -292,13 +296,15 @@ def main():
         prof_w = ctx.profile_read(reset=True)
         ctx.profile(False)
         sur_info = {"solves_per_s": warm_rate,
-                    "gmres_iters": iter_stats(st_w, 30, 1e-3, "complex FP16, scaled"),
+                    "gmres_iters": iter_stats(st_w, args.restart, args.inner_tol,
+                                              "complex FP16, scaled"),
                     "inference_ms_per_sweep": prof_w.get("surrogate", {}).get("ms", 0.0) / args.steps,
                     "train_systems": 8 * args.train_dirs, "train_steps": args.train_steps,
                     "final_loss": losses[-1][1], "train_s": train_s,
                     "network": "32-256-256-256-2 SiLU, BF16 MFMA GEMMs, FP32 Adam"}
 
-    gmres_iters = iter_stats(st, 30, 1e-3, "complex FP16, scaled (FP64 residual c128)")
+    gmres_iters = iter_stats(st, args.restart, args.inner_tol,
+                             "complex FP16, scaled (FP64 residual c128)")
     pd.solver = headline
 
     def timed(solver, x0_fn=None):
