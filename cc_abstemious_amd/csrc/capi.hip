@@ -363,6 +363,7 @@ int ccab_create(int device, ccab_ctx** out) {
     }
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
     if (const char* lc = getenv("CCAB_LU_COOP_ROWS")) ctx->lu_coop_rows = std::max(0, atoi(lc));
+    if (const char* mc = getenv("CCAB_MGS_COOP")) ctx->mgs_coop = atoi(mc) != 0;
     if (const char* mb = getenv("CCAB_MEM_BUDGET_MB"))
         ctx->mem_budget = size_t(std::max(0L, atol(mb))) << 20;
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;

@@ -107,7 +107,7 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_RAGGED, SLOT_COUNT
+    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -149,6 +149,9 @@ struct ccab_ctx {
     // panels of at least this many rows are factored by many workgroups per
     // system when the batch is small (0: never); ccab_set_lu_coop
     int64_t lu_coop_rows = 2048;
+    // Gram-Schmidt of few large systems over many workgroups per system
+    // (environment CCAB_MGS_COOP=0 keeps one workgroup per system)
+    bool mgs_coop = true;
     // GEMV: workgroups per launch the column split aims for
     int gemv_wg_target = 1024;
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of

@@ -16,6 +16,7 @@
 // C' are the interleaved memory itself and A' is the 2x2 real expansion
 // [[Re, -Im], [Im, Re]] built on the fly from LDS.
 #include "ccab_internal.h"
+#include "coop.h"
 
 namespace {
 
@@ -440,10 +441,6 @@ __global__ __launch_bounds__(kSlabThreads) void slab_factor_kernel(Cx<T>* __rest
 // so the kernel always exits.
 // ---------------------------------------------------------------------------
 constexpr int kCoopThreads = 256;
-constexpr int kCoopHeaderWords = 32;
-// polls (each an agent-scope load plus a ~128-cycle sleep) a barrier may
-// wait, a few seconds, before it gives up
-constexpr unsigned kCoopSpinLimit = 1u << 21;
 
 template <typename T>
 struct CoopWords;
@@ -482,43 +479,6 @@ inline int64_t coop_parity_words(int G, int W, int wpc) {
 }
 inline int64_t coop_system_words(int G, int W, int wpc) {
     return kCoopHeaderWords + 2 * coop_parity_words(G, W, wpc);
-}
-
-__global__ void coop_reset_kernel(uint64_t* xbuf, int64_t sys_words, int64_t batch) {
-    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b >= batch) return;
-    xbuf[b * sys_words] = 0;      // barrier counter
-    xbuf[b * sys_words + 8] = 0;  // abort flag
-}
-
-// Grid barrier of one system: waits until the counter reaches target (all G
-// workgroups have arrived at this barrier).  Returns false, uniformly in the
-// workgroup, when the system's abort flag is set (by this or another
-// workgroup's timed-out wait).
-__device__ __forceinline__ bool coop_barrier(uint64_t* hdr, uint64_t target, unsigned spin_limit,
-                                             int* s_ok) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(hdr, uint64_t(1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned spins = 0;
-        int ok = 1;
-        while (__hip_atomic_load(hdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (__hip_atomic_load(hdr + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                ok = 0;
-                break;
-            }
-            if (++spins > spin_limit) {
-                __hip_atomic_store(hdr + 8, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        *s_ok = ok;
-    }
-    __syncthreads();
-    return *s_ok != 0;
 }
 
 template <typename T, int W, int RQ>

@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include "ccab_internal.h"
+#include "coop.h"
 
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
@@ -82,6 +83,183 @@ __global__ __launch_bounds__(kMgsThreads) void mgs_kernel(int64_t n, int64_t k, 
     // scale = 1 / (hn + 0i) as a complex reciprocal
     const c128 sc = crecip(c128{hn, 0.0});
     for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) w[r] = w[r] * sc;
+}
+
+// The same step for few large systems: G workgroups per system (grid.x), each
+// holding rows g*256*RQ .. of w in registers.  Per basis vector every
+// workgroup publishes its partial dot product, one grid barrier, and all
+// workgroups add the G partials in the same fixed order (so they agree on
+// the coefficient bit for bit), then update their rows.  k + 2 barriers per
+// step; Q is read once.  After a failed barrier (see coop.h) h[k+1] is NaN,
+// which the caller's Givens step reports.
+constexpr int kMgsCoopThreads = 256;
+
+template <bool REF, int RQ>
+__global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
+    int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ, c128* h, int64_t sh,
+    const int32_t* __restrict__ active, uint64_t* __restrict__ xbuf, int64_t sys_words,
+    unsigned spin_limit) {
+    constexpr int NW = kMgsCoopThreads / 64;
+    __shared__ double red[2 * NW];
+    __shared__ double tot[2];
+    __shared__ int s_ok;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int64_t z = blockIdx.y;
+    const int64_t b = active ? active[z] : z;
+    uint64_t* hdr = xbuf + z * sys_words;
+    c128* q = Q + b * sQ;
+    c128* w = q + (k + 1) * ldq;
+    c128* hb = h + b * sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = int64_t(g) * kMgsCoopThreads * RQ;
+    c128 wv[RQ];
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+        const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+        wv[u] = row < n ? w[row] : c128{0.0, 0.0};
+    }
+    // the grid-wide sum of (a, b) over the workgroups' partials, barrier
+    // number nb; false after a failed barrier
+    auto grid_sum = [&](double& a, double& bb, int64_t nb) -> bool {
+        a = wave_sum(a);
+        bb = wave_sum(bb);
+        if (lane == 0) {
+            red[2 * wave] = a;
+            red[2 * wave + 1] = bb;
+        }
+        __syncthreads();
+        uint64_t* par = hdr + kCoopHeaderWords + (nb & 1) * 2 * int64_t(G);
+        if (tid == 0) {
+            double pa = 0.0, pb = 0.0;
+            for (int t = 0; t < NW; ++t) {
+                pa += red[2 * t];
+                pb += red[2 * t + 1];
+            }
+            __hip_atomic_store(par + 2 * g, uint64_t(__double_as_longlong(pa)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(par + 2 * g + 1, uint64_t(__double_as_longlong(pb)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!coop_barrier(hdr, uint64_t(nb + 1) * uint64_t(G), spin_limit, &s_ok)) return false;
+        if (wave == 0) {
+            double sa = 0.0, sb = 0.0;
+            for (int t = lane; t < G; t += 64) {
+                sa += __longlong_as_double((long long)__hip_atomic_load(par + 2 * t, __ATOMIC_RELAXED,
+                                                                      __HIP_MEMORY_SCOPE_AGENT));
+                sb += __longlong_as_double((long long)__hip_atomic_load(
+                    par + 2 * t + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            }
+            sa = wave_sum(sa);
+            sb = wave_sum(sb);
+            if (lane == 0) {
+                tot[0] = sa;
+                tot[1] = sb;
+            }
+        }
+        __syncthreads();
+        a = tot[0];
+        bb = tot[1];
+        __syncthreads();  // tot and red are reused by the next sum
+        return true;
+    };
+    bool ok = true;
+    int64_t nb = 0;
+    for (int64_t i = 0; i <= k && ok; ++i) {
+        const c128* qi = q + i * ldq;
+        c128 qv[RQ];
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int u = 0; u < RQ; ++u) {
+            const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+            qv[u] = row < n ? qi[row] : c128{0.0, 0.0};
+            const c128 a = qv[u], x = wv[u];
+            if (REF) {  // conj(w) * q
+                sr += x.re * a.re + x.im * a.im;
+                si += x.re * a.im - x.im * a.re;
+            } else {    // conj(q) * w
+                sr += a.re * x.re + a.im * x.im;
+                si += a.re * x.im - a.im * x.re;
+            }
+        }
+        ok = grid_sum(sr, si, nb++);
+        if (!ok) break;
+        const c128 hi{sr, si};
+#pragma unroll
+        for (int u = 0; u < RQ; ++u) wv[u] = wv[u] - hi * qv[u];
+        if (g == 0 && tid == 0) hb[i] = hi;
+    }
+    double nn = 0.0, dummy = 0.0;
+    if (ok) {
+#pragma unroll
+        for (int u = 0; u < RQ; ++u) nn += wv[u].re * wv[u].re + wv[u].im * wv[u].im;
+        ok = grid_sum(nn, dummy, nb++);
+    }
+    if (!ok) {
+        if (g == 0 && tid == 0) hb[k + 1] = c128{__builtin_nan(""), 0.0};
+        return;
+    }
+    const double hn = sqrt(nn);
+    if (g == 0 && tid == 0) hb[k + 1] = c128{hn, 0.0};
+    const c128 sc = crecip(c128{hn, 0.0});
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+        const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+        if (row < n) w[row] = wv[u] * sc;
+    }
+}
+
+// Modified Gram-Schmidt step of nact systems (active list, or all when
+// null): one workgroup per system, or, for few large systems that would
+// leave most CUs idle, mgs_coop_kernel over up to one workgroup per CU.
+int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ,
+               c128* h, int64_t sh, const int32_t* act, int64_t nact) {
+    hipStream_t s = ctx->stream;
+    if (ctx->mgs_coop && n >= 4096) {
+        int rq = 0;
+        int64_t G = 0;
+        for (int cand : {1, 2, 4, 8}) {
+            G = (n + int64_t(kMgsCoopThreads) * cand - 1) / (int64_t(kMgsCoopThreads) * cand);
+            if (G * nact <= ctx->num_cus) {
+                rq = cand;
+                break;
+            }
+        }
+        if (rq) {
+            const int64_t sys_words = kCoopHeaderWords + 4 * int64_t(std::max(ctx->num_cus, 1));
+            uint64_t* xbuf = static_cast<uint64_t*>(
+                ctx_buf(ctx, SLOT_MGS_COOP, size_t(nact) * sys_words * 8));
+            if (!xbuf) return CCAB_ENOMEM;
+            hipLaunchKernelGGL(coop_reset_kernel, dim3((unsigned)((nact + 63) / 64)), dim3(64), 0, s,
+                               xbuf, sys_words, nact);
+            const dim3 grid((unsigned)G, (unsigned)nact);
+#define CCAB_MGS_COOP(R)                                                                        \
+    do {                                                                                        \
+        if (ref)                                                                                \
+            hipLaunchKernelGGL((mgs_coop_kernel<true, R>), grid, dim3(kMgsCoopThreads), 0, s, n, \
+                               k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit);     \
+        else                                                                                    \
+            hipLaunchKernelGGL((mgs_coop_kernel<false, R>), grid, dim3(kMgsCoopThreads), 0, s,   \
+                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit);  \
+    } while (0)
+            switch (rq) {
+            case 1: CCAB_MGS_COOP(1); break;
+            case 2: CCAB_MGS_COOP(2); break;
+            case 4: CCAB_MGS_COOP(4); break;
+            default: CCAB_MGS_COOP(8); break;
+            }
+#undef CCAB_MGS_COOP
+            CCAB_HIP(hipGetLastError());
+            return CCAB_OK;
+        }
+    }
+    if (ref)
+        hipLaunchKernelGGL(mgs_kernel<true>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k, Q,
+                           ldq, sQ, h, sh, act);
+    else
+        hipLaunchKernelGGL(mgs_kernel<false>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k,
+                           Q, ldq, sQ, h, sh, act);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
 }
 
 __global__ void scale_kernel(int64_t n, const c128* __restrict__ src, c128 s, c128* dst) {
@@ -283,12 +461,7 @@ int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, 
             ++iters;
             CCAB_TRY(matvec(Q + k * n, one, zero, Q + (k + 1) * n));
             ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0);
-            if (p.ref_mgs)
-                hipLaunchKernelGGL(mgs_kernel<true>, dim3(1), dim3(kMgsThreads), 0, s, n, k, Q, n,
-                                   0, h, 0, nullptr);
-            else
-                hipLaunchKernelGGL(mgs_kernel<false>, dim3(1), dim3(kMgsThreads), 0, s, n, k, Q, n,
-                                   0, h, 0, nullptr);
+            CCAB_TRY(launch_mgs(ctx, p.ref_mgs != 0, n, k, Q, n, 0, h, 0, nullptr, 1));
             CCAB_HIP(hipMemcpyAsync(hp, h, 16 * (k + 2), hipMemcpyDeviceToHost, s));
             CCAB_HIP(hipStreamSynchronize(s));
             for (int64_t i = 0; i < k + 2; ++i) hk[i] = cd(hp[2 * i], hp[2 * i + 1]);
@@ -567,12 +740,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                                  one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg));
             {
                 ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
-                if (ref_mgs)
-                    hipLaunchKernelGGL(mgs_kernel<true>, dim3(ni), dim3(kMgsThreads), 0, s, n, k, Q,
-                                       n, sq, Hd, m + 2, act);
-                else
-                    hipLaunchKernelGGL(mgs_kernel<false>, dim3(ni), dim3(kMgsThreads), 0, s, n, k,
-                                       Q, n, sq, Hd, m + 2, act);
+                CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni));
             }
             CCAB_HIP(hipMemcpyAsync(hH, Hd, pin_h, hipMemcpyDeviceToHost, s));
             CCAB_HIP(hipStreamSynchronize(s));
