@@ -10,7 +10,10 @@
 // The matvec writes every block's contribution to a per-(block,row) slot and
 // a row-wise CSR reduction adds them in a fixed order (no atomics).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -486,19 +489,20 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(SysParams p, const dou
 
 // --- recompression (aca.rs:137-179): QR(U), QR(V) by CGS2, SVD of R_u R_v^T
 //     by one-sided Jacobi (one wavefront, LDS), truncation, new U and V -------
+template <int RM>
 __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
                                                                  const int32_t* __restrict__ list,
                                                                  c128* U, c128* V, c128* U2,
                                                                  c128* V2, c128* Rws, double tol) {
-    __shared__ c128 Wm[RMAX][RMAX + 1];  // [row][col]
-    __shared__ c128 Vs[RMAX][RMAX + 1];
+    __shared__ c128 Wm[RM][RM + 1];  // [row][col]
+    __shared__ c128 Vs[RM][RM + 1];
     __shared__ double red[2 * kAcaThreads / 64];
-    __shared__ double sv[RMAX];
-    __shared__ int ord[RMAX];
-    __shared__ double s_tmp[RMAX][2][kAcaThreads / 64];
+    __shared__ double sv[RM];
+    __shared__ int ord[RM];
+    __shared__ double s_tmp[RM][2][kAcaThreads / 64];
     HBlkDev& bk = blks[list[blockIdx.x]];
     if (bk.status != 0) return;
-    const int r = bk.rank;
+    const int r = bk.rank;  // <= RM (the host buckets blocks by rank)
     const int64_t nr = bk.nr, nc = bk.nc;
     c128* Ru = Rws + int64_t(blockIdx.x) * 2 * RMAX * RMAX;  // [c][i] = R[i][c]
     c128* Rv = Ru + RMAX * RMAX;
@@ -801,9 +805,32 @@ struct ccab_hmatrix {
     }
 };
 
+namespace {
+// phase times of the build on stderr when CCAB_TRACE_HBUILD is set (each mark
+// waits for the stream, so traced builds are slower)
+struct BuildTrace {
+    hipStream_t s;
+    bool on;
+    std::chrono::steady_clock::time_point t0, t;
+    explicit BuildTrace(hipStream_t st) : s(st), on(std::getenv("CCAB_TRACE_HBUILD") != nullptr) {
+        t0 = t = std::chrono::steady_clock::now();
+    }
+    void mark(const char* what) {
+        if (!on) return;
+        (void)hipStreamSynchronize(s);
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[hbuild] %-28s %8.2f ms (total %8.2f ms)\n", what,
+                     std::chrono::duration<double, std::milli>(now - t).count(),
+                     std::chrono::duration<double, std::milli>(now - t0).count());
+        t = now;
+    }
+};
+}  // namespace
+
 int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
                   int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out) {
     hipStream_t s = ctx->stream;
+    BuildTrace trace(s);
     std::unique_ptr<ccab_hmatrix> owner(new ccab_hmatrix());
     ccab_hmatrix* H = owner.get();
     H->n = n;
@@ -819,21 +846,29 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     std::vector<HostBlock> bl;
     build_blocks(T, 0, 0, eta, bl);
     const int64_t nb = (int64_t)bl.size();
-    std::vector<int64_t> rows, cols;
+    trace.mark("cluster and block trees");
+    // every cluster's index list stored once (n x tree depth entries); a
+    // block's rows / columns are its clusters' lists, so rows[] and cols[]
+    // are the same array
+    std::vector<int64_t> node_off(T.size());
+    int64_t nidx = 0;
+    for (size_t c = 0; c < T.size(); ++c) {
+        node_off[c] = nidx;
+        nidx += (int64_t)T[c].idx.size();
+    }
+    std::vector<int64_t> rows(nidx);
+    for (size_t c = 0; c < T.size(); ++c)
+        std::copy(T[c].idx.begin(), T[c].idx.end(), rows.begin() + node_off[c]);
     H->blk.resize(nb);
     int64_t u_off = 0, v_off = 0, d_off = 0, p_off = 0, w_off = 0, f_off = 0;
     std::vector<int32_t> dl, ll;
     for (int64_t b = 0; b < nb; ++b) {
         HBlkDev& d = H->blk[b];
         std::memset(&d, 0, sizeof(d));
-        const auto& R = T[bl[b].r].idx;
-        const auto& C = T[bl[b].c].idx;
-        d.r_off = (int64_t)rows.size();
-        d.nr = (int64_t)R.size();
-        d.c_off = (int64_t)cols.size();
-        d.nc = (int64_t)C.size();
-        rows.insert(rows.end(), R.begin(), R.end());
-        cols.insert(cols.end(), C.begin(), C.end());
+        d.r_off = node_off[bl[b].r];
+        d.nr = (int64_t)T[bl[b].r].idx.size();
+        d.c_off = node_off[bl[b].c];
+        d.nc = (int64_t)T[bl[b].c].idx.size();
         d.adm = bl[b].adm ? 1 : 0;
         d.p_off = p_off;
         p_off += d.nr;
@@ -855,8 +890,9 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             dl.push_back((int32_t)b);
         }
     }
+    trace.mark("block layout");
     H->d_rows = dalloc<int64_t>(H->owned, rows.size());
-    H->d_cols = dalloc<int64_t>(H->owned, cols.size());
+    H->d_cols = H->d_rows;
     H->d_blk = dalloc<HBlkDev>(H->owned, nb);
     H->d_U = dalloc<c128>(H->owned, u_off);
     H->d_V = dalloc<c128>(H->owned, v_off);
@@ -875,10 +911,10 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     std::vector<uint32_t> words(kRngTable);
     chacha12_words(10, words.data(), kRngTable);
     CCAB_HIP(hipMemcpy(H->d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_cols, cols.data(), cols.size() * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(d_rng, words.data(), kRngTable * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll.data(), ll.size() * 4, hipMemcpyHostToDevice));
+    trace.mark("allocation and upload");
     const c128 fa = side == CCAB_LHS ? p.lhs_a : p.rhs_a;
     const c128 fb = side == CCAB_LHS ? p.lhs_b : p.rhs_b;
     // batched ACA + recompression of the admissible blocks
@@ -886,8 +922,43 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         hipLaunchKernelGGL(aca_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, p, d_soa,
                            n, H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, fa, fb, H->d_U,
                            H->d_V, Wws, Fws, d_rng, tol);
-        hipLaunchKernelGGL(recompress_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s,
-                           H->d_blk, H->d_lr_list, H->d_U, H->d_V, U2, V2, Rws, tol);
+        // recompression with LDS sized by rank: blocks bucketed by their ACA
+        // rank (most are far below RMAX, and the RMAX-sized kernel holds one
+        // workgroup per CU)
+        CCAB_HIP(hipMemcpyAsync(H->blk.data(), H->d_blk, nb * sizeof(HBlkDev), hipMemcpyDeviceToHost, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+        trace.mark("ACA");
+        constexpr int kBuckets = 4;
+        const int cap[kBuckets] = {8, 16, 32, RMAX};
+        std::vector<int32_t> bl[kBuckets];
+        for (int32_t b : ll) {
+            const HBlkDev& d = H->blk[b];
+            if (d.status != 0) continue;  // becomes dense below
+            int k = 0;
+            while (k < kBuckets - 1 && d.rank > cap[k]) ++k;
+            bl[k].push_back(b);
+        }
+        std::vector<int32_t> order;
+        for (auto& v : bl) order.insert(order.end(), v.begin(), v.end());
+        if (!order.empty()) {
+            int32_t* d_order = dalloc<int32_t>(H->owned, order.size());
+            if (!d_order) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+            CCAB_HIP(hipMemcpy(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+            size_t off = 0;
+            for (int k = 0; k < kBuckets; ++k) {
+                const unsigned cnt = (unsigned)bl[k].size();
+                if (cnt == 0) continue;
+                const int32_t* lst = d_order + off;
+                switch (k) {
+                case 0: hipLaunchKernelGGL(recompress_kernel<8>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
+                case 1: hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
+                case 2: hipLaunchKernelGGL(recompress_kernel<32>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
+                default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
+                }
+                off += cnt;
+            }
+        }
+        trace.mark("recompression");
         std::swap(H->d_U, U2);
         std::swap(H->d_V, V2);
         hipLaunchKernelGGL(aca_norm_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s,
@@ -928,6 +999,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     CCAB_HIP(hipGetLastError());
     CCAB_HIP(hipMemcpyAsync(H->blk.data(), H->d_blk, nb * sizeof(HBlkDev), hipMemcpyDeviceToHost, s));
     CCAB_HIP(hipStreamSynchronize(s));
+    trace.mark("norms and dense blocks");
     // norm = sum of dense Frobenius norms, then the low-rank estimates (h_matrix.rs:189-196)
     double nd = 0.0, nl = 0.0;
     for (int32_t b : dl) nd += H->blk[b].norm;
@@ -953,6 +1025,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     if (!H->d_rowptr || !H->d_ent) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
     CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), p_off * 8, hipMemcpyHostToDevice));
+    trace.mark("matvec plan");
     // release the build workspace
     for (void* w : {static_cast<void*>(U2), static_cast<void*>(V2), static_cast<void*>(Wws),
                     static_cast<void*>(Fws), static_cast<void*>(Rws)}) {
@@ -962,6 +1035,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             H->owned.erase(it);
         }
     }
+    trace.mark("workspace release");
     *out = owner.release();
     return CCAB_OK;
 }
