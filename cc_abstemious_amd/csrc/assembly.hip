@@ -13,6 +13,10 @@
 // is 64 consecutive entries of one column: 512 B (c64) / 1 KiB (c128), full
 // 128-byte lines.  The kernel is HBM-store bound: 8 B (c64) or 16 B (c128)
 // per stored entry and matrix.
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+
 #include "ccab_internal.h"
 #include "greens.h"
 
@@ -97,6 +101,88 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
         double* dst = rowsum_part + 2 * ((b * nchunks + blockIdx.y) * n + i);
         dst[0] = rs_re;
         dst[1] = rs_im;
+    }
+}
+
+// ASM_LHS assembly of a frequency sweep over one mesh, FP64 arithmetic, kF
+// systems per thread: the pair geometry (r, 1/r, the normal projections) is
+// computed once for the kF systems.  With equally spaced wavenumbers (step
+// dk != 0) the phasor e^{i k r} comes from sincos for the first system of the
+// group and from e^{i k_f r} = e^{i k_(f-1) r} e^{i dk r} for the others (two
+// sincos per kF entries instead of kF); otherwise from sincos per system.
+// Same layout, outputs and row-sum partials as assemble_kernel.
+constexpr int kF = 8;
+
+template <bool BM, typename OutT>
+__global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
+    const SysParams* __restrict__ params, const double* __restrict__ soa_base, int64_t n,
+    int64_t batch, void* out0, void* out1, int64_t ld, int64_t stride,
+    double* __restrict__ rowsum_part, int64_t nchunks, double dk) {
+    const int64_t b0 = int64_t(blockIdx.z) * kF;
+    const int nf = (int)min<int64_t>(kF, batch - b0);
+    const Soa s = soa_of(soa_base, n);
+    const int64_t i = int64_t(blockIdx.x) * kRows + threadIdx.x;
+    const int64_t j0 = int64_t(blockIdx.y) * kColChunk;
+    const int64_t j1 = min(j0 + kColChunk, n);
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    const double yx = s.x[ic], yy = s.y[ic], yz = s.z[ic];
+    const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
+    const double k0 = params[b0].k;
+    // per-lane row-sum accumulators of the kF systems (private LDS slots)
+    __shared__ double2 rs[kF][kRows];
+    for (int f = 0; f < kF; ++f) rs[f][threadIdx.x] = double2{0.0, 0.0};
+    for (int64_t j = j0; j < j1; ++j) {
+        const double wa = s.wt[j] * s.area[j];
+        const bool diag = j == ic;
+        // the diagonal lane evaluates a unit-distance pair and discards it
+        const double rx = diag ? 1.0 : s.x[j] - yx;
+        const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
+                                                         s.ny[j], s.nz[j], nyx, nyy, nyz);
+        double sn, cs, sd = 0.0, cd = 1.0;
+        sincos(k0 * q.rd, &sn, &cs);
+        if (dk != 0.0) sincos(dk * q.rd, &sd, &cd);
+#pragma unroll 1
+        for (int f = 0; f < nf; ++f) {
+            const SysParams& p = params[b0 + f];
+            if (f > 0) {
+                if (dk != 0.0) {
+                    const double c2 = cs * cd - sn * sd;
+                    sn = sn * cd + cs * sd;
+                    cs = c2;
+                } else {
+                    sincos(p.k * q.rd, &sn, &cs);
+                }
+            }
+            c128 G, H;
+            if (diag) {
+                diag_entry(p, s.area[j], s.wt[j], G, H);
+            } else {
+                c128 g, h;
+                greens_at<double, BM>(q, p.k, p.beta, cs, sn, g, h, p.dh_sign);
+                G = c128{g.re * wa, g.im * wa};
+                H = c128{h.re * wa, h.im * wa};
+            }
+            if (active) {
+                const int64_t off = (b0 + f) * stride + i + j * ld;
+                const c128 L = p.lhs_a * H + p.lhs_b * G;
+                if (out0) store<c128>(out0, off, L);
+                if (out1) store<OutT>(out1, off, L, p.h_scale);
+            }
+            if (p.rowsum_of != 0) {
+                const c128 a = p.rowsum_of == 1 ? G : H;
+                double2& r = rs[f][threadIdx.x];
+                r.x += a.re;
+                r.y += a.im;
+            }
+        }
+    }
+    if (rowsum_part == nullptr || !active) return;
+    for (int f = 0; f < nf; ++f) {
+        if (params[b0 + f].rowsum_of == 0) continue;
+        double* dst = rowsum_part + 2 * (((b0 + f) * nchunks + blockIdx.y) * n + i);
+        dst[0] = rs[f][threadIdx.x].x;
+        dst[1] = rs[f][threadIdx.x].y;
     }
 }
 
@@ -247,7 +333,30 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? (prec_compute == PREC_F64_H ? 4.0 : 8.0) : 0.0);
     }
     ProfScope ps(ctx, PROF_ASSEMBLY, double(n) * double(n) * double(batch) * bytes_per_entry);
-    if (mode == ASM_HG) {
+    static const bool sweep_kernel = [] {
+        const char* e = std::getenv("CCAB_ASM_SWEEP");
+        return !(e && e[0] == '0');
+    }();
+    if (mode == ASM_LHS && prec_compute != PREC_F32 && batch >= 2 && sweep_kernel) {
+        // equally spaced wavenumbers (to 4 ulp of the largest) let the sweep
+        // kernel step the phasor instead of calling sincos per system
+        const double k0 = h_params[0].k;
+        const double step = (h_params[batch - 1].k - k0) / double(batch - 1);
+        double kmax = 0.0;
+        for (int64_t b = 0; b < batch; ++b) kmax = std::max(kmax, std::fabs(h_params[b].k));
+        bool uniform = step != 0.0 && std::isfinite(step);
+        for (int64_t b = 0; b < batch && uniform; ++b)
+            uniform = std::fabs(h_params[b].k - (k0 + double(b) * step)) <= 4.0 * DBL_EPSILON * kmax;
+        const double dk = uniform ? step : 0.0;
+        dim3 g2(grid.x, grid.y, (unsigned)((batch + kF - 1) / kF));
+        if (prec_compute == PREC_F64_H) {
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+        } else {
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+        }
+    } else if (mode == ASM_HG) {
         if (prec_compute == PREC_F64) {
             if (bm) launch_one<double, true, ASM_HG, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);
             else launch_one<double, false, ASM_HG, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);

@@ -18,36 +18,60 @@ __device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(
 __device__ __forceinline__ double sqrt_t(double a) { return sqrt(a); }
 __device__ __forceinline__ float sqrt_t(float a) { return sqrtf(a); }
 
+// Geometry of the pair (source point j at x with normal n_x, collocation
+// point i at y with normal n_y), r = x - y: everything of influence_matrix.rs:
+// 12-37 that does not depend on the wavenumber.
+template <typename T>
+struct PairGeom {
+    T rd, inv_r, rdoty, rdotx, nn;
+};
+
+template <typename T, bool BM>
+__device__ __forceinline__ PairGeom<T> pair_geom(T rx, T ry, T rz, T nxx, T nxy, T nxz, T nyx,
+                                                 T nyy, T nyz) {
+    PairGeom<T> q;
+    q.rd = sqrt_t(rx * rx + ry * ry + rz * rz);
+    q.inv_r = T(1) / q.rd;
+    const T erx = rx * q.inv_r, ery = ry * q.inv_r, erz = rz * q.inv_r;
+    q.rdoty = erx * nyx + ery * nyy + erz * nyz;
+    q.rdotx = BM ? erx * nxx + ery * nxy + erz * nxz : T(0);
+    q.nn = BM ? nyx * nxx + nyy * nxy + nyz * nxz : T(0);
+    return q;
+}
+
+// Kernel pair (g, h) at wavenumber k from the geometry and the phasor
+// (c, s) = (cos kr, sin kr).  dh_sign: -1 as the reference
+// (influence_matrix.rs:30), +1 for d^2 G / dn_x dn_y.
+template <typename T, bool BM>
+__device__ __forceinline__ void greens_at(const PairGeom<T>& q, T k, Cx<T> beta, T c, T s,
+                                          Cx<T>& g, Cx<T>& h, T dh_sign) {
+    const T gs = q.inv_r * T(1.0 / (4.0 * kPi));
+    g = Cx<T>{c * gs, s * gs};
+    const Cx<T> f1{-q.inv_r, k};
+    const Cx<T> gf1 = g * f1;
+    h = gf1 * (-q.rdoty);
+    if (BM) {
+        const Cx<T> dg = gf1 * q.rdotx;
+        const T rr = q.rdotx * q.rdoty * dh_sign;
+        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)  (dh_sign = -1)
+        const Cx<T> term{-f1.re * q.nn + (k * k * q.rd + T(3) * f1.re) * rr,
+                         -f1.im * q.nn + T(3) * f1.im * rr};
+        const Cx<T> dh = (g * q.inv_r) * term;
+        g += beta * dg;
+        h += beta * dh;
+    }
+}
+
 // Off-diagonal kernel pair (g, h) for source point j (x, n_x) and collocation
-// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.  dh_sign: -1 as
-// the reference (influence_matrix.rs:30), +1 for d^2 G / dn_x dn_y.
+// point i (y, n_y), r = x - y.  influence_matrix.rs:12-37.
 template <typename T, bool BM>
 __device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx, T nxy, T nxz,
                                        T nyx, T nyy, T nyz, Cx<T>& g, Cx<T>& h,
                                        T dh_sign = T(-1)) {
-    const T rd = sqrt_t(rx * rx + ry * ry + rz * rz);
-    const T inv_r = T(1) / rd;
-    const T erx = rx * inv_r, ery = ry * inv_r, erz = rz * inv_r;
+    const PairGeom<T> q = pair_geom<T, BM>(rx, ry, rz, nxx, nxy, nxz, nyx, nyy, nyz);
     T s, c;
-    sincos_t(k * rd, &s, &c);
-    const T gs = inv_r * T(1.0 / (4.0 * kPi));
-    g = Cx<T>{c * gs, s * gs};
-    const Cx<T> f1{-inv_r, k};
-    const T rdoty = erx * nyx + ery * nyy + erz * nyz;
-    const Cx<T> gf1 = g * f1;
-    h = gf1 * (-rdoty);
-    if (BM) {
-        const T rdotx = erx * nxx + ery * nxy + erz * nxz;
-        const Cx<T> dg = gf1 * rdotx;
-        const T nn = nyx * nxx + nyy * nxy + nyz * nxz;
-        const T rr = rdotx * rdoty * dh_sign;
-        // -f1 (n_y.n_x) - (k^2 r + 3 f1)(e.n_x)(e.n_y)  (dh_sign = -1)
-        const Cx<T> term{-f1.re * nn + (k * k * rd + T(3) * f1.re) * rr,
-                         -f1.im * nn + T(3) * f1.im * rr};
-        const Cx<T> dh = (g * inv_r) * term;
-        g += beta * dg;
-        h += beta * dh;
-    }
+    sincos_t(k * q.rd, &s, &c);
+    greens_at<T, BM>(q, k, beta, c, s, g, h, dh_sign);
 }
 
 // Singular diagonal (influence_matrix.rs:42-50) incl. the global shifts of

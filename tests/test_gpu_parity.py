@@ -101,6 +101,50 @@ def test_assemble_system_fused(sphere, bc, bcv):
     assert _rel(rhs[0].cpu().numpy(), R) < 1e-12
 
 
+@pytest.mark.parametrize("spacing", ["uniform", "irregular"])
+@pytest.mark.parametrize("bc,bcv", [(capi.BC_NORMAL_VELOCITY, (0.5, -0.2)),
+                                    (capi.BC_PRESSURE, (0.3, 0.1))])
+def test_assemble_system_sweep(sphere, spacing, bc, bcv):
+    """One batched assembly over 11 frequencies (8 systems per thread sharing
+    the pair geometry; with equally spaced wavenumbers e^{ikr} is stepped by
+    e^{i dk r} from the first system of each group of 8) against one assembly
+    per frequency and, for the last stepped system of the full group (7) and
+    of the partial group (10), against the oracle."""
+    if spacing == "uniform":
+        ka = np.linspace(0.5, 6.0, 11)
+    else:
+        ka = np.array([0.5, 0.7, 1.3, 1.4, 2.9, 3.0, 3.7, 4.4, 5.2, 5.3, 6.0])
+    fs = ka * 343.0 / (2 * math.pi)
+    _note(f"batched assembly, {spacing} frequencies, bc={bc}")
+    pd = _predata(sphere, capi.BURTON_MILLER, bc, bcv, f=fs[0], c=343.0, rho=1.21)
+    n = pd.get_num_eqn()
+    g = torch.Generator().manual_seed(3)
+    rhs_inc = torch.complex(torch.randn(len(fs), n, generator=g, dtype=torch.float64),
+                            torch.randn(len(fs), n, generator=g, dtype=torch.float64)).to(pd.device)
+    l128, l64, rhs = influence_matrix.assemble_system(pd, rhs_inc, frequencies=fs,
+                                                      want_c128=True, want_c64=True)
+    worst = 0.0
+    for b, f in enumerate(fs):
+        s128, s64, srhs = influence_matrix.assemble_system(pd, rhs_inc[b:b + 1], frequencies=[f],
+                                                           want_c128=True, want_c64=True)
+        e = _rel(l128[b].cpu().numpy(), s128[0].cpu().numpy())
+        worst = max(worst, e)
+        assert e < 1e-14, (b, e)
+        assert _rel(l64[b].cpu().numpy(), s64[0].cpu().numpy()) < 2e-7, b
+        assert _rel(rhs[b].cpu().numpy(), srhs[0].cpu().numpy()) < 1e-13, b
+    print(f"max entry difference batched vs per frequency: {worst:.2e}")
+    v = complex(*bcv)
+    for b in (7, 10):
+        ph = O.physics(fs[b], 343.0, 1.21, capi.BURTON_MILLER, pd.problem_type, bc, bcv)
+        H, G = O.dense_surface_matrices(ph, pd.cpts)
+        L = H if bc == capi.BC_NORMAL_VELOCITY else G
+        assert _rel(l128[b].cpu().numpy(), L) < 1e-13, b
+        r = G @ np.full(n, v) if bc == capi.BC_NORMAL_VELOCITY else -(H @ np.full(n, v))
+        ri = rhs_inc[b].cpu().numpy()
+        expect = r - ri if bc == capi.BC_NORMAL_VELOCITY else r + ri
+        assert _rel(rhs[b].cpu().numpy(), expect) < 1e-12, b
+
+
 @pytest.mark.parametrize("side", [capi.LHS, capi.RHS])
 def test_row_or_column(sphere, side):
     _note(f"row/column side={side}")
