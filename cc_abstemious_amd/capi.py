@@ -88,7 +88,7 @@ class SolverCfg(ct.Structure):
         ("gmres", GmresParams),
         ("ir_tolerance", ct.c_double),
         ("ir_max_steps", ct.c_int32),
-        ("pad_", ct.c_int32),
+        ("residual_matrix", ct.c_int32),
         ("aca_tolerance", ct.c_double),
         ("eta", ct.c_double),
     ]
@@ -230,12 +230,14 @@ def gmres_params(max_iterations=1000, tolerance=1e-5, restart=0, ref_mgs=True,
 
 
 def solver_cfg(solver=SOLVER_DIRECT, gmres: GmresParams | None = None, ir_tolerance=1e-12,
-               ir_max_steps=10, leaf_size=32, aca_tolerance=1e-4, eta=4.0) -> SolverCfg:
+               ir_max_steps=10, leaf_size=32, aca_tolerance=1e-4, eta=4.0,
+               residual_matrix=C128) -> SolverCfg:
     c = SolverCfg()
     c.solver = int(solver)
     c.gmres = gmres if gmres is not None else gmres_params()
     c.ir_tolerance = float(ir_tolerance)
     c.ir_max_steps = int(ir_max_steps)
+    c.residual_matrix = int(residual_matrix)
     c.leaf_size = int(leaf_size)
     c.aca_tolerance = float(aca_tolerance)
     c.eta = float(eta)

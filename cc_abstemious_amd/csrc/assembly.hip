@@ -110,13 +110,16 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
 // dk != 0) the phasor e^{i k r} comes from sincos for the first system of the
 // group and from e^{i k_f r} = e^{i k_(f-1) r} e^{i dk r} for the others (two
 // sincos per kF entries instead of kF); otherwise from sincos per system.
-// Same layout, outputs and row-sum partials as assemble_kernel.
+// Same layout, outputs and row-sum partials as assemble_kernel; with OutT =
+// c32 and out2 set, out2 also receives the complex-FP32 remainder
+// L - fp16(L h_scale) / h_scale (the FP16 copy plus the remainder stand for
+// the FP64 matrix to about 2^-35 relative per entry).
 constexpr int kF = 8;
 
 template <bool BM, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     const SysParams* __restrict__ params, const double* __restrict__ soa_base, int64_t n,
-    int64_t batch, void* out0, void* out1, int64_t ld, int64_t stride,
+    int64_t batch, void* out0, void* out1, void* out2, int64_t ld, int64_t stride,
     double* __restrict__ rowsum_part, int64_t nchunks, double dk) {
     const int64_t b0 = int64_t(blockIdx.z) * kF;
     const int nf = (int)min<int64_t>(kF, batch - b0);
@@ -167,7 +170,22 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                 const int64_t off = (b0 + f) * stride + i + j * ld;
                 const c128 L = p.lhs_a * H + p.lhs_b * G;
                 if (out0) store<c128>(out0, off, L);
-                if (out1) store<OutT>(out1, off, L, p.h_scale);
+                if constexpr (sizeof(OutT) == 4) {
+                    if (out2) {
+                        const uint16_t hr = to_half_sat(L.re * p.h_scale);
+                        const uint16_t hi = to_half_sat(L.im * p.h_scale);
+                        reinterpret_cast<c32*>(out1)[off] = c32{hr, hi};
+                        const double inv = 1.0 / p.h_scale;  // a power of two: exact
+                        const double br = double(__half2float(__ushort_as_half(hr))) * inv;
+                        const double bi = double(__half2float(__ushort_as_half(hi))) * inv;
+                        reinterpret_cast<c64*>(out2)[off] =
+                            c64{static_cast<float>(L.re - br), static_cast<float>(L.im - bi)};
+                    } else if (out1) {
+                        store<OutT>(out1, off, L, p.h_scale);
+                    }
+                } else {
+                    if (out1) store<OutT>(out1, off, L, p.h_scale);
+                }
             }
             if (p.rowsum_of != 0) {
                 const c128 a = p.rowsum_of == 1 ? G : H;
@@ -316,7 +334,7 @@ int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d
 int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                     int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
                     void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
-                    int64_t* nchunks_out) {
+                    int64_t* nchunks_out, void* out2) {
     const int64_t nchunks = (n + kColChunk - 1) / kColChunk;
     if (nchunks_out) *nchunks_out = nchunks;
     CCAB_CHECK(batch >= 1 && batch <= 65535, "batch out of range");
@@ -330,18 +348,21 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         const double e = prec_compute == PREC_F64 ? 16.0 : 8.0;
         bytes_per_entry = (out0 ? e : 0.0) + (out1 ? e : 0.0);
     } else {
-        bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? (prec_compute == PREC_F64_H ? 4.0 : 8.0) : 0.0);
+        bytes_per_entry = (out0 ? 16.0 : 0.0) + (out1 ? (prec_compute == PREC_F64_H ? 4.0 : 8.0) : 0.0) +
+                          (out2 ? 8.0 : 0.0);
     }
+    CCAB_CHECK(out2 == nullptr || (mode == ASM_LHS && prec_compute == PREC_F64_H && out1),
+               "the FP32 remainder goes with the FP16 copy of the LHS");
     ProfScope ps(ctx, PROF_ASSEMBLY, double(n) * double(n) * double(batch) * bytes_per_entry);
     static const bool sweep_kernel = [] {
         const char* e = std::getenv("CCAB_ASM_SWEEP");
         return !(e && e[0] == '0');
     }();
-    if (mode == ASM_LHS && prec_compute != PREC_F32 && batch >= 2 && sweep_kernel) {
+    if (mode == ASM_LHS && prec_compute != PREC_F32 && ((batch >= 2 && sweep_kernel) || out2)) {
         // equally spaced wavenumbers (to 4 ulp of the largest) let the sweep
         // kernel step the phasor instead of calling sincos per system
         const double k0 = h_params[0].k;
-        const double step = (h_params[batch - 1].k - k0) / double(batch - 1);
+        const double step = batch >= 2 ? (h_params[batch - 1].k - k0) / double(batch - 1) : 0.0;
         double kmax = 0.0;
         for (int64_t b = 0; b < batch; ++b) kmax = std::max(kmax, std::fabs(h_params[b].k));
         bool uniform = step != 0.0 && std::isfinite(step);
@@ -350,11 +371,11 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         const double dk = uniform ? step : 0.0;
         dim3 g2(grid.x, grid.y, (unsigned)((batch + kF - 1) / kF));
         if (prec_compute == PREC_F64_H) {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
         } else {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, ld, stride, d_rowsum_part, nchunks, dk);
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
         }
     } else if (mode == ASM_HG) {
         if (prec_compute == PREC_F64) {

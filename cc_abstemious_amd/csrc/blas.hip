@@ -175,6 +175,91 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
         if (i0 + r < M) o[r] = c128{re[r], im[r]};
 }
 
+// Product with a matrix held as a scaled complex-FP16 copy H plus a
+// complex-FP32 remainder L: entry = H * inv_scale[b] + L (FP64).  VEC: four
+// consecutive rows per lane, one 16-byte load of H and two of L per column
+// (needs lda, sA, the ragged alignment and the row start multiples of 4);
+// otherwise one row per lane.  Same partial-sum layout as the kernels above.
+template <bool VEC>
+__global__ __launch_bounds__(256) void gemv_split_kernel(int64_t M, int64_t N,
+                                                         const c32* __restrict__ Ah,
+                                                         const c64* __restrict__ Al,
+                                                         const double* __restrict__ inv_scale,
+                                                         int64_t lda, int64_t sA,
+                                                         const c128* __restrict__ x, int64_t sx,
+                                                         int64_t chunk, c128* __restrict__ part,
+                                                         int64_t nsplit,
+                                                         const int32_t* __restrict__ active,
+                                                         const int64_t* __restrict__ rn,
+                                                         const int64_t* __restrict__ raoff) {
+    constexpr int R = VEC ? 4 : 1;
+    const int64_t z = blockIdx.z;
+    const int64_t b = active ? active[z] : z;
+    const int64_t i0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) * R;
+    const int64_t nb = rn ? rn[b] : N;
+    const int64_t mb = rn ? nb : M;
+    const int64_t j0 = int64_t(blockIdx.y) * chunk;
+    const int64_t j1 = i0 >= mb ? j0 : min(j0 + chunk, nb);
+    if (i0 >= M) return;
+    if (rn) lda = nb;
+    const int64_t base = (rn ? raoff[b] : b * sA) + i0;
+    const c32* ah = Ah + base;
+    const c64* al = Al + base;
+    const double isc = inv_scale[b];
+    const c128* xb = x + b * sx;
+    double re[R], im[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) re[r] = im[r] = 0.0;
+    if (VEC && i0 + R <= mb) {
+        auto column = [&](const uint4& vh, const float4& v0, const float4& v1, const c128& xv) {
+            c128 hv[4];
+            unpack16<4>(vh, hv);
+            const double lr[4] = {double(v0.x), double(v0.z), double(v1.x), double(v1.z)};
+            const double li[4] = {double(v0.y), double(v0.w), double(v1.y), double(v1.w)};
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double er = fma(hv[r].re, isc, lr[r]);
+                const double ei = fma(hv[r].im, isc, li[r]);
+                re[r] += er * xv.re - ei * xv.im;
+                im[r] += er * xv.im + ei * xv.re;
+            }
+        };
+        int64_t j = j0;
+        for (; j + 1 < j1; j += 2) {
+            uint4 vh[2];
+            float4 v0[2], v1[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                vh[u] = *reinterpret_cast<const uint4*>(ah + (j + u) * lda);
+                v0[u] = *reinterpret_cast<const float4*>(al + (j + u) * lda);
+                v1[u] = *reinterpret_cast<const float4*>(al + (j + u) * lda + 2);
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) column(vh[u], v0[u], v1[u], xb[j + u]);
+        }
+        if (j < j1)
+            column(*reinterpret_cast<const uint4*>(ah + j * lda),
+                   *reinterpret_cast<const float4*>(al + j * lda),
+                   *reinterpret_cast<const float4*>(al + j * lda + 2), xb[j]);
+    } else {
+        for (int64_t j = j0; j < j1; ++j) {
+            const c128 xv = xb[j];
+            for (int r = 0; r < R && i0 + r < mb; ++r) {
+                const c128 h = ld_c(ah + r, j * lda);
+                const c128 l = ld_c(al + r, j * lda);
+                const double er = fma(h.re, isc, l.re);
+                const double ei = fma(h.im, isc, l.im);
+                re[r] += er * xv.re - ei * xv.im;
+                im[r] += er * xv.im + ei * xv.re;
+            }
+        }
+    }
+    c128* o = part + (z * nsplit + blockIdx.y) * M + i0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (i0 + r < M) o[r] = c128{re[r], im[r]};
+}
+
 __global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int64_t nsplit,
                                    c128 alpha, c128 beta, c128* y, int64_t sy,
                                    const int32_t* __restrict__ active,
@@ -254,9 +339,11 @@ size_t gemv_workspace_bytes(int64_t n, int64_t batch) { return 0; }
 int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
-                int64_t stridey, const double* d_inv_scale, const Ragged* rg) {
+                int64_t stridey, const double* d_inv_scale, const Ragged* rg, const c64* d_Alo) {
     if (M <= 0 || nact <= 0) return CCAB_OK;
     if (prec == CCAB_C32 && !d_inv_scale) return ctx_fail(ctx, CCAB_EINVAL, "c32 GEMV needs scales");
+    if (d_Alo && prec != CCAB_C32)
+        return ctx_fail(ctx, CCAB_EINVAL, "the FP32 remainder goes with the c32 matrix");
     if (rg && (!rg->n || !rg->aoff || M != N))
         return ctx_fail(ctx, CCAB_EINVAL, "ragged GEMV needs sizes, offsets and square systems");
     const int64_t* rn = rg ? rg->n : nullptr;
@@ -264,7 +351,8 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     // rows per lane: one 16-byte load per column where the layout allows it
     int R = prec == CCAB_C32 ? 4 : prec == CCAB_C64 ? 2 : 1;
     if ((rg ? rg->align % R != 0 : (lda % R != 0 || strideA % R != 0)) ||
-        (reinterpret_cast<uintptr_t>(d_A) % 16) != 0)
+        (reinterpret_cast<uintptr_t>(d_A) % 16) != 0 ||
+        (d_Alo && reinterpret_cast<uintptr_t>(d_Alo) % 16 != 0))
         R = 1;
     const int64_t rowblocks = (M + 256 * R - 1) / (256 * R);
     const int64_t finish_blocks = (M + 255) / 256;
@@ -279,11 +367,19 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
     dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)nact);
-    ProfScope ps(ctx, prec == CCAB_C128 ? PROF_GEMV128 : PROF_GEMV,
+    ProfScope ps(ctx, (prec == CCAB_C128 || d_Alo) ? PROF_GEMV128 : PROF_GEMV,
                  (rg ? rg->elems : double(M) * double(N) * double(nact)) *
-                     (prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
+                     (d_Alo ? 12.0 : prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
     if (N > 0) {
-        if (prec == CCAB_C128)
+        if (d_Alo && R == 4)
+            hipLaunchKernelGGL(gemv_split_kernel<true>, grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c32*>(d_A), d_Alo, d_inv_scale, lda, strideA, d_x,
+                               stridex, chunk, part, nsplit, d_active, rn, raoff);
+        else if (d_Alo)
+            hipLaunchKernelGGL(gemv_split_kernel<false>, grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c32*>(d_A), d_Alo, d_inv_scale, lda, strideA, d_x,
+                               stridex, chunk, part, nsplit, d_active, rn, raoff);
+        else if (prec == CCAB_C128)
             hipLaunchKernelGGL(gemv_part_kernel<c128>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
                                part, nsplit, d_active, rn, raoff);
@@ -308,7 +404,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     }
     hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)finish_blocks, (unsigned)nact), dim3(256),
                        0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active,
-                       prec == CCAB_C32 ? d_inv_scale : nullptr);
+                       (prec == CCAB_C32 && !d_Alo) ? d_inv_scale : nullptr);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

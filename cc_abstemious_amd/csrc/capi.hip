@@ -17,7 +17,8 @@ int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
-                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg = nullptr);
+                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg = nullptr,
+                 const c64* Alo = nullptr);
 
 // ---------------------------------------------------------------------------
 // context plumbing
@@ -656,6 +657,14 @@ int ccab_gmres_dense(ccab_ctx* ctx, int64_t n, const void* d_A, int64_t lda, cca
 
 }  // extern "C"
 
+static int check_residual_matrix(ccab_ctx* ctx, const ccab_solver_cfg& cfg) {
+    CCAB_CHECK(cfg.residual_matrix == CCAB_C128 ||
+                   (cfg.residual_matrix == CCAB_C32 && cfg.solver == CCAB_SOLVER_ITERATIVE_MIXED &&
+                    cfg.gmres.matrix_prec == CCAB_C32),
+               "residual_matrix CCAB_C32 needs ITERATIVE_MIXED with the CCAB_C32 operator");
+    return CCAB_OK;
+}
+
 // Shared by ccab_solve_surface (one frequency, given rhs_inc) and ccab_sweep
 // (incident wave evaluated per frequency).
 static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
@@ -663,6 +672,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
                       const ccab_solver_cfg* cfg, ccab_c128* d_phi, ccab_c128* d_vn,
                       ccab_solve_stats* stats, const ccab_c128* d_x0 = nullptr) {
     CCAB_CHECK(ph && d_cpts && cfg && d_phi && d_vn && n > 0 && nfreq > 0, "invalid arguments");
+    CCAB_TRY(check_residual_matrix(ctx, *cfg));
     const int bc = ph[0].bc_type;
     for (int64_t f = 1; f < nfreq; ++f) {
         CCAB_CHECK(ph[f].bc_type == bc && ph[f].method == ph[0].method,
@@ -692,6 +702,9 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         return CCAB_OK;
     }
     const size_t n2 = size_t(n) * n;
+    const bool half_op = solver == CCAB_SOLVER_ITERATIVE_MIXED && cfg->gmres.matrix_prec == CCAB_C32;
+    // FP64 residuals against the FP16 operator + its FP32 remainder
+    const bool split_res = half_op && cfg->residual_matrix == CCAB_C32;
     // process frequencies in chunks that fit a memory budget
     size_t per = 0;
     if (solver == CCAB_SOLVER_DIRECT) per = n2 * 16;
@@ -699,7 +712,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     else if (solver == CCAB_SOLVER_ITERATIVE)
         per = n2 * (cfg->gmres.matrix_prec == CCAB_C64 ? 8 : 16);
     else if (solver == CCAB_SOLVER_ITERATIVE_MIXED)
-        per = n2 * (cfg->gmres.matrix_prec == CCAB_C32 ? 20 : 24);
+        per = n2 * (split_res ? 12 : cfg->gmres.matrix_prec == CCAB_C32 ? 20 : 24);
     else return ctx_fail(ctx, CCAB_EINVAL, "solver %d not available on the dense path", solver);
     const size_t budget = mem_budget(ctx);
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(nfreq, int64_t(budget / per)));
@@ -713,13 +726,16 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
     }
     void* A = nullptr;
     c64* A64 = nullptr;
-    if (solver == CCAB_SOLVER_DIRECT || solver == CCAB_SOLVER_DIRECT_MIXED ||
-        solver == CCAB_SOLVER_ITERATIVE_MIXED ||
-        (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C128)) {
+    c64* Alo = nullptr;
+    if (split_res) {
+        Alo = static_cast<c64*>(ctx_buf(ctx, SLOT_SOLVE_A, size_t(chunk) * n2 * 8));
+        if (!Alo) return CCAB_ENOMEM;
+    } else if (solver == CCAB_SOLVER_DIRECT || solver == CCAB_SOLVER_DIRECT_MIXED ||
+               solver == CCAB_SOLVER_ITERATIVE_MIXED ||
+               (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C128)) {
         A = ctx_buf(ctx, SLOT_SOLVE_A, size_t(chunk) * n2 * 16);
         if (!A) return CCAB_ENOMEM;
     }
-    const bool half_op = solver == CCAB_SOLVER_ITERATIVE_MIXED && cfg->gmres.matrix_prec == CCAB_C32;
     if (solver == CCAB_SOLVER_DIRECT_MIXED || solver == CCAB_SOLVER_ITERATIVE_MIXED ||
         (solver == CCAB_SOLVER_ITERATIVE && cfg->gmres.matrix_prec == CCAB_C64)) {
         // c64 copy, or the scaled c32 copy for the half-precision GMRES operator
@@ -762,7 +778,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         int64_t nch = 0;
         CCAB_TRY(launch_assemble(ctx, dp + f0, hp.data() + f0, nb, soa, n,
                                  half_op ? PREC_F64_H : PREC_F64, ASM_LHS, A, A64, n, int64_t(n2),
-                                 part, &nch));
+                                 part, &nch, Alo));
         ccab_c128* x = rhs_c + f0 * n;
         CCAB_TRY(launch_rhs_finish(ctx, dp + f0, nb, n, part, nch, x, x));
         if (solver == CCAB_SOLVER_DIRECT) {
@@ -781,7 +797,8 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
                                   half_op ? CCAB_C32 : CCAB_C64, half_op ? inv_scale + f0 : nullptr,
                                   n, int64_t(n2), reinterpret_cast<c128*>(x), x0, cfg->gmres.restart,
                                   cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
-                                  cfg->gmres.ref_mgs != 0, stats ? stats + f0 : nullptr));
+                                  cfg->gmres.ref_mgs != 0, stats ? stats + f0 : nullptr, nullptr,
+                                  Alo));
         } else if (solver == CCAB_SOLVER_DIRECT_MIXED) {
             CCAB_TRY(mixed_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
                                reinterpret_cast<c128*>(x), cfg->ir_tolerance, cfg->ir_max_steps,
@@ -854,9 +871,13 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
                                 d_x0 ? d_x0 + coff[b] : nullptr));
         return CCAB_OK;
     }
+    CCAB_TRY(check_residual_matrix(ctx, *cfg));
     const int bc = ph[0].bc_type;
     const bool half_op = cfg->gmres.matrix_prec == CCAB_C32;
     const size_t esz_op = half_op ? 4 : 8;
+    // FP64 residuals against the FP16 operator + its FP32 remainder
+    const bool split_res = half_op && cfg->residual_matrix == CCAB_C32;
+    const size_t esz_res = split_res ? 8 : 16;
     std::vector<SysParams> hp;
     SysParams* dp;
     CCAB_TRY(stage_params(ctx, ph, nbody, hp, &dp));
@@ -866,7 +887,7 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         int64_t b1 = b0;
         size_t bytes = 0;
         while (b1 < nbody) {
-            const size_t add = size_t(nh[b1]) * nh[b1] * (16 + esz_op);
+            const size_t add = size_t(nh[b1]) * nh[b1] * (esz_res + esz_op);
             if (b1 > b0 && bytes + add > budget) break;
             bytes += add;
             ++b1;
@@ -886,12 +907,14 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         for (int64_t c = 0; c < nc; ++c)
             while (align > 1 && (hn[c] % align != 0 || haoff[c] % align != 0)) align /= 2;
         const int64_t npts = coff[b1] - coff[b0];
-        c128* A128 = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_A, size_t(entries) * 16));
+        void* Ares = ctx_buf(ctx, SLOT_SOLVE_A, size_t(entries) * esz_res);
+        c128* A128 = split_res ? nullptr : static_cast<c128*>(Ares);
+        c64* Alo = split_res ? static_cast<c64*>(Ares) : nullptr;
         char* Aop = static_cast<char*>(ctx_buf(ctx, SLOT_SOLVE_A64, size_t(entries) * esz_op));
         c128* X = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_RHS, size_t(nc) * nmax * 16));
         double* soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(npts) * 64));
         int64_t* dr = static_cast<int64_t*>(ctx_buf(ctx, SLOT_RAGGED, size_t(nc) * 16));
-        if (!A128 || !Aop || !X || !soa || !dr) return CCAB_ENOMEM;
+        if (!Ares || !Aop || !X || !soa || !dr) return CCAB_ENOMEM;
         c128* X0 = nullptr;
         if (d_x0) {
             X0 = static_cast<c128*>(ctx_buf(ctx, SLOT_SOLVE_X, size_t(nc) * nmax * 16));
@@ -954,8 +977,10 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
             CCAB_TRY(launch_incident(ctx, &hp[b], ph[b], w[b], soa_b, n, xb));
             int64_t nch = 0;
             CCAB_TRY(launch_assemble(ctx, dp + b, hp.data() + b, 1, soa_b, n,
-                                     half_op ? PREC_F64_H : PREC_F64, ASM_LHS, A128 + haoff[c],
-                                     Aop + size_t(haoff[c]) * esz_op, n, n * n, part, &nch));
+                                     half_op ? PREC_F64_H : PREC_F64, ASM_LHS,
+                                     A128 ? A128 + haoff[c] : nullptr,
+                                     Aop + size_t(haoff[c]) * esz_op, n, n * n, part, &nch,
+                                     Alo ? Alo + haoff[c] : nullptr));
             CCAB_TRY(launch_rhs_finish(ctx, dp + b, 1, n, part, nch, xb, xb));
             if (X0)
                 CCAB_HIP(hipMemcpyAsync(X0 + c * nmax, d_x0 + coff[b], size_t(n) * 16,
@@ -970,7 +995,7 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         CCAB_TRY(gmres_ir_run(ctx, nmax, nc, A128, Aop, half_op ? CCAB_C32 : CCAB_C64,
                               half_op ? inv_scale : nullptr, nmax, 0, X, X0, cfg->gmres.restart,
                               cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
-                              cfg->gmres.ref_mgs != 0, stats ? stats + b0 : nullptr, &rg));
+                              cfg->gmres.ref_mgs != 0, stats ? stats + b0 : nullptr, &rg, Alo));
         for (int64_t c = 0; c < nc; ++c) {
             const int64_t b = b0 + c;
             CCAB_TRY(finish_surface(ctx, hp.data() + b, dp + b, bc, hn[c], 1,

@@ -271,7 +271,7 @@ constexpr double kPi = 3.14159265358979323846;
 int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                     int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
                     void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
-                    int64_t* nchunks_out);
+                    int64_t* nchunks_out, void* out2 = nullptr);
 int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, int64_t n,
                       const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
                       ccab_c128* d_rhs);
@@ -323,7 +323,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
                 int64_t stridey, const double* d_inv_scale = nullptr,
-                const Ragged* rg = nullptr);
+                const Ragged* rg = nullptr, const c64* d_Alo = nullptr);
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

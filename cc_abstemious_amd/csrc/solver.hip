@@ -610,8 +610,9 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 // ---------------------------------------------------------------------------
 // Batched GMRES with FP64 iterative refinement (no reference counterpart: it
 // replaces per-frequency LU/GMRES in the sweep).  Every cycle computes the
-// FP64 residual r = b - A x with the c128 matrix (or the c64 one when A128 is
-// null), then runs one GMRES(m) cycle on the c64 operator for A d = r (FP64
+// FP64 residual r = b - A x with the c128 matrix (with the FP16 operator plus
+// its complex-FP32 remainder Alo when Alo is given; with the c64 operator
+// when neither is), then runs one GMRES(m) cycle on the c64 operator for A d = r (FP64
 // Krylov vectors, modified Gram-Schmidt in FP64) until ||r - A d|| <=
 // inner_tol ||r||, and updates x += d.  All systems advance together: one
 // batched c64 matvec and one batched MGS launch per Arnoldi step, the Givens
@@ -621,9 +622,12 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
-                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg) {
-    if (op_prec == CCAB_C32 && (!A128 || !d_inv_scale))
-        return ctx_fail(ctx, CCAB_EINVAL, "the FP16 operator needs the c128 matrix and scales");
+                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg, const c64* Alo) {
+    if (op_prec == CCAB_C32 && (!(A128 || Alo) || !d_inv_scale))
+        return ctx_fail(ctx, CCAB_EINVAL,
+                        "the FP16 operator needs scales and the c128 matrix or its FP32 remainder");
+    if (Alo && op_prec != CCAB_C32)
+        return ctx_fail(ctx, CCAB_EINVAL, "the FP32 remainder goes with the FP16 operator");
     if (m <= 0) m = 50;
     // a Krylov space cannot exceed the smallest system
     m = (int)std::min<int64_t>(m, rg ? rg->n_min : n);
@@ -650,8 +654,10 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     int32_t* hAct = reinterpret_cast<int32_t*>(pin + pin_h + pin_y);
     double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + pin_a);
     const c128 one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
-    const void* Ares = A128 ? static_cast<const void*>(A128) : Aop;
-    const int pres = A128 ? CCAB_C128 : CCAB_C64;
+    // residual matrix: c128; or the FP16 operator + FP32 remainder; or the
+    // c64 operator itself
+    const void* Ares = Alo ? Aop : A128 ? static_cast<const void*>(A128) : Aop;
+    const int pres = Alo ? CCAB_C32 : A128 ? CCAB_C128 : CCAB_C64;
 
     hipLaunchKernelGGL(copy_c128_kernel, grid1(nb), dim3(256), 0, s, nb, X, Bv);
     if (X0) hipLaunchKernelGGL(copy_c128_kernel, grid1(nb), dim3(256), 0, s, nb, X0, X);
@@ -702,7 +708,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         if (cyc > 0 || X0) {
             set_elems(active);
             CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n,
-                                 nullptr, rg));
+                                 Alo ? d_inv_scale : nullptr, rg, Alo));
             for (int32_t b : active) nmv[b]++;
         }
         if (cyc == 0 && X0)

@@ -65,12 +65,16 @@ class Solver:
     leaf_size: int = 32              # hierarchical: solve.rs:91,117
     aca_tolerance: float = 1e-4      # hierarchical: solve.rs:92,118
     eta: float = 4.0                 # hierarchical: h_matrix.rs:136
+    # ITERATIVE_MIXED with the C32 operator: FP64 residuals against the c128
+    # matrix (C128) or the FP16 operator plus its complex-FP32 remainder (C32)
+    residual_matrix: int = capi.C128
 
     def to_c(self) -> capi.SolverCfg:
         g = capi.gmres_params(self.max_iterations, self.tolerance, self.restart, self.ref_mgs,
                               self.stop_rule, self.matrix_prec)
         return capi.solver_cfg(self.kind, g, self.ir_tolerance, self.ir_max_steps,
-                               self.leaf_size, self.aca_tolerance, self.eta)
+                               self.leaf_size, self.aca_tolerance, self.eta,
+                               self.residual_matrix)
 
 
 class PreData:

@@ -122,7 +122,12 @@ typedef struct {
     ccab_gmres_params gmres; /* ITERATIVE(_MIXED), HIERARCHICAL */
     double ir_tolerance;     /* mixed: stop when ||dx||/||x|| < this (<=0 -> 1e-12) */
     int32_t ir_max_steps;    /* mixed: <=0 -> 10 */
-    int32_t pad_;
+    /* ITERATIVE_MIXED with the CCAB_C32 operator: the matrix of the FP64
+     * residuals.  CCAB_C128 (0): the c128 matrix (16 B + 4 B stored per
+     * entry).  CCAB_C32: the FP16 operator plus its complex-FP32 remainder
+     * (4 B + 8 B per entry; the sum equals the FP64 entry to about 2^-35
+     * relative).  No reference counterpart. */
+    int32_t residual_matrix;
     double aca_tolerance;    /* HIERARCHICAL: ACA tolerance (<=0 -> 1e-4, solve.rs:92,118) */
     double eta;              /* HIERARCHICAL: admissibility (<=0 -> 4.0, h_matrix.rs:136) */
 } ccab_solver_cfg;

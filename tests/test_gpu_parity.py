@@ -309,6 +309,18 @@ def test_sweep_mixed_matches_fp64():
         assert e_d < 1e-12 and e_m < 1e-10, (k, e_d, e_m)
 
 
+def test_residual_matrix_needs_fp16_operator():
+    _note("residual_matrix C32 with a c64 operator is refused")
+    pd = _predata(mesh.icosphere(1), capi.BURTON_MILLER,
+                  solver=Solver(kind=capi.SOLVER_ITERATIVE_MIXED, matrix_prec=capi.C64,
+                                residual_matrix=capi.C32))
+    with pytest.raises(capi.CcabError, match="residual_matrix"):
+        solve.sweep(pd)
+    pd.solver = Solver(kind=capi.SOLVER_DIRECT, residual_matrix=capi.C32)
+    with pytest.raises(capi.CcabError, match="residual_matrix"):
+        solve.sweep(pd)
+
+
 def test_error_paths():
     _note("error paths")
     ctx = capi.default_context()
@@ -373,11 +385,15 @@ def test_sweep_gmres_ir_matches_fp64(x0_kind):
 
 
 
-@pytest.mark.parametrize("level", [2, 3])
-def test_sweep_gmres_ir_fp16_operator_matches_fp64(level):
+@pytest.mark.parametrize("level,residual", [(2, capi.C128), (3, capi.C128), (3, capi.C32)])
+def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
     """The complex-FP16 (scaled) GMRES operator inside FP64 refinement reaches
-    the FP64 LU solution; only the inner solve sees FP16 entries."""
-    _note(f"sweep GMRES-IR with c32 operator vs c128 LU, icosphere level {level}, BM")
+    the FP64 LU solution; only the inner solve sees FP16 entries.  With
+    residual C32 the refinement's matrix is the FP16 operator plus its
+    complex-FP32 remainder (about 2^-35 relative per entry; without the
+    remainder the error would stay near 1e-4)."""
+    _note(f"sweep GMRES-IR with c32 operator vs c128 LU, icosphere level {level}, BM, "
+          f"residual matrix {residual}")
     m = mesh.icosphere(level)
     ka = np.linspace(0.5, 8.0, 6)
     fs = ka * 343.0 / (2 * math.pi)
@@ -386,7 +402,8 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level):
                  Solver(kind=capi.SOLVER_DIRECT))
     phi_d, vn_d, _ = solve.sweep(pd)
     pd.solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-10, restart=50,
-                       ir_tolerance=1e-3, ir_max_steps=30, ref_mgs=False, matrix_prec=capi.C32)
+                       ir_tolerance=1e-3, ir_max_steps=30, ref_mgs=False, matrix_prec=capi.C32,
+                       residual_matrix=residual)
     phi_h, _, st = solve.sweep(pd)
     for k in range(len(fs)):
         e = float(torch.linalg.vector_norm(phi_h[k] - phi_d[k]) / torch.linalg.vector_norm(phi_d[k]))
