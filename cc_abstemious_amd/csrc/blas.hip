@@ -34,7 +34,18 @@ __device__ __forceinline__ c128 ld_c<c32>(const c32* p, int64_t off) {
 
 // Ragged batches (rn != nullptr): system b is rn[b] x rn[b] at A + raoff[b]
 // with leading dimension rn[b]; rows beyond rn[b] get zero partial sums.
-template <typename TA>
+template <bool NT, typename TA>
+__device__ __forceinline__ c128 ld_m(const TA* p, int64_t off) {
+    if constexpr (NT && sizeof(TA) == 16) {
+        typedef double f64x2_t __attribute__((ext_vector_type(2)));
+        const f64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f64x2_t*>(p + off));
+        return c128{v.x, v.y};
+    } else {
+        return ld_c(p, off);
+    }
+}
+
+template <typename TA, bool NT = false>
 __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, const TA* __restrict__ A,
                                                         int64_t lda, int64_t sA,
                                                         const c128* __restrict__ x, int64_t sx,
@@ -42,9 +53,11 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
                                                         int64_t nsplit,
                                                         const int32_t* __restrict__ active,
                                                         const int64_t* __restrict__ rn,
-                                                        const int64_t* __restrict__ raoff) {
+                                                        const int64_t* __restrict__ raoff,
+                                                        const int32_t* __restrict__ skip) {
     const int64_t z = blockIdx.z;
     const int64_t b = active ? active[z] : z;
+    if (skip && skip[b]) return;
     const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
     const int64_t nb = rn ? rn[b] : N;
     const int64_t j0 = int64_t(blockIdx.y) * chunk;
@@ -59,7 +72,7 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
         c128 av[4], xv[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            av[u] = ld_c(a, (j + u) * lda);
+            av[u] = ld_m<NT>(a, (j + u) * lda);
             xv[u] = xb[j + u];
         }
 #pragma unroll
@@ -69,7 +82,7 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
         }
     }
     for (; j < j1; ++j) {
-        const c128 a0 = ld_c(a, j * lda);
+        const c128 a0 = ld_m<NT>(a, j * lda);
         const c128 x0 = xb[j];
         re[0] += a0.re * x0.re - a0.im * x0.im;
         im[0] += a0.re * x0.im + a0.im * x0.re;
@@ -107,7 +120,22 @@ __device__ __forceinline__ void unpack16(const uint4& v, c128* a) {
     a[3] = c128{hf(v.w & 0xffffu), hf(v.w >> 16)};
 }
 
-template <typename TA, int R>
+// one 16-byte matrix load; NT: non-temporal (the matrix streams through once
+// per product, x and the partial sums are the data worth caching)
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+template <bool NT, typename V>
+__device__ __forceinline__ V ld16(const void* p) {
+    if constexpr (NT) {
+        const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+        V out;
+        __builtin_memcpy(&out, &v, 16);
+        return out;
+    } else {
+        return *reinterpret_cast<const V*>(p);
+    }
+}
+
+template <typename TA, int R, bool NT = false>
 __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N, const TA* __restrict__ A,
                                                             int64_t lda, int64_t sA,
                                                             const c128* __restrict__ x, int64_t sx,
@@ -115,10 +143,12 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
                                                             int64_t nsplit,
                                                             const int32_t* __restrict__ active,
                                                             const int64_t* __restrict__ rn,
-                                                            const int64_t* __restrict__ raoff) {
+                                                            const int64_t* __restrict__ raoff,
+                                                            const int32_t* __restrict__ skip) {
     using V = typename Vec16<TA, R>::type;
     const int64_t z = blockIdx.z;
     const int64_t b = active ? active[z] : z;
+    if (skip && skip[b]) return;
     const int64_t i0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) * R;
     const int64_t nb = rn ? rn[b] : N;     // columns (and, ragged, rows) of this system
     const int64_t mb = rn ? nb : M;
@@ -136,7 +166,7 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
         for (; j + 3 < j1; j += 4) {
             V v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const V*>(a + (j + u) * lda);
+            for (int u = 0; u < 4; ++u) v[u] = ld16<NT, V>(a + (j + u) * lda);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 c128 av[R];
@@ -151,7 +181,7 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
         }
         for (; j < j1; ++j) {
             c128 av[R];
-            unpack16<R>(*reinterpret_cast<const V*>(a + j * lda), av);
+            unpack16<R>(ld16<NT, V>(a + j * lda), av);
             const c128 xv = xb[j];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -191,10 +221,12 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(int64_t M, int64_t N,
                                                          int64_t nsplit,
                                                          const int32_t* __restrict__ active,
                                                          const int64_t* __restrict__ rn,
-                                                         const int64_t* __restrict__ raoff) {
+                                                         const int64_t* __restrict__ raoff,
+                                                         const int32_t* __restrict__ skip) {
     constexpr int R = VEC ? 4 : 1;
     const int64_t z = blockIdx.z;
     const int64_t b = active ? active[z] : z;
+    if (skip && skip[b]) return;
     const int64_t i0 = (int64_t(blockIdx.x) * 256 + threadIdx.x) * R;
     const int64_t nb = rn ? rn[b] : N;
     const int64_t mb = rn ? nb : M;
@@ -263,9 +295,11 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(int64_t M, int64_t N,
 __global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int64_t nsplit,
                                    c128 alpha, c128 beta, c128* y, int64_t sy,
                                    const int32_t* __restrict__ active,
-                                   const double* __restrict__ inv_scale) {
+                                   const double* __restrict__ inv_scale,
+                                   const int32_t* __restrict__ skip) {
     const int64_t z = blockIdx.y;
     const int64_t b = active ? active[z] : z;
+    if (skip && skip[b]) return;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
     c128 s{0.0, 0.0};
@@ -339,7 +373,8 @@ size_t gemv_workspace_bytes(int64_t n, int64_t batch) { return 0; }
 int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
-                int64_t stridey, const double* d_inv_scale, const Ragged* rg, const c64* d_Alo) {
+                int64_t stridey, const double* d_inv_scale, const Ragged* rg, const c64* d_Alo,
+                const int32_t* d_skip) {
     if (M <= 0 || nact <= 0) return CCAB_OK;
     if (prec == CCAB_C32 && !d_inv_scale) return ctx_fail(ctx, CCAB_EINVAL, "c32 GEMV needs scales");
     if (d_Alo && prec != CCAB_C32)
@@ -371,40 +406,55 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                  (rg ? rg->elems : double(M) * double(N) * double(nact)) *
                      (d_Alo ? 12.0 : prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
     if (N > 0) {
+        const bool nt = ctx->gemv_nt;
+        // (the split product keeps ordinary loads: non-temporal ones measured
+        // slower for its paired 16-byte remainder loads)
         if (d_Alo && R == 4)
             hipLaunchKernelGGL(gemv_split_kernel<true>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c32*>(d_A), d_Alo, d_inv_scale, lda, strideA, d_x,
-                               stridex, chunk, part, nsplit, d_active, rn, raoff);
+                               stridex, chunk, part, nsplit, d_active, rn, raoff, d_skip);
         else if (d_Alo)
             hipLaunchKernelGGL(gemv_split_kernel<false>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c32*>(d_A), d_Alo, d_inv_scale, lda, strideA, d_x,
-                               stridex, chunk, part, nsplit, d_active, rn, raoff);
+                               stridex, chunk, part, nsplit, d_active, rn, raoff, d_skip);
+        else if (prec == CCAB_C128 && nt)
+            hipLaunchKernelGGL((gemv_part_kernel<c128, true>), grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active, rn, raoff, d_skip);
         else if (prec == CCAB_C128)
             hipLaunchKernelGGL(gemv_part_kernel<c128>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c128*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit, d_active, rn, raoff);
+                               part, nsplit, d_active, rn, raoff, d_skip);
+        else if (prec == CCAB_C32 && R == 4 && nt)
+            hipLaunchKernelGGL((gemv_part_vec_kernel<c32, 4, true>), grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c32*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active, rn, raoff, d_skip);
         else if (prec == CCAB_C32 && R == 4)
             hipLaunchKernelGGL((gemv_part_vec_kernel<c32, 4>), grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c32*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit, d_active, rn, raoff);
+                               part, nsplit, d_active, rn, raoff, d_skip);
+        else if (prec == CCAB_C64 && R == 2 && nt)
+            hipLaunchKernelGGL((gemv_part_vec_kernel<c64, 2, true>), grid, dim3(256), 0, ctx->stream, M, N,
+                               static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
+                               part, nsplit, d_active, rn, raoff, d_skip);
         else if (prec == CCAB_C64 && R == 2)
             hipLaunchKernelGGL((gemv_part_vec_kernel<c64, 2>), grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit, d_active, rn, raoff);
+                               part, nsplit, d_active, rn, raoff, d_skip);
         else if (prec == CCAB_C32)
             hipLaunchKernelGGL(gemv_part_kernel<c32>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c32*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit, d_active, rn, raoff);
+                               part, nsplit, d_active, rn, raoff, d_skip);
         else
             hipLaunchKernelGGL(gemv_part_kernel<c64>, grid, dim3(256), 0, ctx->stream, M, N,
                                static_cast<const c64*>(d_A), lda, strideA, d_x, stridex, chunk,
-                               part, nsplit, d_active, rn, raoff);
+                               part, nsplit, d_active, rn, raoff, d_skip);
     } else {
         nsplit = 0;
     }
     hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)finish_blocks, (unsigned)nact), dim3(256),
                        0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active,
-                       (prec == CCAB_C32 && !d_Alo) ? d_inv_scale : nullptr);
+                       (prec == CCAB_C32 && !d_Alo) ? d_inv_scale : nullptr, d_skip);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

@@ -113,6 +113,10 @@ void prof_end(ccab_ctx* ctx, hipEvent_t a, int cls, double work) {
     ctx->prof_pending.push_back({a, b, cls, work});
 }
 
+void prof_adjust(ccab_ctx* ctx, int cls, double dwork) {
+    if (ctx->prof_on && cls >= 0 && cls < 16) ctx->prof_work[cls] += dwork;
+}
+
 static void prof_resolve(ccab_ctx* ctx) {
     if (ctx->prof_pending.empty()) return;
     (void)hipStreamSynchronize(ctx->stream);
@@ -365,9 +369,11 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
     if (const char* lc = getenv("CCAB_LU_COOP_ROWS")) ctx->lu_coop_rows = std::max(0, atoi(lc));
     if (const char* mc = getenv("CCAB_MGS_COOP")) ctx->mgs_coop = atoi(mc) != 0;
+    if (const char* gl = getenv("CCAB_GIR_LAG")) ctx->gir_lag = std::max(1, std::min(8, atoi(gl)));
     if (const char* mb = getenv("CCAB_MEM_BUDGET_MB"))
         ctx->mem_budget = size_t(std::max(0L, atol(mb))) << 20;
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
+    if (const char* nt = getenv("CCAB_GEMV_NT")) ctx->gemv_nt = atoi(nt) != 0;
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
     *out = ctx;
@@ -449,6 +455,7 @@ int ccab_destroy(ccab_ctx* ctx) {
     graph_clear(ctx);
     if (ctx->wait_event) (void)hipEventDestroy(ctx->wait_event);
     if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
+    for (hipEvent_t e : ctx->gir_events) (void)hipEventDestroy(e);
     prof_resolve(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int s = 0; s < SLOT_COUNT; ++s)

@@ -107,7 +107,7 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_COUNT
+    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -152,13 +152,21 @@ struct ccab_ctx {
     // Gram-Schmidt of few large systems over many workgroups per system
     // (environment CCAB_MGS_COOP=0 keeps one workgroup per system)
     bool mgs_coop = true;
+    // GMRES-IR: the host applies the device's done flags this many Arnoldi
+    // steps late (environment CCAB_GIR_LAG, 1-8)
+    int gir_lag = 1;
     // GEMV: workgroups per launch the column split aims for
     int gemv_wg_target = 1024;
+    // GEMV matrix loads non-temporal (the matrix streams through once per
+    // product; x, the partial sums and the Krylov basis keep the caches);
+    // environment CCAB_GEMV_NT=0 turns it off
+    bool gemv_nt = true;
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of
     // the free memory, at most 96 GiB); environment CCAB_MEM_BUDGET_MB
     size_t mem_budget = 0;
     hipEvent_t wait_event = nullptr;  // ccab_stream_wait
     hipEvent_t join_event = nullptr;  // ccab_stream_join
+    std::vector<hipEvent_t> gir_events;  // GMRES-IR: per-step convergence-flag copies
     // replay captured HIP graphs of fixed launch sequences (the LU)
     bool use_graphs = false;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
@@ -196,6 +204,9 @@ enum ProfClass {
 };
 void prof_begin(ccab_ctx* ctx, hipEvent_t* ev);
 void prof_end(ccab_ctx* ctx, hipEvent_t ev, int cls, double work);
+// correct the algorithmic work of a class once it is known (launches whose
+// systems turned out to be done on the device did none)
+void prof_adjust(ccab_ctx* ctx, int cls, double dwork);
 struct ProfScope {
     ccab_ctx* ctx;
     hipEvent_t ev = nullptr;
@@ -323,7 +334,8 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
                 int64_t stridey, const double* d_inv_scale = nullptr,
-                const Ragged* rg = nullptr, const c64* d_Alo = nullptr);
+                const Ragged* rg = nullptr, const c64* d_Alo = nullptr,
+                const int32_t* d_skip = nullptr);
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

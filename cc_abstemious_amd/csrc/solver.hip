@@ -50,9 +50,11 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
 template <bool REF>
 __global__ __launch_bounds__(kMgsThreads) void mgs_kernel(int64_t n, int64_t k, c128* Q, int64_t ldq,
                                                           int64_t sQ, c128* h, int64_t sh,
-                                                          const int32_t* __restrict__ active) {
+                                                          const int32_t* __restrict__ active,
+                                                          const int32_t* __restrict__ skip) {
     __shared__ double red[2 * kMgsThreads / 64];
     const int64_t b = active ? active[blockIdx.x] : blockIdx.x;
+    if (skip && skip[b]) return;
     c128* q = Q + b * sQ;
     c128* w = q + (k + 1) * ldq;
     c128* hb = h + b * sh;
@@ -98,7 +100,7 @@ template <bool REF, int RQ>
 __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
     int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ, c128* h, int64_t sh,
     const int32_t* __restrict__ active, uint64_t* __restrict__ xbuf, int64_t sys_words,
-    unsigned spin_limit) {
+    unsigned spin_limit, const int32_t* __restrict__ skip) {
     constexpr int NW = kMgsCoopThreads / 64;
     __shared__ double red[2 * NW];
     __shared__ double tot[2];
@@ -106,6 +108,8 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
     const int G = gridDim.x, g = blockIdx.x;
     const int64_t z = blockIdx.y;
     const int64_t b = active ? active[z] : z;
+    // every workgroup of a skipped system leaves before its first barrier
+    if (skip && skip[b]) return;
     uint64_t* hdr = xbuf + z * sys_words;
     c128* q = Q + b * sQ;
     c128* w = q + (k + 1) * ldq;
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
 // null): one workgroup per system, or, for few large systems that would
 // leave most CUs idle, mgs_coop_kernel over up to one workgroup per CU.
 int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ,
-               c128* h, int64_t sh, const int32_t* act, int64_t nact) {
+               c128* h, int64_t sh, const int32_t* act, int64_t nact, const int32_t* skip = nullptr) {
     hipStream_t s = ctx->stream;
     if (ctx->mgs_coop && n >= 4096) {
         int rq = 0;
@@ -236,10 +240,10 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
     do {                                                                                        \
         if (ref)                                                                                \
             hipLaunchKernelGGL((mgs_coop_kernel<true, R>), grid, dim3(kMgsCoopThreads), 0, s, n, \
-                               k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit);     \
+                               k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip); \
         else                                                                                    \
             hipLaunchKernelGGL((mgs_coop_kernel<false, R>), grid, dim3(kMgsCoopThreads), 0, s,   \
-                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit);  \
+                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip); \
     } while (0)
             switch (rq) {
             case 1: CCAB_MGS_COOP(1); break;
@@ -254,10 +258,10 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
     }
     if (ref)
         hipLaunchKernelGGL(mgs_kernel<true>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k, Q,
-                           ldq, sQ, h, sh, act);
+                           ldq, sQ, h, sh, act, skip);
     else
         hipLaunchKernelGGL(mgs_kernel<false>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k,
-                           Q, ldq, sQ, h, sh, act);
+                           Q, ldq, sQ, h, sh, act, skip);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
@@ -347,6 +351,103 @@ __global__ __launch_bounds__(256) void warm_scale_kernel(int64_t n, const c128* 
         xx[i] = al * xx[i];
         rr[i] = bb[i] - al * u;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Device side of the batched GMRES cycle (gmres.rs:158-200 for many systems):
+// the Givens rotations, the rotated right-hand side beta and the stop test
+// run next to the data, so the host never waits on an Arnoldi step.  Per
+// system b: H (m+1) x m rotated Hessenberg columns, cs/sn the rotations, be =
+// beta, kdone[b] = number of Arnoldi steps once the system's cycle is done
+// (0 while it runs).  One thread per active system; same arithmetic as the
+// host loop of gmres.rs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cabs_d(c128 z) { return hypot(z.re, z.im); }
+
+__global__ void gir_cycle_init_kernel(int m, const int32_t* __restrict__ act, int nact,
+                                      int32_t* kdone, c128* be, const double* __restrict__ rn) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    kdone[b] = 0;
+    be[b * (m + 1)] = c128{rn[b], 0.0};
+}
+
+// Column k from the Gram-Schmidt step (h[0..k+1], stride sh per system)
+__global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act, int nact,
+                                  const c128* __restrict__ hd, int64_t sh, c128* Hg, c128* csd,
+                                  c128* snd, c128* bed, int32_t* kdone,
+                                  const double* __restrict__ rn, const double* __restrict__ bn,
+                                  double inner_tol, double tol) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    if (kdone[b] != 0) return;
+    const c128* h = hd + b * sh;
+    c128* col = Hg + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
+    c128* cs = csd + b * m;
+    c128* sn = snd + b * m;
+    c128* be = bed + b * (m + 1);
+    for (int i = 0; i <= k + 1; ++i) col[i] = h[i];
+    for (int i = 0; i < k; ++i) {
+        const c128 a = col[i], c = col[i + 1];
+        col[i] = cs[i] * a + sn[i] * c;
+        col[i + 1] = cs[i] * c - conj(sn[i]) * a;
+    }
+    const c128 v1 = col[k], v2 = col[k + 1];
+    const double a1 = cabs_d(v1);
+    const c128 signf = (v1.re == 0.0 && v1.im == 0.0) ? c128{1.0, 0.0} : c128{v1.re / a1, v1.im / a1};
+    const double rr = sqrt(norm_sqr(v1) + norm_sqr(v2));
+    const c128 ck{a1 / rr, 0.0};
+    const c128 t = signf * conj(v2);
+    const c128 sk{t.re / rr, t.im / rr};
+    cs[k] = ck;
+    sn[k] = sk;
+    col[k] = ck * v1 + sk * v2;
+    col[k + 1] = c128{0.0, 0.0};
+    const c128 bk1 = -(conj(sk) * be[k]);
+    be[k + 1] = bk1;
+    be[k] = ck * be[k];
+    const double res = cabs_d(bk1);
+    const double est = res / rn[b];
+    if (est <= inner_tol || k == m - 1 || res <= 0.5 * tol * bn[b] || !isfinite(est))
+        kdone[b] = k + 1;
+}
+
+// y = R^{-1} beta over the system's kdone[b] steps (R the rotated, upper
+// triangular H), zero beyond; column-oriented back substitution.
+__global__ void gir_solve_y_kernel(int m, const int32_t* __restrict__ act, int nact,
+                                   const c128* __restrict__ Hg, const c128* __restrict__ bed,
+                                   const int32_t* __restrict__ kdone, c128* Yd) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    const int kk = kdone[b];
+    const c128* H = Hg + b * int64_t(m + 1) * m;
+    const c128* be = bed + b * (m + 1);
+    c128* y = Yd + b * (m + 1);
+    for (int i = 0; i <= m; ++i) y[i] = i < kk ? be[i] : c128{0.0, 0.0};
+    for (int j = kk - 1; j >= 0; --j) {
+        const c128 yj = cdiv(y[j], H[j + int64_t(j) * (m + 1)]);
+        y[j] = yj;
+        for (int r = 0; r < j; ++r) y[r] = y[r] - yj * H[r + int64_t(j) * (m + 1)];
+    }
+}
+
+// x_b += Q_b y_b over the system's own kdone[b] basis vectors only (columns
+// beyond them may hold anything: never written, or left from another cycle)
+__global__ void gir_update_x_kernel(int64_t n, int m, const int32_t* __restrict__ act,
+                                    const int32_t* __restrict__ kdone, const c128* __restrict__ Q,
+                                    int64_t sq, const c128* __restrict__ Yd, c128* X) {
+    const int64_t b = act[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int kk = kdone[b];
+    const c128* q = Q + b * sq + i;
+    const c128* y = Yd + b * (m + 1);
+    c128 acc{0.0, 0.0};
+    for (int j = 0; j < kk; ++j) acc += q[int64_t(j) * n] * y[j];
+    X[b * n + i] = X[b * n + i] + acc;
 }
 
 using cd = std::complex<double>;
@@ -645,14 +746,37 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     int32_t* act = static_cast<int32_t*>(ctx_buf(ctx, SLOT_GIR_ACT, size_t(B) * 4 * 2));
     if (!Bv || !R || !Q || !Hd || !Yd || !nrm || !act) return CCAB_ENOMEM;
     const int64_t sq = n * (m + 1);
-    const size_t pin_h = ws_align(size_t(B) * (m + 2) * 16), pin_y = ws_align(size_t(B) * (m + 1) * 16);
-    const size_t pin_a = ws_align(size_t(B) * 4);
-    CCAB_TRY(pinned_reserve(ctx, pin_h + pin_y + pin_a + size_t(B) * 8));
+    // device Givens state (see gir_givens_kernel)
+    const size_t g_h = ws_align(size_t(B) * (m + 1) * m * 16), g_cs = ws_align(size_t(B) * m * 16);
+    const size_t g_be = ws_align(size_t(B) * (m + 1) * 16), g_i = ws_align(size_t(B) * 4);
+    char* gv = static_cast<char*>(ctx_buf(ctx, SLOT_GIR_GIV, g_h + 2 * g_cs + g_be + g_i + size_t(B) * 8));
+    if (!gv) return CCAB_ENOMEM;
+    c128* Hg = reinterpret_cast<c128*>(gv);
+    c128* csd = reinterpret_cast<c128*>(gv + g_h);
+    c128* snd = reinterpret_cast<c128*>(gv + g_h + g_cs);
+    c128* bed = reinterpret_cast<c128*>(gv + g_h + 2 * g_cs);
+    int32_t* kdone_d = reinterpret_cast<int32_t*>(gv + g_h + 2 * g_cs + g_be);
+    double* bn_d = reinterpret_cast<double*>(gv + g_h + 2 * g_cs + g_be + g_i);
+    // pinned: the synchronous active-list staging, norms, and per Arnoldi
+    // step one active-list slot and one done-flag slot (asynchronous copies)
+    const size_t pin_a = ws_align(size_t(B) * 4), pin_n = ws_align(size_t(B) * 8);
+    CCAB_TRY(pinned_reserve(ctx, pin_a + pin_n + 2 * size_t(m + 1) * pin_a));
     char* pin = static_cast<char*>(ctx->pinned);
-    double* hH = reinterpret_cast<double*>(pin);
-    double* hY = reinterpret_cast<double*>(pin + pin_h);
-    int32_t* hAct = reinterpret_cast<int32_t*>(pin + pin_h + pin_y);
-    double* hN = reinterpret_cast<double*>(pin + pin_h + pin_y + pin_a);
+    int32_t* hAct = reinterpret_cast<int32_t*>(pin);
+    double* hN = reinterpret_cast<double*>(pin + pin_a);
+    auto ring_act = [&](int k) { return reinterpret_cast<int32_t*>(pin + pin_a + pin_n + size_t(k) * pin_a); };
+    auto ring_done = [&](int k) {
+        return reinterpret_cast<int32_t*>(pin + pin_a + pin_n + size_t(m + 1 + k) * pin_a);
+    };
+    // an event per Arnoldi step marks its done-flag copy
+    while ((int)ctx->gir_events.size() < m + 1) {
+        hipEvent_t e;
+        CCAB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->gir_events.push_back(e);
+    }
+    // the host learns which systems finished kLag steps late (it waits for
+    // step k - kLag after enqueuing step k, so the GPU always has work)
+    const int kLag = ctx->gir_lag;
     const c128 one{1.0, 0.0}, mone{-1.0, 0.0}, zero{0.0, 0.0};
     // residual matrix: c128; or the FP16 operator + FP32 remainder; or the
     // c64 operator itself
@@ -696,9 +820,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     std::vector<char> conv(B, 0);
     CCAB_TRY(upload_active(active, act));
     CCAB_TRY(read_norms(Bv, n, active, bn));
-    std::vector<cd> cs(size_t(B) * m), sn(size_t(B) * m), beta(size_t(B) * (m + 1)),
-        Hh(size_t(B) * (m + 1) * m);
-    std::vector<int> kdone(B);
+    CCAB_HIP(hipMemcpy(bn_d, bn.data(), size_t(B) * 8, hipMemcpyHostToDevice));
     for (int cyc = 0; cyc <= max_cycles; ++cyc) {
         // FP64 residual of the active systems
         CCAB_TRY(upload_active(active, act));
@@ -728,81 +850,86 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         // one GMRES(m) cycle for A64 d = r on the active systems
         hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
                            (unsigned)active.size()), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
-        for (int32_t b : active) {
-            std::fill(cs.begin() + size_t(b) * m, cs.begin() + size_t(b + 1) * m, cd(0));
-            std::fill(sn.begin() + size_t(b) * m, sn.begin() + size_t(b + 1) * m, cd(0));
-            std::fill(beta.begin() + size_t(b) * (m + 1), beta.begin() + size_t(b + 1) * (m + 1), cd(0));
-            beta[size_t(b) * (m + 1)] = cd(rn[b], 0.0);
-            kdone[b] = -1;
-            cycles[b]++;
-        }
-        std::memset(hY, 0, pin_y);
+        const int na_c = (int)active.size();
+        hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
+                           m, act, na_c, kdone_d, bed, nrm);
+        for (int32_t b : active) cycles[b]++;
         std::vector<int32_t> inner = active;
+        std::vector<int> launched(B, 0);  // Arnoldi steps enqueued per system (profiler)
+        int known = -1;  // the latest step whose done flags the host has applied
         for (int k = 0; k < m && !inner.empty(); ++k) {
-            CCAB_TRY(upload_active(inner, act));
+            if (inner != uploaded) {  // a fresh pinned slot per step: no host wait
+                std::memcpy(ring_act(k), inner.data(), inner.size() * 4);
+                CCAB_HIP(hipMemcpyAsync(act, ring_act(k), inner.size() * 4, hipMemcpyHostToDevice, s));
+                uploaded = inner;
+            }
             const unsigned ni = (unsigned)inner.size();
             set_elems(inner);
+            for (int32_t b : inner) launched[b]++;
+            // systems already done on the device skip the product, the
+            // Gram-Schmidt step and the rotation
             CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq,
-                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg));
+                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr,
+                                 kdone_d));
             {
                 ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
-                CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni));
+                CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d));
             }
-            CCAB_HIP(hipMemcpyAsync(hH, Hd, pin_h, hipMemcpyDeviceToHost, s));
-            CCAB_HIP(hipStreamSynchronize(s));
-            std::vector<int32_t> next;
-            for (int32_t b : inner) {
-                nmv[b]++;
-                iters[b]++;
-                cd* csb = cs.data() + size_t(b) * m;
-                cd* snb = sn.data() + size_t(b) * m;
-                cd* be = beta.data() + size_t(b) * (m + 1);
-                cd* H = Hh.data() + size_t(b) * (m + 1) * m;
-                std::vector<cd> hk(k + 2);
-                const double* hb = hH + size_t(b) * (m + 2) * 2;
-                for (int i = 0; i < k + 2; ++i) hk[i] = cd(hb[2 * i], hb[2 * i + 1]);
-                for (int i = 0; i < k; ++i) {
-                    const cd temp = csb[i] * hk[i] + snb[i] * hk[i + 1];
-                    hk[i + 1] = -std::conj(snb[i]) * hk[i] + csb[i] * hk[i + 1];
-                    hk[i] = temp;
-                }
-                const cd v1 = hk[k], v2 = hk[k + 1];
-                const cd signf = (v1 == cd(0.0, 0.0)) ? cd(1.0, 0.0) : v1 / std::abs(v1);
-                const double rr = std::sqrt(std::norm(v1) + std::norm(v2));
-                csb[k] = cd(std::abs(v1) / rr, 0.0);
-                snb[k] = signf * std::conj(v2) / rr;
-                hk[k] = csb[k] * hk[k] + snb[k] * hk[k + 1];
-                for (int i = 0; i <= k; ++i) H[i + size_t(k) * (m + 1)] = hk[i];
-                be[k + 1] = -std::conj(snb[k]) * be[k];
-                be[k] *= csb[k];
-                const double est = std::abs(be[k + 1]) / rn[b];
-                const bool done = est <= inner_tol || k == m - 1 ||
-                                  std::abs(be[k + 1]) <= 0.5 * tol * bn[b] || !std::isfinite(est);
-                if (done) {
-                    const int kk = k + 1;
-                    std::vector<cd> Hk(size_t(kk) * kk), yv(kk);
-                    for (int c = 0; c < kk; ++c)
-                        for (int q = 0; q < kk; ++q) Hk[q + size_t(c) * kk] = H[q + size_t(c) * (m + 1)];
-                    for (int i = 0; i < kk; ++i) yv[i] = be[i];
-                    host_lu_solve(kk, Hk, yv);
-                    double* yb = hY + size_t(b) * (m + 1) * 2;
-                    for (int i = 0; i < kk; ++i) {
-                        yb[2 * i] = yv[i].real();
-                        yb[2 * i + 1] = yv[i].imag();
-                    }
-                    kdone[b] = kk;
-                } else {
-                    next.push_back(b);
-                }
+            hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act,
+                               (int)ni, Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d,
+                               inner_tol, tol);
+            CCAB_HIP(hipGetLastError());
+            CCAB_HIP(hipMemcpyAsync(ring_done(k), kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
+            CCAB_HIP(hipEventRecord(ctx->gir_events[k], s));
+            // newest finished flags: wait for step k - kLag, take a later one if ready
+            int target = -1;
+            if (k - kLag > known) {
+                CCAB_HIP(hipEventSynchronize(ctx->gir_events[k - kLag]));
+                target = k - kLag;
             }
-            inner.swap(next);
+            for (int t = k; t > std::max(target, known); --t) {
+                const hipError_t q = hipEventQuery(ctx->gir_events[t]);
+                if (q == hipSuccess) {
+                    target = t;
+                    break;
+                }
+                if (q != hipErrorNotReady) CCAB_HIP(q);
+            }
+            if (target > known) {
+                known = target;
+                const int32_t* fl = ring_done(target);
+                std::vector<int32_t> next;
+                for (int32_t b : inner)
+                    if (fl[b] == 0) next.push_back(b);
+                inner.swap(next);
+            }
         }
-        // x += Q y for every system of this cycle (y zero beyond its own size)
+        // every system of the cycle is done on the device (step m - 1 ends
+        // the cycle): y = R^{-1} beta, then x += Q y
         CCAB_TRY(upload_active(active, act));
-        CCAB_HIP(hipMemcpyAsync(Yd, hY, pin_y, hipMemcpyHostToDevice, s));
-        CCAB_TRY(gemv_active(ctx, CCAB_C128, n, m, (int64_t)active.size(), act, Q, n, sq, Yd, m + 1,
-                             one, one, X, n));
+        hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s, m,
+                           act, na_c, Hg, bed, kdone_d, Yd);
+        hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
+                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X);
+        CCAB_HIP(hipMemcpyAsync(hAct, kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
         CCAB_HIP(hipStreamSynchronize(s));
+        uploaded.clear();  // hAct now holds flags, not the list
+        double gemv_over = 0.0, mgs_over = 0.0;
+        const double esz = op_prec == CCAB_C32 ? 4.0 : 8.0;
+        for (int32_t b : active) {
+            if (hAct[b] <= 0 || hAct[b] > m)
+                return ctx_fail(ctx, CCAB_EHIP, "GMRES-IR cycle of system %d did not finish", b);
+            iters[b] += hAct[b];
+            nmv[b] += hAct[b];
+            // steps enqueued after the system was done did no work
+            const double nb = rg ? double(rg->hn[b]) : double(n);
+            for (int k = hAct[b]; k < launched[b]; ++k) {
+                gemv_over += nb * nb * esz;
+                mgs_over += double(k + 2) * 2.0 * double(n) * 16.0;
+            }
+        }
+        prof_adjust(ctx, PROF_GEMV, -gemv_over);
+        prof_adjust(ctx, PROF_MGS, -mgs_over);
     }
     if (st) {
         for (int64_t b = 0; b < B; ++b) {
