@@ -429,8 +429,9 @@ def main():
         "lu_mixed_rel_err_vs_fp64_lu": lu_err,
         "kernel_ms": {k: round(v["ms"] / args.steps, 3) for k, v in prof.items()},
         "roofline_residual": ({"bound": "hbm",
-                               "kernel": ("gemv_split_kernel<true> (FP64 residual, FP16 operator + "
-                                          "FP32 remainder, 12 B/entry)"
+                               "kernel": ("gemv_part_vec_kernel<c64,2,true> (FP64 residual "
+                                          "r -= A_l d over the FP32 remainder, 8 B/entry; the "
+                                          "FP16 operator's share A_h d = Q Hbar y)"
                                           if args.residual_matrix == "split" else
                                           "gemv_part_kernel<c128> (FP64 residual)"),
                                "achieved": gemv128["work"] / (gemv128["ms"] * 1e-3) / 1e9,

@@ -107,7 +107,7 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_COUNT
+    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -335,7 +335,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
                 int64_t stridey, const double* d_inv_scale = nullptr,
                 const Ragged* rg = nullptr, const c64* d_Alo = nullptr,
-                const int32_t* d_skip = nullptr);
+                const int32_t* d_skip = nullptr, int prof_cls = -1);
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

@@ -378,13 +378,17 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
                                   const c128* __restrict__ hd, int64_t sh, c128* Hg, c128* csd,
                                   c128* snd, c128* bed, int32_t* kdone,
                                   const double* __restrict__ rn, const double* __restrict__ bn,
-                                  double inner_tol, double tol) {
+                                  double inner_tol, double tol, c128* Hbar) {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     if (z >= nact) return;
     const int64_t b = act[z];
     if (kdone[b] != 0) return;
     const c128* h = hd + b * sh;
     c128* col = Hg + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
+    if (Hbar) {  // the unrotated column, for A Q_k = Q_(k+1) Hbar_k
+        c128* hb = Hbar + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
+        for (int i = 0; i <= k + 1; ++i) hb[i] = h[i];
+    }
     c128* cs = csd + b * m;
     c128* sn = snd + b * m;
     c128* be = bed + b * (m + 1);
@@ -415,10 +419,12 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
 }
 
 // y = R^{-1} beta over the system's kdone[b] steps (R the rotated, upper
-// triangular H), zero beyond; column-oriented back substitution.
+// triangular H), zero beyond; column-oriented back substitution.  With Hbar:
+// also z = Hbar y (kdone[b] + 1 entries), so that A d = Q z for d = Q y.
 __global__ void gir_solve_y_kernel(int m, const int32_t* __restrict__ act, int nact,
                                    const c128* __restrict__ Hg, const c128* __restrict__ bed,
-                                   const int32_t* __restrict__ kdone, c128* Yd) {
+                                   const int32_t* __restrict__ kdone, c128* Yd,
+                                   const c128* __restrict__ Hbar, c128* Zd) {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     if (z >= nact) return;
     const int64_t b = act[z];
@@ -432,13 +438,25 @@ __global__ void gir_solve_y_kernel(int m, const int32_t* __restrict__ act, int n
         y[j] = yj;
         for (int r = 0; r < j; ++r) y[r] = y[r] - yj * H[r + int64_t(j) * (m + 1)];
     }
+    if (!Hbar) return;
+    const c128* Hb = Hbar + b * int64_t(m + 1) * m;
+    c128* zz = Zd + b * (m + 1);
+    for (int i = 0; i <= m; ++i) {
+        c128 acc{0.0, 0.0};
+        if (i <= kk)  // column j holds rows 0..j+1
+            for (int j = (i > 0 ? i - 1 : 0); j < kk; ++j) acc += Hb[i + int64_t(j) * (m + 1)] * y[j];
+        zz[i] = acc;
+    }
 }
 
 // x_b += Q_b y_b over the system's own kdone[b] basis vectors only (columns
-// beyond them may hold anything: never written, or left from another cycle)
+// beyond them may hold anything: never written, or left from another cycle).
+// With Zd: also d_b = Q_b y_b into D and r_b -= Q_b z_b (kdone[b] + 1
+// vectors), the operator's share of r - A d.
 __global__ void gir_update_x_kernel(int64_t n, int m, const int32_t* __restrict__ act,
                                     const int32_t* __restrict__ kdone, const c128* __restrict__ Q,
-                                    int64_t sq, const c128* __restrict__ Yd, c128* X) {
+                                    int64_t sq, const c128* __restrict__ Yd, c128* X,
+                                    const c128* __restrict__ Zd, c128* D, c128* R) {
     const int64_t b = act[blockIdx.y];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -448,6 +466,13 @@ __global__ void gir_update_x_kernel(int64_t n, int m, const int32_t* __restrict_
     c128 acc{0.0, 0.0};
     for (int j = 0; j < kk; ++j) acc += q[int64_t(j) * n] * y[j];
     X[b * n + i] = X[b * n + i] + acc;
+    if (!Zd) return;
+    D[b * n + i] = acc;
+    const c128* z = Zd + b * (m + 1);
+    c128 az{0.0, 0.0};
+    for (int j = 0; j <= kk; ++j)  // (q_kk is not finite after an exact breakdown, where z_kk = 0)
+        if (z[j].re != 0.0 || z[j].im != 0.0) az += q[int64_t(j) * n] * z[j];
+    R[b * n + i] = R[b * n + i] - az;
 }
 
 using cd = std::complex<double>;
@@ -749,8 +774,17 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     // device Givens state (see gir_givens_kernel)
     const size_t g_h = ws_align(size_t(B) * (m + 1) * m * 16), g_cs = ws_align(size_t(B) * m * 16);
     const size_t g_be = ws_align(size_t(B) * (m + 1) * 16), g_i = ws_align(size_t(B) * 4);
-    char* gv = static_cast<char*>(ctx_buf(ctx, SLOT_GIR_GIV, g_h + 2 * g_cs + g_be + g_i + size_t(B) * 8));
+    const size_t g_all = g_h + 2 * g_cs + g_be + g_i + ws_align(size_t(B) * 8);
+    // with the FP32 remainder the residual follows r <- r - Q (Hbar y) - A_l d
+    // (A_h d = Q_(k+1) Hbar y by the Arnoldi relation of the FP16 operator), so
+    // each refinement residual reads the 8-byte remainder instead of 12 bytes
+    const bool recur = Alo != nullptr;
+    char* gv = static_cast<char*>(ctx_buf(ctx, SLOT_GIR_GIV, g_all + (recur ? g_h + g_be : 0)));
     if (!gv) return CCAB_ENOMEM;
+    c128* Hbar = recur ? reinterpret_cast<c128*>(gv + g_all) : nullptr;
+    c128* Zd = recur ? reinterpret_cast<c128*>(gv + g_all + g_h) : nullptr;
+    c128* Dv = recur ? static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_D, nb * 16)) : nullptr;
+    if (recur && !Dv) return CCAB_ENOMEM;
     c128* Hg = reinterpret_cast<c128*>(gv);
     c128* csd = reinterpret_cast<c128*>(gv + g_h);
     c128* snd = reinterpret_cast<c128*>(gv + g_h + g_cs);
@@ -825,13 +859,21 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         // FP64 residual of the active systems
         CCAB_TRY(upload_active(active, act));
         const unsigned na = (unsigned)active.size();
-        hipLaunchKernelGGL(copy_active_kernel, dim3((unsigned)((n + 255) / 256), na), dim3(256), 0,
-                           s, n, Bv, R, act);
-        if (cyc > 0 || X0) {
+        if (recur && cyc > 0) {
+            // R = r_old - Q z already; subtract the remainder's share A_l d
             set_elems(active);
-            CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n,
-                                 Alo ? d_inv_scale : nullptr, rg, Alo));
+            CCAB_TRY(gemv_active(ctx, CCAB_C64, n, n, na, act, Alo, lda, sA, Dv, n, mone, one, R, n,
+                                 nullptr, rg, nullptr, nullptr, PROF_GEMV128));
             for (int32_t b : active) nmv[b]++;
+        } else {
+            hipLaunchKernelGGL(copy_active_kernel, dim3((unsigned)((n + 255) / 256), na), dim3(256),
+                               0, s, n, Bv, R, act);
+            if (cyc > 0 || X0) {
+                set_elems(active);
+                CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n,
+                                     Alo ? d_inv_scale : nullptr, rg, Alo));
+                for (int32_t b : active) nmv[b]++;
+            }
         }
         if (cyc == 0 && X0)
             hipLaunchKernelGGL(warm_scale_kernel, dim3(na), dim3(256), 0, s, n, Bv, R, X, act);
@@ -877,7 +919,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
             }
             hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act,
                                (int)ni, Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d,
-                               inner_tol, tol);
+                               inner_tol, tol, Hbar);
             CCAB_HIP(hipGetLastError());
             CCAB_HIP(hipMemcpyAsync(ring_done(k), kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
             CCAB_HIP(hipEventRecord(ctx->gir_events[k], s));
@@ -908,9 +950,9 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         // the cycle): y = R^{-1} beta, then x += Q y
         CCAB_TRY(upload_active(active, act));
         hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s, m,
-                           act, na_c, Hg, bed, kdone_d, Yd);
+                           act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
         hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
-                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X);
+                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X, Zd, Dv, R);
         CCAB_HIP(hipMemcpyAsync(hAct, kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
         CCAB_HIP(hipStreamSynchronize(s));
         uploaded.clear();  // hAct now holds flags, not the list

@@ -391,7 +391,9 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
     the FP64 LU solution; only the inner solve sees FP16 entries.  With
     residual C32 the refinement's matrix is the FP16 operator plus its
     complex-FP32 remainder (about 2^-35 relative per entry; without the
-    remainder the error would stay near 1e-4)."""
+    remainder the error would stay near 1e-4), and its residual is carried
+    as r - Q (Hbar y) - A_l d between cycles: the true residual against the
+    c128 matrix must agree with the reported one."""
     _note(f"sweep GMRES-IR with c32 operator vs c128 LU, icosphere level {level}, BM, "
           f"residual matrix {residual}")
     m = mesh.icosphere(level)
@@ -410,3 +412,15 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
         assert e < 1e-8, (k, e, st[k])
         assert st[k]["converged"] == 1 and st[k]["rel_residual"] <= 1e-10
     print("iterations", [s["iterations"] for s in st], "cycles", [s["ir_steps"] for s in st])
+    if residual == capi.C32:
+        for k in (0, len(fs) - 1):
+            pk = PreData(m, [fs[k]], 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR,
+                         capi.BC_NORMAL_VELOCITY, (0, 0),
+                         IncidentWave(capi.WAVE_PLANE, (0.2, 0.3, 0.9), (1.0, 0.0)), Solver())
+            rhs_inc = incident_wave.get_incident_surface(pk)
+            l128, _, rhs = influence_matrix.assemble_system(pk, rhs_inc)
+            r = l128[0] @ phi_h[k] - rhs[0]
+            true_rel = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(rhs[0]))
+            print(f"frequency {k}: reported {st[k]['rel_residual']:.3e} true {true_rel:.3e}")
+            # tolerance 1e-10 plus the remainder's 2^-35 representation error
+            assert true_rel <= 3e-10
