@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     const double yx = s.x[ic], yy = s.y[ic], yz = s.z[ic];
     const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
     const double k0 = params[b0].k;
+    const double dh_sign = params[b0].dh_sign;  // one value per batch (checked by the launcher)
     // per-lane row-sum accumulators of the kF systems (private LDS slots)
     __shared__ double2 rs[kF][kRows];
     for (int f = 0; f < kF; ++f) rs[f][threadIdx.x] = double2{0.0, 0.0};
@@ -142,9 +143,35 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
         const double rx = diag ? 1.0 : s.x[j] - yx;
         const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
                                                          s.ny[j], s.nz[j], nyx, nyy, nyz);
+        // With beta = i/k (Burton-Miller) the pair of influence_matrix.rs:
+        // 12-37 times the weight w = wa / (4 pi r) is
+        //   H = w e^{ikr} (a0, k a1 + a2 / k),  G = w e^{ikr} (1 - e.n_x, -(e.n_x / r) / k)
+        // with E = n_y.n_x - 3 rr, rr = (e.n_x)(e.n_y) dh_sign, a0 = (e.n_y + E) / r,
+        // a1 = rr - e.n_y, a2 = E / r^2 (classical: H = w e^{ikr} (e.n_y)(1/r, -k),
+        // G = w e^{ikr}); only the phasor and k, 1/k change with the frequency.
+        const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
+        double a0, a1, a2, g0, g1;
+        if constexpr (BM) {
+            const double rr = q.rdotx * q.rdoty * dh_sign;
+            const double E = q.nn - 3.0 * rr;
+            a0 = q.inv_r * (q.rdoty + E);
+            a1 = rr - q.rdoty;
+            a2 = q.inv_r * q.inv_r * E;
+            g0 = 1.0 - q.rdotx;
+            g1 = -q.inv_r * q.rdotx;
+        } else {
+            a0 = q.inv_r * q.rdoty;
+            a1 = -q.rdoty;
+            a2 = 0.0;
+            g0 = 1.0;
+            g1 = 0.0;
+        }
         double sn, cs, sd = 0.0, cd = 1.0;
         sincos(k0 * q.rd, &sn, &cs);
         if (dk != 0.0) sincos(dk * q.rd, &sd, &cd);
+        // the phasor carries the weight
+        cs *= w;
+        sn *= w;
 #pragma unroll 1
         for (int f = 0; f < nf; ++f) {
             const SysParams& p = params[b0 + f];
@@ -155,16 +182,18 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     cs = c2;
                 } else {
                     sincos(p.k * q.rd, &sn, &cs);
+                    cs *= w;
+                    sn *= w;
                 }
             }
             c128 G, H;
             if (diag) {
                 diag_entry(p, s.area[j], s.wt[j], G, H);
             } else {
-                c128 g, h;
-                greens_at<double, BM>(q, p.k, p.beta, cs, sn, g, h, p.dh_sign);
-                G = c128{g.re * wa, g.im * wa};
-                H = c128{h.re * wa, h.im * wa};
+                const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
+                const c128 ph{cs, sn};
+                H = ph * c128{a0, p.k * a1 + a2 * ik};
+                G = ph * c128{g0, g1 * ik};
             }
             if (active) {
                 const int64_t off = (b0 + f) * stride + i + j * ld;
@@ -369,6 +398,13 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         for (int64_t b = 0; b < batch && uniform; ++b)
             uniform = std::fabs(h_params[b].k - (k0 + double(b) * step)) <= 4.0 * DBL_EPSILON * kmax;
         const double dk = uniform ? step : 0.0;
+        for (int64_t b = 0; b < batch; ++b) {
+            CCAB_CHECK(h_params[b].dh_sign == h_params[0].dh_sign,
+                       "the systems of a batch share one dh convention");
+            CCAB_CHECK(!bm || (h_params[b].beta.re == 0.0 &&
+                               std::fabs(h_params[b].beta.im * h_params[b].k - 1.0) < 1e-14),
+                       "Burton-Miller coupling beta = i/k expected");
+        }
         dim3 g2(grid.x, grid.y, (unsigned)((batch + kF - 1) / kF));
         if (prec_compute == PREC_F64_H) {
             if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
