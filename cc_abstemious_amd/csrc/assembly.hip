@@ -106,7 +106,8 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
 
 // ASM_LHS assembly of a frequency sweep over one mesh, FP64 arithmetic, kF
 // systems per thread: the pair geometry (r, 1/r, the normal projections) is
-// computed once for the kF systems.  With equally spaced wavenumbers (step
+// computed once for the kF systems (16: measured 6.6 ms at 8, 5.7 at 16, 5.7
+// at 32 for 64 frequencies at n = 5120).  With equally spaced wavenumbers (step
 // dk != 0) the phasor e^{i k r} comes from sincos for the first system of the
 // group and from e^{i k_f r} = e^{i k_(f-1) r} e^{i dk r} for the others (two
 // sincos per kF entries instead of kF); otherwise from sincos per system.
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
 // c32 and out2 set, out2 also receives the complex-FP32 remainder
 // L - fp16(L h_scale) / h_scale (the FP16 copy plus the remainder stand for
 // the FP64 matrix to about 2^-35 relative per entry).
-constexpr int kF = 8;
+constexpr int kF = 16;
 
 template <bool BM, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
@@ -133,9 +134,12 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
     const double k0 = params[b0].k;
     const double dh_sign = params[b0].dh_sign;  // one value per batch (checked by the launcher)
-    // per-lane row-sum accumulators of the kF systems (private LDS slots)
-    __shared__ double2 rs[kF][kRows];
-    for (int f = 0; f < kF; ++f) rs[f][threadIdx.x] = double2{0.0, 0.0};
+    // per-lane row-sum accumulators of the kF systems (private LDS slots;
+    // dynamic LDS of kF * kRows * 16 bytes, launched only with row sums)
+    extern __shared__ double2 rs_lds[];
+    auto rs = [&](int f) -> double2& { return rs_lds[f * kRows + threadIdx.x]; };
+    if (rowsum_part)
+        for (int f = 0; f < kF; ++f) rs(f) = double2{0.0, 0.0};
     for (int64_t j = j0; j < j1; ++j) {
         const double wa = s.wt[j] * s.area[j];
         const bool diag = j == ic;
@@ -216,9 +220,9 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     if (out1) store<OutT>(out1, off, L, p.h_scale);
                 }
             }
-            if (p.rowsum_of != 0) {
+            if (rowsum_part && p.rowsum_of != 0) {
                 const c128 a = p.rowsum_of == 1 ? G : H;
-                double2& r = rs[f][threadIdx.x];
+                double2& r = rs(f);
                 r.x += a.re;
                 r.y += a.im;
             }
@@ -228,8 +232,8 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     for (int f = 0; f < nf; ++f) {
         if (params[b0 + f].rowsum_of == 0) continue;
         double* dst = rowsum_part + 2 * (((b0 + f) * nchunks + blockIdx.y) * n + i);
-        dst[0] = rs[f][threadIdx.x].x;
-        dst[1] = rs[f][threadIdx.x].y;
+        dst[0] = rs(f).x;
+        dst[1] = rs(f).y;
     }
 }
 
@@ -406,12 +410,14 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
                        "Burton-Miller coupling beta = i/k expected");
         }
         dim3 g2(grid.x, grid.y, (unsigned)((batch + kF - 1) / kF));
+        // row sums in dynamic LDS, 16 B per frequency and lane (64 KiB)
+        const size_t lds = d_rowsum_part ? size_t(kF) * kRows * sizeof(double2) : 0;
         if (prec_compute == PREC_F64_H) {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
         } else {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), 0, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
         }
     } else if (mode == ASM_HG) {
         if (prec_compute == PREC_F64) {
