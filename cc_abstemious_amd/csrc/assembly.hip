@@ -266,12 +266,8 @@ __global__ void cpts_to_soa_kernel(const double* __restrict__ aos, int64_t n, do
 }
 
 // incident_wave.rs:7-63
-__global__ void incident_kernel(SysParams p, int kind, double d0, double d1, double d2,
-                                c128 amp, const double* __restrict__ soa_base, int64_t n,
-                                ccab_c128* rhs) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Soa s = soa_of(soa_base, n);
+__device__ __forceinline__ c128 incident_value(const SysParams& p, int kind, double d0, double d1,
+                                               double d2, c128 amp, const Soa& s, int64_t i) {
     const double cx_ = s.x[i], cy = s.y[i], cz = s.z[i];
     c128 out;
     if (kind == CCAB_WAVE_PLANE) {
@@ -298,7 +294,28 @@ __global__ void incident_kernel(SysParams p, int kind, double d0, double d1, dou
             out += p.beta * vn;
         }
     }
-    rhs[i] = ccab_c128{out.re, out.im};
+    return out;
+}
+
+__global__ void incident_kernel(SysParams p, int kind, double d0, double d1, double d2,
+                                c128 amp, const double* __restrict__ soa_base, int64_t n,
+                                ccab_c128* rhs) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const c128 v = incident_value(p, kind, d0, d1, d2, amp, soa_of(soa_base, n), i);
+    rhs[i] = ccab_c128{v.re, v.im};
+}
+
+// one wave, every frequency of a sweep (grid.y): system b's amplitude amps[b]
+__global__ void incident_sweep_kernel(const SysParams* __restrict__ params,
+                                      const c128* __restrict__ amps, int kind, double d0,
+                                      double d1, double d2, const double* __restrict__ soa_base,
+                                      int64_t n, ccab_c128* rhs) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const c128 v = incident_value(params[b], kind, d0, d1, d2, amps[b], soa_of(soa_base, n), i);
+    rhs[b * n + i] = ccab_c128{v.re, v.im};
 }
 
 // influence_matrix.rs:170-231.  Diagonal shift at the first position t whose
@@ -450,6 +467,39 @@ int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, i
     dim3 grid((unsigned)((n + 255) / 256), (unsigned)batch);
     hipLaunchKernelGGL(rhs_finish_kernel, grid, dim3(256), 0, ctx->stream, d_params, n,
                        d_rowsum_part, nchunks, d_rhs_inc, d_rhs);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_incident_sweep(ccab_ctx* ctx, const SysParams* dp, const SysParams* hp,
+                          const ccab_physics* ph, int64_t nfreq, const ccab_wave& w,
+                          const double* d_soa, int64_t n, ccab_c128* d_rhs) {
+    bool same = true;
+    for (int64_t f = 1; f < nfreq; ++f)
+        same &= (ph[f].corrections & CCAB_FIX_SPHERICAL_ORIGIN) ==
+                (ph[0].corrections & CCAB_FIX_SPHERICAL_ORIGIN);
+    if (!same || nfreq == 1 || nfreq > 65535) {
+        for (int64_t f = 0; f < nfreq; ++f)
+            CCAB_TRY(launch_incident(ctx, &hp[f], ph[f], w, d_soa, n, d_rhs + f * n));
+        return CCAB_OK;
+    }
+    double d[3] = {w.vec[0], w.vec[1], w.vec[2]};
+    const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(nn > 0.0)) return ctx_fail(ctx, CCAB_ENAN, "incident wave vector has zero length");
+    if (w.kind == CCAB_WAVE_PLANE || !(ph[0].corrections & CCAB_FIX_SPHERICAL_ORIGIN))
+        for (double& v : d) v /= nn;
+    const c128 pamp{w.amplitude[0], w.amplitude[1]};
+    std::vector<c128> amps(nfreq);
+    for (int64_t f = 0; f < nfreq; ++f)
+        amps[f] = cdiv(pamp, c128{0.0, hp[f].omega * ph[f].mass_density});
+    c128* d_amps = static_cast<c128*>(ctx_buf(ctx, SLOT_INC_AMP, size_t(nfreq) * 16));
+    if (!d_amps) return CCAB_ENOMEM;
+    CCAB_HIP(hipMemcpyAsync(d_amps, amps.data(), size_t(nfreq) * 16, hipMemcpyHostToDevice,
+                            ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));  // amps is a host temporary
+    hipLaunchKernelGGL(incident_sweep_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nfreq),
+                       dim3(256), 0, ctx->stream, dp, d_amps, w.kind, d[0], d[1], d[2], d_soa, n,
+                       d_rhs);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

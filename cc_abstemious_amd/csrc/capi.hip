@@ -193,24 +193,6 @@ __global__ void fill_kernel(int64_t n, c128 v, ccab_c128* dst, int64_t stride) {
     if (i < n) dst[blockIdx.y * stride + i] = ccab_c128{v.re, v.im};
 }
 
-__global__ void scale_copy_kernel(int64_t n, const SysParams* __restrict__ params,
-                                  const ccab_c128* __restrict__ src, ccab_c128* dst) {
-    // vn = f * phi for the impedance BC (solve.rs:67), or its negation with
-    // CCAB_FIX_IMPEDANCE_VN
-    const int64_t b = blockIdx.y;
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const c128 f = params[b].vn_factor;
-    const ccab_c128 s = src[b * n + i];
-    const c128 v = f * c128{s.re, s.im};
-    dst[b * n + i] = ccab_c128{v.re, v.im};
-}
-
-__global__ void copy_vec_kernel(int64_t n, const ccab_c128* __restrict__ src, ccab_c128* dst) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
 __global__ void negate_kernel(int64_t n, ccab_c128* v) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) v[i] = ccab_c128{-v[i].re, -v[i].im};
@@ -316,26 +298,38 @@ int stage_params(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch, std::vect
 }
 
 // phi / vn from the solved unknown x per BC (solve.rs:36-69)
+// phi and vn of every system from the solution x (solve.rs:24-71): pressure
+// BC phi = value, vn = x; velocity BC phi = x, vn = value; impedance BC phi =
+// x, vn = vn_factor x (solve.rs:67, negated with CCAB_FIX_IMPEDANCE_VN).  x may alias phi (velocity, impedance) or vn (pressure).
+__global__ void finish_surface_kernel(int64_t n, const SysParams* __restrict__ params, int bc,
+                                      const ccab_c128* x, ccab_c128* phi, ccab_c128* vn) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ccab_c128 xv = x[b * n + i];
+    const c128 bcv = params[b].bcv;
+    if (bc == CCAB_BC_PRESSURE) {
+        phi[b * n + i] = ccab_c128{bcv.re, bcv.im};
+        vn[b * n + i] = xv;
+    } else if (bc == CCAB_BC_NORMAL_VELOCITY) {
+        phi[b * n + i] = xv;
+        vn[b * n + i] = ccab_c128{bcv.re, bcv.im};
+    } else {
+        const c128 v = params[b].vn_factor * c128{xv.re, xv.im};
+        phi[b * n + i] = xv;
+        vn[b * n + i] = ccab_c128{v.re, v.im};
+    }
+}
+
 int finish_surface(ccab_ctx* ctx, const SysParams* hp, const SysParams* dp, int bc, int64_t n,
                    int64_t batch, const ccab_c128* x, ccab_c128* phi, ccab_c128* vn) {
-    const dim3 g((unsigned)((n + 255) / 256));
-    for (int64_t b = 0; b < batch; ++b) {
-        const ccab_c128* xb = x + b * n;
-        ccab_c128* pb = phi + b * n;
-        ccab_c128* vb = vn + b * n;
-        if (bc == CCAB_BC_PRESSURE) {
-            hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, hp[b].bcv, pb, 0);
-            if (vb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, vb);
-        } else if (bc == CCAB_BC_NORMAL_VELOCITY) {
-            if (pb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, pb);
-            hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, hp[b].bcv, vb, 0);
-        } else {
-            if (pb != xb) hipLaunchKernelGGL(copy_vec_kernel, g, dim3(256), 0, ctx->stream, n, xb, pb);
-        }
+    (void)hp;
+    for (int64_t b0 = 0; b0 < batch; b0 += 65535) {
+        const int64_t nb = std::min<int64_t>(65535, batch - b0);
+        hipLaunchKernelGGL(finish_surface_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)nb),
+                           dim3(256), 0, ctx->stream, n, dp + b0, bc, x + b0 * n, phi + b0 * n,
+                           vn + b0 * n);
     }
-    if (bc == CCAB_BC_IMPEDANCE)
-        hipLaunchKernelGGL(scale_copy_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
-                           dim3(256), 0, ctx->stream, n, dp, phi, vn);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
@@ -698,8 +692,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         CCAB_HIP(hipMemcpyAsync(rhs, d_rhs_inc_given, size_t(n) * 16 * nfreq,
                                 hipMemcpyDeviceToDevice, ctx->stream));
     } else {
-        for (int64_t f = 0; f < nfreq; ++f)
-            CCAB_TRY(launch_incident(ctx, &hp[f], ph[f], *w, soa, n, rhs_c + f * n));
+        CCAB_TRY(launch_incident_sweep(ctx, dp, hp.data(), ph, nfreq, *w, soa, n, rhs_c));
     }
     const int solver = cfg->solver;
     if (solver == CCAB_SOLVER_HIERARCHICAL) {

@@ -107,7 +107,7 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_COUNT
+    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -152,6 +152,10 @@ struct ccab_ctx {
     // Gram-Schmidt of few large systems over many workgroups per system
     // (environment CCAB_MGS_COOP=0 keeps one workgroup per system)
     bool mgs_coop = true;
+    // the cooperative Gram-Schmidt's barrier headers: buffer and how many
+    // system slots are known clear
+    void* mgs_xbuf = nullptr;
+    int64_t mgs_xbuf_ready = 0;
     // GMRES-IR: the host applies the device's done flags this many Arnoldi
     // steps late (environment CCAB_GIR_LAG, 1-8)
     int gir_lag = 1;
@@ -287,6 +291,10 @@ int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, i
                       const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
                       ccab_c128* d_rhs);
 int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d_soa);
+// every frequency of a sweep in one launch (d_params: the device copy of h_params)
+int launch_incident_sweep(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
+                          const ccab_physics* ph, int64_t nfreq, const ccab_wave& w,
+                          const double* d_soa, int64_t n, ccab_c128* d_rhs);
 int launch_incident(ccab_ctx* ctx, const SysParams* h_params, const ccab_physics& ph,
                     const ccab_wave& w, const double* d_soa, int64_t n, ccab_c128* d_rhs);
 int launch_rowcol(ccab_ctx* ctx, const SysParams& p, const double* d_soa, int64_t n,

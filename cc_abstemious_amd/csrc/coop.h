@@ -19,8 +19,9 @@ constexpr unsigned kCoopSpinLimit = 1u << 21;
 __global__ void coop_reset_kernel(uint64_t* xbuf, int64_t sys_words, int64_t batch) {
     const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= batch) return;
-    xbuf[b * sys_words] = 0;      // barrier counter
-    xbuf[b * sys_words + 8] = 0;  // abort flag
+    xbuf[b * sys_words] = 0;       // barrier counter
+    xbuf[b * sys_words + 8] = 0;   // abort flag
+    xbuf[b * sys_words + 16] = 0;  // leave counter (kernels that clear their own header)
 }
 
 // Grid barrier of one system: waits until the counter reaches target (all G

@@ -166,6 +166,20 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
         __syncthreads();  // tot and red are reused by the next sum
         return true;
     };
+    // the last workgroup of the system to leave clears its header (barrier
+    // counter, abort flag, leave counter at word 16) for the next launch
+    auto leave = [&]() {
+        __syncthreads();
+        if (tid == 0) {
+            const uint64_t before = __hip_atomic_fetch_add(hdr + 16, uint64_t(1), __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            if (before == uint64_t(G - 1)) {
+                __hip_atomic_store(hdr, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(hdr + 8, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(hdr + 16, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    };
     bool ok = true;
     int64_t nb = 0;
     for (int64_t i = 0; i <= k && ok; ++i) {
@@ -200,6 +214,7 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
     }
     if (!ok) {
         if (g == 0 && tid == 0) hb[k + 1] = c128{__builtin_nan(""), 0.0};
+        leave();
         return;
     }
     const double hn = sqrt(nn);
@@ -210,6 +225,7 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
         const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
         if (row < n) w[row] = wv[u] * sc;
     }
+    leave();
 }
 
 // Modified Gram-Schmidt step of nact systems (active list, or all when
@@ -233,8 +249,14 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
             uint64_t* xbuf = static_cast<uint64_t*>(
                 ctx_buf(ctx, SLOT_MGS_COOP, size_t(nact) * sys_words * 8));
             if (!xbuf) return CCAB_ENOMEM;
-            hipLaunchKernelGGL(coop_reset_kernel, dim3((unsigned)((nact + 63) / 64)), dim3(64), 0, s,
-                               xbuf, sys_words, nact);
+            // headers clear themselves at the end of every launch; a new or
+            // grown buffer is cleared once
+            if (xbuf != ctx->mgs_xbuf || nact > ctx->mgs_xbuf_ready) {
+                hipLaunchKernelGGL(coop_reset_kernel, dim3((unsigned)((nact + 63) / 64)), dim3(64), 0,
+                                   s, xbuf, sys_words, nact);
+                ctx->mgs_xbuf = xbuf;
+                ctx->mgs_xbuf_ready = nact;
+            }
             const dim3 grid((unsigned)G, (unsigned)nact);
 #define CCAB_MGS_COOP(R)                                                                        \
     do {                                                                                        \
@@ -375,14 +397,23 @@ __global__ void gir_cycle_init_kernel(int m, const int32_t* __restrict__ act, in
 
 // Column k from the Gram-Schmidt step (h[0..k+1], stride sh per system)
 __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act, int nact,
-                                  const c128* __restrict__ hd, int64_t sh, c128* Hg, c128* csd,
-                                  c128* snd, c128* bed, int32_t* kdone,
+                                  const c128* __restrict__ hd, int64_t sh, c128* __restrict__ Hg,
+                                  c128* __restrict__ csd, c128* __restrict__ snd,
+                                  c128* __restrict__ bed, int32_t* __restrict__ kdone,
                                   const double* __restrict__ rn, const double* __restrict__ bn,
-                                  double inner_tol, double tol, c128* Hbar) {
+                                  double inner_tol, double tol, c128* __restrict__ Hbar,
+                                  int32_t* __restrict__ flags_out) {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     if (z >= nact) return;
     const int64_t b = act[z];
-    if (kdone[b] != 0) return;
+    const int32_t kd = kdone[b];
+    if (kd != 0) {
+        if (flags_out) {
+            flags_out[b] = kd;
+            __threadfence_system();
+        }
+        return;
+    }
     const c128* h = hd + b * sh;
     c128* col = Hg + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
     if (Hbar) {  // the unrotated column, for A Q_k = Q_(k+1) Hbar_k
@@ -392,13 +423,17 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
     c128* cs = csd + b * m;
     c128* sn = snd + b * m;
     c128* be = bed + b * (m + 1);
-    for (int i = 0; i <= k + 1; ++i) col[i] = h[i];
+    // the previous rotations applied to the new column: entry i+1 enters
+    // rotation i unrotated, entry i arrives rotated (carried in registers,
+    // so the loads of cs, sn and h do not wait on the chain)
+    c128 carry = h[0];
+#pragma unroll 4
     for (int i = 0; i < k; ++i) {
-        const c128 a = col[i], c = col[i + 1];
-        col[i] = cs[i] * a + sn[i] * c;
-        col[i + 1] = cs[i] * c - conj(sn[i]) * a;
+        const c128 c = cs[i], sv = sn[i], nx = h[i + 1];
+        col[i] = c * carry + sv * nx;
+        carry = c * nx - conj(sv) * carry;
     }
-    const c128 v1 = col[k], v2 = col[k + 1];
+    const c128 v1 = carry, v2 = h[k + 1];
     const double a1 = cabs_d(v1);
     const c128 signf = (v1.re == 0.0 && v1.im == 0.0) ? c128{1.0, 0.0} : c128{v1.re / a1, v1.im / a1};
     const double rr = sqrt(norm_sqr(v1) + norm_sqr(v2));
@@ -414,8 +449,13 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
     be[k] = ck * be[k];
     const double res = cabs_d(bk1);
     const double est = res / rn[b];
-    if (est <= inner_tol || k == m - 1 || res <= 0.5 * tol * bn[b] || !isfinite(est))
-        kdone[b] = k + 1;
+    const bool done = est <= inner_tol || k == m - 1 || res <= 0.5 * tol * bn[b] || !isfinite(est);
+    if (done) kdone[b] = k + 1;
+    // the host's copy of the flag (pinned, device-visible memory)
+    if (flags_out) {
+        flags_out[b] = done ? k + 1 : 0;
+        __threadfence_system();
+    }
 }
 
 // y = R^{-1} beta over the system's kdone[b] steps (R the rotated, upper
@@ -445,6 +485,46 @@ __global__ void gir_solve_y_kernel(int m, const int32_t* __restrict__ act, int n
         c128 acc{0.0, 0.0};
         if (i <= kk)  // column j holds rows 0..j+1
             for (int j = (i > 0 ? i - 1 : 0); j < kk; ++j) acc += Hb[i + int64_t(j) * (m + 1)] * y[j];
+        zz[i] = acc;
+    }
+}
+
+// The same for restart lengths m <= kSolveYLdsM: one wave per system, H (and
+// Hbar) staged in LDS by all lanes at once, then the column-oriented back
+// substitution with the row updates spread over the lanes, and z row-parallel.
+constexpr int kSolveYLdsM = 40;
+
+__global__ __launch_bounds__(64) void gir_solve_y_wave_kernel(
+    int m, const int32_t* __restrict__ act, const c128* __restrict__ Hg,
+    const c128* __restrict__ bed, const int32_t* __restrict__ kdone, c128* Yd,
+    const c128* __restrict__ Hbar, c128* Zd) {
+    __shared__ c128 sH[kSolveYLdsM * (kSolveYLdsM + 1)];
+    __shared__ c128 sy[kSolveYLdsM + 1];
+    const int lane = threadIdx.x;
+    const int64_t b = act[blockIdx.x];
+    const int kk = kdone[b];
+    const int ld = m + 1;
+    const c128* H = Hg + b * int64_t(ld) * m;
+    for (int e = lane; e < ld * kk; e += 64) sH[e] = H[e];
+    for (int i = lane; i <= m; i += 64) sy[i] = i < kk ? bed[b * ld + i] : c128{0.0, 0.0};
+    __syncthreads();
+    for (int j = kk - 1; j >= 0; --j) {
+        if (lane == 0) sy[j] = cdiv(sy[j], sH[j + j * ld]);
+        __syncthreads();
+        const c128 yj = sy[j];
+        for (int r = lane; r < j; r += 64) sy[r] = sy[r] - yj * sH[r + j * ld];
+        __syncthreads();
+    }
+    c128* y = Yd + b * ld;
+    for (int i = lane; i <= m; i += 64) y[i] = sy[i];
+    if (!Hbar) return;
+    // z = Hbar y: column j of Hbar holds rows 0..j+1
+    const c128* Hb = Hbar + b * int64_t(ld) * m;
+    c128* zz = Zd + b * ld;
+    for (int i = lane; i <= m; i += 64) {
+        c128 acc{0.0, 0.0};
+        if (i <= kk)
+            for (int j = (i > 0 ? i - 1 : 0); j < kk; ++j) acc += Hb[i + int64_t(j) * ld] * sy[j];
         zz[i] = acc;
     }
 }
@@ -802,6 +882,13 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     auto ring_done = [&](int k) {
         return reinterpret_cast<int32_t*>(pin + pin_a + pin_n + size_t(m + 1 + k) * pin_a);
     };
+    // the Givens kernel writes the step's done flags straight into the pinned
+    // slot (device address of the same memory)
+    char* pin_dev = nullptr;
+    CCAB_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&pin_dev), ctx->pinned, 0));
+    auto ring_done_dev = [&](int k) {
+        return reinterpret_cast<int32_t*>(pin_dev + pin_a + pin_n + size_t(m + 1 + k) * pin_a);
+    };
     // an event per Arnoldi step marks its done-flag copy
     while ((int)ctx->gir_events.size() < m + 1) {
         hipEvent_t e;
@@ -919,9 +1006,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
             }
             hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act,
                                (int)ni, Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d,
-                               inner_tol, tol, Hbar);
+                               inner_tol, tol, Hbar, ring_done_dev(k));
             CCAB_HIP(hipGetLastError());
-            CCAB_HIP(hipMemcpyAsync(ring_done(k), kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
             CCAB_HIP(hipEventRecord(ctx->gir_events[k], s));
             // newest finished flags: wait for step k - kLag, take a later one if ready
             int target = -1;
@@ -949,8 +1035,12 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         // every system of the cycle is done on the device (step m - 1 ends
         // the cycle): y = R^{-1} beta, then x += Q y
         CCAB_TRY(upload_active(active, act));
-        hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s, m,
-                           act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
+        if (m <= kSolveYLdsM)
+            hipLaunchKernelGGL(gir_solve_y_wave_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m, act,
+                               Hg, bed, kdone_d, Yd, Hbar, Zd);
+        else
+            hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
+                               m, act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
         hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
                            dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X, Zd, Dv, R);
         CCAB_HIP(hipMemcpyAsync(hAct, kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
