@@ -135,6 +135,10 @@ __device__ __forceinline__ V ld16(const void* p) {
     }
 }
 
+// columns whose loads a lane has in flight at once (64-system n = 5120 sweep:
+// 4 -> 38.5 ms of matvecs, 8 -> 37.9 ms, 16 -> 39.1 ms)
+constexpr int kVecCols = 8;
+
 template <typename TA, int R, bool NT = false>
 __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N, const TA* __restrict__ A,
                                                             int64_t lda, int64_t sA,
@@ -163,12 +167,12 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
     for (int r = 0; r < R; ++r) re[r] = im[r] = 0.0;
     if (i0 + R <= mb) {
         int64_t j = j0;
-        for (; j + 3 < j1; j += 4) {
-            V v[4];
+        for (; j + kVecCols - 1 < j1; j += kVecCols) {
+            V v[kVecCols];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = ld16<NT, V>(a + (j + u) * lda);
+            for (int u = 0; u < kVecCols; ++u) v[u] = ld16<NT, V>(a + (j + u) * lda);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kVecCols; ++u) {
                 c128 av[R];
                 unpack16<R>(v[u], av);
                 const c128 xv = xb[j + u];
