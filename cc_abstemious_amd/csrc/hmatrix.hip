@@ -7,8 +7,11 @@
 // Block storage in HBM: row / column index lists back to back; dense blocks
 // column-major back to back (c128); low-rank blocks as U (nr x r) and V
 // (nc x r) with A ~ U V^T (non-conjugated, as the reference's ACA::gemv).
-// The matvec writes every block's contribution to a per-(block,row) slot and
-// a row-wise CSR reduction adds them in a fixed order (no atomics).
+// The matvec (no atomics, fixed summation order): t_b = V_b^T x for every
+// low-rank block; then one wave per 64-row tile of every row cluster adds,
+// over the cluster's blocks (dense first, then low-rank, in block order),
+// D_b x_cols and U_b t_b into one slot per (cluster, row); then every row sums
+// the slots of the clusters that contain it (about one per tree level).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -31,7 +34,7 @@ struct HBlkDev {
     int64_t r_off, nr, c_off, nc;  // into rows[] / cols[]
     int64_t u_off, v_off;          // into U, V (capacity nr*RMAX, nc*RMAX)
     int64_t d_off;                 // into D (dense blocks)
-    int64_t p_off;                 // into the matvec partial buffer
+    int64_t p_off;                 // low-rank: offset of V^T x in the matvec's t buffer
     int64_t w_off;                 // ACA workspace (vectors), in c128 units
     int64_t f_off;                 // ACA exclusion flags, in bytes
     int32_t adm, rank;
@@ -682,57 +685,84 @@ __global__ __launch_bounds__(kAcaThreads) void aca_norm_kernel(HBlkDev* blks,
 }
 
 // --- matvec ------------------------------------------------------------------
-// part[p_off + i] = alpha * (D x_cols)[i]  (dense blocks)
-__global__ void hmv_dense_kernel(const HBlkDev* __restrict__ blks, const int32_t* __restrict__ list,
-                                 const int64_t* __restrict__ cols, const c128* __restrict__ D,
-                                 const c128* __restrict__ x, c128 alpha, c128* part) {
+// t_b = V_b^T x_cols, rank entries per low-rank block at t + p_off: one
+// wave per block, lanes over the block's columns, four rank columns of V per
+// pass, then fixed-order wave reductions (one-wave workgroups measured faster
+// than four blocks per workgroup)
+__global__ __launch_bounds__(64) void hmv_vt_kernel(const HBlkDev* __restrict__ blks,
+                                                    const int32_t* __restrict__ list,
+                                                    const int64_t* __restrict__ cols,
+                                                    const c128* __restrict__ V,
+                                                    const c128* __restrict__ x, c128* t) {
     const HBlkDev bk = blks[list[blockIdx.x]];
-    for (int64_t i = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; i < bk.nr;
-         i += int64_t(blockDim.x) * gridDim.y) {
-        c128 s{0.0, 0.0};
-        for (int64_t j = 0; j < bk.nc; ++j) s += D[bk.d_off + i + j * bk.nr] * x[cols[bk.c_off + j]];
-        part[bk.p_off + i] = alpha * s;
+    const int r = bk.rank;
+    const int64_t nc = bk.nc;
+    const c128* Vb = V + bk.v_off;
+    const int64_t* cj = cols + bk.c_off;
+    const int lane = threadIdx.x;
+    for (int k0 = 0; k0 < r; k0 += 4) {
+        c128 acc[4] = {};
+        for (int64_t j = lane; j < nc; j += 64) {
+            const c128 xj = x[cj[j]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k0 + u < r) acc[u] += Vb[j + (k0 + u) * nc] * xj;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
+            if (lane == 0 && k0 + u < r) t[bk.p_off + k0 + u] = c128{re, im};
+        }
     }
 }
 
-// part[p_off + i] = alpha * (U (V^T x_cols))[i]  (low-rank blocks)
-__global__ __launch_bounds__(256) void hmv_lowrank_kernel(const HBlkDev* __restrict__ blks,
-                                                          const int32_t* __restrict__ list,
-                                                          const int64_t* __restrict__ cols,
-                                                          const c128* __restrict__ U,
-                                                          const c128* __restrict__ V,
-                                                          const c128* __restrict__ x, c128 alpha,
-                                                          c128* part) {
-    __shared__ c128 t[RMAX];
-    __shared__ double red[2 * 4 * RMAX];
-    const HBlkDev bk = blks[list[blockIdx.x]];
-    const int r = bk.rank;
-    for (int k = 0; k < r; ++k) {
-        double re = 0, im = 0;
-        for (int64_t j = threadIdx.x; j < bk.nc; j += 256) {
-            const c128 z = V[bk.v_off + j + k * bk.nc] * x[cols[bk.c_off + j]];
-            re += z.re;
-            im += z.im;
+// One wave per 64-row tile of a row cluster: acc = sum over the cluster's
+// blocks of D_b x_cols (dense) and U_b t_b (low-rank), nacc[slot] = alpha acc
+__global__ __launch_bounds__(64) void hmv_node_kernel(
+    const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_r0,
+    const int64_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
+    const int64_t* __restrict__ node_off, const HBlkDev* __restrict__ blks,
+    const int64_t* __restrict__ cols, const c128* __restrict__ D, const c128* __restrict__ U,
+    const c128* __restrict__ t, const c128* __restrict__ x, c128 alpha, c128* nacc) {
+    const int c = tile_node[blockIdx.x];
+    const int64_t p = tile_r0[blockIdx.x] + threadIdx.x;
+    const int64_t e0 = nptr[c], e1 = nptr[c + 1];
+    const int64_t nr = blks[nlist[e0]].nr;  // every block of the cluster has its rows
+    const bool on = p < nr;
+    const int64_t pp = on ? p : 0;          // lanes past the cluster read row 0
+    c128 acc{0.0, 0.0};
+    for (int64_t e = e0; e < e1; ++e) {
+        const HBlkDev bk = blks[nlist[e]];
+        if (!bk.adm) {
+            const c128* d = D + bk.d_off + pp;
+            const int64_t* cj = cols + bk.c_off;
+            int64_t j = 0;
+            for (; j + 7 < bk.nc; j += 8) {
+                c128 a[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    a[u] = d[(j + u) * nr];
+                    xv[u] = x[cj[j + u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
+            }
+            for (; j < bk.nc; ++j) acc += d[j * nr] * x[cj[j]];
+        } else {
+            const c128* uu = U + bk.u_off + pp;
+            const c128* tb = t + bk.p_off;
+            int k = 0;
+            for (; k + 3 < bk.rank; k += 4) {
+                c128 a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] = uu[(k + u) * nr];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc += a[u] * tb[k + u];
+            }
+            for (; k < bk.rank; ++k) acc += uu[k * nr] * tb[k];
         }
-        re = wave_sum(re);
-        im = wave_sum(im);
-        if ((threadIdx.x & 63) == 0) {
-            red[(k * 4 + (threadIdx.x >> 6)) * 2] = re;
-            red[(k * 4 + (threadIdx.x >> 6)) * 2 + 1] = im;
-        }
     }
-    __syncthreads();
-    if (threadIdx.x < r) {
-        c128 s{0.0, 0.0};
-        for (int w = 0; w < 4; ++w) s += c128{red[(threadIdx.x * 4 + w) * 2], red[(threadIdx.x * 4 + w) * 2 + 1]};
-        t[threadIdx.x] = alpha * s;
-    }
-    __syncthreads();
-    for (int64_t i = threadIdx.x; i < bk.nr; i += 256) {
-        c128 s{0.0, 0.0};
-        for (int k = 0; k < r; ++k) s += U[bk.u_off + i + k * bk.nr] * t[k];
-        part[bk.p_off + i] = s;
-    }
+    if (on) nacc[node_off[c] + p] = alpha * acc;
 }
 
 // y[i] = beta y[i] + sum of the contributions of row i (fixed order)
@@ -791,9 +821,17 @@ struct ccab_hmatrix {
     c128* d_U = nullptr;
     c128* d_V = nullptr;
     c128* d_D = nullptr;
-    c128* d_part = nullptr;
-    int64_t* d_rowptr = nullptr;
+    // matvec plan: t = V^T x per low-rank block, one slot per (cluster, row)
+    c128* d_t = nullptr;
+    c128* d_nacc = nullptr;
+    int32_t* d_tile_node = nullptr;  // 64-row tiles of the clusters that own blocks
+    int32_t* d_tile_r0 = nullptr;
+    int64_t* d_nptr = nullptr;       // cluster -> its blocks (CSR over d_nlist)
+    int32_t* d_nlist = nullptr;
+    int64_t* d_node_off = nullptr;   // cluster -> first slot
+    int64_t* d_rowptr = nullptr;     // row -> its slots (CSR over d_ent)
     int64_t* d_ent = nullptr;
+    int64_t n_tiles = 0;
     int32_t* d_dense_list = nullptr;
     int32_t* d_lr_list = nullptr;
     int64_t n_dense = 0, n_lr = 0;
@@ -860,7 +898,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     for (size_t c = 0; c < T.size(); ++c)
         std::copy(T[c].idx.begin(), T[c].idx.end(), rows.begin() + node_off[c]);
     H->blk.resize(nb);
-    int64_t u_off = 0, v_off = 0, d_off = 0, p_off = 0, w_off = 0, f_off = 0;
+    int64_t u_off = 0, v_off = 0, d_off = 0, w_off = 0, f_off = 0;
     std::vector<int32_t> dl, ll;
     for (int64_t b = 0; b < nb; ++b) {
         HBlkDev& d = H->blk[b];
@@ -870,8 +908,6 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         d.c_off = node_off[bl[b].c];
         d.nc = (int64_t)T[bl[b].c].idx.size();
         d.adm = bl[b].adm ? 1 : 0;
-        d.p_off = p_off;
-        p_off += d.nr;
         if (d.adm) {
             // the ACA takes at most min(nr, nc) terms
             const int64_t cap = std::min<int64_t>(std::min(d.nr, d.nc), RMAX);
@@ -984,9 +1020,15 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     std::sort(dl.begin(), dl.end());
     H->n_dense = (int64_t)dl.size();
     H->n_lr = (int64_t)ll2.size();
+    // V^T x of every low-rank block: rank entries at p_off
+    int64_t t_total = 0;
+    for (int32_t b : ll2) {
+        H->blk[b].p_off = t_total;
+        t_total += H->blk[b].rank;
+    }
     H->d_D = dalloc<c128>(H->owned, d_off);
-    H->d_part = dalloc<c128>(H->owned, p_off);
-    if (!H->d_D || !H->d_part) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    H->d_t = dalloc<c128>(H->owned, t_total);
+    if (!H->d_D || !H->d_t) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
     CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
@@ -1007,24 +1049,60 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     H->norm = nd + nl;
     for (const auto& d : H->blk)
         H->mv_bytes += 16.0 * (d.adm ? double(d.rank) * double(d.nr + d.nc) : double(d.nr) * double(d.nc));
-    // matvec plan: row CSR over the per-(block,row) contribution slots
+    // matvec plan: every cluster's blocks (dense first, then low-rank, each in
+    // block order), 64-row tiles of the clusters that own blocks, and for
+    // every row the slots of those clusters (cluster order)
+    const int64_t nn = (int64_t)T.size();
+    std::vector<int64_t> nptr(nn + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) nptr[bl[b].r + 1]++;
+    for (int64_t c = 0; c < nn; ++c) nptr[c + 1] += nptr[c];
+    std::vector<int32_t> nlist(nb);
+    {
+        std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int64_t b = 0; b < nb; ++b)
+                if ((pass == 0) == (H->blk[b].adm == 0)) nlist[fill[bl[b].r]++] = (int32_t)b;
+    }
+    std::vector<int32_t> tile_node, tile_r0;
     std::vector<int64_t> cnt(n + 1, 0);
-    for (int64_t b = 0; b < nb; ++b)
-        for (int64_t i = 0; i < H->blk[b].nr; ++i) cnt[rows[H->blk[b].r_off + i] + 1]++;
-    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-    std::vector<int64_t> ent(p_off), fillp(cnt.begin(), cnt.end() - 1);
-    // dense contributions first, then low-rank, each in block order (h_matrix.rs:228-233)
-    for (int pass = 0; pass < 2; ++pass)
-        for (int64_t b = 0; b < nb; ++b) {
-            const HBlkDev& d = H->blk[b];
-            if ((pass == 0) != (d.adm == 0)) continue;
-            for (int64_t i = 0; i < d.nr; ++i) ent[fillp[rows[d.r_off + i]]++] = d.p_off + i;
+    for (int64_t c = 0; c < nn; ++c) {
+        if (nptr[c + 1] == nptr[c]) continue;
+        const int64_t sz = (int64_t)T[c].idx.size();
+        for (int64_t r0 = 0; r0 < sz; r0 += 64) {
+            tile_node.push_back((int32_t)c);
+            tile_r0.push_back((int32_t)r0);
         }
+        for (int64_t q = 0; q < sz; ++q) cnt[T[c].idx[q] + 1]++;
+    }
+    for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int64_t> ent(cnt[n]);
+    {
+        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t c = 0; c < nn; ++c) {
+            if (nptr[c + 1] == nptr[c]) continue;
+            for (int64_t q = 0; q < (int64_t)T[c].idx.size(); ++q)
+                ent[fill[T[c].idx[q]]++] = node_off[c] + q;
+        }
+    }
+    H->n_tiles = (int64_t)tile_node.size();
+    H->d_nacc = dalloc<c128>(H->owned, nidx);
+    H->d_tile_node = dalloc<int32_t>(H->owned, tile_node.size());
+    H->d_tile_r0 = dalloc<int32_t>(H->owned, tile_r0.size());
+    H->d_nptr = dalloc<int64_t>(H->owned, nn + 1);
+    H->d_nlist = dalloc<int32_t>(H->owned, nb);
+    H->d_node_off = dalloc<int64_t>(H->owned, nn);
     H->d_rowptr = dalloc<int64_t>(H->owned, n + 1);
-    H->d_ent = dalloc<int64_t>(H->owned, p_off);
-    if (!H->d_rowptr || !H->d_ent) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    H->d_ent = dalloc<int64_t>(H->owned, ent.size());
+    if (!H->d_nacc || !H->d_tile_node || !H->d_tile_r0 || !H->d_nptr || !H->d_nlist ||
+        !H->d_node_off || !H->d_rowptr || !H->d_ent)
+        return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    CCAB_HIP(hipMemcpy(H->d_tile_node, tile_node.data(), tile_node.size() * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_tile_r0, tile_r0.data(), tile_r0.size() * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_nptr, nptr.data(), (nn + 1) * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_nlist, nlist.data(), nb * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_node_off, node_off.data(), nn * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), p_off * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), ent.size() * 8, hipMemcpyHostToDevice));
     trace.mark("matvec plan");
     // release the build workspace
     for (void* w : {static_cast<void*>(U2), static_cast<void*>(V2), static_cast<void*>(Wws),
@@ -1044,14 +1122,15 @@ int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x
                  c128* y) {
     hipStream_t s = ctx->stream;
     ProfScope ps(ctx, PROF_HMAT, H->mv_bytes);
-    if (H->n_dense)
-        hipLaunchKernelGGL(hmv_dense_kernel, dim3((unsigned)H->n_dense, 1), dim3(256), 0, s,
-                           H->d_blk, H->d_dense_list, H->d_cols, H->d_D, x, alpha, H->d_part);
     if (H->n_lr)
-        hipLaunchKernelGGL(hmv_lowrank_kernel, dim3((unsigned)H->n_lr), dim3(256), 0, s, H->d_blk,
-                           H->d_lr_list, H->d_cols, H->d_U, H->d_V, x, alpha, H->d_part);
+        hipLaunchKernelGGL(hmv_vt_kernel, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
+                           H->d_lr_list, H->d_cols, H->d_V, x, H->d_t);
+    if (H->n_tiles)
+        hipLaunchKernelGGL(hmv_node_kernel, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
+                           H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
+                           H->d_blk, H->d_cols, H->d_D, H->d_U, H->d_t, x, alpha, H->d_nacc);
     hipLaunchKernelGGL(hmv_reduce_kernel, dim3((unsigned)((H->n + 255) / 256)), dim3(256), 0, s,
-                       H->n, H->d_rowptr, H->d_ent, H->d_part, beta, y);
+                       H->n, H->d_rowptr, H->d_ent, H->d_nacc, beta, y);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
