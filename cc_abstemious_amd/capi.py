@@ -151,6 +151,8 @@ SIGNATURES = {
     "ccab_solve_bodies": (_I, [_P, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ccab_hmatrix_build": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _I64, ct.c_double,
                                 ct.c_double, ct.POINTER(_P)]),
+    "ccab_hmatrix_build_ex": (_I, [_P, ct.POINTER(Physics), _P, _I64, _I, _I64, ct.c_double,
+                                   ct.c_double, _I, ct.POINTER(_P)]),
     "ccab_hmatrix_destroy": (_I, [_P]),
     "ccab_hmatrix_gemv": (_I, [_P, _P, _P, _P, _P, _P]),
     "ccab_hmatrix_to_full": (_I, [_P, _P, _P, _I64]),

@@ -226,6 +226,10 @@ int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
     if (!tmp) return CCAB_ENOMEM;
     const dim3 g((unsigned)((n + 255) / 256));
     const c128 one{1.0, 0.0}, mone{-1.0, 0.0};
+    // block storage: c128 as the reference, or complex FP32 (matrix_prec C64)
+    CCAB_CHECK(cfg->gmres.matrix_prec == CCAB_C128 || cfg->gmres.matrix_prec == CCAB_C64,
+               "H-matrix storage is CCAB_C128 or CCAB_C64");
+    const int hprec = cfg->gmres.matrix_prec;
     for (int64_t f = 0; f < nfreq; ++f) {
         const SysParams& p = hp[f];
         c128* x = rhs + f * n;
@@ -239,7 +243,7 @@ int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
                 hipLaunchKernelGGL(negate_kernel, g, dim3(256), 0, ctx->stream, n, xc);
         } else {
             ccab_hmatrix* hr_raw = nullptr;
-            CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_RHS, leaf, eta, tol, &hr_raw));
+            CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_RHS, leaf, eta, tol, &hr_raw, hprec));
             HPtr hr(hr_raw);
             hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, p.bcv,
                                reinterpret_cast<ccab_c128*>(tmp), 0);
@@ -249,7 +253,7 @@ int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
             CCAB_HIP(hipStreamSynchronize(ctx->stream));
         }
         ccab_hmatrix* hl_raw = nullptr;
-        CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_LHS, leaf, eta, tol, &hl_raw));
+        CCAB_TRY(hmatrix_build(ctx, p, soa, h.data(), n, CCAB_LHS, leaf, eta, tol, &hl_raw, hprec));
         HPtr hl(hl_raw);
         MatvecFn op = [&](const c128* xin, c128 a, c128 b, c128* yout) -> int {
             return hmatrix_gemv(ctx, hl.get(), a, xin, b, yout);
@@ -1054,8 +1058,16 @@ extern "C" int ccab_cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, 
 extern "C" int ccab_hmatrix_build(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
                                   int64_t n, int side, int64_t leaf_size, double eta,
                                   double aca_tolerance, ccab_hmatrix** out) {
+    return ccab_hmatrix_build_ex(ctx, ph, d_cpts, n, side, leaf_size, eta, aca_tolerance,
+                                 CCAB_C128, out);
+}
+
+extern "C" int ccab_hmatrix_build_ex(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts,
+                                     int64_t n, int side, int64_t leaf_size, double eta,
+                                     double aca_tolerance, int prec, ccab_hmatrix** out) {
     if (!ctx) return CCAB_EINVAL;
     CCAB_CHECK(ph && d_cpts && out && n > 0, "invalid arguments");
+    CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "H-matrix storage is CCAB_C128 or CCAB_C64");
     *out = nullptr;
     CCAB_CHECK(side == CCAB_LHS || side == CCAB_RHS, "bad side %d", side);
     CCAB_CHECK(leaf_size >= 1, "leaf_size must be positive");
@@ -1067,7 +1079,8 @@ extern "C" int ccab_hmatrix_build(ccab_ctx* ctx, const ccab_physics* ph, const d
     CCAB_TRY(stage_soa(ctx, d_cpts, n, &soa));
     std::vector<double> h;
     CCAB_TRY(host_cpts(ctx, d_cpts, n, h));
-    return hmatrix_build(ctx, hp[0], soa, h.data(), n, side, leaf_size, eta, aca_tolerance, out);
+    return hmatrix_build(ctx, hp[0], soa, h.data(), n, side, leaf_size, eta, aca_tolerance, out,
+                         prec);
 }
 
 extern "C" int ccab_hmatrix_destroy(ccab_hmatrix* h) {

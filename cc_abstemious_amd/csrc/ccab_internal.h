@@ -107,7 +107,8 @@ enum BufSlot {
     SLOT_SOLVE_PIV, SLOT_SOLVE_X, SLOT_IDX, SLOT_GIR_B, SLOT_GIR_R, SLOT_GIR_Q, SLOT_GIR_H,
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
-    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_COUNT
+    SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
+    SLOT_HM_F, SLOT_HM_R, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -364,7 +365,8 @@ int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, 
 
 // hmatrix.hip
 int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
-                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out);
+                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
+                  int prec = CCAB_C128);  // CCAB_C64: blocks stored in complex FP32
 int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
                  c128* y);
 int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda);

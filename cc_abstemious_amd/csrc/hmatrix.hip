@@ -32,11 +32,13 @@ constexpr int kRngTable = 1024; // ChaCha12 words available to one block's draws
 
 struct HBlkDev {
     int64_t r_off, nr, c_off, nc;  // into rows[] / cols[]
-    int64_t u_off, v_off;          // into U, V (capacity nr*RMAX, nc*RMAX)
+    int64_t u_off, v_off;          // into U, V (during the ACA: the capacity buffers, nr x cap
+                                   // and nc x cap; after recompression: the packed factors)
     int64_t d_off;                 // into D (dense blocks)
     int64_t p_off;                 // low-rank: offset of V^T x in the matvec's t buffer
     int64_t w_off;                 // ACA workspace (vectors), in c128 units
     int64_t f_off;                 // ACA exclusion flags, in bytes
+    int64_t u2_off, v2_off;        // recompressed U, V (packed to the ACA rank)
     int32_t adm, rank;
     int32_t status;                // ACA: 0 ok, 1 rank capacity reached, 2 rng exhausted
     int32_t pad_;
@@ -507,13 +509,13 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
     if (bk.status != 0) return;
     const int r = bk.rank;  // <= RM (the host buckets blocks by rank)
     const int64_t nr = bk.nr, nc = bk.nc;
-    c128* Ru = Rws + int64_t(blockIdx.x) * 2 * RMAX * RMAX;  // [c][i] = R[i][c]
-    c128* Rv = Ru + RMAX * RMAX;
+    c128* Ru = Rws + int64_t(blockIdx.x) * 2 * RM * RM;  // [c][i] = R[i][c]
+    c128* Rv = Ru + RM * RM;
     for (int side = 0; side < 2; ++side) {
         c128* Q = side == 0 ? U + bk.u_off : V + bk.v_off;
         const int64_t len = side == 0 ? nr : nc;
         c128* R = side == 0 ? Ru : Rv;
-        for (int e = threadIdx.x; e < RMAX * RMAX; e += kAcaThreads) R[e] = c128{0.0, 0.0};
+        for (int e = threadIdx.x; e < RM * RM; e += kAcaThreads) R[e] = c128{0.0, 0.0};
         __syncthreads();
         for (int c = 0; c < r; ++c) {
             c128* qc = Q + c * len;
@@ -549,7 +551,7 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
                     c128 h{0.0, 0.0};
                     for (int w = 0; w < kAcaThreads / 64; ++w)
                         h += c128{s_tmp[threadIdx.x][0][w], s_tmp[threadIdx.x][1][w]};
-                    R[c * RMAX + threadIdx.x] += h;
+                    R[c * RM + threadIdx.x] += h;
                 }
                 __syncthreads();
             }
@@ -561,7 +563,7 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
             const double rn = sqrt(red[0] + red[1] + red[2] + red[3]);
             const double inv = rn > 1e-300 ? 1.0 / rn : 0.0;
             for (int64_t t = threadIdx.x; t < len; t += kAcaThreads) qc[t] = qc[t] * inv;
-            if (threadIdx.x == 0) R[c * RMAX + c] = c128{rn, 0.0};
+            if (threadIdx.x == 0) R[c * RM + c] = c128{rn, 0.0};
             __syncthreads();
         }
     }
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
     for (int e = threadIdx.x; e < r * r; e += kAcaThreads) {
         const int i = e % r, c = e / r;
         c128 sum{0.0, 0.0};
-        for (int t = 0; t < r; ++t) sum += Ru[t * RMAX + i] * Rv[t * RMAX + c];
+        for (int t = 0; t < r; ++t) sum += Ru[t * RM + i] * Rv[t * RM + c];
         Wm[i][c] = sum;
         Vs[i][c] = c128{i == c ? 1.0 : 0.0, 0.0};
     }
@@ -646,12 +648,12 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
         for (int64_t i = threadIdx.x; i < nr; i += kAcaThreads) {
             c128 acc{0.0, 0.0};
             for (int c = 0; c < r; ++c) acc += Qu[i + c * nr] * Wm[c][o];
-            U2[bk.u_off + i + t * nr] = acc;
+            U2[bk.u2_off + i + t * nr] = acc;
         }
         for (int64_t j = threadIdx.x; j < nc; j += kAcaThreads) {
             c128 acc{0.0, 0.0};
             for (int c = 0; c < r; ++c) acc += Qv[j + c * nc] * conj(Vs[c][o]);
-            V2[bk.v_off + j + t * nc] = acc;
+            V2[bk.v2_off + j + t * nc] = acc;
         }
     }
     __syncthreads();
@@ -685,19 +687,27 @@ __global__ __launch_bounds__(kAcaThreads) void aca_norm_kernel(HBlkDev* blks,
 }
 
 // --- matvec ------------------------------------------------------------------
+// stored block entries (c128, or c64 with FP32 storage) as FP64
+__device__ __forceinline__ c128 ldx(const c128* p) { return *p; }
+__device__ __forceinline__ c128 ldx(const c64* p) {
+    const c64 v = *p;
+    return c128{double(v.re), double(v.im)};
+}
+
 // t_b = V_b^T x_cols, rank entries per low-rank block at t + p_off: one
 // wave per block, lanes over the block's columns, four rank columns of V per
 // pass, then fixed-order wave reductions (one-wave workgroups measured faster
 // than four blocks per workgroup)
+template <typename T>
 __global__ __launch_bounds__(64) void hmv_vt_kernel(const HBlkDev* __restrict__ blks,
                                                     const int32_t* __restrict__ list,
                                                     const int64_t* __restrict__ cols,
-                                                    const c128* __restrict__ V,
+                                                    const T* __restrict__ V,
                                                     const c128* __restrict__ x, c128* t) {
     const HBlkDev bk = blks[list[blockIdx.x]];
     const int r = bk.rank;
     const int64_t nc = bk.nc;
-    const c128* Vb = V + bk.v_off;
+    const T* Vb = V + bk.v_off;
     const int64_t* cj = cols + bk.c_off;
     const int lane = threadIdx.x;
     for (int k0 = 0; k0 < r; k0 += 4) {
@@ -706,7 +716,7 @@ __global__ __launch_bounds__(64) void hmv_vt_kernel(const HBlkDev* __restrict__ 
             const c128 xj = x[cj[j]];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (k0 + u < r) acc[u] += Vb[j + (k0 + u) * nc] * xj;
+                if (k0 + u < r) acc[u] += ldx(Vb + j + (k0 + u) * nc) * xj;
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -718,11 +728,12 @@ __global__ __launch_bounds__(64) void hmv_vt_kernel(const HBlkDev* __restrict__ 
 
 // One wave per 64-row tile of a row cluster: acc = sum over the cluster's
 // blocks of D_b x_cols (dense) and U_b t_b (low-rank), nacc[slot] = alpha acc
+template <typename T>
 __global__ __launch_bounds__(64) void hmv_node_kernel(
     const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_r0,
     const int64_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
     const int64_t* __restrict__ node_off, const HBlkDev* __restrict__ blks,
-    const int64_t* __restrict__ cols, const c128* __restrict__ D, const c128* __restrict__ U,
+    const int64_t* __restrict__ cols, const T* __restrict__ D, const T* __restrict__ U,
     const c128* __restrict__ t, const c128* __restrict__ x, c128 alpha, c128* nacc) {
     const int c = tile_node[blockIdx.x];
     const int64_t p = tile_r0[blockIdx.x] + threadIdx.x;
@@ -734,32 +745,32 @@ __global__ __launch_bounds__(64) void hmv_node_kernel(
     for (int64_t e = e0; e < e1; ++e) {
         const HBlkDev bk = blks[nlist[e]];
         if (!bk.adm) {
-            const c128* d = D + bk.d_off + pp;
+            const T* d = D + bk.d_off + pp;
             const int64_t* cj = cols + bk.c_off;
             int64_t j = 0;
             for (; j + 7 < bk.nc; j += 8) {
                 c128 a[8], xv[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    a[u] = d[(j + u) * nr];
+                    a[u] = ldx(d + (j + u) * nr);
                     xv[u] = x[cj[j + u]];
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
             }
-            for (; j < bk.nc; ++j) acc += d[j * nr] * x[cj[j]];
+            for (; j < bk.nc; ++j) acc += ldx(d + j * nr) * x[cj[j]];
         } else {
-            const c128* uu = U + bk.u_off + pp;
+            const T* uu = U + bk.u_off + pp;
             const c128* tb = t + bk.p_off;
             int k = 0;
             for (; k + 3 < bk.rank; k += 4) {
                 c128 a[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) a[u] = uu[(k + u) * nr];
+                for (int u = 0; u < 4; ++u) a[u] = ldx(uu + (k + u) * nr);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) acc += a[u] * tb[k + u];
             }
-            for (; k < bk.rank; ++k) acc += uu[k * nr] * tb[k];
+            for (; k < bk.rank; ++k) acc += ldx(uu + k * nr) * tb[k];
         }
     }
     if (on) nacc[node_off[c] + p] = alpha * acc;
@@ -781,9 +792,10 @@ __global__ void hmv_reduce_kernel(int64_t n, const int64_t* __restrict__ rowptr,
 
 // A[rows[i] + cols[j] lda] = block entry (the blocks partition the matrix,
 // so every entry is written once; h_matrix.rs:235-247)
+template <typename T>
 __global__ void hmat_full_kernel(const HBlkDev* __restrict__ blks, const int64_t* __restrict__ rows,
-                                 const int64_t* __restrict__ cols, const c128* __restrict__ U,
-                                 const c128* __restrict__ V, const c128* __restrict__ D, c128* A,
+                                 const int64_t* __restrict__ cols, const T* __restrict__ U,
+                                 const T* __restrict__ V, const T* __restrict__ D, c128* A,
                                  int64_t lda) {
     const HBlkDev bk = blks[blockIdx.x];
     const int64_t cnt = bk.nr * bk.nc;
@@ -792,11 +804,36 @@ __global__ void hmat_full_kernel(const HBlkDev* __restrict__ blks, const int64_t
         const int64_t i = e % bk.nr, j = e / bk.nr;
         c128 v{0.0, 0.0};
         if (bk.adm) {
-            for (int k = 0; k < bk.rank; ++k) v += U[bk.u_off + i + k * bk.nr] * V[bk.v_off + j + k * bk.nc];
+            for (int k = 0; k < bk.rank; ++k)
+                v += ldx(U + bk.u_off + i + k * bk.nr) * ldx(V + bk.v_off + j + k * bk.nc);
         } else {
-            v = D[bk.d_off + e];
+            v = ldx(D + bk.d_off + e);
         }
         A[rows[bk.r_off + i] + cols[bk.c_off + j] * lda] = v;
+    }
+}
+
+// FP32 storage: the first rank columns of every low-rank block's U and V
+// (contiguous, nr * rank and nc * rank entries) packed at the new offsets
+// held in w_off / f_off, and every dense block (same offsets), as c64
+__global__ void hmat_to_c64_kernel(const HBlkDev* __restrict__ blks, const c128* __restrict__ U,
+                                   const c128* __restrict__ V, const c128* __restrict__ D,
+                                   c64* U32, c64* V32, c64* D32) {
+    const HBlkDev bk = blks[blockIdx.x];
+    auto cvt = [](c128 v) { return c64{static_cast<float>(v.re), static_cast<float>(v.im)}; };
+    if (bk.adm) {
+        const int64_t cu = bk.nr * bk.rank, cv = bk.nc * bk.rank;
+        for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cu;
+             e += int64_t(blockDim.x) * gridDim.y)
+            U32[bk.w_off + e] = cvt(U[bk.u_off + e]);
+        for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cv;
+             e += int64_t(blockDim.x) * gridDim.y)
+            V32[bk.f_off + e] = cvt(V[bk.v_off + e]);
+    } else {
+        const int64_t cd = bk.nr * bk.nc;
+        for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cd;
+             e += int64_t(blockDim.x) * gridDim.y)
+            D32[bk.d_off + e] = cvt(D[bk.d_off + e]);
     }
 }
 
@@ -821,6 +858,12 @@ struct ccab_hmatrix {
     c128* d_U = nullptr;
     c128* d_V = nullptr;
     c128* d_D = nullptr;
+    // FP32 storage (the blocks built in FP64, then stored as c64; U, V packed
+    // to their ranks): the c128 arrays above are released
+    bool fp32 = false;
+    c64* d_U32 = nullptr;
+    c64* d_V32 = nullptr;
+    c64* d_D32 = nullptr;
     // matvec plan: t = V^T x per low-rank block, one slot per (cluster, row)
     c128* d_t = nullptr;
     c128* d_nacc = nullptr;
@@ -866,7 +909,8 @@ struct BuildTrace {
 }  // namespace
 
 int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
-                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out) {
+                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
+                  int prec) {
     hipStream_t s = ctx->stream;
     BuildTrace trace(s);
     std::unique_ptr<ccab_hmatrix> owner(new ccab_hmatrix());
@@ -930,19 +974,20 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     H->d_rows = dalloc<int64_t>(H->owned, rows.size());
     H->d_cols = H->d_rows;
     H->d_blk = dalloc<HBlkDev>(H->owned, nb);
-    H->d_U = dalloc<c128>(H->owned, u_off);
-    H->d_V = dalloc<c128>(H->owned, v_off);
-    c128* U2 = dalloc<c128>(H->owned, u_off);
-    c128* V2 = dalloc<c128>(H->owned, v_off);
-    c128* Wws = dalloc<c128>(H->owned, w_off);
-    uint8_t* Fws = dalloc<uint8_t>(H->owned, f_off);
-    c128* Rws = dalloc<c128>(H->owned, std::max<size_t>(ll.size(), 1) * 2 * RMAX * RMAX);
+    // the ACA's factors at full rank capacity and its vectors and flags live
+    // in context buffers reused by every build (tens of GB at n = 81920:
+    // allocating them per build made build times erratic); the H matrix owns
+    // only the recompressed factors, packed to the ACA ranks
+    c128* Ua = static_cast<c128*>(ctx_buf(ctx, SLOT_HM_UA, size_t(std::max<int64_t>(u_off, 1)) * 16));
+    c128* Va = static_cast<c128*>(ctx_buf(ctx, SLOT_HM_VA, size_t(std::max<int64_t>(v_off, 1)) * 16));
+    c128* Wws = static_cast<c128*>(ctx_buf(ctx, SLOT_HM_W, size_t(std::max<int64_t>(w_off, 1)) * 16));
+    uint8_t* Fws = static_cast<uint8_t*>(ctx_buf(ctx, SLOT_HM_F, size_t(std::max<int64_t>(f_off, 1))));
     uint32_t* d_rng = dalloc<uint32_t>(H->owned, kRngTable);
     // fallbacks move blocks from the low-rank to the dense list: size both for every block
     H->d_dense_list = dalloc<int32_t>(H->owned, nb + 1);
     H->d_lr_list = dalloc<int32_t>(H->owned, nb + 1);
-    if (!H->d_rows || !H->d_cols || !H->d_blk || !H->d_U || !H->d_V || !U2 || !V2 || !Wws ||
-        !Fws || !Rws || !d_rng || !H->d_dense_list || !H->d_lr_list)
+    if (!H->d_rows || !H->d_cols || !H->d_blk || !Ua || !Va || !Wws || !Fws || !d_rng ||
+        !H->d_dense_list || !H->d_lr_list)
         return fail(CCAB_ENOMEM, "H-matrix allocation failed");
     std::vector<uint32_t> words(kRngTable);
     chacha12_words(10, words.data(), kRngTable);
@@ -956,8 +1001,8 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     // batched ACA + recompression of the admissible blocks
     if (!ll.empty()) {
         hipLaunchKernelGGL(aca_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, p, d_soa,
-                           n, H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, fa, fb, H->d_U,
-                           H->d_V, Wws, Fws, d_rng, tol);
+                           n, H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, fa, fb, Ua, Va, Wws,
+                           Fws, d_rng, tol);
         // recompression with LDS sized by rank: blocks bucketed by their ACA
         // rank (most are far below RMAX, and the RMAX-sized kernel holds one
         // workgroup per CU)
@@ -976,27 +1021,55 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         }
         std::vector<int32_t> order;
         for (auto& v : bl) order.insert(order.end(), v.begin(), v.end());
+        // recompressed factors packed to the ACA ranks (the recompressed rank
+        // is at most the ACA rank); R factors sized per bucket (the bucket
+        // launches run one after another)
+        int64_t cu = 0, cv = 0;
+        size_t rws = 1;
+        for (int k = 0; k < kBuckets; ++k)
+            rws = std::max(rws, bl[k].size() * 2 * size_t(cap[k]) * size_t(cap[k]));
+        for (int32_t b : order) {
+            HBlkDev& d = H->blk[b];
+            d.u2_off = cu;
+            cu += d.nr * d.rank;
+            d.v2_off = cv;
+            cv += d.nc * d.rank;
+        }
+        H->d_U = dalloc<c128>(H->owned, cu);
+        H->d_V = dalloc<c128>(H->owned, cv);
+        c128* Rws = static_cast<c128*>(ctx_buf(ctx, SLOT_HM_R, rws * 16));
+        if (!H->d_U || !H->d_V || !Rws) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
         if (!order.empty()) {
             int32_t* d_order = dalloc<int32_t>(H->owned, order.size());
             if (!d_order) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
             CCAB_HIP(hipMemcpy(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+            c128* U2 = H->d_U;
+            c128* V2 = H->d_V;
             size_t off = 0;
             for (int k = 0; k < kBuckets; ++k) {
                 const unsigned cnt = (unsigned)bl[k].size();
                 if (cnt == 0) continue;
                 const int32_t* lst = d_order + off;
                 switch (k) {
-                case 0: hipLaunchKernelGGL(recompress_kernel<8>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
-                case 1: hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
-                case 2: hipLaunchKernelGGL(recompress_kernel<32>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
-                default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, H->d_U, H->d_V, U2, V2, Rws, tol); break;
+                case 0: hipLaunchKernelGGL(recompress_kernel<8>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
+                case 1: hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
+                case 2: hipLaunchKernelGGL(recompress_kernel<32>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
+                default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
                 }
                 off += cnt;
             }
         }
+        // the recompressed ranks come back; the factors now sit at the packed offsets
+        CCAB_HIP(hipMemcpyAsync(H->blk.data(), H->d_blk, nb * sizeof(HBlkDev), hipMemcpyDeviceToHost, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+        for (int32_t b : order) {
+            HBlkDev& d = H->blk[b];
+            d.u_off = d.u2_off;
+            d.v_off = d.v2_off;
+        }
+        CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
         trace.mark("recompression");
-        std::swap(H->d_U, U2);
-        std::swap(H->d_V, V2);
         hipLaunchKernelGGL(aca_norm_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s,
                            H->d_blk, H->d_lr_list, H->d_U, H->d_V);
         CCAB_HIP(hipGetLastError());
@@ -1104,9 +1177,42 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), ent.size() * 8, hipMemcpyHostToDevice));
     trace.mark("matvec plan");
+    std::vector<void*> release;
+    if (prec == CCAB_C64) {
+        // FP32 storage: U and V packed to their ranks at new offsets (staged
+        // in w_off / f_off for the conversion), dense blocks at their offsets
+        int64_t cu = 0, cv = 0;
+        for (int32_t b : ll2) {
+            HBlkDev& d = H->blk[b];
+            d.w_off = cu;
+            cu += d.nr * d.rank;
+            d.f_off = cv;
+            cv += d.nc * d.rank;
+        }
+        H->d_U32 = dalloc<c64>(H->owned, cu);
+        H->d_V32 = dalloc<c64>(H->owned, cv);
+        H->d_D32 = dalloc<c64>(H->owned, d_off);
+        if (!H->d_U32 || !H->d_V32 || !H->d_D32) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+        if (nb)
+            hipLaunchKernelGGL(hmat_to_c64_kernel, dim3((unsigned)nb, 4), dim3(256), 0, s, H->d_blk,
+                               H->d_U, H->d_V, H->d_D, H->d_U32, H->d_V32, H->d_D32);
+        CCAB_HIP(hipGetLastError());
+        for (int32_t b : ll2) {
+            HBlkDev& d = H->blk[b];
+            d.u_off = d.w_off;
+            d.v_off = d.f_off;
+        }
+        CCAB_HIP(hipMemcpyAsync(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+        release.insert(release.end(), {H->d_U, H->d_V, H->d_D});
+        H->d_U = H->d_V = H->d_D = nullptr;
+        H->fp32 = true;
+        H->mv_bytes *= 0.5;
+        trace.mark("FP32 storage");
+    }
     // release the build workspace
-    for (void* w : {static_cast<void*>(U2), static_cast<void*>(V2), static_cast<void*>(Wws),
-                    static_cast<void*>(Fws), static_cast<void*>(Rws)}) {
+    for (void* w : release) {
         auto it = std::find(H->owned.begin(), H->owned.end(), w);
         if (it != H->owned.end()) {
             (void)hipFree(*it);
@@ -1122,13 +1228,23 @@ int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x
                  c128* y) {
     hipStream_t s = ctx->stream;
     ProfScope ps(ctx, PROF_HMAT, H->mv_bytes);
-    if (H->n_lr)
-        hipLaunchKernelGGL(hmv_vt_kernel, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
-                           H->d_lr_list, H->d_cols, H->d_V, x, H->d_t);
-    if (H->n_tiles)
-        hipLaunchKernelGGL(hmv_node_kernel, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
-                           H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
-                           H->d_blk, H->d_cols, H->d_D, H->d_U, H->d_t, x, alpha, H->d_nacc);
+    if (H->fp32) {
+        if (H->n_lr)
+            hipLaunchKernelGGL(hmv_vt_kernel<c64>, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
+                               H->d_lr_list, H->d_cols, H->d_V32, x, H->d_t);
+        if (H->n_tiles)
+            hipLaunchKernelGGL(hmv_node_kernel<c64>, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
+                               H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
+                               H->d_blk, H->d_cols, H->d_D32, H->d_U32, H->d_t, x, alpha, H->d_nacc);
+    } else {
+        if (H->n_lr)
+            hipLaunchKernelGGL(hmv_vt_kernel<c128>, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
+                               H->d_lr_list, H->d_cols, H->d_V, x, H->d_t);
+        if (H->n_tiles)
+            hipLaunchKernelGGL(hmv_node_kernel<c128>, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
+                               H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
+                               H->d_blk, H->d_cols, H->d_D, H->d_U, H->d_t, x, alpha, H->d_nacc);
+    }
     hipLaunchKernelGGL(hmv_reduce_kernel, dim3((unsigned)((H->n + 255) / 256)), dim3(256), 0, s,
                        H->n, H->d_rowptr, H->d_ent, H->d_nacc, beta, y);
     CCAB_HIP(hipGetLastError());
@@ -1157,8 +1273,11 @@ void hmatrix_stats(const ccab_hmatrix* H, int64_t* st, double* norm) {
 
 int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda) {
     const int64_t nb = (int64_t)H->blk.size();
-    if (nb)
-        hipLaunchKernelGGL(hmat_full_kernel, dim3((unsigned)nb, 4), dim3(256), 0, ctx->stream,
+    if (nb && H->fp32)
+        hipLaunchKernelGGL(hmat_full_kernel<c64>, dim3((unsigned)nb, 4), dim3(256), 0, ctx->stream,
+                           H->d_blk, H->d_rows, H->d_cols, H->d_U32, H->d_V32, H->d_D32, A, lda);
+    else if (nb)
+        hipLaunchKernelGGL(hmat_full_kernel<c128>, dim3((unsigned)nb, 4), dim3(256), 0, ctx->stream,
                            H->d_blk, H->d_rows, H->d_cols, H->d_U, H->d_V, H->d_D, A, lda);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;

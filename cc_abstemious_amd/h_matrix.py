@@ -34,13 +34,15 @@ class HMatrix:
 
     @classmethod
     def new_from(cls, predata: PreData, side: int = capi.LHS, leaf_cardinality: int = 32,
-                 tolerance: float = 1e-4, eta: float = 4.0, ctx=None) -> "HMatrix":
-        """H matrix of the LHS (a H + b G) or RHS side matrix at the current frequency."""
+                 tolerance: float = 1e-4, eta: float = 4.0, ctx=None,
+                 storage: int = capi.C128) -> "HMatrix":
+        """H matrix of the LHS (a H + b G) or RHS side matrix at the current frequency.
+        storage C64 keeps the blocks (built in FP64) in complex FP32."""
         ctx = ctx or capi.default_context(predata.device.index or 0)
         h = ct.c_void_p()
-        ctx.call("ccab_hmatrix_build", ct.byref(predata.physics()), _ptr(predata.d_cpts),
+        ctx.call("ccab_hmatrix_build_ex", ct.byref(predata.physics()), _ptr(predata.d_cpts),
                  predata.get_num_eqn(), int(side), int(leaf_cardinality), float(eta),
-                 float(tolerance), ct.byref(h))
+                 float(tolerance), int(storage), ct.byref(h))
         return cls(h, predata.get_num_eqn(), predata.device, ctx)
 
     def get_num_eqn(self) -> int:

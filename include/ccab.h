@@ -320,6 +320,15 @@ int ccab_dense_field_matrices(ccab_ctx* ctx, const ccab_physics* ph, const doubl
 int ccab_hmatrix_build(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts, int64_t n,
                        int side, int64_t leaf_size, double eta, double aca_tolerance,
                        ccab_hmatrix** out);
+/* ccab_hmatrix_build with the block storage chosen (no reference counterpart):
+ * prec CCAB_C128 (= ccab_hmatrix_build) or CCAB_C64, blocks built in FP64
+ * (ACA, recompression, norms) and stored in complex FP32 (U and V packed to
+ * their ranks), which halves the bytes of every matvec; the products still
+ * accumulate in FP64.  The SOLVER_HIERARCHICAL sweep selects it with
+ * gmres.matrix_prec = CCAB_C64. */
+int ccab_hmatrix_build_ex(ccab_ctx* ctx, const ccab_physics* ph, const double* d_cpts, int64_t n,
+                          int side, int64_t leaf_size, double eta, double aca_tolerance, int prec,
+                          ccab_hmatrix** out);
 int ccab_hmatrix_destroy(ccab_hmatrix* h);
 /* src/solve/h_matrix.rs:218-234 (HMatrix::gemv): d_y = alpha H d_x + beta d_y,
  * alpha and beta complex {re, im} (beta = 0: d_y is not read, as in BLAS);

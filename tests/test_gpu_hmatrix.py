@@ -201,6 +201,49 @@ def test_hierarchical_sweep_matches_dense():
     assert all(s["converged"] for s in st)
 
 
+def test_hmatrix_fp32_storage():
+    """Blocks built in FP64 and stored in complex FP32 (ccab_hmatrix_build_ex,
+    storage C64): the represented matrix equals the FP64-stored one to FP32
+    rounding (entries within 2^-24 of the largest), the matvec agrees with it,
+    the stored storage statistics are unchanged, and the hierarchical sweep
+    with matrix_prec C64 reaches the FP64-storage solution."""
+    m = _lumpy(8)
+    pd = _predata(m, capi.BURTON_MILLER, f=300.0, c=343.0, rho=1.21)
+    n = pd.get_num_eqn()
+    _note(f"H matrix FP32 storage lumpy sphere n={n}")
+    H64 = h_matrix.HMatrix.new_from(pd, capi.LHS)
+    H32 = h_matrix.HMatrix.new_from(pd, capi.LHS, storage=capi.C64)
+    s64, s32 = H64.stats(), H32.stats()
+    assert s64 == s32
+    f64, f32 = H64.to_full(), H32.to_full()
+    e = float((f32 - f64).abs().max() / f64.abs().max())
+    print(f"to_full FP32 vs FP64 storage: {e:.3e}")
+    assert 0.0 < e < 2.0 ** -23
+    rng = np.random.default_rng(9)
+    xt = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n)).to(pd.device)
+    y = torch.zeros(n, dtype=torch.complex128, device=pd.device)
+    H32.gemv(0.7 - 0.2j, xt, 0.0, y)
+    ref = (0.7 - 0.2j) * (f32 @ xt)
+    e = float(torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref))
+    print(f"FP32-storage gemv vs its to_full: {e:.3e}")
+    assert e < 1e-13
+    # the sweep
+    freqs = np.array([1.5, 3.0]) * 343.0 / (2 * np.pi)
+    sp = PreData(mesh.icosphere(3), freqs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR,
+                 capi.BC_NORMAL_VELOCITY, (0.0, 0.0),
+                 IncidentWave(capi.WAVE_PLANE, (0.0, 0.6, 0.8), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_HIERARCHICAL, tolerance=1e-8, max_iterations=1000,
+                        aca_tolerance=1e-6))
+    p64, _, st64 = solve.sweep(sp)
+    sp.solver.matrix_prec = capi.C64
+    p32, _, st32 = solve.sweep(sp)
+    err = torch.linalg.vector_norm(p32 - p64, dim=1) / torch.linalg.vector_norm(p64, dim=1)
+    print("sweep FP32 vs FP64 storage", err.tolist(), [s["num_mv"] for s in st64],
+          [s["num_mv"] for s in st32])
+    assert float(err.max()) < 1e-6
+    assert all(s["converged"] for s in st32)
+
+
 def test_gmres_hmatrix_class_api():
     """GMRES(max_it, thresh) with .hmatrix set (gmres.rs:18, solve.rs:119-125)."""
     m = mesh.icosphere(2)
