@@ -492,7 +492,7 @@ __global__ void gir_solve_y_kernel(int m, const int32_t* __restrict__ act, int n
 // The same for restart lengths m <= kSolveYLdsM: one wave per system, H (and
 // Hbar) staged in LDS by all lanes at once, then the column-oriented back
 // substitution with the row updates spread over the lanes, and z row-parallel.
-constexpr int kSolveYLdsM = 40;
+constexpr int kSolveYLdsM = 50;
 
 __global__ __launch_bounds__(64) void gir_solve_y_wave_kernel(
     int m, const int32_t* __restrict__ act, const c128* __restrict__ Hg,
@@ -957,6 +957,9 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                                0, s, n, Bv, R, act);
             if (cyc > 0 || X0) {
                 set_elems(active);
+                // (a warm start's first residual with the FP32 remainder reads
+                // operator + remainder: leaving A_l x0 to a later cycle costs
+                // that cycle more steps than it saves, measured)
                 CCAB_TRY(gemv_active(ctx, pres, n, n, na, act, Ares, lda, sA, X, n, mone, one, R, n,
                                      Alo ? d_inv_scale : nullptr, rg, Alo));
                 for (int32_t b : active) nmv[b]++;

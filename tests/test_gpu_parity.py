@@ -424,3 +424,22 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
             print(f"frequency {k}: reported {st[k]['rel_residual']:.3e} true {true_rel:.3e}")
             # tolerance 1e-10 plus the remainder's 2^-35 representation error
             assert true_rel <= 3e-10
+        # warm start: the first residual uses the FP16 operator alone and the
+        # remainder's share of x0 joins the first carried update
+        g = torch.Generator().manual_seed(11)
+        x0 = phi_d * (1 + 0.01 * torch.randn(phi_d.shape, generator=g, dtype=torch.float64).to(phi_d.device))
+        phi_w, _, st_w = solve.sweep(pd, x0=x0)
+        for k in range(len(fs)):
+            e = float(torch.linalg.vector_norm(phi_w[k] - phi_d[k]) / torch.linalg.vector_norm(phi_d[k]))
+            assert e < 1e-8, (k, e, st_w[k])
+            assert st_w[k]["converged"] == 1 and st_w[k]["rel_residual"] <= 1e-10
+        assert sum(s["iterations"] for s in st_w) < sum(s["iterations"] for s in st)
+        pk = PreData(m, [fs[-1]], 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR,
+                     capi.BC_NORMAL_VELOCITY, (0, 0),
+                     IncidentWave(capi.WAVE_PLANE, (0.2, 0.3, 0.9), (1.0, 0.0)), Solver())
+        l128, _, rhs = influence_matrix.assemble_system(pk, incident_wave.get_incident_surface(pk))
+        r = l128[0] @ phi_w[-1] - rhs[0]
+        true_rel = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(rhs[0]))
+        print(f"warm start: iterations {sum(s['iterations'] for s in st_w)} vs "
+              f"{sum(s['iterations'] for s in st)}; true residual {true_rel:.3e}")
+        assert true_rel <= 3e-10
