@@ -19,7 +19,11 @@ other directions and frequencies; inference inside the timed sweep), the
 c64-operator sweep, the plain FP64 dense LU sweep (c128, FP64 MFMA; the
 accuracy reference, autotuned) and the c64-LU + FP64-IR sweep; and, as its
 own line ("bodies"), 256 perturbed-sphere bodies of 1280-5120 unknowns solved
-in one call (ragged batched GMRES-IR) against the same bodies body by body.
+in one call (ragged batched GMRES-IR) against the same bodies body by body;
+"large_dense", the impedance-BC ka = 12 case on 20480 triangles and 24576
+quads (3 right-hand sides, GMRES(50)-IR vs the c128 LU); and "hmatrix", the
+ka = 20 H-matrix solve on 81920 triangles (c128 and complex-FP32 block
+storage) with its error against the dense LU on 20480 triangles.
 
 Prints ONE JSON line.  value = sweep solves/s (frequencies solved per second).
 """
@@ -71,6 +75,10 @@ def parse():
     ap.add_argument("--calibrate", action="store_true",
                     help="after the runs, one read-only (sum) and one write-only (fill) kernel "
                          "over 1 GiB, to scale hardware traffic counters")
+    ap.add_argument("--large", type=int, default=1,
+                    help="1: also time the large dense iterative case (n = 20480, 24576)")
+    ap.add_argument("--hmatrix", type=int, default=1,
+                    help="1: also time the H-matrix case (n = 81920, and 20480 vs dense)")
     ap.add_argument("--bodies", type=int, default=256,
                     help="uneven bodies of the batched-bodies line (0 skips it)")
     ap.add_argument("--train-dirs", type=int, default=12)
@@ -217,6 +225,153 @@ def bodies_block(args, ctx):
             "warm_start": warm,
             "ragged_gemv_gbs": (gemv["work"] / (gemv["ms"] * 1e-3) / 1e9) if gemv["count"] else None,
             "kernel_ms": {k: round(v["ms"], 3) for k, v in prof.items()}}
+
+
+def large_dense_block(args, ctx):
+    """Large dense iterative case: icosphere of 20480 triangles and cube-sphere
+    of 24576 quads, Burton-Miller, impedance BC Z = rho c (1 + 0.2i), ka = 12,
+    two plane waves and one spherical wave at a seeded source inside the
+    body (3 right-hand sides).  GMRES(50) with FP64 refinement to relative
+    residual 1e-7 on the complex-FP16 operator (refinement on FP16 + FP32
+    remainder) and on the c64 operator (refinement on c128), each right-hand
+    side a call of its own (assembly included), against the c128 LU."""
+    import torch
+    from cc_abstemious_amd import capi, mesh, solve
+    from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+    rho, c = 1.21, 343.0
+    ka = 12.0
+    f = ka * c / (2 * math.pi)
+    Z = (rho * c, 0.2 * rho * c)
+    rng = np.random.default_rng(11)
+    waves = []
+    for _ in range(2):
+        d = rng.standard_normal(3)
+        waves.append(IncidentWave(capi.WAVE_PLANE, tuple(d / np.linalg.norm(d)), (1.0, 0.0)))
+    sp = rng.standard_normal(3)
+    waves.append(IncidentWave(capi.WAVE_SPHERICAL, tuple(0.5 * sp / np.linalg.norm(sp)), (1.0, 0.0)))
+    solvers = {
+        "gmres_ir_c16": Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=50,
+                               ir_tolerance=1e-3, ir_max_steps=40, ref_mgs=False,
+                               stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C32,
+                               residual_matrix=capi.C32),
+        "gmres_ir_c64": Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=50,
+                               ir_tolerance=1e-5, ir_max_steps=40, ref_mgs=False,
+                               stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C64),
+        "lu_c128": Solver(kind=capi.SOLVER_DIRECT),
+    }
+    out = {}
+    for label, m in (("icosphere_20480_tri", mesh.icosphere(5)),
+                     ("cube_sphere_24576_quad", mesh.cube_sphere(64))):
+        sols, rec = {}, {}
+        for key, solver in solvers.items():
+            phis, its, t = [], [], 0.0
+            for w in waves:
+                pd = PreData(m, [f], c, rho, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_IMPEDANCE,
+                             Z, w, solver)
+                if key == "gmres_ir_c16" and not phis:  # untimed warm-up of this shape
+                    solve.sweep(pd, ctx=ctx)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                phi, _, st = solve.sweep(pd, ctx=ctx)
+                torch.cuda.synchronize()
+                t += time.perf_counter() - t0
+                phis.append(phi[0])
+                its.append(int(st[0]["iterations"]))
+                n = pd.get_num_eqn()
+                del pd
+            sols[key] = phis
+            rec[key] = {"seconds_per_rhs": t / len(waves), "gmres_iters": its}
+        for key in ("gmres_ir_c16", "gmres_ir_c64"):
+            rec[key]["rel_err_vs_fp64_lu_max"] = max(
+                float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+                for a, b in zip(sols[key], sols["lu_c128"]))
+            rec[key]["speedup_vs_fp64_lu"] = (rec["lu_c128"]["seconds_per_rhs"] /
+                                              rec[key]["seconds_per_rhs"])
+        rec["n"] = n
+        out[label] = rec
+        del sols
+        torch.cuda.empty_cache()
+    out["workload"] = ("Burton-Miller, impedance Z = rho c (1 + 0.2i), ka = 12, 2 plane waves + "
+                       "1 spherical source inside the body; GMRES(50)-IR to relative residual "
+                       "1e-7, one call per right-hand side including assembly; reference c128 LU")
+    return out
+
+
+def hmatrix_block(args, ctx):
+    """H-matrix path: icosphere of 81920 triangles, Burton-Miller, rigid, plane
+    wave, ka = 20, leaf 64, eta = 1, ACA tolerance 1e-4, GMRES(50) to relative
+    residual 1e-6: build, compression, matvec and solve with c128 and with
+    complex-FP32 block storage; and on 20480 triangles the solution against
+    the dense c128 LU."""
+    import torch
+    from cc_abstemious_amd import capi, mesh, solve, h_matrix, incident_wave
+    from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+    rho, c = 1.21, 343.0
+    f = 20.0 * c / (2 * math.pi)
+    d = np.array([0.3, -0.4, 0.866])
+    d /= np.linalg.norm(d)
+    out = {}
+    for level in (6, 5):
+        m = mesh.icosphere(level)
+        rec = {}
+        for key, prec in (("c128", capi.C128), ("fp32_storage", capi.C64)):
+            pd = PreData(m, [f], c, rho, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                         (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)),
+                         Solver(kind=capi.SOLVER_HIERARCHICAL, tolerance=1e-6, max_iterations=3000,
+                                restart=50, leaf_size=64, eta=1.0, aca_tolerance=1e-4,
+                                stop_rule=capi.STOP_RELATIVE_RESIDUAL, ref_mgs=False,
+                                matrix_prec=prec))
+            n = pd.get_num_eqn()
+            rhs = incident_wave.get_incident_surface(pd, ctx=ctx)
+            solve.solve_for_surface(pd, rhs, ctx=ctx)  # warm-up (context buffers)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            H = h_matrix.HMatrix.new_from(pd, capi.LHS, 64, 1e-4, 1.0, ctx=ctx, storage=prec)
+            torch.cuda.synchronize()
+            tb = time.perf_counter() - t0
+            st = H.stats()
+            x = torch.randn(n, dtype=torch.complex128, device="cuda")
+            y = torch.zeros_like(x)
+            H.gemv(1.0, x, 0.0, y)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                H.gemv(1.0, x, 0.0, y)
+            torch.cuda.synchronize()
+            tm = (time.perf_counter() - t0) / 10
+            del H
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            phi, _ = solve.solve_for_surface(pd, rhs, ctx=ctx)
+            torch.cuda.synchronize()
+            ts = time.perf_counter() - t0
+            rec[key] = {"build_ms": tb * 1e3, "matvec_ms": tm * 1e3,
+                        "matvec_gbs": st["storage"] * (16 if prec == capi.C128 else 8) / tm / 1e9,
+                        "solve_ms_incl_build": ts * 1e3, "gmres_matvecs": int(pd.last_stats["num_mv"]),
+                        "rel_residual": pd.last_stats["rel_residual"]}
+            rec["compression_ratio_pct"] = st["compression_ratio"]
+            rec["blocks"] = {"admissible": st["admissible"], "inadmissible": st["inadmissible"],
+                             "rank_sum": st["rank_sum"], "dense_fallback": st["dense_fallback"]}
+            if level == 5:
+                rec[key]["phi"] = phi
+            del pd
+        rec["n"] = n
+        if level == 5:
+            pd = PreData(m, [f], c, rho, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                         (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)),
+                         Solver(kind=capi.SOLVER_DIRECT))
+            phi_d, _, _ = solve.sweep(pd, ctx=ctx)
+            for key in ("c128", "fp32_storage"):
+                ph = rec[key].pop("phi")
+                rec[key]["rel_err_vs_fp64_lu"] = float(torch.linalg.vector_norm(ph - phi_d[0]) /
+                                                       torch.linalg.vector_norm(phi_d[0]))
+            del pd, phi_d
+        out[f"icosphere_{20 * 4 ** level}_tri"] = rec
+        torch.cuda.empty_cache()
+    out["workload"] = ("Burton-Miller rigid sphere, plane wave, ka = 20, leaf 64, eta = 1, ACA "
+                       "tolerance 1e-4, GMRES(50) to relative residual 1e-6; error against the "
+                       "dense c128 LU at 20480 triangles")
+    return out
 
 
 def main():
@@ -454,6 +609,8 @@ def main():
         "roofline_lu": lu_roof,
     }
     result["bodies"] = bodies_block(args, ctx) if args.bodies > 0 else None
+    result["large_dense"] = large_dense_block(args, ctx) if args.large else None
+    result["hmatrix"] = hmatrix_block(args, ctx) if args.hmatrix else None
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pd.cpts, (freqs[-1], c, rho, 1, 0, 1, (0.0, 0.0)),
                                               args.cpu_lu_n)
