@@ -104,6 +104,58 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
     }
 }
 
+// With beta = i/k (Burton-Miller) the pair of influence_matrix.rs:12-37
+// times the weight w = wa / (4 pi r) is
+//   H = w e^{ikr} (a0, k a1 + a2 / k),  G = w e^{ikr} (g0, g1 / k)
+// with E = n_y.n_x - 3 rr, rr = (e.n_x)(e.n_y) dh_sign, a0 = (e.n_y + E) / r,
+// a1 = rr - e.n_y, a2 = E / r^2, g0 = 1 - e.n_x, g1 = -(e.n_x) / r
+// (classical: H = w e^{ikr} (e.n_y)(1/r, -k), G = w e^{ikr}); only the
+// phasor and k, 1/k change with the frequency.
+template <bool BM>
+__device__ __forceinline__ void pair_coeffs(const PairGeom<double>& q, double dh_sign, double& a0,
+                                            double& a1, double& a2, double& g0, double& g1) {
+    if constexpr (BM) {
+        const double rr = q.rdotx * q.rdoty * dh_sign;
+        const double E = q.nn - 3.0 * rr;
+        a0 = q.inv_r * (q.rdoty + E);
+        a1 = rr - q.rdoty;
+        a2 = q.inv_r * q.inv_r * E;
+        g0 = 1.0 - q.rdotx;
+        g1 = -q.inv_r * q.rdotx;
+    } else {
+        a0 = q.inv_r * q.rdoty;
+        a1 = -q.rdoty;
+        a2 = 0.0;
+        g0 = 1.0;
+        g1 = 0.0;
+    }
+}
+
+// one LHS entry L to the outputs: out0 c128; out1 OutT (c64, or c32 scaled by
+// h_scale); with OutT = c32 and out2, out2 the complex-FP32 remainder
+// L - fp16(L h_scale) / h_scale
+template <typename OutT>
+__device__ __forceinline__ void store_lhs(const SysParams& p, int64_t off, c128 L, void* out0,
+                                          void* out1, void* out2) {
+    if (out0) store<c128>(out0, off, L);
+    if constexpr (sizeof(OutT) == 4) {
+        if (out2) {
+            const uint16_t hr = to_half_sat(L.re * p.h_scale);
+            const uint16_t hi = to_half_sat(L.im * p.h_scale);
+            reinterpret_cast<c32*>(out1)[off] = c32{hr, hi};
+            const double inv = 1.0 / p.h_scale;  // a power of two: exact
+            const double br = double(__half2float(__ushort_as_half(hr))) * inv;
+            const double bi = double(__half2float(__ushort_as_half(hi))) * inv;
+            reinterpret_cast<c64*>(out2)[off] =
+                c64{static_cast<float>(L.re - br), static_cast<float>(L.im - bi)};
+        } else if (out1) {
+            store<OutT>(out1, off, L, p.h_scale);
+        }
+    } else {
+        if (out1) store<OutT>(out1, off, L, p.h_scale);
+    }
+}
+
 // ASM_LHS assembly of a frequency sweep over one mesh, FP64 arithmetic, kF
 // systems per thread: the pair geometry (r, 1/r, the normal projections) is
 // computed once for the kF systems (16: measured 6.6 ms at 8, 5.7 at 16, 5.7
@@ -147,29 +199,9 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
         const double rx = diag ? 1.0 : s.x[j] - yx;
         const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
                                                          s.ny[j], s.nz[j], nyx, nyy, nyz);
-        // With beta = i/k (Burton-Miller) the pair of influence_matrix.rs:
-        // 12-37 times the weight w = wa / (4 pi r) is
-        //   H = w e^{ikr} (a0, k a1 + a2 / k),  G = w e^{ikr} (1 - e.n_x, -(e.n_x / r) / k)
-        // with E = n_y.n_x - 3 rr, rr = (e.n_x)(e.n_y) dh_sign, a0 = (e.n_y + E) / r,
-        // a1 = rr - e.n_y, a2 = E / r^2 (classical: H = w e^{ikr} (e.n_y)(1/r, -k),
-        // G = w e^{ikr}); only the phasor and k, 1/k change with the frequency.
         const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
         double a0, a1, a2, g0, g1;
-        if constexpr (BM) {
-            const double rr = q.rdotx * q.rdoty * dh_sign;
-            const double E = q.nn - 3.0 * rr;
-            a0 = q.inv_r * (q.rdoty + E);
-            a1 = rr - q.rdoty;
-            a2 = q.inv_r * q.inv_r * E;
-            g0 = 1.0 - q.rdotx;
-            g1 = -q.inv_r * q.rdotx;
-        } else {
-            a0 = q.inv_r * q.rdoty;
-            a1 = -q.rdoty;
-            a2 = 0.0;
-            g0 = 1.0;
-            g1 = 0.0;
-        }
+        pair_coeffs<BM>(q, dh_sign, a0, a1, a2, g0, g1);
         double sn, cs, sd = 0.0, cd = 1.0;
         sincos(k0 * q.rd, &sn, &cs);
         if (dk != 0.0) sincos(dk * q.rd, &sd, &cd);
@@ -199,27 +231,9 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                 H = ph * c128{a0, p.k * a1 + a2 * ik};
                 G = ph * c128{g0, g1 * ik};
             }
-            if (active) {
-                const int64_t off = (b0 + f) * stride + i + j * ld;
-                const c128 L = p.lhs_a * H + p.lhs_b * G;
-                if (out0) store<c128>(out0, off, L);
-                if constexpr (sizeof(OutT) == 4) {
-                    if (out2) {
-                        const uint16_t hr = to_half_sat(L.re * p.h_scale);
-                        const uint16_t hi = to_half_sat(L.im * p.h_scale);
-                        reinterpret_cast<c32*>(out1)[off] = c32{hr, hi};
-                        const double inv = 1.0 / p.h_scale;  // a power of two: exact
-                        const double br = double(__half2float(__ushort_as_half(hr))) * inv;
-                        const double bi = double(__half2float(__ushort_as_half(hi))) * inv;
-                        reinterpret_cast<c64*>(out2)[off] =
-                            c64{static_cast<float>(L.re - br), static_cast<float>(L.im - bi)};
-                    } else if (out1) {
-                        store<OutT>(out1, off, L, p.h_scale);
-                    }
-                } else {
-                    if (out1) store<OutT>(out1, off, L, p.h_scale);
-                }
-            }
+            if (active)
+                store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, p.lhs_a * H + p.lhs_b * G, out0,
+                                out1, out2);
             if (rowsum_part && p.rowsum_of != 0) {
                 const c128 a = p.rowsum_of == 1 ? G : H;
                 double2& r = rs(f);
@@ -372,7 +386,182 @@ void launch_one(dim3 grid, hipStream_t st, const SysParams* d_params, const doub
                        soa, n, o0, o1, ld, stride, part, nchunks);
 }
 
+// --- ragged batches of bodies --------------------------------------------------
+__device__ __forceinline__ int body_of_wg(const BodyDesc* __restrict__ desc, int nbody, int64_t wg) {
+    int lo = 0, hi = nbody;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (desc[mid].wg0 <= wg) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// grid (rows / 256, bodies): SoA copy of body c's points
+__global__ void bodies_soa_kernel(const BodyDesc* __restrict__ desc, const double* __restrict__ aos,
+                                  double* soa) {
+    const BodyDesc d = desc[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const double* src = aos + 8 * (d.pts + i);
+    double* dst = soa + 8 * d.pts;
+#pragma unroll
+    for (int f = 0; f < 8; ++f) dst[f * d.n + i] = src[f];
+}
+
+// grid (rows / 256, bodies): incident right-hand side and optional guess of body c
+__global__ void bodies_incident_kernel(const BodyDesc* __restrict__ desc,
+                                       const SysParams* __restrict__ params,
+                                       const BodyWave* __restrict__ waves,
+                                       const double* __restrict__ soa, int64_t nmax, ccab_c128* X,
+                                       const ccab_c128* __restrict__ x0, ccab_c128* X0) {
+    const int64_t c = blockIdx.y;
+    const BodyDesc d = desc[c];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const BodyWave w = waves[c];
+    const c128 v = incident_value(params[c], w.kind, w.d[0], w.d[1], w.d[2], w.amp,
+                                  soa_of(soa + 8 * d.pts, d.n), i);
+    X[c * nmax + i] = ccab_c128{v.re, v.im};
+    if (X0) X0[c * nmax + i] = x0[d.pts + i];
+}
+
+// Fused LHS assembly of every body (one system each, FP64 arithmetic): the
+// 1-D grid is every body's (row block of 256, column chunk of 64) tiles back
+// to back; the pair as in the sweep kernel with the phasor from sincos.
+template <bool BM, typename OutT>
+__global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
+    const SysParams* __restrict__ params, const BodyDesc* __restrict__ desc, int nbody,
+    const double* __restrict__ soa_all, void* out0, void* out1, void* out2,
+    double* __restrict__ rowsum_part) {
+    const int c = body_of_wg(desc, nbody, blockIdx.x);
+    const BodyDesc d = desc[c];
+    const SysParams& p = params[c];
+    const int64_t n = d.n;
+    const int64_t rowblocks = (n + kRows - 1) / kRows;
+    const int64_t local = int64_t(blockIdx.x) - d.wg0;
+    const int64_t rb = local % rowblocks, cb = local / rowblocks;
+    const Soa s = soa_of(soa_all + 8 * d.pts, n);
+    const int64_t i = rb * kRows + threadIdx.x;
+    const int64_t j0 = cb * kColChunk;
+    const int64_t j1 = min(j0 + kColChunk, n);
+    const bool active = i < n;
+    const int64_t ic = active ? i : n - 1;
+    const double yx = s.x[ic], yy = s.y[ic], yz = s.z[ic];
+    const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
+    const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
+    double rs_re = 0.0, rs_im = 0.0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double wa = s.wt[j] * s.area[j];
+        const bool diag = j == ic;
+        const double rx = diag ? 1.0 : s.x[j] - yx;
+        const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
+                                                         s.ny[j], s.nz[j], nyx, nyy, nyz);
+        c128 G, H;
+        if (diag) {
+            diag_entry(p, s.area[j], s.wt[j], G, H);
+        } else {
+            double a0, a1, a2, g0, g1;
+            pair_coeffs<BM>(q, p.dh_sign, a0, a1, a2, g0, g1);
+            const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
+            double sn, cs;
+            sincos(p.k * q.rd, &sn, &cs);
+            const c128 ph{cs * w, sn * w};
+            H = ph * c128{a0, p.k * a1 + a2 * ik};
+            G = ph * c128{g0, g1 * ik};
+        }
+        if (active) store_lhs<OutT>(p, d.aoff + i + j * n, p.lhs_a * H + p.lhs_b * G, out0, out1, out2);
+        if (p.rowsum_of != 0) {
+            const c128 a = p.rowsum_of == 1 ? G : H;
+            rs_re += a.re;
+            rs_im += a.im;
+        }
+    }
+    if (rowsum_part != nullptr && active && p.rowsum_of != 0) {
+        double* dst = rowsum_part + d.part_off + 2 * (cb * n + i);
+        dst[0] = rs_re;
+        dst[1] = rs_im;
+    }
+}
+
+// grid (rows / 256, bodies): X = rhs_inc_sign X + rhs_scale sum of the row
+// partials (fixed chunk order), as rhs_finish_kernel per body
+__global__ void rhs_finish_bodies_kernel(const BodyDesc* __restrict__ desc,
+                                         const SysParams* __restrict__ params,
+                                         const double* __restrict__ part, int64_t nmax,
+                                         ccab_c128* X) {
+    const int64_t c = blockIdx.y;
+    const BodyDesc d = desc[c];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= d.n) return;
+    const SysParams& p = params[c];
+    c128 acc{0.0, 0.0};
+    if (p.rowsum_of != 0 && part != nullptr) {
+        const int64_t nch = (d.n + kColChunk - 1) / kColChunk;
+        for (int64_t k = 0; k < nch; ++k) {
+            const double* src = part + d.part_off + 2 * (k * d.n + i);
+            acc.re += src[0];
+            acc.im += src[1];
+        }
+        acc = p.rhs_scale * acc;
+    }
+    const ccab_c128 ri = X[c * nmax + i];
+    X[c * nmax + i] = ccab_c128{acc.re + p.rhs_inc_sign * ri.re, acc.im + p.rhs_inc_sign * ri.im};
+}
+
 }  // namespace
+
+int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
+                        const double* d_cpts, double* soa, const SysParams* d_params,
+                        const BodyWave* d_waves, ccab_c128* X, const ccab_c128* d_x0,
+                        ccab_c128* X0) {
+    CCAB_CHECK(nbody >= 1 && nbody <= 65535, "bodies per launch out of range");
+    const dim3 g((unsigned)((nmax + 255) / 256), (unsigned)nbody);
+    hipLaunchKernelGGL(bodies_soa_kernel, g, dim3(256), 0, ctx->stream, d_desc, d_cpts, soa);
+    hipLaunchKernelGGL(bodies_incident_kernel, g, dim3(256), 0, ctx->stream, d_desc, d_params,
+                       d_waves, soa, nmax, X, d_x0, X0);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_assemble_bodies(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
+                           const BodyDesc* h_desc, const SysParams* d_params,
+                           const SysParams* h_params, const double* soa, int prec, void* out0,
+                           void* out1, void* out2, double* part, ccab_c128* X) {
+    CCAB_CHECK(nbody >= 1 && nbody <= 65535, "bodies per launch out of range");
+    CCAB_CHECK(prec == PREC_F64 || prec == PREC_F64_H, "bodies assemble in FP64");
+    CCAB_CHECK(out2 == nullptr || (prec == PREC_F64_H && out1), "the FP32 remainder goes with the FP16 copy");
+    const bool bm = h_params[0].bm != 0;
+    double entries = 0.0;
+    for (int64_t c = 0; c < nbody; ++c) {
+        CCAB_CHECK((h_params[c].bm != 0) == bm, "all bodies of a batch share one method");
+        CCAB_CHECK(!bm || (h_params[c].beta.re == 0.0 &&
+                           std::fabs(h_params[c].beta.im * h_params[c].k - 1.0) < 1e-14),
+                   "Burton-Miller coupling beta = i/k expected");
+        entries += double(h_desc[c].n) * double(h_desc[c].n);
+    }
+    const int64_t wgs = h_desc[nbody].wg0;
+    CCAB_CHECK(wgs >= 1 && wgs < (int64_t(1) << 31), "assembly grid out of range");
+    const double bytes = (out0 ? 16.0 : 0.0) + (out1 ? (prec == PREC_F64_H ? 4.0 : 8.0) : 0.0) +
+                         (out2 ? 8.0 : 0.0);
+    hipStream_t st = ctx->stream;
+    {
+        ProfScope ps(ctx, PROF_ASSEMBLY, entries * bytes);
+        const dim3 grid((unsigned)wgs);
+        const int nb = (int)nbody;
+        if (prec == PREC_F64_H) {
+            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+        } else {
+            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+        }
+    }
+    hipLaunchKernelGGL(rhs_finish_bodies_kernel, dim3((unsigned)((nmax + 255) / 256), (unsigned)nbody),
+                       dim3(256), 0, st, d_desc, d_params, part, nmax, X);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
 
 int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d_soa) {
     hipLaunchKernelGGL(cpts_to_soa_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,

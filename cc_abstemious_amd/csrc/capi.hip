@@ -926,12 +926,7 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         }
         bool need_rowsum = false;
         for (int64_t c = 0; c < nc; ++c) need_rowsum |= hp[b0 + c].rowsum_of != 0;
-        double* part = nullptr;
-        if (need_rowsum) {
-            part = static_cast<double*>(
-                ctx_buf(ctx, SLOT_ROWSUM, size_t((nmax + 63) / 64) * nmax * 16));
-            if (!part) return CCAB_ENOMEM;
-        }
+        double* part = nullptr;  // row-sum partials of every body (below)
         double* inv_scale = nullptr;
         if (half_op) {
             // FP16 scale per body, as in the sweep: the largest diagonal entry
@@ -973,23 +968,45 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
             hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((nc * nmax + 255) / 256)), dim3(256), 0,
                                ctx->stream, nc * nmax, c128{0.0, 0.0},
                                reinterpret_cast<ccab_c128*>(X0), 0);
+        // the chunk's bodies in one launch per stage: descriptors (points,
+        // matrix, assembly tiles, row-sum partials) and incident waves
+        std::vector<BodyDesc> desc(nc + 1);
+        std::vector<BodyWave> bw(nc);
+        int64_t wg = 0, poff = 0;
         for (int64_t c = 0; c < nc; ++c) {
             const int64_t b = b0 + c, n = hn[c];
-            double* soa_b = soa + 8 * (coff[b] - coff[b0]);
-            ccab_c128* xb = reinterpret_cast<ccab_c128*>(X + c * nmax);
-            CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts + 8 * coff[b], n, soa_b));
-            CCAB_TRY(launch_incident(ctx, &hp[b], ph[b], w[b], soa_b, n, xb));
-            int64_t nch = 0;
-            CCAB_TRY(launch_assemble(ctx, dp + b, hp.data() + b, 1, soa_b, n,
-                                     half_op ? PREC_F64_H : PREC_F64, ASM_LHS,
-                                     A128 ? A128 + haoff[c] : nullptr,
-                                     Aop + size_t(haoff[c]) * esz_op, n, n * n, part, &nch,
-                                     Alo ? Alo + haoff[c] : nullptr));
-            CCAB_TRY(launch_rhs_finish(ctx, dp + b, 1, n, part, nch, xb, xb));
-            if (X0)
-                CCAB_HIP(hipMemcpyAsync(X0 + c * nmax, d_x0 + coff[b], size_t(n) * 16,
-                                        hipMemcpyDeviceToDevice, ctx->stream));
+            desc[c] = BodyDesc{n, coff[b] - coff[b0], haoff[c], wg, poff, 0};
+            wg += ((n + 255) / 256) * ((n + 63) / 64);
+            poff += 2 * ((n + 63) / 64) * n;
+            double d[3] = {w[b].vec[0], w[b].vec[1], w[b].vec[2]};
+            const double nn = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (!(nn > 0.0)) return ctx_fail(ctx, CCAB_ENAN, "incident wave vector has zero length");
+            // normalize(); the spherical origin stays put with CCAB_FIX_SPHERICAL_ORIGIN
+            if (w[b].kind == CCAB_WAVE_PLANE || !(ph[b].corrections & CCAB_FIX_SPHERICAL_ORIGIN))
+                for (double& v : d) v /= nn;
+            bw[c] = BodyWave{w[b].kind, 0, {d[0], d[1], d[2]},
+                             cdiv(c128{w[b].amplitude[0], w[b].amplitude[1]},
+                                  c128{0.0, hp[b].omega * ph[b].mass_density})};
         }
+        desc[nc] = BodyDesc{0, npts, entries, wg, poff, 0};
+        const size_t desc_bytes = ws_align(size_t(nc + 1) * sizeof(BodyDesc));
+        char* dd = static_cast<char*>(ctx_buf(ctx, SLOT_BODY_DESC, desc_bytes + size_t(nc) * sizeof(BodyWave)));
+        if (!dd) return CCAB_ENOMEM;
+        BodyDesc* d_desc = reinterpret_cast<BodyDesc*>(dd);
+        BodyWave* d_waves = reinterpret_cast<BodyWave*>(dd + desc_bytes);
+        CCAB_HIP(hipMemcpy(d_desc, desc.data(), size_t(nc + 1) * sizeof(BodyDesc), hipMemcpyHostToDevice));
+        CCAB_HIP(hipMemcpy(d_waves, bw.data(), size_t(nc) * sizeof(BodyWave), hipMemcpyHostToDevice));
+        if (need_rowsum) {
+            part = static_cast<double*>(ctx_buf(ctx, SLOT_ROWSUM, size_t(std::max<int64_t>(poff, 1)) * 8));
+            if (!part) return CCAB_ENOMEM;
+        }
+        CCAB_TRY(launch_bodies_setup(ctx, nc, nmax, d_desc, d_cpts + 8 * coff[b0], soa, dp + b0,
+                                     d_waves, reinterpret_cast<ccab_c128*>(X),
+                                     d_x0 ? d_x0 + coff[b0] : nullptr,
+                                     reinterpret_cast<ccab_c128*>(X0)));
+        CCAB_TRY(launch_assemble_bodies(ctx, nc, nmax, d_desc, desc.data(), dp + b0, hp.data() + b0,
+                                        soa, half_op ? PREC_F64_H : PREC_F64, A128, Aop, Alo, part,
+                                        reinterpret_cast<ccab_c128*>(X)));
         Ragged rg;
         rg.n = dr;
         rg.aoff = dr + nc;

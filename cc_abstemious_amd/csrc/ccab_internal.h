@@ -108,7 +108,7 @@ enum BufSlot {
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
     SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
-    SLOT_HM_F, SLOT_HM_R, SLOT_COUNT
+    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -292,6 +292,31 @@ int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, i
                       const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
                       ccab_c128* d_rhs);
 int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d_soa);
+// One body of a ragged batch (bodies of uneven size in one call): its
+// unknowns, first point (collocation records and SoA, relative to the
+// batch's first body), first matrix entry (n x n column-major, ld n), first
+// assembly workgroup and first row-sum partial; one more entry ends the list
+// (wg0 = total workgroups).  Vectors of body c sit at c * nmax.
+struct BodyDesc {
+    int64_t n, pts, aoff, wg0, part_off, pad_;
+};
+struct BodyWave {
+    int32_t kind, pad_;
+    double d[3];
+    c128 amp;
+};
+// SoA points, incident right-hand sides (X, zero-filled, stride nmax) and
+// optional guesses X0 of every body: three launches for the batch
+int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
+                        const double* d_cpts, double* soa, const SysParams* d_params,
+                        const BodyWave* d_waves, ccab_c128* X, const ccab_c128* d_x0,
+                        ccab_c128* X0);
+// fused LHS assembly and BC algebra of every body in one launch, then their
+// right-hand sides (row sums when the BC needs them): prec PREC_F64 or PREC_F64_H
+int launch_assemble_bodies(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
+                           const BodyDesc* h_desc, const SysParams* d_params,
+                           const SysParams* h_params, const double* soa, int prec, void* out0,
+                           void* out1, void* out2, double* part, ccab_c128* X);
 // every frequency of a sweep in one launch (d_params: the device copy of h_params)
 int launch_incident_sweep(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                           const ccab_physics* ph, int64_t nfreq, const ccab_wave& w,

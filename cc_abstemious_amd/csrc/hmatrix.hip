@@ -1021,6 +1021,14 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         }
         std::vector<int32_t> order;
         for (auto& v : bl) order.insert(order.end(), v.begin(), v.end());
+        if (trace.on) {
+            for (int k = 0; k < kBuckets; ++k) {
+                double rows = 0.0;
+                for (int32_t b : bl[k]) rows += double(H->blk[b].nr + H->blk[b].nc);
+                std::fprintf(stderr, "[hbuild] rank <= %2d: %6zu blocks, mean nr + nc %.0f\n", cap[k],
+                             bl[k].size(), bl[k].empty() ? 0.0 : rows / double(bl[k].size()));
+            }
+        }
         // recompressed factors packed to the ACA ranks (the recompressed rank
         // is at most the ACA rank); R factors sized per bucket (the bucket
         // launches run one after another)
