@@ -71,7 +71,9 @@ __global__ __launch_bounds__(256) void gemv_part_kernel(int64_t M, int64_t N, co
     const int64_t nb = rn ? rn[b] : N;
     const int64_t j0 = int64_t(blockIdx.y) * chunk;
     const int64_t j1 = (rn && i >= nb) ? j0 : min(j0 + chunk, nb);
-    if (i >= M) return;
+    // ragged: rows past the body and chunks past its columns write nothing
+    // (the finish reads only the body's own rows and chunks)
+    if (i >= M || (rn && (i >= nb || j0 >= nb))) return;
     if (rn) lda = nb;
     const TA* a = (rn ? A + raoff[b] : A + b * sA) + i;
     const c128* xb = x + b * sx;
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void gemv_part_vec_kernel(int64_t M, int64_t N
     const int64_t mb = rn ? nb : M;
     const int64_t j0 = int64_t(blockIdx.y) * chunk;
     const int64_t j1 = i0 >= mb ? j0 : min(j0 + chunk, nb);
-    if (i0 >= M) return;
+    if (i0 >= M || (rn && (i0 >= mb || j0 >= nb))) return;  // (as above)
     if (rn) lda = nb;
     const TA* a = (rn ? A + raoff[b] : A + b * sA) + i0;
     const c128* xb = x + b * sx;
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(256) void gemv_split_kernel(int64_t M, int64_t N,
     const int64_t mb = rn ? nb : M;
     const int64_t j0 = int64_t(blockIdx.y) * chunk;
     const int64_t j1 = i0 >= mb ? j0 : min(j0 + chunk, nb);
-    if (i0 >= M) return;
+    if (i0 >= M || (rn && (i0 >= mb || j0 >= nb))) return;  // (as above)
     if (rn) lda = nb;
     const int64_t base = (rn ? raoff[b] : b * sA) + i0;
     const c32* ah = Ah + base;
@@ -302,14 +304,18 @@ __global__ void gemv_finish_kernel(int64_t M, const c128* __restrict__ part, int
                                    c128 alpha, c128 beta, c128* y, int64_t sy,
                                    const int32_t* __restrict__ active,
                                    const double* __restrict__ inv_scale,
-                                   const int32_t* __restrict__ skip) {
+                                   const int32_t* __restrict__ skip,
+                                   const int64_t* __restrict__ rn, int64_t chunk) {
     const int64_t z = blockIdx.y;
     const int64_t b = active ? active[z] : z;
     if (skip && skip[b]) return;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= M) return;
+    // ragged: the body's chunks that hold columns; rows past the body are 0
+    int64_t np = nsplit;
+    if (rn) np = i < rn[b] ? min(nsplit, (rn[b] + chunk - 1) / chunk) : 0;
     c128 s{0.0, 0.0};
-    for (int64_t p = 0; p < nsplit; ++p) s += part[(z * nsplit + p) * M + i];
+    for (int64_t p = 0; p < np; ++p) s += part[(z * nsplit + p) * M + i];
     if (inv_scale) s = s * inv_scale[b];
     c128* yy = y + b * sy + i;
     const bool zero_beta = beta.re == 0.0 && beta.im == 0.0;
@@ -403,7 +409,10 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     const int64_t target = ctx->gemv_wg_target * (rg ? 4 : 1);
     int64_t nsplit = (target + rowblocks * nact - 1) / (rowblocks * nact);
     nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
-    const int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
+    int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
+    // ragged: a fixed column chunk, so that a large body's workgroups stream
+    // no more than a small body's and the launch has no long tail
+    if (rg && ctx->gemv_ragged_chunk > 0 && N > 0) chunk = std::min<int64_t>(N, ctx->gemv_ragged_chunk);
     nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;
     c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
@@ -460,7 +469,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     }
     hipLaunchKernelGGL(gemv_finish_kernel, dim3((unsigned)finish_blocks, (unsigned)nact), dim3(256),
                        0, ctx->stream, M, part, nsplit, alpha, beta, d_y, stridey, d_active,
-                       (prec == CCAB_C32 && !d_Alo) ? d_inv_scale : nullptr, d_skip);
+                       (prec == CCAB_C32 && !d_Alo) ? d_inv_scale : nullptr, d_skip, rn, chunk);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

@@ -372,6 +372,7 @@ int ccab_create(int device, ccab_ctx** out) {
         ctx->mem_budget = size_t(std::max(0L, atol(mb))) << 20;
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
     if (const char* nt = getenv("CCAB_GEMV_NT")) ctx->gemv_nt = atoi(nt) != 0;
+    if (const char* rc = getenv("CCAB_GEMV_RAGGED_CHUNK")) ctx->gemv_ragged_chunk = std::max(0, atoi(rc));
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
     *out = ctx;

@@ -166,6 +166,7 @@ struct ccab_ctx {
     // product; x, the partial sums and the Krylov basis keep the caches);
     // environment CCAB_GEMV_NT=0 turns it off
     bool gemv_nt = true;
+    int gemv_ragged_chunk = 512;  // columns per workgroup of ragged products (0: from the workgroup target)
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of
     // the free memory, at most 96 GiB); environment CCAB_MEM_BUDGET_MB
     size_t mem_budget = 0;
