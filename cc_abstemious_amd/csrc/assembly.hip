@@ -509,7 +509,46 @@ __global__ void rhs_finish_bodies_kernel(const BodyDesc* __restrict__ desc,
     X[c * nmax + i] = ccab_c128{acc.re + p.rhs_inc_sign * ri.re, acc.im + p.rhs_inc_sign * ri.im};
 }
 
+// Largest |lhs_a H_ii + lhs_b G_ii| over the diagonal of each system, one
+// workgroup per system (the FP16 operator's scale follows from it).  desc
+// null: every system has the points 0..n-1 of cpts (a sweep); otherwise
+// system s has desc[s].n points from point desc[s].pts.  cpts: 8 doubles
+// per point, area and weight at 6 and 7.
+__global__ __launch_bounds__(256) void diag_max_kernel(const SysParams* __restrict__ params,
+                                                       const double* __restrict__ cpts,
+                                                       const BodyDesc* __restrict__ desc, int64_t n,
+                                                       double* __restrict__ out) {
+    __shared__ double red[256];
+    const int64_t s = blockIdx.x;
+    const SysParams& p = params[s];
+    const int64_t ns = desc ? desc[s].n : n;
+    const double* c = cpts + 8 * (desc ? desc[s].pts : 0);
+    double mx = 0.0;
+    for (int64_t i = threadIdx.x; i < ns; i += 256) {
+        c128 G, H;
+        diag_entry(p, c[8 * i + 6], c[8 * i + 7], G, H);
+        const c128 L = p.lhs_a * H + p.lhs_b * G;
+        mx = fmax(mx, sqrt(norm_sqr(L)));
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s] = red[0];
+}
+
 }  // namespace
+
+int launch_diag_max(ccab_ctx* ctx, int64_t nsys, const SysParams* d_params, const double* d_cpts,
+                    const BodyDesc* d_desc, int64_t n, double* d_out) {
+    if (nsys <= 0) return CCAB_OK;
+    hipLaunchKernelGGL(diag_max_kernel, dim3((unsigned)nsys), dim3(256), 0, ctx->stream, d_params,
+                       d_cpts, d_desc, n, d_out);
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
 
 int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
                         const double* d_cpts, double* soa, const SysParams* d_params,

@@ -752,25 +752,21 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         // FP16 scale per system: the largest diagonal entry (which bounds the
         // near-singular entries of the row) maps to about 2^12, 16x below the
         // largest finite half
-        std::vector<double> h(size_t(n) * 8);
-        CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts, size_t(n) * 64, hipMemcpyDeviceToHost, ctx->stream));
-        CCAB_HIP(hipStreamSynchronize(ctx->stream));
+        // (the diagonal maxima on the device, one number per system back)
+        inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nfreq) * 8));
+        if (!inv_scale) return CCAB_ENOMEM;
+        CCAB_TRY(launch_diag_max(ctx, nfreq, dp, d_cpts, nullptr, n, inv_scale));
         std::vector<double> isc(nfreq);
+        CCAB_HIP(hipMemcpyAsync(isc.data(), inv_scale, size_t(nfreq) * 8, hipMemcpyDeviceToHost,
+                                ctx->stream));
+        CCAB_HIP(hipStreamSynchronize(ctx->stream));
         for (int64_t f = 0; f < nfreq; ++f) {
-            double mx = 0.0;
-            for (int64_t i = 0; i < n; ++i) {
-                c128 G, H;
-                diag_entry(hp[f], h[8 * i + 6], h[8 * i + 7], G, H);
-                const c128 L = hp[f].lhs_a * H + hp[f].lhs_b * G;
-                mx = std::max(mx, std::sqrt(norm_sqr(L)));
-            }
+            const double mx = isc[f];
             CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
             hp[f].h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
             isc[f] = 1.0 / hp[f].h_scale;
         }
         CCAB_HIP(hipMemcpy(dp, hp.data(), sizeof(SysParams) * nfreq, hipMemcpyHostToDevice));
-        inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nfreq) * 8));
-        if (!inv_scale) return CCAB_ENOMEM;
         CCAB_HIP(hipMemcpy(inv_scale, isc.data(), size_t(nfreq) * 8, hipMemcpyHostToDevice));
     }
     int32_t* piv = nullptr;
@@ -929,33 +925,6 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         for (int64_t c = 0; c < nc; ++c) need_rowsum |= hp[b0 + c].rowsum_of != 0;
         double* part = nullptr;  // row-sum partials of every body (below)
         double* inv_scale = nullptr;
-        if (half_op) {
-            // FP16 scale per body, as in the sweep: the largest diagonal entry
-            // maps to about 2^12
-            std::vector<double> h(size_t(npts) * 8);
-            CCAB_HIP(hipMemcpyAsync(h.data(), d_cpts + 8 * coff[b0], size_t(npts) * 64,
-                                    hipMemcpyDeviceToHost, ctx->stream));
-            CCAB_HIP(hipStreamSynchronize(ctx->stream));
-            std::vector<double> isc(nc);
-            for (int64_t c = 0; c < nc; ++c) {
-                SysParams& p = hp[b0 + c];
-                const double* hb = h.data() + 8 * (coff[b0 + c] - coff[b0]);
-                double mx = 0.0;
-                for (int64_t i = 0; i < hn[c]; ++i) {
-                    c128 G, H;
-                    diag_entry(p, hb[8 * i + 6], hb[8 * i + 7], G, H);
-                    const c128 L = p.lhs_a * H + p.lhs_b * G;
-                    mx = std::max(mx, std::sqrt(norm_sqr(L)));
-                }
-                CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
-                p.h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
-                isc[c] = 1.0 / p.h_scale;
-            }
-            CCAB_HIP(hipMemcpy(dp + b0, hp.data() + b0, sizeof(SysParams) * nc, hipMemcpyHostToDevice));
-            inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nc) * 8));
-            if (!inv_scale) return CCAB_ENOMEM;
-            CCAB_HIP(hipMemcpy(inv_scale, isc.data(), size_t(nc) * 8, hipMemcpyHostToDevice));
-        }
         {
             std::vector<int64_t> up(2 * nc);
             std::copy(hn.begin(), hn.end(), up.begin());
@@ -997,6 +966,26 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         BodyWave* d_waves = reinterpret_cast<BodyWave*>(dd + desc_bytes);
         CCAB_HIP(hipMemcpy(d_desc, desc.data(), size_t(nc + 1) * sizeof(BodyDesc), hipMemcpyHostToDevice));
         CCAB_HIP(hipMemcpy(d_waves, bw.data(), size_t(nc) * sizeof(BodyWave), hipMemcpyHostToDevice));
+        if (half_op) {
+            // FP16 scale per body, as in the sweep: the largest diagonal entry
+            // maps to about 2^12 (the maxima on the device, one number per body back)
+            inv_scale = static_cast<double*>(ctx_buf(ctx, SLOT_SOLVE_ISCALE, size_t(nc) * 8));
+            if (!inv_scale) return CCAB_ENOMEM;
+            CCAB_TRY(launch_diag_max(ctx, nc, dp + b0, d_cpts + 8 * coff[b0], d_desc, 0, inv_scale));
+            std::vector<double> isc(nc);
+            CCAB_HIP(hipMemcpyAsync(isc.data(), inv_scale, size_t(nc) * 8, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            CCAB_HIP(hipStreamSynchronize(ctx->stream));
+            for (int64_t c = 0; c < nc; ++c) {
+                SysParams& p = hp[b0 + c];
+                const double mx = isc[c];
+                CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
+                p.h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
+                isc[c] = 1.0 / p.h_scale;
+            }
+            CCAB_HIP(hipMemcpy(dp + b0, hp.data() + b0, sizeof(SysParams) * nc, hipMemcpyHostToDevice));
+            CCAB_HIP(hipMemcpy(inv_scale, isc.data(), size_t(nc) * 8, hipMemcpyHostToDevice));
+        }
         if (need_rowsum) {
             part = static_cast<double*>(ctx_buf(ctx, SLOT_ROWSUM, size_t(std::max<int64_t>(poff, 1)) * 8));
             if (!part) return CCAB_ENOMEM;

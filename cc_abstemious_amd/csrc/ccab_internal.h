@@ -308,6 +308,9 @@ struct BodyWave {
 };
 // SoA points, incident right-hand sides (X, zero-filled, stride nmax) and
 // optional guesses X0 of every body: three launches for the batch
+// per system, the largest modulus of the LHS diagonal (d_out: nsys doubles)
+int launch_diag_max(ccab_ctx* ctx, int64_t nsys, const SysParams* d_params, const double* d_cpts,
+                    const BodyDesc* d_desc, int64_t n, double* d_out);
 int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
                         const double* d_cpts, double* soa, const SysParams* d_params,
                         const BodyWave* d_waves, ccab_c128* X, const ccab_c128* d_x0,
