@@ -146,6 +146,39 @@ def test_bodies_in_memory_chunks():
         assert _rel(b.cpu().numpy(), a.cpu().numpy()) < 1e-8
 
 
+@pytest.mark.parametrize("odd", [False, True])
+def test_ragged_column_chunks(odd):
+    """Bodies of 80-1280 unknowns with 64-column chunks of the ragged matvec
+    (the small bodies' rows and chunks past their size write no partial
+    sums) and with the workgroup-target split: every body to 1e-8 of its
+    oracle FP64 LU either way.  odd: one body of 1279 unknowns (one entry per
+    lane); else all sizes multiples of 4 (four rows per lane)."""
+    specs = [_body(600, 2), _body(601, 8), _body(602, 4)]
+    if odd:
+        specs.append(_minus_one_element(mesh.icosphere(3)))
+    solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-12, restart=40,
+                    ir_tolerance=1e-3, ir_max_steps=20, ref_mgs=False, matrix_prec=capi.C32,
+                    residual_matrix=capi.C32)
+    bodies = _bodies(specs, solver=solver, seed=7)
+    refs = [_oracle(pd)[0] for pd in bodies]
+    out = {}
+    for chunk in ("64", "0"):
+        os.environ["CCAB_GEMV_RAGGED_CHUNK"] = chunk
+        try:
+            ctx = capi.Context()
+        finally:
+            del os.environ["CCAB_GEMV_RAGGED_CHUNK"]
+        _note(f"ragged chunk={chunk} odd={odd} n={[pd.get_num_eqn() for pd in bodies]}")
+        phis, _, st = solve.solve_bodies(bodies, ctx=ctx)
+        out[chunk] = [p.cpu().numpy() for p in phis]
+        ctx.close()
+        for pd, phi, s_, ref in zip(bodies, out[chunk], st, refs):
+            assert s_["converged"], s_
+            assert _rel(phi, ref) < 1e-8, (pd.get_num_eqn(), _rel(phi, ref))
+    for a, b in zip(out["64"], out["0"]):
+        assert _rel(a, b) < 1e-8
+
+
 def test_bodies_errors():
     specs = [_body(500, 4), _body(501, 5)]
     bodies = _bodies(specs, seed=6)
