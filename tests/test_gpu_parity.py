@@ -443,3 +443,32 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
         print(f"warm start: iterations {sum(s['iterations'] for s in st_w)} vs "
               f"{sum(s['iterations'] for s in st)}; true residual {true_rel:.3e}")
         assert true_rel <= 3e-10
+
+
+def test_sweep_fp16_operator_n5120():
+    """n = 5120, where the Gram-Schmidt step runs as the multi-workgroup
+    kernel (grid barriers between its partial dot products): the
+    FP16-operator GMRES-IR sweep reaches the FP64 LU solution to the
+    refinement's tolerance, with the same iteration counts on a repeat
+    (the kernel's fixed-order sums make runs reproducible)."""
+    _note("sweep n=5120, FP16-operator GMRES-IR (multi-workgroup Gram-Schmidt) vs FP64 LU")
+    m = mesh.icosphere(4)
+    fs = np.array([1.0, 3.5, 6.0]) * 343.0 / (2 * math.pi)
+    pd = PreData(m, fs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
+                 (0, 0), IncidentWave(capi.WAVE_PLANE, (0.2, 0.3, 0.9), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_DIRECT))
+    phi_d, _, _ = solve.sweep(pd)
+    pd.solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=30,
+                       ir_tolerance=1e-3, ir_max_steps=20, ref_mgs=False, matrix_prec=capi.C32,
+                       residual_matrix=capi.C32)
+    runs = []
+    for _ in range(2):
+        phi, _, st = solve.sweep(pd)
+        runs.append((phi.cpu().numpy(), [s_["iterations"] for s_ in st]))
+        for k in range(len(fs)):
+            e = float(np.linalg.norm(runs[-1][0][k] - phi_d[k].cpu().numpy())
+                      / np.linalg.norm(phi_d[k].cpu().numpy()))
+            assert st[k]["converged"] == 1 and e < 1e-6, (k, e)
+    print("iterations", runs[0][1])
+    assert runs[0][1] == runs[1][1]
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
