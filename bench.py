@@ -386,7 +386,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the frequency "
-              "sweep shards by frequency across GPUs, which this build does not launch.",
+              "sweep shards by frequency across GPUs, which bench.py does not launch.",
               file=sys.stderr)
         sys.exit(2)
     import torch
