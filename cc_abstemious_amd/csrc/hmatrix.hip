@@ -660,6 +660,163 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
     if (threadIdx.x == 0) bk.rank = index;
 }
 
+// The same recompression with one wavefront per block and four blocks per
+// workgroup, for blocks of rank <= RM <= 16 (the bulk of them: a few hundred
+// rows per factor, where a workgroup per block waits mostly at its barriers):
+// every reduction a wave_sum, no workgroup barrier.  R factors in the same
+// scratch, one slot per block of the list.
+template <int RM>
+__global__ __launch_bounds__(256) void recompress_wave_kernel(HBlkDev* blks,
+                                                              const int32_t* __restrict__ list,
+                                                              int64_t cnt, c128* U, c128* V,
+                                                              c128* U2, c128* V2, c128* Rws,
+                                                              double tol) {
+    constexpr int NWV = 4;
+    __shared__ c128 Wm_all[NWV][RM][RM + 1];
+    __shared__ c128 Vs_all[NWV][RM][RM + 1];
+    __shared__ double sv_all[NWV][RM];
+    __shared__ int ord_all[NWV][RM];
+    __shared__ c128 hs_all[NWV][RM];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t idx = int64_t(blockIdx.x) * NWV + wv;
+    if (idx >= cnt) return;  // (no workgroup barrier below)
+    HBlkDev& bk = blks[list[idx]];
+    if (bk.status != 0) return;
+    c128(&Wm)[RM][RM + 1] = Wm_all[wv];
+    c128(&Vs)[RM][RM + 1] = Vs_all[wv];
+    double* sv = sv_all[wv];
+    int* ord = ord_all[wv];
+    c128* hs = hs_all[wv];
+    const int r = bk.rank;  // <= RM (bucketed by the host)
+    const int64_t nr = bk.nr, nc = bk.nc;
+    c128* Ru = Rws + idx * 2 * RM * RM;  // [c][i] = R[i][c]
+    c128* Rv = Ru + RM * RM;
+    for (int side = 0; side < 2; ++side) {
+        c128* Q = side == 0 ? U + bk.u_off : V + bk.v_off;
+        const int64_t len = side == 0 ? nr : nc;
+        c128* R = side == 0 ? Ru : Rv;
+        for (int e = lane; e < RM * RM; e += 64) R[e] = c128{0.0, 0.0};
+        for (int c = 0; c < r; ++c) {
+            c128* qc = Q + c * len;
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int i = 0; i < c; ++i) {  // hs_i = q_i^H q_c
+                    const c128* qi = Q + i * len;
+                    double hr = 0, hi = 0;
+                    for (int64_t t = lane; t < len; t += 64) {
+                        const c128 z = conj(qi[t]) * qc[t];
+                        hr += z.re;
+                        hi += z.im;
+                    }
+                    hr = wave_sum(hr);
+                    hi = wave_sum(hi);
+                    if (lane == 0) hs[i] = c128{hr, hi};
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                for (int64_t t = lane; t < len; t += 64) {
+                    c128 v = qc[t];
+                    for (int i = 0; i < c; ++i) v -= hs[i] * Q[i * len + t];
+                    qc[t] = v;
+                }
+                if (lane < c) R[c * RM + lane] += hs[lane];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            }
+            double nn = 0.0;
+            for (int64_t t = lane; t < len; t += 64) nn += norm_sqr(qc[t]);
+            const double rn = sqrt(wave_sum(nn));
+            const double inv = rn > 1e-300 ? 1.0 / rn : 0.0;
+            for (int64_t t = lane; t < len; t += 64) qc[t] = qc[t] * inv;
+            if (lane == 0) R[c * RM + c] = c128{rn, 0.0};
+        }
+    }
+    // R written by single lanes is read by all lanes below
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // W = R_u R_v^T (r x r)
+    for (int e = lane; e < r * r; e += 64) {
+        const int i = e % r, c = e / r;
+        c128 sum{0.0, 0.0};
+        for (int t = 0; t < r; ++t) sum += Ru[t * RM + i] * Rv[t * RM + c];
+        Wm[i][c] = sum;
+        Vs[i][c] = c128{i == c ? 1.0 : 0.0, 0.0};
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // one-sided Jacobi on the columns of Wm (lane = row)
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < r - 1; ++p)
+            for (int q = p + 1; q < r; ++q) {
+                const c128 wp = lane < r ? Wm[lane][p] : c128{0, 0};
+                const c128 wq = lane < r ? Wm[lane][q] : c128{0, 0};
+                const double app = wave_sum(norm_sqr(wp));
+                const double aqq = wave_sum(norm_sqr(wq));
+                const c128 z = conj(wp) * wq;
+                const c128 apq{wave_sum(z.re), wave_sum(z.im)};
+                const double mag = hypot(apq.re, apq.im);
+                if (mag == 0.0 || mag <= 1e-15 * sqrt(app * aqq)) continue;
+                off = fmax(off, mag / sqrt(app * aqq));
+                const c128 ph{apq.re / mag, apq.im / mag};
+                const double tau = (aqq - app) / (2.0 * mag);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                if (lane < r) {
+                    Wm[lane][p] = wp * cs - (conj(ph) * wq) * sn;
+                    Wm[lane][q] = (ph * wp) * sn + wq * cs;
+                    const c128 vp = Vs[lane][p], vq = Vs[lane][q];
+                    Vs[lane][p] = vp * cs - (conj(ph) * vq) * sn;
+                    Vs[lane][q] = (ph * vp) * sn + vq * cs;
+                }
+            }
+        if (off < 1e-15) break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (lane < r) {  // singular values = column norms of the rotated W
+        double sq = 0.0;
+        for (int i = 0; i < r; ++i) sq += norm_sqr(Wm[i][lane]);
+        sv[lane] = sqrt(sq);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    if (lane == 0) {  // descending order, stable
+        for (int c = 0; c < r; ++c) ord[c] = c;
+        for (int x = 1; x < r; ++x) {
+            const int t = ord[x];
+            int y = x - 1;
+            while (y >= 0 && sv[ord[y]] < sv[t]) {
+                ord[y + 1] = ord[y];
+                --y;
+            }
+            ord[y + 1] = t;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // truncation: first sigma_i < tol * sum_{<=i} sigma, keep >= 1
+    int index = 0;
+    {
+        double sum = 0.0;
+        for (int t = 0; t < r; ++t) {
+            sum += sv[ord[t]];
+            if (sv[ord[t]] < tol * sum) break;
+            ++index;
+        }
+        index = index + 1 < r ? index + 1 : r;
+        if (index < 1) index = 1;
+    }
+    const c128* Qu = U + bk.u_off;
+    const c128* Qv = V + bk.v_off;
+    for (int t = 0; t < index; ++t) {
+        const int o = ord[t];
+        for (int64_t i = lane; i < nr; i += 64) {
+            c128 acc{0.0, 0.0};
+            for (int c = 0; c < r; ++c) acc += Qu[i + c * nr] * Wm[c][o];
+            U2[bk.u2_off + i + t * nr] = acc;
+        }
+        for (int64_t j = lane; j < nc; j += 64) {
+            c128 acc{0.0, 0.0};
+            for (int c = 0; c < r; ++c) acc += Qv[j + c * nc] * conj(Vs[c][o]);
+            V2[bk.v2_off + j + t * nc] = acc;
+        }
+    }
+    if (lane == 0) bk.rank = index;
+}
+
 // norm estimate of the recompressed factors ab initio (aca.rs:212-222)
 __global__ __launch_bounds__(kAcaThreads) void aca_norm_kernel(HBlkDev* blks,
                                                                const int32_t* __restrict__ list,
@@ -1055,13 +1212,26 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             c128* U2 = H->d_U;
             c128* V2 = H->d_V;
             size_t off = 0;
+            // blocks of rank <= 16: one wavefront each (CCAB_HM_WAVE_RECOMPRESS=0: a workgroup each)
+            const char* wrc = std::getenv("CCAB_HM_WAVE_RECOMPRESS");
+            const bool wave_rc = !wrc || std::atoi(wrc) != 0;
             for (int k = 0; k < kBuckets; ++k) {
                 const unsigned cnt = (unsigned)bl[k].size();
                 if (cnt == 0) continue;
                 const int32_t* lst = d_order + off;
                 switch (k) {
-                case 0: hipLaunchKernelGGL(recompress_kernel<8>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
-                case 1: hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
+                case 0:
+                    if (wave_rc)
+                        hipLaunchKernelGGL(recompress_wave_kernel<8>, dim3((cnt + 3) / 4), dim3(256), 0, s, H->d_blk, lst, int64_t(cnt), Ua, Va, U2, V2, Rws, tol);
+                    else
+                        hipLaunchKernelGGL(recompress_kernel<8>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol);
+                    break;
+                case 1:
+                    if (wave_rc)
+                        hipLaunchKernelGGL(recompress_wave_kernel<16>, dim3((cnt + 3) / 4), dim3(256), 0, s, H->d_blk, lst, int64_t(cnt), Ua, Va, U2, V2, Rws, tol);
+                    else
+                        hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol);
+                    break;
                 case 2: hipLaunchKernelGGL(recompress_kernel<32>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
                 default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
                 }
