@@ -455,6 +455,12 @@ int ccab_destroy(ccab_ctx* ctx) {
     if (ctx->wait_event) (void)hipEventDestroy(ctx->wait_event);
     if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     for (hipEvent_t e : ctx->gir_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->side_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->side_stream) {
+        (void)hipStreamSynchronize(ctx->side_stream);
+        (void)hipStreamDestroy(ctx->side_stream);
+    }
     prof_resolve(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (int s = 0; s < SLOT_COUNT; ++s)

@@ -173,6 +173,8 @@ struct ccab_ctx {
     hipEvent_t wait_event = nullptr;  // ccab_stream_wait
     hipEvent_t join_event = nullptr;  // ccab_stream_join
     std::vector<hipEvent_t> gir_events;  // GMRES-IR: per-step convergence-flag copies
+    hipStream_t side_stream = nullptr;  // default priority; work that overlaps the main stream
+    hipEvent_t side_ev[2] = {nullptr, nullptr};
     // replay captured HIP graphs of fixed launch sequences (the LU)
     bool use_graphs = false;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;

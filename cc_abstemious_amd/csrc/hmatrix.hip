@@ -1191,8 +1191,12 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         // launches run one after another)
         int64_t cu = 0, cv = 0;
         size_t rws = 1;
-        for (int k = 0; k < kBuckets; ++k)
+        for (int k = 0; k + 1 < kBuckets; ++k)
             rws = std::max(rws, bl[k].size() * 2 * size_t(cap[k]) * size_t(cap[k]));
+        // the largest-rank bucket (few blocks, one workgroup per CU each) runs
+        // on the side stream beside the others, in R space of its own
+        const size_t rws_big = rws;
+        rws += bl[kBuckets - 1].size() * 2 * size_t(RMAX) * size_t(RMAX);
         for (int32_t b : order) {
             HBlkDev& d = H->blk[b];
             d.u2_off = cu;
@@ -1212,6 +1216,17 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             c128* U2 = H->d_U;
             c128* V2 = H->d_V;
             size_t off = 0;
+            hipStream_t sb = s;  // stream of the largest-rank bucket
+            if (!bl[kBuckets - 1].empty()) {
+                if (!ctx->side_stream) {
+                    CCAB_HIP(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
+                    for (hipEvent_t& e : ctx->side_ev)
+                        CCAB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                }
+                sb = ctx->side_stream;
+                CCAB_HIP(hipEventRecord(ctx->side_ev[0], s));
+                CCAB_HIP(hipStreamWaitEvent(sb, ctx->side_ev[0], 0));
+            }
             // blocks of rank <= 16: one wavefront each (CCAB_HM_WAVE_RECOMPRESS=0: a workgroup each)
             const char* wrc = std::getenv("CCAB_HM_WAVE_RECOMPRESS");
             const bool wave_rc = !wrc || std::atoi(wrc) != 0;
@@ -1233,9 +1248,14 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
                         hipLaunchKernelGGL(recompress_kernel<16>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol);
                     break;
                 case 2: hipLaunchKernelGGL(recompress_kernel<32>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
-                default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, s, H->d_blk, lst, Ua, Va, U2, V2, Rws, tol); break;
+                default: hipLaunchKernelGGL(recompress_kernel<RMAX>, dim3(cnt), dim3(kAcaThreads), 0, sb, H->d_blk, lst, Ua, Va, U2, V2, Rws + rws_big, tol); break;
                 }
                 off += cnt;
+            }
+            CCAB_HIP(hipGetLastError());
+            if (sb != s) {  // the main stream waits for the side stream's bucket
+                CCAB_HIP(hipEventRecord(ctx->side_ev[1], sb));
+                CCAB_HIP(hipStreamWaitEvent(s, ctx->side_ev[1], 0));
             }
         }
         // the recompressed ranks come back; the factors now sit at the packed offsets
