@@ -23,7 +23,8 @@ SOLVER_DIRECT_MIXED, SOLVER_ITERATIVE_MIXED = 3, 4
 STOP_BACKWARD_ERROR, STOP_RELATIVE_RESIDUAL = 0, 1
 # physics corrections (opt-in; 0 = the reference's numerics), include/ccab.h
 FIX_BM_DH, FIX_FIELD_SIGN, FIX_IMPEDANCE_VN, FIX_SPHERICAL_ORIGIN, FIX_QUAD_AREA = 1, 2, 4, 8, 16
-FIX_ALL = 31
+FIX_PRESSURE_SIGN = 32
+FIX_ALL = 63
 
 
 class CcabError(RuntimeError):
@@ -157,6 +158,10 @@ SIGNATURES = {
     "ccab_hmatrix_gemv": (_I, [_P, _P, _P, _P, _P, _P]),
     "ccab_hmatrix_to_full": (_I, [_P, _P, _P, _I64]),
     "ccab_hmatrix_stats": (_I, [_P, _P, _P]),
+    "ccab_hmatrix_stats_ex": (_I, [_P, _P, _I, _P]),
+    "ccab_hmatrix_build_matrix": (_I, [_P, _I64, _P, _I64, _P, _I64, ct.c_double, ct.c_double,
+                                       _I, ct.POINTER(_P)]),
+    "ccab_aca_build": (_I, [_P, _I64, _I64, _P, _I64, ct.c_double, ct.POINTER(_P)]),
     "ccab_gmres_hmatrix": (_I, [_P, _P, _P, _P, ct.POINTER(GmresParams), _P]),
     "ccab_field": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
     "ccab_dense_field_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _P, _I64, _P, _P, _I64]),

@@ -144,12 +144,15 @@ SysParams make_params(const ccab_physics& ph) {
     p.bcv = c128{ph.bc_value[0], ph.bc_value[1]};
     const bool bc_zero = ph.bc_value[0] == 0.0 && ph.bc_value[1] == 0.0;
     switch (ph.bc_type) {
-    case CCAB_BC_PRESSURE:
-        p.lhs_a = {0, 0}; p.lhs_b = {1, 0}; p.rhs_a = {-1, 0}; p.rhs_b = {0, 0};
+    case CCAB_BC_PRESSURE: {
+        // G vn = rhs_inc - H p (solve.rs:36-43); + H p with CCAB_FIX_PRESSURE_SIGN
+        const double s = (ph.corrections & CCAB_FIX_PRESSURE_SIGN) ? 1.0 : -1.0;
+        p.lhs_a = {0, 0}; p.lhs_b = {1, 0}; p.rhs_a = {s, 0}; p.rhs_b = {0, 0};
         p.rowsum_of = bc_zero ? 0 : 2;
-        p.rhs_scale = c128{-p.bcv.re, -p.bcv.im};
+        p.rhs_scale = c128{s * p.bcv.re, s * p.bcv.im};
         p.rhs_inc_sign = 1.0;
         break;
+    }
     case CCAB_BC_NORMAL_VELOCITY:
         p.lhs_a = {1, 0}; p.lhs_b = {0, 0}; p.rhs_a = {0, 0}; p.rhs_b = {1, 0};
         p.rowsum_of = bc_zero ? 0 : 1;
@@ -180,7 +183,7 @@ int check_physics(ccab_ctx* ctx, const ccab_physics& ph) {
     CCAB_CHECK(ph.problem_type == CCAB_EXTERIOR || ph.problem_type == CCAB_INTERIOR,
                "bad problem type");
     CCAB_CHECK(ph.bc_type >= CCAB_BC_PRESSURE && ph.bc_type <= CCAB_BC_IMPEDANCE, "bad bc type");
-    CCAB_CHECK((ph.corrections & ~31) == 0, "unknown correction bits");
+    CCAB_CHECK((ph.corrections & ~63) == 0, "unknown correction bits");
     if (ph.bc_type == CCAB_BC_IMPEDANCE)
         CCAB_CHECK(ph.bc_value[0] != 0.0 || ph.bc_value[1] != 0.0, "impedance must be non-zero");
     return CCAB_OK;
@@ -247,7 +250,7 @@ int hierarchical_sweep(ccab_ctx* ctx, const std::vector<SysParams>& hp, int bc,
             HPtr hr(hr_raw);
             hipLaunchKernelGGL(fill_kernel, g, dim3(256), 0, ctx->stream, n, p.bcv,
                                reinterpret_cast<ccab_c128*>(tmp), 0);
-            // Pressure: rhs = H phi + rhs_inc; NormalVelocity: rhs = G vn - rhs_inc
+            // Pressure: rhs = (rhs_a H) phi + rhs_inc; NormalVelocity: rhs = G vn - rhs_inc
             CCAB_TRY(hmatrix_gemv(ctx, hr.get(), one, tmp,
                                   bc == CCAB_BC_PRESSURE ? one : mone, x));
             CCAB_HIP(hipStreamSynchronize(ctx->stream));
@@ -375,6 +378,10 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* rc = getenv("CCAB_GEMV_RAGGED_CHUNK")) ctx->gemv_ragged_chunk = std::max(0, atoi(rc));
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
+    // the grid-barrier kernels assume one resident workgroup per CU: without
+    // room for one, their paths stay off
+    if (!mgs_coop_fits()) ctx->mgs_coop = false;
+    if (!lu_coop_fits()) ctx->lu_coop_rows = 0;
     *out = ctx;
     return CCAB_OK;
 }
@@ -1096,6 +1103,30 @@ extern "C" int ccab_hmatrix_build_ex(ccab_ctx* ctx, const ccab_physics* ph, cons
                          prec);
 }
 
+extern "C" int ccab_hmatrix_build_matrix(ccab_ctx* ctx, int64_t n, const ccab_c128* d_A,
+                                         int64_t lda, const double* h_xyz, int64_t leaf_size,
+                                         double eta, double aca_tolerance, int prec,
+                                         ccab_hmatrix** out) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A && h_xyz && out && n > 0 && lda >= n, "invalid arguments");
+    *out = nullptr;
+    CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "H-matrix storage is CCAB_C128 or CCAB_C64");
+    CCAB_CHECK(leaf_size >= 1, "leaf_size must be positive");
+    CCAB_CHECK(eta > 0.0 && aca_tolerance > 0.0, "eta and aca_tolerance must be positive");
+    return hmatrix_build_matrix(ctx, reinterpret_cast<const c128*>(d_A), lda, h_xyz, n, leaf_size,
+                                eta, aca_tolerance, out, prec);
+}
+
+extern "C" int ccab_aca_build(ccab_ctx* ctx, int64_t m, int64_t n, const ccab_c128* d_A,
+                              int64_t lda, double tol, ccab_hmatrix** out) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(d_A && out && m > 0 && n > 0 && lda >= m, "invalid arguments");
+    *out = nullptr;
+    CCAB_CHECK(tol > 0.0, "tol must be positive");
+    return aca_build_matrix(ctx, reinterpret_cast<const c128*>(d_A), lda, m, n, tol, out,
+                            CCAB_C128);
+}
+
 extern "C" int ccab_hmatrix_destroy(ccab_hmatrix* h) {
     if (h) hmatrix_free(h);
     return CCAB_OK;
@@ -1124,12 +1155,20 @@ extern "C" int ccab_hmatrix_stats(const ccab_hmatrix* h, int64_t* st, double* no
     return CCAB_OK;
 }
 
+extern "C" int ccab_hmatrix_stats_ex(const ccab_hmatrix* h, int64_t* st, int nst, double* norm) {
+    if (!h || !st || nst < 0) return CCAB_EINVAL;
+    hmatrix_stats_ex(h, st, nst);
+    if (norm) *norm = hmatrix_norm(h);
+    return CCAB_OK;
+}
+
 extern "C" int ccab_gmres_hmatrix(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_x,
                                   const ccab_c128* d_x0, const ccab_gmres_params* p,
                                   ccab_solve_stats* stats) {
     if (!ctx) return CCAB_EINVAL;
     CCAB_CHECK(h && d_x && p, "invalid arguments");
     CCAB_CHECK(hmatrix_device(h) == ctx->device, "H matrix lives on another device");
+    CCAB_CHECK(hmatrix_ncol(h) == hmatrix_n(h), "GMRES needs a square operator");
     MatvecFn op = [&](const c128* xin, c128 a, c128 b, c128* yout) -> int {
         return hmatrix_gemv(ctx, h, a, xin, b, yout);
     };

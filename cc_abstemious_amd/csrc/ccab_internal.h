@@ -108,7 +108,7 @@ enum BufSlot {
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
     SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
-    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COUNT
+    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_COUNT
 };
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
@@ -157,6 +157,9 @@ struct ccab_ctx {
     // system slots are known clear
     void* mgs_xbuf = nullptr;
     int64_t mgs_xbuf_ready = 0;
+    // set (never cleared) by a cooperative Gram-Schmidt step whose grid
+    // barrier timed out; the host reads and clears it when a NaN shows up
+    uint64_t* coop_abort = nullptr;
     // GMRES-IR: the host applies the device's done flags this many Arnoldi
     // steps late (environment CCAB_GIR_LAG, 1-8)
     int gir_lag = 1;
@@ -393,6 +396,13 @@ int convert_c64_to_c128(ccab_ctx* ctx, int64_t count, const c64* src, c128* dst)
 using MatvecFn = std::function<int(const c128* x, c128 alpha, c128 beta, c128* y)>;
 int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, c128* d_x,
                  const c128* d_x0, const ccab_gmres_params& p, ccab_solve_stats* st);
+// CCAB_EHIP with a message if a cooperative Gram-Schmidt barrier timed out
+// since the last check (else CCAB_OK); called where a NaN is reported
+int coop_abort_check(ccab_ctx* ctx);
+// every workgroup of the multi-workgroup (grid-barrier) kernels fits on a CU
+// beside nothing else: checked once per context, the paths are off otherwise
+bool mgs_coop_fits();
+bool lu_coop_fits();
 
 // hmatrix.hip
 int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
@@ -402,6 +412,13 @@ int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x
                  c128* y);
 int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda);
 void hmatrix_stats(const ccab_hmatrix* H, int64_t* st, double* norm);
+void hmatrix_stats_ex(const ccab_hmatrix* H, int64_t* st, int nst);
+int hmatrix_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, const double* h_xyz,
+                         int64_t n, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
+                         int prec);
+int aca_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, int64_t m, int64_t ncol,
+                     double tol, ccab_hmatrix** out, int prec);
+int64_t hmatrix_ncol(const ccab_hmatrix* H);
 void hmatrix_free(ccab_hmatrix* H);
 int64_t hmatrix_n(const ccab_hmatrix* H);
 int hmatrix_device(const ccab_hmatrix* H);

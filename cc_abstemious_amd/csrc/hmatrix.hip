@@ -169,19 +169,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __device__ __forceinline__ double cabs(c128 z) { return hypot(z.re, z.im); }
 
+// Entry sources of the blocks: the BEM side matrix a H + b G
+// (get_surface_row_or_column, influence_matrix.rs:170-231), or an explicit
+// column-major matrix in HBM (the closures of the reference's ACA and HMatrix
+// tests, aca.rs:303-304, h_matrix.rs:282-298)
+struct BemEntry {
+    SysParams p;
+    const double* soa;
+    int64_t n;
+    c128 fa, fb;
+    __device__ __forceinline__ c128 operator()(int64_t i, int64_t j) const {
+        return side_entry(p, soa_of(soa, n), i, j, fa, fb);
+    }
+};
+
+struct MatEntry {
+    const c128* A;
+    int64_t lda;
+    __device__ __forceinline__ c128 operator()(int64_t i, int64_t j) const { return A[i + j * lda]; }
+};
+
 // dense near-field blocks: D[i + j nr] = entry(rows[i], cols[j])
-__global__ void dense_block_kernel(SysParams p, const double* __restrict__ soa_base, int64_t n,
-                                   const HBlkDev* __restrict__ blks,
+template <typename E>
+__global__ void dense_block_kernel(E entry, const HBlkDev* __restrict__ blks,
                                    const int32_t* __restrict__ list, const int64_t* __restrict__ rows,
-                                   const int64_t* __restrict__ cols, c128 fa, c128 fb,
-                                   c128* __restrict__ D) {
+                                   const int64_t* __restrict__ cols, c128* __restrict__ D) {
     const HBlkDev bk = blks[list[blockIdx.x]];
-    const Soa s = soa_of(soa_base, n);
     const int64_t cnt = bk.nr * bk.nc;
     for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cnt;
          e += int64_t(blockDim.x) * gridDim.y) {
         const int64_t i = e % bk.nr, j = e / bk.nr;
-        D[bk.d_off + e] = side_entry(p, s, rows[bk.r_off + i], cols[bk.c_off + j], fa, fb);
+        D[bk.d_off + e] = entry(rows[bk.r_off + i], cols[bk.c_off + j]);
     }
 }
 
@@ -200,32 +218,33 @@ __global__ void dense_norm_kernel(HBlkDev* blks, const int32_t* __restrict__ lis
 }
 
 // --- ACA (aca.rs:57-136), one workgroup of kAcaThreads per admissible block ---
+template <typename E>
 struct AcaCtx {
-    SysParams p;
-    Soa s;
+    E entry;
     const int64_t* rows;
     const int64_t* cols;
     int64_t nr, nc;
     c128* U;  // nr x RMAX
     c128* V;  // nc x RMAX
-    c128 fa, fb;
 };
 
 // r[j] = A(rows[i], cols[j]) - sum_k U[i,k] V[j,k]   (residual row)
-__device__ void aca_row(const AcaCtx& a, int64_t i, int rank, c128* r) {
+template <typename E>
+__device__ void aca_row(const AcaCtx<E>& a, int64_t i, int rank, c128* r) {
     const int64_t gi = a.rows[i];
     for (int64_t j = threadIdx.x; j < a.nc; j += kAcaThreads) {
-        c128 v = side_entry(a.p, a.s, gi, a.cols[j], a.fa, a.fb);
+        c128 v = a.entry(gi, a.cols[j]);
         for (int k = 0; k < rank; ++k) v -= a.U[i + k * a.nr] * a.V[j + k * a.nc];
         r[j] = v;
     }
 }
 
 // r[i] = A(rows[i], cols[j]) - sum_k V[j,k] U[i,k]   (residual column)
-__device__ void aca_col(const AcaCtx& a, int64_t j, int rank, c128* r) {
+template <typename E>
+__device__ void aca_col(const AcaCtx<E>& a, int64_t j, int rank, c128* r) {
     const int64_t gj = a.cols[j];
     for (int64_t i = threadIdx.x; i < a.nr; i += kAcaThreads) {
-        c128 v = side_entry(a.p, a.s, a.rows[i], gj, a.fa, a.fb);
+        c128 v = a.entry(a.rows[i], gj);
         for (int k = 0; k < rank; ++k) v -= a.V[j + k * a.nc] * a.U[i + k * a.nr];
         r[i] = v;
     }
@@ -303,13 +322,12 @@ __device__ int64_t rng_uniform(const uint32_t* words, int* pos, uint32_t range, 
     return 0;
 }
 
-__global__ __launch_bounds__(kAcaThreads) void aca_kernel(SysParams p, const double* __restrict__ soa_base,
-                                                          int64_t n, HBlkDev* blks,
+template <typename E>
+__global__ __launch_bounds__(kAcaThreads) void aca_kernel(E entry, HBlkDev* blks,
                                                           const int32_t* __restrict__ list,
                                                           const int64_t* __restrict__ rows,
                                                           const int64_t* __restrict__ cols,
-                                                          c128 fa, c128 fb, c128* U, c128* V,
-                                                          c128* W, uint8_t* F,
+                                                          c128* U, c128* V, c128* W, uint8_t* F,
                                                           const uint32_t* __restrict__ rng,
                                                           double tol) {
     __shared__ double red_v[kAcaThreads / 64];
@@ -319,8 +337,7 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(SysParams p, const dou
     __shared__ int64_t s_ref[2];
     __shared__ int s_status;
     HBlkDev& bk = blks[list[blockIdx.x]];
-    AcaCtx a{p, soa_of(soa_base, n), rows + bk.r_off, cols + bk.c_off, bk.nr, bk.nc,
-             U + bk.u_off, V + bk.v_off, fa, fb};
+    AcaCtx<E> a{entry, rows + bk.r_off, cols + bk.c_off, bk.nr, bk.nc, U + bk.u_off, V + bk.v_off};
     const int64_t nr = bk.nr, nc = bk.nc;
     c128* riref = W + bk.w_off;       // nc
     c128* rjref = riref + nc;         // nr
@@ -1005,7 +1022,8 @@ T* dalloc(std::vector<void*>& owned, size_t count) {
 }  // namespace
 
 struct ccab_hmatrix {
-    int64_t n = 0;
+    int64_t n = 0;     // rows
+    int64_t ncol = 0;  // columns (= n except for a single ACA block)
     int device = 0;
     std::vector<void*> owned;  // device allocations
     std::vector<HBlkDev> blk;  // host copy (ranks, norms after build)
@@ -1038,6 +1056,7 @@ struct ccab_hmatrix {
     double norm = 0.0;
     double mv_bytes = 0.0;  // block data one matvec reads
     int64_t aca_fallbacks = 0;
+    int64_t aca_bucket[4] = {0, 0, 0, 0};  // admissible blocks by ACA rank: <= 8, 16, 32, 64
     ~ccab_hmatrix() {
         for (void* p : owned) (void)hipFree(p);
     }
@@ -1063,29 +1082,25 @@ struct BuildTrace {
         t = now;
     }
 };
-}  // namespace
 
-int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
-                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
-                  int prec) {
+// The blocks of a block list (bl, over the clusters T) built on the device
+// from an entry source: batched ACA + recompression of the admissible
+// blocks, dense inadmissible blocks, norms and the matvec plan.  The operator
+// has nrow rows and ncol columns (square for the cluster-tree H matrices; one
+// rectangular admissible block for a plain ACA).
+template <typename E>
+int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>& T,
+                        const std::vector<HostBlock>& bl, int64_t nrow, int64_t ncol, double tol,
+                        ccab_hmatrix** out, int prec, BuildTrace& trace) {
     hipStream_t s = ctx->stream;
-    BuildTrace trace(s);
     std::unique_ptr<ccab_hmatrix> owner(new ccab_hmatrix());
     ccab_hmatrix* H = owner.get();
-    H->n = n;
+    H->n = nrow;
+    H->ncol = ncol;
     H->device = ctx->device;
     auto fail = [&](int code, const char* msg) { return ctx_fail(ctx, code, "%s", msg); };
-    // cluster and block trees on the host (index order as cluster.rs)
-    std::vector<HNode> T;
-    {
-        std::vector<int64_t> all(n);
-        for (int64_t i = 0; i < n; ++i) all[i] = i;
-        build_cluster(T, h_cpts, 8, std::move(all), leaf);
-    }
-    std::vector<HostBlock> bl;
-    build_blocks(T, 0, 0, eta, bl);
     const int64_t nb = (int64_t)bl.size();
-    trace.mark("cluster and block trees");
+    const int64_t n = nrow;
     // every cluster's index list stored once (n x tree depth entries); a
     // block's rows / columns are its clusters' lists, so rows[] and cols[]
     // are the same array
@@ -1153,13 +1168,11 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     CCAB_HIP(hipMemcpy(d_rng, words.data(), kRngTable * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll.data(), ll.size() * 4, hipMemcpyHostToDevice));
     trace.mark("allocation and upload");
-    const c128 fa = side == CCAB_LHS ? p.lhs_a : p.rhs_a;
-    const c128 fb = side == CCAB_LHS ? p.lhs_b : p.rhs_b;
     // batched ACA + recompression of the admissible blocks
     if (!ll.empty()) {
-        hipLaunchKernelGGL(aca_kernel, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, p, d_soa,
-                           n, H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, fa, fb, Ua, Va, Wws,
-                           Fws, d_rng, tol);
+        hipLaunchKernelGGL(aca_kernel<E>, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, entry,
+                           H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, Ua, Va, Wws, Fws, d_rng,
+                           tol);
         // recompression with LDS sized by rank: blocks bucketed by their ACA
         // rank (most are far below RMAX, and the RMAX-sized kernel holds one
         // workgroup per CU)
@@ -1168,22 +1181,23 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         trace.mark("ACA");
         constexpr int kBuckets = 4;
         const int cap[kBuckets] = {8, 16, 32, RMAX};
-        std::vector<int32_t> bl[kBuckets];
+        std::vector<int32_t> bucket[kBuckets];
         for (int32_t b : ll) {
             const HBlkDev& d = H->blk[b];
             if (d.status != 0) continue;  // becomes dense below
             int k = 0;
             while (k < kBuckets - 1 && d.rank > cap[k]) ++k;
-            bl[k].push_back(b);
+            bucket[k].push_back(b);
         }
         std::vector<int32_t> order;
-        for (auto& v : bl) order.insert(order.end(), v.begin(), v.end());
+        for (auto& v : bucket) order.insert(order.end(), v.begin(), v.end());
+        for (int k = 0; k < kBuckets; ++k) H->aca_bucket[k] = (int64_t)bucket[k].size();
         if (trace.on) {
             for (int k = 0; k < kBuckets; ++k) {
                 double rows = 0.0;
-                for (int32_t b : bl[k]) rows += double(H->blk[b].nr + H->blk[b].nc);
+                for (int32_t b : bucket[k]) rows += double(H->blk[b].nr + H->blk[b].nc);
                 std::fprintf(stderr, "[hbuild] rank <= %2d: %6zu blocks, mean nr + nc %.0f\n", cap[k],
-                             bl[k].size(), bl[k].empty() ? 0.0 : rows / double(bl[k].size()));
+                             bucket[k].size(), bucket[k].empty() ? 0.0 : rows / double(bucket[k].size()));
             }
         }
         // recompressed factors packed to the ACA ranks (the recompressed rank
@@ -1192,11 +1206,11 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
         int64_t cu = 0, cv = 0;
         size_t rws = 1;
         for (int k = 0; k + 1 < kBuckets; ++k)
-            rws = std::max(rws, bl[k].size() * 2 * size_t(cap[k]) * size_t(cap[k]));
+            rws = std::max(rws, bucket[k].size() * 2 * size_t(cap[k]) * size_t(cap[k]));
         // the largest-rank bucket (few blocks, one workgroup per CU each) runs
         // on the side stream beside the others, in R space of its own
         const size_t rws_big = rws;
-        rws += bl[kBuckets - 1].size() * 2 * size_t(RMAX) * size_t(RMAX);
+        rws += bucket[kBuckets - 1].size() * 2 * size_t(RMAX) * size_t(RMAX);
         for (int32_t b : order) {
             HBlkDev& d = H->blk[b];
             d.u2_off = cu;
@@ -1217,7 +1231,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             c128* V2 = H->d_V;
             size_t off = 0;
             hipStream_t sb = s;  // stream of the largest-rank bucket
-            if (!bl[kBuckets - 1].empty()) {
+            if (!bucket[kBuckets - 1].empty()) {
                 if (!ctx->side_stream) {
                     CCAB_HIP(hipStreamCreateWithFlags(&ctx->side_stream, hipStreamNonBlocking));
                     for (hipEvent_t& e : ctx->side_ev)
@@ -1231,7 +1245,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
             const char* wrc = std::getenv("CCAB_HM_WAVE_RECOMPRESS");
             const bool wave_rc = !wrc || std::atoi(wrc) != 0;
             for (int k = 0; k < kBuckets; ++k) {
-                const unsigned cnt = (unsigned)bl[k].size();
+                const unsigned cnt = (unsigned)bucket[k].size();
                 if (cnt == 0) continue;
                 const int32_t* lst = d_order + off;
                 switch (k) {
@@ -1304,8 +1318,8 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
     if (!dl.empty()) {
-        hipLaunchKernelGGL(dense_block_kernel, dim3((unsigned)dl.size(), 4), dim3(256), 0, s, p,
-                           d_soa, n, H->d_blk, H->d_dense_list, H->d_rows, H->d_cols, fa, fb, H->d_D);
+        hipLaunchKernelGGL(dense_block_kernel<E>, dim3((unsigned)dl.size(), 4), dim3(256), 0, s,
+                           entry, H->d_blk, H->d_dense_list, H->d_rows, H->d_cols, H->d_D);
         hipLaunchKernelGGL(dense_norm_kernel, dim3((unsigned)dl.size()), dim3(256), 0, s, H->d_blk,
                            H->d_dense_list, H->d_D);
     }
@@ -1422,6 +1436,54 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     return CCAB_OK;
 }
 
+// cluster tree of the n points (stride doubles apart, coordinates first) and
+// its block tree (index order as cluster.rs / block.rs)
+void trees(const double* xyz, int64_t stride, int64_t n, int64_t leaf, double eta,
+           std::vector<HNode>& T, std::vector<HostBlock>& bl) {
+    std::vector<int64_t> all(n);
+    for (int64_t i = 0; i < n; ++i) all[i] = i;
+    build_cluster(T, xyz, stride, std::move(all), leaf);
+    build_blocks(T, 0, 0, eta, bl);
+}
+}  // namespace
+
+int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,
+                  int64_t n, int side, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
+                  int prec) {
+    BuildTrace trace(ctx->stream);
+    std::vector<HNode> T;
+    std::vector<HostBlock> bl;
+    trees(h_cpts, 8, n, leaf, eta, T, bl);
+    trace.mark("cluster and block trees");
+    const BemEntry entry{p, d_soa, n, side == CCAB_LHS ? p.lhs_a : p.rhs_a,
+                         side == CCAB_LHS ? p.lhs_b : p.rhs_b};
+    return hmatrix_from_blocks(ctx, entry, T, bl, n, n, tol, out, prec, trace);
+}
+
+int hmatrix_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, const double* h_xyz,
+                         int64_t n, int64_t leaf, double eta, double tol, ccab_hmatrix** out,
+                         int prec) {
+    BuildTrace trace(ctx->stream);
+    std::vector<HNode> T;
+    std::vector<HostBlock> bl;
+    trees(h_xyz, 3, n, leaf, eta, T, bl);
+    trace.mark("cluster and block trees");
+    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, T, bl, n, n, tol, out, prec, trace);
+}
+
+int aca_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, int64_t m, int64_t ncol,
+                     double tol, ccab_hmatrix** out, int prec) {
+    BuildTrace trace(ctx->stream);
+    // two "clusters", the rows and the columns, and one admissible block
+    std::vector<HNode> T(2);
+    T[0].idx.resize(m);
+    T[1].idx.resize(ncol);
+    for (int64_t i = 0; i < m; ++i) T[0].idx[i] = i;
+    for (int64_t j = 0; j < ncol; ++j) T[1].idx[j] = j;
+    const std::vector<HostBlock> bl{{0, 1, true}};
+    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, T, bl, m, ncol, tol, out, prec, trace);
+}
+
 int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
                  c128* y) {
     hipStream_t s = ctx->stream;
@@ -1469,6 +1531,19 @@ void hmatrix_stats(const ccab_hmatrix* H, int64_t* st, double* norm) {
     if (norm) *norm = H->norm;
 }
 
+void hmatrix_stats_ex(const ccab_hmatrix* H, int64_t* st, int nst) {
+    int64_t all[12];
+    hmatrix_stats(H, all, nullptr);
+    for (int k = 0; k < 4; ++k) all[5 + k] = H->aca_bucket[k];
+    int64_t rmax = 0;
+    for (const auto& d : H->blk)
+        if (d.adm) rmax = std::max<int64_t>(rmax, d.rank);
+    all[9] = rmax;
+    all[10] = H->n;
+    all[11] = H->ncol;
+    for (int k = 0; k < nst && k < 12; ++k) st[k] = all[k];
+}
+
 int hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* H, c128* A, int64_t lda) {
     const int64_t nb = (int64_t)H->blk.size();
     if (nb && H->fp32)
@@ -1485,4 +1560,5 @@ void hmatrix_free(ccab_hmatrix* H) { delete H; }
 int hmatrix_device(const ccab_hmatrix* H) { return H->device; }
 
 int64_t hmatrix_n(const ccab_hmatrix* H) { return H->n; }
+int64_t hmatrix_ncol(const ccab_hmatrix* H) { return H->ncol; }
 double hmatrix_norm(const ccab_hmatrix* H) { return H->norm; }

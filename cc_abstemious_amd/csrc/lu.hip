@@ -1450,3 +1450,15 @@ int cgemm_update(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t K, const
                                  static_cast<const c64*>(B), ldb, 0, static_cast<c64*>(C), ldc,
                                  0, 1);
 }
+
+bool lu_coop_fits() {
+    auto fits = [](const void* f) {
+        int nb = 0;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kCoopThreads, 0) ==
+                   hipSuccess && nb >= 1;
+    };
+    return fits(reinterpret_cast<const void*>(&coop_slab_kernel<double, coop_width<double, 1>(), 1>)) &&
+           fits(reinterpret_cast<const void*>(&coop_slab_kernel<double, coop_width<double, 2>(), 2>)) &&
+           fits(reinterpret_cast<const void*>(&coop_slab_kernel<float, coop_width<float, 1>(), 1>)) &&
+           fits(reinterpret_cast<const void*>(&coop_slab_kernel<float, coop_width<float, 2>(), 2>));
+}

@@ -92,15 +92,16 @@ __global__ __launch_bounds__(kMgsThreads) void mgs_kernel(int64_t n, int64_t k, 
 // workgroup publishes its partial dot product, one grid barrier, and all
 // workgroups add the G partials in the same fixed order (so they agree on
 // the coefficient bit for bit), then update their rows.  k + 2 barriers per
-// step; Q is read once.  After a failed barrier (see coop.h) h[k+1] is NaN,
-// which the caller's Givens step reports.
+// step; Q is read once.  After a failed barrier (see coop.h) h[k+1] is NaN
+// and *sticky is set: the caller's Givens step finds the NaN and
+// coop_abort_check reports the timeout.
 constexpr int kMgsCoopThreads = 256;
 
 template <bool REF, int RQ>
 __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
     int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ, c128* h, int64_t sh,
     const int32_t* __restrict__ active, uint64_t* __restrict__ xbuf, int64_t sys_words,
-    unsigned spin_limit, const int32_t* __restrict__ skip) {
+    unsigned spin_limit, const int32_t* __restrict__ skip, uint64_t* __restrict__ sticky) {
     constexpr int NW = kMgsCoopThreads / 64;
     __shared__ double red[2 * NW];
     __shared__ double tot[2];
@@ -213,7 +214,10 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
         ok = grid_sum(nn, dummy, nb++);
     }
     if (!ok) {
-        if (g == 0 && tid == 0) hb[k + 1] = c128{__builtin_nan(""), 0.0};
+        if (g == 0 && tid == 0) {
+            hb[k + 1] = c128{__builtin_nan(""), 0.0};
+            __hip_atomic_store(sticky, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         leave();
         return;
     }
@@ -257,15 +261,24 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
                 ctx->mgs_xbuf = xbuf;
                 ctx->mgs_xbuf_ready = nact;
             }
+            if (!ctx->coop_abort) {
+                uint64_t* st = static_cast<uint64_t*>(ctx_buf(ctx, SLOT_COOP_ABORT, 8));
+                if (!st) return CCAB_ENOMEM;
+                hipLaunchKernelGGL(coop_reset_kernel, dim3(1), dim3(64), 0, s, st, int64_t(1),
+                                   int64_t(1));
+                ctx->coop_abort = st;
+            }
             const dim3 grid((unsigned)G, (unsigned)nact);
 #define CCAB_MGS_COOP(R)                                                                        \
     do {                                                                                        \
         if (ref)                                                                                \
             hipLaunchKernelGGL((mgs_coop_kernel<true, R>), grid, dim3(kMgsCoopThreads), 0, s, n, \
-                               k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip); \
+                               k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip, \
+                               ctx->coop_abort);                                       \
         else                                                                                    \
             hipLaunchKernelGGL((mgs_coop_kernel<false, R>), grid, dim3(kMgsCoopThreads), 0, s,   \
-                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip); \
+                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip, \
+                               ctx->coop_abort);                                       \
     } while (0)
             switch (rq) {
             case 1: CCAB_MGS_COOP(1); break;
@@ -287,6 +300,35 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
+
+}  // namespace
+
+int coop_abort_check(ccab_ctx* ctx) {
+    if (!ctx->coop_abort) return CCAB_OK;
+    uint64_t v = 0;
+    CCAB_HIP(hipMemcpyAsync(&v, ctx->coop_abort, 8, hipMemcpyDeviceToHost, ctx->stream));
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
+    if (!v) return CCAB_OK;
+    hipLaunchKernelGGL(coop_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->coop_abort,
+                       int64_t(1), int64_t(1));
+    return ctx_fail(ctx, CCAB_EHIP,
+                    "Gram-Schmidt grid barrier timed out (a workgroup of the multi-workgroup "
+                    "step was not resident; CCAB_MGS_COOP=0 keeps one workgroup per system)");
+}
+
+bool mgs_coop_fits() {
+    auto fits = [](const void* f) {
+        int nb = 0;
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kMgsCoopThreads, 0) ==
+                   hipSuccess && nb >= 1;
+    };
+    return fits(reinterpret_cast<const void*>(&mgs_coop_kernel<true, 1>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_coop_kernel<true, 8>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_coop_kernel<false, 1>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_coop_kernel<false, 8>));
+}
+
+namespace {
 
 __global__ void scale_kernel(int64_t n, const c128* __restrict__ src, c128 s, c128* dst) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -690,7 +732,10 @@ int gmres_op_run(ccab_ctx* ctx, int64_t n, const MatvecFn& apply, double anorm, 
             beta[k + 1] = -std::conj(sn[k]) * beta[k];
             beta[k] *= cs[k];
             error = err_of(std::abs(beta[k + 1]), xn);
-            if (!std::isfinite(error)) return ctx_fail(ctx, CCAB_ENAN, "GMRES error is NaN");
+            if (!std::isfinite(error)) {
+                CCAB_TRY(coop_abort_check(ctx));
+                return ctx_fail(ctx, CCAB_ENAN, "GMRES error is NaN");
+            }
             if (error < thresh || k == m - 1) {
                 const int64_t kk = k + 1;
                 std::vector<cd> Hk(kk * kk), yv(kk);
@@ -971,8 +1016,10 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         std::vector<int32_t> still;
         for (int32_t b : active) {
             rel[b] = bn[b] > 0 ? rn[b] / bn[b] : 0.0;
-            if (!std::isfinite(rel[b]))
+            if (!std::isfinite(rel[b])) {
+                CCAB_TRY(coop_abort_check(ctx));
                 return ctx_fail(ctx, CCAB_ENAN, "GMRES-IR residual is NaN in system %d", b);
+            }
             if (rel[b] <= tol || rn[b] == 0.0) conv[b] = 1;
             else still.push_back(b);
         }

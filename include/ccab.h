@@ -71,9 +71,14 @@ enum { CCAB_STOP_BACKWARD_ERROR = 0, /* gmres.rs:142-144 */
  *                             unit sphere)
  *   CCAB_FIX_QUAD_AREA        ccab_collocation_points_ex: quad area = the
  *                             element's area with weight 1 (elements.rs:168
- *                             takes a quarter with weight 4) */
+ *                             takes a quarter with weight 4)
+ *   CCAB_FIX_PRESSURE_SIGN    pressure BC: G vn = rhs_inc + H p, the equation
+ *                             the velocity branch (solve.rs:46-52) implies
+ *                             (solve.rs:36-43 and :99-104 subtract H p, which
+ *                             flips the sign of vn); also the RHS-side factors
+ *                             of the row/column getter and the H matrix */
 enum { CCAB_FIX_BM_DH = 1, CCAB_FIX_FIELD_SIGN = 2, CCAB_FIX_IMPEDANCE_VN = 4,
-       CCAB_FIX_SPHERICAL_ORIGIN = 8, CCAB_FIX_QUAD_AREA = 16 };
+       CCAB_FIX_SPHERICAL_ORIGIN = 8, CCAB_FIX_QUAD_AREA = 16, CCAB_FIX_PRESSURE_SIGN = 32 };
 
 /* The physics of one frequency (UserInput fields + PreData frequency,
  * input_data.rs:79-91, preprocess.rs:35-45). */
@@ -343,6 +348,36 @@ int ccab_hmatrix_to_full(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_A, i
  * ranks, st[4] admissible blocks stored dense; *norm = sum of the dense
  * Frobenius norms and the ACA norm estimates (the ||A|| of the GMRES stop). */
 int ccab_hmatrix_stats(const ccab_hmatrix* h, int64_t* st, double* norm);
+/* ccab_hmatrix_stats with more entries (no reference counterpart): st[0..4]
+ * as there; st[5..8] admissible blocks by ACA rank before recompression
+ * (<= 8, 9-16, 17-32, 33-64: the LDS sizes of the recompression kernels);
+ * st[9] largest rank after recompression; st[10] rows, st[11] columns.
+ * nst = entries wanted (at most 12 are written). */
+int ccab_hmatrix_stats_ex(const ccab_hmatrix* h, int64_t* st, int nst, double* norm);
+/* src/solve/h_matrix.rs:127-148 (HMatrix::new_from) over an explicit n x n
+ * operator: its get_row_or_column closure reads the column-major c128 matrix
+ * d_A (leading dimension lda, device), as in the reference's tests
+ * (h_matrix.rs:282-298).  h_xyz (host, 3 doubles per point) holds the
+ * coordinates of the CollocationPoints the cluster tree splits; point i is
+ * equation i (an identity eqn_map).  leaf_size, eta (HMatrix::new_from fixes
+ * 4.0, h_matrix.rs:136), aca_tolerance and prec (CCAB_C128, or CCAB_C64 block
+ * storage) as ccab_hmatrix_build_ex.  The handle works with every
+ * ccab_hmatrix_* function and ccab_gmres_hmatrix. */
+int ccab_hmatrix_build_matrix(ccab_ctx* ctx, int64_t n, const ccab_c128* d_A, int64_t lda,
+                              const double* h_xyz, int64_t leaf_size, double eta,
+                              double aca_tolerance, int prec, ccab_hmatrix** out);
+/* src/solve/aca.rs:22-33 (ACA::new(tol, m, n, get_row, get_column)) over an
+ * explicit m x n matrix d_A (column-major c128, leading dimension lda,
+ * device; the closures of aca.rs:303-304 read its rows and columns):
+ * partial-pivot ACA with the reference's seeded reference rows
+ * (aca.rs:57-136) and SVD recompression at tol (aca.rs:137-179).  The result
+ * is an m x n ccab_hmatrix holding one low-rank block: ccab_hmatrix_gemv is
+ * ACA::gemv (aca.rs:181-195; x has n entries, y has m), the norm of
+ * ccab_hmatrix_stats is ACA::get_norm (aca.rs:212-241) and st[3] is
+ * ACA::get_num_uv.  A matrix the ACA cannot represent with 64 terms is stored
+ * dense (st[4] = 1).  ccab_hmatrix_to_full writes m x n (lda >= m). */
+int ccab_aca_build(ccab_ctx* ctx, int64_t m, int64_t n, const ccab_c128* d_A, int64_t lda,
+                   double tol, ccab_hmatrix** out);
 /* src/solve/gmres.rs:35-202 with the H matrix as operator (src/solve.rs:119-125).
  * d_x: b on entry, x on exit; d_x0 initial guess or NULL. */
 int ccab_gmres_hmatrix(ccab_ctx* ctx, const ccab_hmatrix* h, ccab_c128* d_x,

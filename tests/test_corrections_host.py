@@ -62,5 +62,5 @@ def test_quad_area_sphere_total():
 def test_physics_struct_carries_corrections():
     assert ct.sizeof(capi.Physics) == 56
     ph = capi.physics(100.0, corrections=capi.FIX_ALL)
-    assert ph.corrections == 31
+    assert ph.corrections == 63
     assert capi.Physics.corrections.offset == 36

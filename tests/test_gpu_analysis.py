@@ -93,9 +93,9 @@ def test_verification_cases_through_analysis(ref_tree, case):
     p = complex(fp[0].scattered[0])
     assert abs(p.real - expect.real) <= eps and abs(p.imag - expect.imag) <= eps
     if case == "direct":
-        assert abs(abs(p) - 0.0004776828778512934) <= 1e-10   # verification.rs:116
+        assert abs(abs(p) - 0.0004776828778512934) <= 1e-10   # verification.rs:117
     if case == "burton_miller":
-        assert abs(abs(p) - 0.0004964814282687255) <= 1e-10   # verification.rs:201
+        assert abs(abs(p) - 0.0004964814282687255) <= 1e-10   # verification.rs:202
 
 
 def test_sweep_vs_oracle_per_frequency(ref_tree):

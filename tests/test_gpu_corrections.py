@@ -174,3 +174,51 @@ def test_quad_area_pulsating_sphere():
     assert errs["tri"] < 0.03
     assert errs["quad, full area"] < 0.03
     assert errs["quad, quarter area"] > 0.2
+
+
+@pytest.mark.parametrize("kind", [capi.SOLVER_DIRECT, capi.SOLVER_HIERARCHICAL])
+def test_pressure_sign(kind):
+    """Dirichlet pulsating sphere (phi = p on r = 1, ka = 1): the physical
+    normal velocity is p (ik - 1/a).  The parity branch (solve.rs:36-43,
+    get_surface_hmatrix's solve.rs:99-104) returns its negative; with
+    FIX_PRESSURE_SIGN the mean vn is within 2 % of it.  With an incident
+    wave as well, the corrected vn solves G vn = rhs_inc + H p (the oracle's
+    H and G, numpy solve)."""
+    import oracle as O
+    m = mesh.icosphere(3)
+    ka = 1.0
+    f = ka * 343.0 / (2 * math.pi)
+    pbar = complex(0.8, -0.3)
+    physical = pbar * (1j * ka - 1.0)
+    solver = Solver(kind=kind, tolerance=1e-10, max_iterations=1000)
+    if kind == capi.SOLVER_HIERARCHICAL:
+        solver.aca_tolerance = 1e-6
+    res = {}
+    for fix in (0, capi.FIX_PRESSURE_SIGN):
+        pd = PreData(m, [f], 343.0, 1.21, capi.CLASSICAL, capi.EXTERIOR, capi.BC_PRESSURE,
+                     (pbar.real, pbar.imag),
+                     IncidentWave(capi.WAVE_PLANE, (1.0, 0.0, 0.0), (0.0, 0.0)), solver,
+                     corrections=fix)
+        phi, vn, _ = solve.sweep(pd)
+        res[fix] = vn[0].cpu().numpy()
+        np.testing.assert_allclose(phi[0].cpu().numpy(), pbar)
+    _note(f"pressure sign correction solver={kind}")
+    e0 = abs(res[0].mean() + physical) / abs(physical)
+    e1 = abs(res[capi.FIX_PRESSURE_SIGN].mean() - physical) / abs(physical)
+    print("parity vs -physical", e0, "corrected vs physical", e1)
+    assert e0 < 0.02 and e1 < 0.02
+    tol = 1e-10 if kind == capi.SOLVER_DIRECT else 1e-4
+    np.testing.assert_allclose(res[capi.FIX_PRESSURE_SIGN], -res[0],
+                               atol=tol * np.abs(res[0]).max())
+    # with an incident plane wave: G vn = rhs_inc + H p
+    w = IncidentWave(capi.WAVE_PLANE, (0.3, -0.4, 0.5), (1.0, 0.5))
+    pd = PreData(m, [f], 343.0, 1.21, capi.CLASSICAL, capi.EXTERIOR, capi.BC_PRESSURE,
+                 (pbar.real, pbar.imag), w, solver, corrections=capi.FIX_PRESSURE_SIGN)
+    _, vn, _ = solve.sweep(pd)
+    ph = O.physics(f, 343.0, 1.21, O.CLASSICAL, O.EXTERIOR, O.BC_PRESSURE, (pbar.real, pbar.imag))
+    H, G = O.dense_surface_matrices(ph, pd.cpts)
+    rhs = O.incident_surface(ph, O.wave(O.WAVE_PLANE, (0.3, -0.4, 0.5), (1.0, 0.5)), pd.cpts)
+    want = np.linalg.solve(G, rhs + H @ np.full(len(rhs), pbar))
+    err = np.linalg.norm(vn[0].cpu().numpy() - want) / np.linalg.norm(want)
+    print("incident + pressure, corrected vs numpy", err)
+    assert err < (1e-10 if kind == capi.SOLVER_DIRECT else 1e-3)

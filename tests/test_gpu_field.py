@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
-P_CLASSICAL = -3.9380091293979903e-05 + 4.7605686656319906e-04j   # verification.rs:116-119
-P_BM = 1.2125591274707629e-04 + 4.814465829556038e-04j            # verification.rs:201-204
+P_CLASSICAL = -3.9380091293979903e-05 + 4.7605686656319906e-04j   # verification.rs:117-119
+P_BM = 1.2125591274707629e-04 + 4.814465829556038e-04j            # verification.rs:202-204
 
 
 def _note(msg):

@@ -1,6 +1,6 @@
 """GPU parity of the hierarchical-matrix path (src/solve/h_matrix.rs, aca.rs,
 h_matrix/cluster.rs, h_matrix/block.rs, solve.rs:74-150) against the oracle
-and the reference's H-matrix fixture (verification.rs:173-175).
+and the reference's H-matrix fixture (verification.rs:174-175).
 
 Both sides take the same ACA pivots (ChaCha12 reference-row draws, seed 10),
 so the GPU blocks must agree with the oracle's to rounding, not only to the
@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLD = os.path.join(ROOT, "tests", "golden")
 FP = np.array([[10.0, 0.0, 0.0]])
-P_HMATRIX = -3.9380308176032198e-05 + 4.7605563272935873e-04j   # verification.rs:173-175
+P_HMATRIX = -3.9380308176032198e-05 + 4.7605563272935873e-04j   # verification.rs:174-175
 
 
 def _note(msg):
@@ -62,7 +62,7 @@ def sphere():
 
 
 def test_hmatrix_fixture_end_to_end(sphere):
-    """verification.rs:173-175: hierarchical solve of the sphere fixture."""
+    """verification.rs:174-175: hierarchical solve of the sphere fixture."""
     _note("H-matrix fixture, sphere.vtk n=324, classical, GMRES tol 1e-5")
     pd = _predata(sphere)
     rhs_inc = incident_wave.get_incident_surface(pd)

@@ -18,9 +18,9 @@ SPHERE = os.path.join(GOLD, "sphere.vtk")
 
 # verification.rs:95-205 common setup
 FP = np.array([[10.0, 0.0, 0.0]])
-P_CLASSICAL = -3.9380091293979903e-05 + 4.7605686656319906e-04j   # :305-307
-P_HMATRIX = -3.9380308176032198e-05 + 4.7605563272935873e-04j     # :362-363
-P_BM = 1.2125591274707629e-04 + 4.814465829556038e-04j            # :390-392
+P_CLASSICAL = -3.9380091293979903e-05 + 4.7605686656319906e-04j   # verification.rs:117-119
+P_HMATRIX = -3.9380308176032198e-05 + 4.7605563272935873e-04j     # verification.rs:174-175
+P_BM = 1.2125591274707629e-04 + 4.814465829556038e-04j            # verification.rs:202-204
 
 
 @pytest.fixture(scope="module")
@@ -265,3 +265,178 @@ def test_bc_algebra_consistency(sphere_cpts):
             assert np.allclose(phi, complex(*val))
             res = G @ vn + H @ phi - rhs
             assert np.linalg.norm(res) < 1e-9 * np.linalg.norm(rhs)
+
+
+# --------------------------------------------------------------------------
+# Closed forms for the impedance branch (solve.rs:57-68), the spherical
+# incident wave (incident_wave.rs:40-62) and the pressure branch
+# (solve.rs:36-43).  The series in oracle/analytic.py are pinned first by
+# the conditions that define them, then the oracle is held against them
+# within the discretisation error of the 1120-triangle sphere (about 2 %).
+# --------------------------------------------------------------------------
+from oracle import analytic as AN  # noqa: E402
+
+REFINED = os.path.join(GOLD, "refined_sphere.vtk")   # radius 0.5, 1120 triangles
+RHO, C0, A_R = 1.21, 343.0, 0.5
+
+
+@pytest.fixture(scope="module")
+def refined_cpts():
+    return O.mesh_cpts_from_vtk(REFINED, 3)
+
+
+def _on_sphere(cp):
+    """Collocation points projected radially onto r = a (the centroids of the
+    faceted sphere sit 0.4 % inside it)."""
+    x = cp[:, :3]
+    return A_R * x / np.linalg.norm(x, axis=1)[:, None]
+
+
+def test_analytic_impedance_series_is_pinned():
+    k, sigma = 2.0, 0.3846 + 1.923j
+    ct_ = np.linspace(-1, 1, 7)
+    hh = 1e-5
+    # d(phi)/dr = sigma phi at r = a (central difference)
+    d = (AN.impedance_sphere_total(A_R, k, sigma, A_R + hh, ct_)
+         - AN.impedance_sphere_total(A_R, k, sigma, A_R - hh, ct_)) / (2 * hh)
+    phi = AN.impedance_sphere_total(A_R, k, sigma, A_R, ct_)
+    np.testing.assert_allclose(d, sigma * phi, rtol=1e-7, atol=1e-9)
+    # sigma = 0 is the rigid sphere: incident minus the reference-signed
+    # scattered series of the module's first function
+    r = 0.9
+    rigid = np.exp(1j * k * r * ct_) - AN.rigid_sphere_scattered(A_R, k, r, ct_)
+    np.testing.assert_allclose(AN.impedance_sphere_total(A_R, k, 0.0, r, ct_), rigid,
+                               rtol=1e-12, atol=1e-12)
+    # the truncated series has converged
+    np.testing.assert_allclose(AN.impedance_sphere_total(A_R, k, sigma, r, ct_, nterms=40),
+                               AN.impedance_sphere_total(A_R, k, sigma, r, ct_, nterms=60),
+                               rtol=1e-13, atol=1e-13)
+
+
+def test_analytic_point_source_series_is_pinned():
+    k, rs = 3.0, 1.0
+    p = np.array([[0.5, 0.1, -0.2], [0.0, 0.45, 0.2], [-0.3, -0.3, 0.3]])
+    s = np.array([0.0, 0.0, rs])
+    R = np.linalg.norm(p - s, axis=1)
+    r = np.linalg.norm(p, axis=1)
+    direct = np.exp(1j * k * R) / (4 * np.pi * R)
+    np.testing.assert_allclose(AN.point_source_series(k, rs, r, p[:, 2] / r), direct,
+                               rtol=1e-10)
+    # rigid surface: the radial derivative of the total field vanishes at r = a
+    u = p / r[:, None]
+    hh = 1e-5
+    dr = (AN.point_source_rigid_sphere_total(A_R, k, s, (A_R + hh) * u)
+          - AN.point_source_rigid_sphere_total(A_R, k, s, (A_R - hh) * u)) / (2 * hh)
+    scale = np.abs(AN.point_source_rigid_sphere_total(A_R, k, s, A_R * u)).max() * k
+    assert np.abs(dr).max() < 1e-7 * scale
+
+
+@pytest.mark.parametrize("ka", [0.5, 1.0, 2.0])
+def test_impedance_branch_vs_impedance_sphere(refined_cpts, ka):
+    """Classical formulation, impedance Z = rho c (1 + 0.2i), plane wave:
+    the oracle's phi against the impedance-sphere series with
+    sigma = i omega rho / Z; the opposite sign of the f G term is 60-130 %
+    off, so a sign slip in solve.rs:57-68's restatement fails here."""
+    cp = refined_cpts
+    k = ka / A_R
+    f = k * C0 / (2 * math.pi)
+    omega = 2 * math.pi * f
+    Z = complex(RHO * C0, 0.2 * RHO * C0)
+    sigma = 1j * omega * RHO / Z
+    d = np.array([0.3, 0.4, 0.5])
+    ph = O.physics(f, C0, RHO, O.CLASSICAL, O.EXTERIOR, O.BC_IMPEDANCE, (Z.real, Z.imag))
+    w = O.wave(O.WAVE_PLANE, tuple(d), (1.0, 0.0))
+    phi, vn, _ = O.solve_surface_dense(ph, cp, O.incident_surface(ph, w, cp))
+    amp = 1.0 / (1j * omega * RHO)
+    x = _on_sphere(cp)
+    exact = AN.impedance_sphere_total_at(x, A_R, k, sigma, d, amp)
+    err = np.linalg.norm(phi - exact) / np.linalg.norm(exact)
+    assert err < 0.03, err
+    wrong = AN.impedance_sphere_total_at(x, A_R, k, -sigma, d, amp)
+    assert np.linalg.norm(phi - wrong) / np.linalg.norm(wrong) > 0.5
+    # the returned vn is f phi = -sigma phi: the opposite of the normal
+    # velocity the solved system implies (the reference's quirk, solve.rs:67)
+    np.testing.assert_allclose(vn, -sigma * phi, rtol=1e-13)
+    assert np.linalg.norm(vn + sigma * exact) / np.linalg.norm(sigma * exact) < 0.03
+
+
+@pytest.mark.parametrize("ka", [0.5, 1.0, 2.0])
+def test_spherical_wave_vs_point_source_series(refined_cpts, ka):
+    """Classical, rigid sphere of radius 0.5 in the field of the spherical
+    wave: the oracle moves the source to origin/|origin| (incident_wave.rs:44),
+    here a point outside the body, so the total surface potential is the
+    point-source rigid-sphere series."""
+    cp = refined_cpts
+    k = ka / A_R
+    f = k * C0 / (2 * math.pi)
+    omega = 2 * math.pi * f
+    origin = np.array([0.2, -0.6, 0.5])
+    ph = O.physics(f, C0, RHO, O.CLASSICAL, O.EXTERIOR, O.BC_NORMAL_VELOCITY, (0.0, 0.0))
+    w = O.wave(O.WAVE_SPHERICAL, tuple(origin), (0.7, -0.3))
+    phi, _, _ = O.solve_surface_dense(ph, cp, O.incident_surface(ph, w, cp))
+    amp = complex(0.7, -0.3) / (1j * omega * RHO)
+    exact = AN.point_source_rigid_sphere_total(A_R, k, origin / np.linalg.norm(origin),
+                                               _on_sphere(cp), amp)
+    err = np.linalg.norm(phi - exact) / np.linalg.norm(exact)
+    assert err < 0.03, err
+
+
+@pytest.mark.parametrize("origin", [(0.2, -0.6, 0.5), (0.05, 0.01, -0.02)])
+def test_spherical_wave_bm_term_finite_difference(refined_cpts, origin):
+    """Burton-Miller right-hand side of the spherical wave
+    (incident_wave.rs:53-58): (rhs_BM - rhs_classical) / beta, beta = i/k,
+    equals the normal derivative of the classical phi_I at each collocation
+    point, taken by central differences along the normal."""
+    cp = refined_cpts
+    f = 300.0
+    k = 2 * math.pi * f / C0
+    w = O.wave(O.WAVE_SPHERICAL, origin, (0.7, -0.3))
+    phC = O.physics(f, C0, RHO, O.CLASSICAL)
+    phB = O.physics(f, C0, RHO, O.BURTON_MILLER)
+    term = (O.incident_surface(phB, w, cp) - O.incident_surface(phC, w, cp)) / (1j / k)
+    hh = 1e-5
+    cpp, cpm = cp.copy(), cp.copy()
+    cpp[:, :3] += hh * cp[:, 3:6]
+    cpm[:, :3] -= hh * cp[:, 3:6]
+    fd = (O.incident_surface(phC, w, cpp) - O.incident_surface(phC, w, cpm)) / (2 * hh)
+    assert np.linalg.norm(term - fd) / np.linalg.norm(fd) < 1e-7
+
+
+def test_plane_wave_bm_term_finite_difference(refined_cpts):
+    """The same check for the plane wave's Burton-Miller term
+    (incident_wave.rs:30-36)."""
+    cp = refined_cpts
+    f = 300.0
+    k = 2 * math.pi * f / C0
+    w = O.wave(O.WAVE_PLANE, (0.3, -0.4, 0.5), (0.7, -0.3))
+    phC = O.physics(f, C0, RHO, O.CLASSICAL)
+    phB = O.physics(f, C0, RHO, O.BURTON_MILLER)
+    term = (O.incident_surface(phB, w, cp) - O.incident_surface(phC, w, cp)) / (1j / k)
+    hh = 1e-5
+    cpp, cpm = cp.copy(), cp.copy()
+    cpp[:, :3] += hh * cp[:, 3:6]
+    cpm[:, :3] -= hh * cp[:, 3:6]
+    fd = (O.incident_surface(phC, w, cpp) - O.incident_surface(phC, w, cpm)) / (2 * hh)
+    assert np.linalg.norm(term - fd) / np.linalg.norm(fd) < 1e-7
+
+
+@pytest.mark.parametrize("ka", [0.5, 1.0])
+def test_pressure_branch_sign_quirk(refined_cpts, ka):
+    """Dirichlet pulsating sphere: phi = p_bar on r = a, outgoing field, so
+    the physical normal velocity is d(phi)/dr = p_bar (ik - 1/a).  The
+    reference's pressure branch solves G vn = rhs_inc - H p_bar
+    (solve.rs:36-43) where its velocity branch (solve.rs:46-52) implies
+    G vn = rhs_inc + H p_bar, so its vn comes out with the opposite sign:
+    mean vn = -p_bar (ik - 1/a) within the discretisation error."""
+    cp = refined_cpts
+    k = ka / A_R
+    f = k * C0 / (2 * math.pi)
+    pbar = complex(0.8, -0.3)
+    ph = O.physics(f, C0, RHO, O.CLASSICAL, O.EXTERIOR, O.BC_PRESSURE, (pbar.real, pbar.imag))
+    phi, vn, _ = O.solve_surface_dense(ph, cp, np.zeros(cp.shape[0], complex))
+    physical = pbar * (1j * k - 1.0 / A_R)
+    assert np.allclose(phi, pbar)
+    # (pointwise vn scatters by 20 % about its mean: G, the single-layer
+    # operator with one-point quadrature, is a first-kind equation)
+    assert abs(vn.mean() + physical) / abs(physical) < 0.02
+    assert abs(vn.mean() - physical) / abs(physical) > 1.9
