@@ -164,6 +164,7 @@ SIGNATURES = {
     "ccab_aca_build": (_I, [_P, _I64, _I64, _P, _I64, ct.c_double, ct.POINTER(_P)]),
     "ccab_gmres_hmatrix": (_I, [_P, _P, _P, _P, ct.POINTER(GmresParams), _P]),
     "ccab_field": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
+    "ccab_field_noalloc": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
     "ccab_dense_field_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _P, _I64, _P, _P, _I64]),
     "ccab_incident_field": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
     "ccab_surrogate_create": (_I, [_P, ct.POINTER(SurrogateCfg), ct.POINTER(_P)]),

@@ -382,6 +382,7 @@ int ccab_create(int device, ccab_ctx** out) {
     // room for one, their paths stay off
     if (!mgs_coop_fits()) ctx->mgs_coop = false;
     if (!lu_coop_fits()) ctx->lu_coop_rows = 0;
+    field_scratch_init(device);
     *out = ctx;
     return CCAB_OK;
 }

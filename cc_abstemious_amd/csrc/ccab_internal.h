@@ -403,6 +403,9 @@ int coop_abort_check(ccab_ctx* ctx);
 // beside nothing else: checked once per context, the paths are off otherwise
 bool mgs_coop_fits();
 bool lu_coop_fits();
+// field.hip: the device address of ccab_field_noalloc's scratch array, looked
+// up once per device (the device must be current)
+void field_scratch_init(int device);
 
 // hmatrix.hip
 int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const double* h_cpts,

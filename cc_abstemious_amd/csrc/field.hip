@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ccab_internal.h"
@@ -129,9 +130,22 @@ extern "C" int ccab_dense_field_matrices(ccab_ctx* ctx, const ccab_physics* ph, 
 
 namespace {
 
-// per-frequency constants on the device
-int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
-                    bool to_pressure, bool need_inc, FieldSys** dsys) {
+// Surface chunks of the field evaluation (grid.y): enough wavefronts to fill
+// the chip, at most max_chunks; *chunk = surface points per chunk.
+int field_chunks(const ccab_ctx* ctx, int64_t n, int64_t nfp, int64_t nfreq, int64_t max_chunks,
+                 int64_t* chunk) {
+    const int64_t pblocks = (nfp + kFieldWave - 1) / kFieldWave;
+    const int64_t want_waves = 8 * std::max(ctx->num_cus, 1) * 4;
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(
+        (n + 127) / 128, (want_waves + pblocks * nfreq - 1) / (pblocks * nfreq)));
+    nchunk = std::max<int64_t>(1, std::min(nchunk, max_chunks));
+    *chunk = (n + nchunk - 1) / nchunk;
+    return (int)((n + *chunk - 1) / *chunk);
+}
+
+// per-frequency constants on the host (hs[nfreq])
+int field_sys_host(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                   bool to_pressure, bool need_inc, FieldSys* hs) {
     if (w) CCAB_CHECK(w->kind == CCAB_WAVE_PLANE || w->kind == CCAB_WAVE_SPHERICAL, "bad wave kind");
     CCAB_CHECK(!need_inc || w, "the incident field needs a wave");
     double d[3] = {0.0, 0.0, 0.0};
@@ -146,7 +160,6 @@ int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
                           (ph[0].corrections & CCAB_FIX_SPHERICAL_ORIGIN);
         for (int t = 0; t < 3; ++t) d[t] = keep ? w->vec[t] : (nn > 0.0 ? w->vec[t] / nn : 0.0);
     }
-    std::vector<FieldSys> hs(nfreq);
     for (int64_t f = 0; f < nfreq; ++f) {
         CCAB_TRY(check_physics(ctx, ph[f]));
         const SysParams p = make_params(ph[f]);
@@ -158,6 +171,14 @@ int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const 
         for (int t = 0; t < 3; ++t) q.d[t] = d[t];
         q.msign = (ph[f].corrections & CCAB_FIX_FIELD_SIGN) ? -1.0 : 1.0;
     }
+    return CCAB_OK;
+}
+
+// per-frequency constants on the device
+int stage_field_sys(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                    bool to_pressure, bool need_inc, FieldSys** dsys) {
+    std::vector<FieldSys> hs(nfreq);
+    CCAB_TRY(field_sys_host(ctx, ph, nfreq, w, to_pressure, need_inc, hs.data()));
     *dsys = static_cast<FieldSys*>(ctx_buf(ctx, SLOT_FIELD_SYS, sizeof(FieldSys) * nfreq));
     if (!*dsys) return CCAB_ENOMEM;
     CCAB_TRY(pinned_reserve(ctx, sizeof(FieldSys) * nfreq));
@@ -195,13 +216,9 @@ extern "C" int ccab_field(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, 
     double* soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(n) * 64));
     if (!soa) return CCAB_ENOMEM;
     CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts, n, soa));
-    // enough wavefronts to fill the chip: split the surface into chunks
     const int64_t pblocks = (nfp + kFieldWave - 1) / kFieldWave;
-    const int64_t want_waves = 8 * std::max(ctx->num_cus, 1) * 4;
-    int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(
-        (n + 127) / 128, (want_waves + pblocks * nfreq - 1) / (pblocks * nfreq)));
-    const int64_t chunk = (n + nchunk - 1) / nchunk;
-    nchunk = (int)((n + chunk - 1) / chunk);
+    int64_t chunk;
+    const int nchunk = field_chunks(ctx, n, nfp, nfreq, INT32_MAX, &chunk);
     c128* part = static_cast<c128*>(
         ctx_buf(ctx, SLOT_FIELD_PART, size_t(nfreq) * nchunk * nfp * sizeof(c128)));
     if (!part) return CCAB_ENOMEM;
@@ -215,5 +232,99 @@ extern "C" int ccab_field(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, 
                            part, want_total ? 1 : 0, need_inc ? 1 : 0, d_out, d_inc);
     }
     CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ccab_field_noalloc: ccab_field with every scratch buffer (the per-frequency
+// constants, the SoA copy of the surface points and the per-chunk partial
+// sums) in one array of the code object, so that a call allocates nothing.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kFieldScratchBytes = size_t(32) << 20;   // per device
+constexpr int64_t kFieldNoAllocMaxFreq = 256;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kFieldScratchBytes];
+
+// the array's device address on every device, from hipGetSymbolAddress
+// (the host-side name `scratch` is not a device address)
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+// Once per device, with that device current (ccab_create calls it).
+void field_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
+extern "C" int ccab_field_noalloc(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq,
+                                  const ccab_wave* w, const double* d_cpts, int64_t n,
+                                  const double* d_fps, int64_t nfp, const ccab_c128* d_phi,
+                                  const ccab_c128* d_vn, int want_total, int to_pressure,
+                                  ccab_c128* d_out, ccab_c128* d_inc) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(ph && d_cpts && d_fps && d_phi && d_vn && d_out && n > 0 && nfp > 0 && nfreq > 0,
+               "invalid arguments");
+    CCAB_CHECK(nfreq <= kFieldNoAllocMaxFreq, "ccab_field_noalloc: at most %lld frequencies",
+               (long long)kFieldNoAllocMaxFreq);
+    const int dev = ctx->device;
+    CCAB_CHECK(dev >= 0 && dev < kMaxDevices, "ccab_field_noalloc: device %d out of range", dev);
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_done[dev] || !g_scratch_ptr[dev])
+            return ctx_fail(ctx, CCAB_EHIP, "ccab_field_noalloc: no scratch address on device %d (%s)",
+                            dev, g_scratch_done[dev] ? hipGetErrorString(g_scratch_err[dev])
+                                                     : "not initialised");
+        base = g_scratch_ptr[dev];
+    }
+    // layout: [FieldSys x nfreq][SoA, 8 n doubles][partials, nfreq x nchunk x nfp c128]
+    const size_t sys_bytes = align256(sizeof(FieldSys) * size_t(nfreq));
+    const size_t soa_bytes = align256(size_t(n) * 64);
+    const size_t part_unit = size_t(nfreq) * size_t(nfp) * sizeof(c128);  // one chunk
+    if (sys_bytes + soa_bytes + part_unit > kFieldScratchBytes)
+        return ctx_fail(ctx, CCAB_EINVAL,
+                        "ccab_field_noalloc: n = %lld surface points, %lld field points and %lld "
+                        "frequencies need more than its %zu MiB scratch (8 n doubles + 16 nfp "
+                        "nfreq bytes per surface chunk); use ccab_field",
+                        (long long)n, (long long)nfp, (long long)nfreq, kFieldScratchBytes >> 20);
+    const int64_t max_chunks = int64_t((kFieldScratchBytes - sys_bytes - soa_bytes) / part_unit);
+    const bool need_inc = want_total || d_inc;
+    FieldSys hs[kFieldNoAllocMaxFreq];
+    CCAB_TRY(field_sys_host(ctx, ph, nfreq, w, to_pressure != 0, need_inc, hs));
+    unsigned char* b = static_cast<unsigned char*>(base);
+    FieldSys* dsys = reinterpret_cast<FieldSys*>(b);
+    double* soa = reinterpret_cast<double*>(b + sys_bytes);
+    c128* part = reinterpret_cast<c128*>(b + sys_bytes + soa_bytes);
+    CCAB_HIP(hipMemcpyAsync(dsys, hs, sizeof(FieldSys) * size_t(nfreq), hipMemcpyHostToDevice,
+                            ctx->stream));
+    CCAB_TRY(launch_cpts_to_soa(ctx, d_cpts, n, soa));
+    const int64_t pblocks = (nfp + kFieldWave - 1) / kFieldWave;
+    int64_t chunk;
+    const int nchunk = field_chunks(ctx, n, nfp, nfreq, max_chunks, &chunk);
+    // every partial slot of the nchunk chunks is written before it is read
+    hipLaunchKernelGGL(field_partial_kernel, dim3((unsigned)pblocks, (unsigned)nchunk, (unsigned)nfreq),
+                       dim3(kFieldWave), 0, ctx->stream, dsys, soa, n, d_fps, nfp, d_phi, d_vn,
+                       chunk, nchunk, part);
+    hipLaunchKernelGGL(field_finish_kernel, dim3((unsigned)((nfp + 255) / 256), (unsigned)nfreq),
+                       dim3(256), 0, ctx->stream, dsys, w ? w->kind : 0, d_fps, nfp, nchunk,
+                       part, want_total ? 1 : 0, need_inc ? 1 : 0, d_out, d_inc);
+    CCAB_HIP(hipGetLastError());
+    // the constants were copied from this stack frame
+    CCAB_HIP(hipStreamSynchronize(ctx->stream));
     return CCAB_OK;
 }

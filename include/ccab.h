@@ -299,6 +299,22 @@ int ccab_field(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_
                const double* d_cpts, int64_t n, const double* d_fps, int64_t nfp,
                const ccab_c128* d_phi, const ccab_c128* d_vn, int want_total, int to_pressure,
                ccab_c128* d_out, ccab_c128* d_inc);
+/* ccab_field without allocation (no reference counterpart): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results as
+ * ccab_field; its scratch (per-frequency constants, a copy of the surface
+ * points, per-chunk partial sums) lives in one 32 MiB array per device in
+ * the library's code object instead of context buffers.  Limits: nfreq <=
+ * 256 and 8 n doubles + 16 nfp nfreq bytes (one surface chunk) within the
+ * 32 MiB, else CCAB_EINVAL; fewer surface chunks than ccab_field may be used
+ * when the partial sums would not fit (same sums, other chunking).  Every
+ * call on a device shares that device's array: calls that overlap on one GPU
+ * (two contexts or streams) race.  The call waits for its stream before it
+ * returns (the constants are staged from the caller's stack). */
+int ccab_field_noalloc(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, const ccab_wave* w,
+                       const double* d_cpts, int64_t n, const double* d_fps, int64_t nfp,
+                       const ccab_c128* d_phi, const ccab_c128* d_vn, int want_total,
+                       int to_pressure, ccab_c128* d_out, ccab_c128* d_inc);
 /* incident_wave.rs:66-100 (get_incident_field): the incident potential at
  * nfp field points (d_out[nfp]) */
 int ccab_incident_field(ccab_ctx* ctx, const ccab_physics* ph, const ccab_wave* w,
