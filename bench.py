@@ -86,30 +86,56 @@ def parse():
     return ap.parse_args()
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(cpts, ph_args, lu_n):
     """Oracle (C restatement of the reference) on the host: full assembly of
     one frequency of the workload on max(nproc/2, 2) threads as the reference
-    (preprocess.rs:12-17), plus the reference's unblocked single-thread LU
-    timed at lu_n and scaled by (n/lu_n)^3."""
+    (preprocess.rs:12-17), with the reference's two mutex locks per entry
+    (influence_matrix.rs:87-90; the value) and without them (reported beside
+    it), plus the reference's unblocked single-thread LU timed at lu_n and
+    scaled by (n/lu_n)^3."""
     import oracle as O
     n = cpts.shape[0]
-    threads = max(os.cpu_count() // 2, 2)
+    nproc = os.cpu_count() or 1
+    threads = max(nproc // 2, 2)
     ph = O.physics(*ph_args)
+    # the locked assembly costs the same per entry at any size (lock-bound):
+    # timed on the first lu_n points and scaled by (n / lu_n)^2 entries
+    ns = min(n, lu_n)
+    t0 = time.perf_counter()
+    O.dense_surface_matrices(ph, np.ascontiguousarray(cpts[:ns]), nthreads=threads, locked=True)
+    t_locked = (time.perf_counter() - t0) * (n / ns) ** 2
     t0 = time.perf_counter()
     O.dense_surface_matrices(ph, cpts, nthreads=threads, locked=False)
-    t_asm = time.perf_counter() - t0
+    t_free = time.perf_counter() - t0
     rng = np.random.default_rng(0)
     a = np.asfortranarray(np.eye(lu_n) * 4 + rng.standard_normal((lu_n, lu_n)) * 0.1 + 0j)
     b = np.ones(lu_n, complex)
     t0 = time.perf_counter()
     O.lu_solve(a, b)
     t_lu = (time.perf_counter() - t0) * (n / lu_n) ** 3
-    per_solve = t_asm + t_lu
-    return {"value": 1.0 / per_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+    return {"value": 1.0 / (t_locked + t_lu), "unit": "solves/s", "cores": threads,
+            "kind": "port", "nproc": nproc, "threads_assembly": threads, "threads_lu": 1,
+            "cpu_model": _cpu_model(),
+            "lock_free_assembly_solves_per_s": 1.0 / (t_free + t_lu),
+            "assembly_s_locked": t_locked, "assembly_s_lock_free": t_free,
+            "lu_s_extrapolated": t_lu,
             "sample": (f"one frequency of the workload: oracle FP64 assembly of the full "
-                       f"n={n} H,G on {threads} threads ({t_asm:.2f} s) + oracle unblocked "
-                       f"partial-pivot LU timed at n={lu_n} and scaled by (n/{lu_n})^3 "
-                       f"({t_lu:.1f} s)")}
+                       f"n={n} H,G on {threads} of {nproc} host threads with the reference's "
+                       f"two locks per entry ({t_locked:.2f} s, timed on {ns} points and scaled "
+                       f"by (n/{ns})^2; {t_free:.2f} s without locks, full n) + "
+                       f"oracle unblocked partial-pivot LU on 1 thread timed at n={lu_n} and "
+                       f"scaled by (n/{lu_n})^3 ({t_lu:.1f} s, extrapolated)")}
 
 
 def bodies_block(args, ctx):

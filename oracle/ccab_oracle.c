@@ -183,7 +183,8 @@ typedef struct {
     cplx* H;
     cplx* G;
     int locked;
-    pthread_mutex_t* mtx;
+    pthread_mutex_t* mtx;   /* H, then G: the two locks of influence_matrix.rs:87-88 */
+    pthread_mutex_t* mtx_g;
     int64_t next_row;
     pthread_mutex_t* qmtx;
 } asm_job;
@@ -198,10 +199,16 @@ static void* asm_worker(void* arg) {
         for (int64_t j = 0; j < jb->n; ++j) {
             cplx g, h;
             gh_entry(jb->k, jb->bm, jb->beta, jb->cpts, i, j, &g, &h);
-            if (jb->locked) pthread_mutex_lock(jb->mtx);
+            if (jb->locked) {
+                pthread_mutex_lock(jb->mtx);
+                pthread_mutex_lock(jb->mtx_g);
+            }
             jb->H[i + j * jb->n] += h;
             jb->G[i + j * jb->n] += g;
-            if (jb->locked) pthread_mutex_unlock(jb->mtx);
+            if (jb->locked) {
+                pthread_mutex_unlock(jb->mtx_g);
+                pthread_mutex_unlock(jb->mtx);
+            }
         }
     }
     return NULL;
@@ -226,9 +233,10 @@ int orc_dense_surface_matrices(const orc_physics* ph, const double* cpts, int64_
         G[i + i * n] = *C(&gd);
     }
     if (nthreads <= 0) nthreads = default_threads();
-    pthread_mutex_t mtx = PTHREAD_MUTEX_INITIALIZER, qmtx = PTHREAD_MUTEX_INITIALIZER;
+    pthread_mutex_t mtx = PTHREAD_MUTEX_INITIALIZER, mtx_g = PTHREAD_MUTEX_INITIALIZER,
+                    qmtx = PTHREAD_MUTEX_INITIALIZER;
     asm_job jb = {k, ph->method == ORC_BURTON_MILLER, *C(&beta), cpts, n, H, G, locked,
-                  &mtx, 0, &qmtx};
+                  &mtx, &mtx_g, 0, &qmtx};
     if (nthreads == 1) {
         asm_worker(&jb);
         return 0;
