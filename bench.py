@@ -81,6 +81,9 @@ def parse():
                     help="1: also time the H-matrix case (n = 81920, and 20480 vs dense)")
     ap.add_argument("--bodies", type=int, default=256,
                     help="uneven bodies of the batched-bodies line (0 skips it)")
+    ap.add_argument("--tune-sweep", type=int, default=1,
+                    help="1: run the sweep autotuner (assembly split, GEMV workgroup target and "
+                         "loads) on the workload before timing (untimed)")
     ap.add_argument("--train-dirs", type=int, default=12)
     ap.add_argument("--train-steps", type=int, default=6000)
     return ap.parse_args()
@@ -436,6 +439,14 @@ def main():
                       matrix_prec=capi.C32, residual_matrix=res_mat)
     pd.solver = headline
 
+    sweep_tuning = None
+    if args.tune_sweep:
+        sweep_tuning = {}
+        for prec, name in ((capi.C32, "fp16_operator"), (capi.C64, "c64_operator")):
+            best, _ = ctx.autotune_sweep(pd, prec)
+            sweep_tuning[name] = best
+        torch.cuda.empty_cache()
+
     for _ in range(args.warmup):
         phi, vn, st = solve.sweep(pd, ctx=ctx)
     torch.cuda.synchronize()
@@ -609,6 +620,7 @@ def main():
         "c64_operator_solves_per_s": c64_rate,
         "c64_operator_gmres_iters": iter_stats(st_c64, 50, 1e-5, "c64"),
         "lu_tuning": tuning,
+        "sweep_tuning": sweep_tuning,
         "assembly_gentries_per_s": gentries,
         "assembly_store_gbs": asm_gbs,
         "gmres_iters": gmres_iters,

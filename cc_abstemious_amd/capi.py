@@ -164,6 +164,8 @@ SIGNATURES = {
     "ccab_aca_build": (_I, [_P, _I64, _I64, _P, _I64, ct.c_double, ct.POINTER(_P)]),
     "ccab_gmres_hmatrix": (_I, [_P, _P, _P, _P, ct.POINTER(GmresParams), _P]),
     "ccab_field": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
+    "ccab_autotune_sweep": (_I, [_P, _P, _I64, _P, _I64, _I, _P, _P]),
+    "ccab_set_sweep_tuning": (_I, [_P, _I, _I64, _P]),
     "ccab_field_noalloc": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I, _I, _P, _P]),
     "ccab_dense_field_matrices": (_I, [_P, ct.POINTER(Physics), _P, _I64, _P, _I64, _P, _P, _I64]),
     "ccab_incident_field": (_I, [_P, ct.POINTER(Physics), ct.POINTER(Wave), _P, _I64, _P]),
@@ -294,6 +296,32 @@ class Context:
         self.call("ccab_autotune", int(prec), int(n), int(batch), ct.byref(v), ct.byref(o), tab)
         times = {(64 if i < 8 else 128, i % 8): tab[i] for i in range(16)}
         return v.value, o.value, times
+
+    def autotune_sweep(self, predata, op_prec: int = C32, frequencies=None):
+        """Tune the batched assembly and GEMVs of predata's sweep (its mesh and
+        frequencies); returns (best settings dict, timing table dict)."""
+        fs = predata.get_frequencies() if frequencies is None else list(frequencies)
+        best = (ct.c_int * 6)()
+        tab = (ct.c_double * 29)()
+        self.call("ccab_autotune_sweep", predata.physics_array(fs), len(fs),
+                  ct.c_void_p(predata.d_cpts.data_ptr()), predata.get_num_eqn(), int(op_prec),
+                  best, tab)
+        keys = ["systems_per_thread", "column_chunk", "op_wg_target", "op_nt", "ref_wg_target",
+                "ref_nt"]
+        t = list(tab)
+        table = {"assembly_ms": {(kf, cc): t[a * 3 + c] for a, kf in enumerate((8, 16, 32))
+                                 for c, cc in enumerate((64, 128, 256))},
+                 "op_gemv_ms": {(wg, nt): t[9 + w * 2 + nt] for w, wg in
+                                enumerate((256, 512, 1024, 2048, 4096)) for nt in (0, 1)},
+                 "ref_gemv_ms": {(wg, nt): t[19 + w * 2 + nt] for w, wg in
+                                 enumerate((256, 512, 1024, 2048, 4096)) for nt in (0, 1)}}
+        return dict(zip(keys, list(best))), table
+
+    def set_sweep_tuning(self, op_prec: int, n: int, cfg=None):
+        """Force the sweep tuning of size n (6 ints as autotune_sweep's best, in
+        its key order) or drop it (None)."""
+        arr = (ct.c_int * 6)(*cfg) if cfg is not None else None
+        self.call("ccab_set_sweep_tuning", int(op_prec), int(n), arr)
 
     def set_graphs(self, on: bool):
         """Replay captured HIP graphs of the LU factorisation."""

@@ -156,10 +156,11 @@ __device__ __forceinline__ void store_lhs(const SysParams& p, int64_t off, c128 
     }
 }
 
-// ASM_LHS assembly of a frequency sweep over one mesh, FP64 arithmetic, kF
-// systems per thread: the pair geometry (r, 1/r, the normal projections) is
-// computed once for the kF systems (16: measured 6.6 ms at 8, 5.7 at 16, 5.7
-// at 32 for 64 frequencies at n = 5120).  With equally spaced wavenumbers (step
+// ASM_LHS assembly of a frequency sweep over one mesh, FP64 arithmetic, KF
+// systems per thread and cc columns per workgroup (defaults 16 and 64, or
+// the tuned values of ccab_autotune_sweep): the pair geometry (r, 1/r, the
+// normal projections) is computed once for the KF systems (measured 6.6 ms
+// at 8, 5.7 at 16, 5.7 at 32 for 64 frequencies at n = 5120).  With equally spaced wavenumbers (step
 // dk != 0) the phasor e^{i k r} comes from sincos for the first system of the
 // group and from e^{i k_f r} = e^{i k_(f-1) r} e^{i dk r} for the others (two
 // sincos per kF entries instead of kF); otherwise from sincos per system.
@@ -169,17 +170,17 @@ __device__ __forceinline__ void store_lhs(const SysParams& p, int64_t off, c128 
 // the FP64 matrix to about 2^-35 relative per entry).
 constexpr int kF = 16;
 
-template <bool BM, typename OutT>
+template <bool BM, typename OutT, int KF>
 __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     const SysParams* __restrict__ params, const double* __restrict__ soa_base, int64_t n,
     int64_t batch, void* out0, void* out1, void* out2, int64_t ld, int64_t stride,
-    double* __restrict__ rowsum_part, int64_t nchunks, double dk) {
-    const int64_t b0 = int64_t(blockIdx.z) * kF;
-    const int nf = (int)min<int64_t>(kF, batch - b0);
+    double* __restrict__ rowsum_part, int64_t nchunks, double dk, int cc) {
+    const int64_t b0 = int64_t(blockIdx.z) * KF;
+    const int nf = (int)min<int64_t>(KF, batch - b0);
     const Soa s = soa_of(soa_base, n);
     const int64_t i = int64_t(blockIdx.x) * kRows + threadIdx.x;
-    const int64_t j0 = int64_t(blockIdx.y) * kColChunk;
-    const int64_t j1 = min(j0 + kColChunk, n);
+    const int64_t j0 = int64_t(blockIdx.y) * cc;
+    const int64_t j1 = min(j0 + cc, n);
     const bool active = i < n;
     const int64_t ic = active ? i : n - 1;
     const double yx = s.x[ic], yy = s.y[ic], yz = s.z[ic];
@@ -191,7 +192,7 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     extern __shared__ double2 rs_lds[];
     auto rs = [&](int f) -> double2& { return rs_lds[f * kRows + threadIdx.x]; };
     if (rowsum_part)
-        for (int f = 0; f < kF; ++f) rs(f) = double2{0.0, 0.0};
+        for (int f = 0; f < KF; ++f) rs(f) = double2{0.0, 0.0};
     for (int64_t j = j0; j < j1; ++j) {
         const double wa = s.wt[j] * s.area[j];
         const bool diag = j == ic;
@@ -654,16 +655,32 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
                                std::fabs(h_params[b].beta.im * h_params[b].k - 1.0) < 1e-14),
                        "Burton-Miller coupling beta = i/k expected");
         }
-        dim3 g2(grid.x, grid.y, (unsigned)((batch + kF - 1) / kF));
-        // row sums in dynamic LDS, 16 B per frequency and lane (64 KiB)
-        const size_t lds = d_rowsum_part ? size_t(kF) * kRows * sizeof(double2) : 0;
-        if (prec_compute == PREC_F64_H) {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-        } else {
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nchunks, dk);
-        }
+        // tuned systems per thread and column chunk (ccab_autotune_sweep); a
+        // chunk is never narrower than kColChunk, so callers size the row-sum
+        // partials for (n + kColChunk - 1) / kColChunk chunks
+        const AsmTune* tn = asm_tune_of(ctx, prec_compute == PREC_F64_H ? CCAB_C32 : CCAB_C64, n);
+        // (32 systems with row sums would need 128 KiB of dynamic LDS: 16 then)
+        const int kf = tn ? (tn->kf == 32 && d_rowsum_part ? 16 : tn->kf) : kF;
+        const int cc = tn ? std::max(tn->col_chunk, kColChunk) : kColChunk;
+        const int64_t nch = (n + cc - 1) / cc;
+        if (nchunks_out) *nchunks_out = nch;
+        dim3 g2(grid.x, (unsigned)nch, (unsigned)((batch + kf - 1) / kf));
+        // row sums in dynamic LDS, 16 B per frequency and lane (64 KiB at kf = 16)
+        const size_t lds = d_rowsum_part ? size_t(kf) * kRows * sizeof(double2) : 0;
+#define CCAB_ASM_SWEEP_LAUNCH(KF)                                                                  \
+    do {                                                                                           \
+        if (prec_compute == PREC_F64_H) {                                                          \
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+        } else {                                                                                   \
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+        }                                                                                          \
+    } while (0)
+        if (kf == 8) CCAB_ASM_SWEEP_LAUNCH(8);
+        else if (kf == 32) CCAB_ASM_SWEEP_LAUNCH(32);
+        else CCAB_ASM_SWEEP_LAUNCH(16);
+#undef CCAB_ASM_SWEEP_LAUNCH
     } else if (mode == ASM_HG) {
         if (prec_compute == PREC_F64) {
             if (bm) launch_one<double, true, ASM_HG, c128>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, nullptr, nchunks);

@@ -406,7 +406,10 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     // ragged batches: finer column chunks balance bodies of uneven size
     // (measured: 4x the uniform target, 4.0 -> 4.7 TB/s on 256 bodies of
     // 1280-5120 unknowns)
-    const int64_t target = ctx->gemv_wg_target * (rg ? 4 : 1);
+    // uniform products of a tuned size (ccab_autotune_sweep) take its
+    // workgroup target and load kind
+    const GemvTune* tn = (!rg && !d_Alo) ? gemv_tune_of(ctx, prec, N) : nullptr;
+    const int64_t target = tn ? tn->wg_target : ctx->gemv_wg_target * (rg ? 4 : 1);
     int64_t nsplit = (target + rowblocks * nact - 1) / (rowblocks * nact);
     nsplit = std::max<int64_t>(1, std::min<int64_t>(nsplit, (N + 63) / 64));
     int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
@@ -421,7 +424,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                  (rg ? rg->elems : double(M) * double(N) * double(nact)) *
                      (d_Alo ? 12.0 : prec == CCAB_C128 ? 16.0 : prec == CCAB_C32 ? 4.0 : 8.0));
     if (N > 0) {
-        const bool nt = ctx->gemv_nt;
+        const bool nt = tn ? tn->nt != 0 : ctx->gemv_nt;
         // (the split product keeps ordinary loads: non-temporal ones measured
         // slower for its paired 16-byte remainder loads)
         if (d_Alo && R == 4)

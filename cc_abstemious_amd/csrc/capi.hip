@@ -279,6 +279,8 @@ size_t mem_budget(ccab_ctx* ctx) {
     return budget;
 }
 
+}  // namespace
+
 // SoA copy of the collocation points (8 n doubles) for the current call
 int stage_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double** soa) {
     *soa = static_cast<double*>(ctx_buf(ctx, SLOT_SOA, size_t(n) * 64));
@@ -303,6 +305,8 @@ int stage_params(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch, std::vect
     CCAB_HIP(hipStreamSynchronize(ctx->stream));
     return CCAB_OK;
 }
+
+namespace {
 
 // phi / vn from the solved unknown x per BC (solve.rs:36-69)
 // phi and vn of every system from the solution x (solve.rs:24-71): pressure

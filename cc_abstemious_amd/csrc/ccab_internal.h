@@ -110,6 +110,22 @@ enum BufSlot {
     SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
     SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_COUNT
 };
+// Sweep tuning (ccab_autotune_sweep / ccab_set_sweep_tuning), per size n:
+// the batched assembly's systems per thread and column chunk for an
+// operator precision, and the uniform GEMV's workgroup target and
+// non-temporal loads for a matrix element precision.
+struct AsmTune {
+    int prec;          // CCAB_C32 (FP16 operator + FP32 remainder) or CCAB_C64 (c64 + c128)
+    int64_t n;
+    int kf;            // 8, 16 or 32 systems per thread
+    int col_chunk;     // 64, 128 or 256 columns per workgroup
+};
+struct GemvTune {
+    int prec;          // element precision of the matrix: CCAB_C32, CCAB_C64, CCAB_C128
+    int64_t n;
+    int wg_target;     // workgroups one product aims for
+    int nt;            // non-temporal matrix loads
+};
 // identity of a captured launch sequence (shape, buffers, tuning)
 struct GraphKey {
     int prec;
@@ -170,6 +186,8 @@ struct ccab_ctx {
     // environment CCAB_GEMV_NT=0 turns it off
     bool gemv_nt = true;
     int gemv_ragged_chunk = 512;  // columns per workgroup of ragged products (0: from the workgroup target)
+    std::vector<AsmTune> asm_tune;
+    std::vector<GemvTune> gemv_tune;
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of
     // the free memory, at most 96 GiB); environment CCAB_MEM_BUDGET_MB
     size_t mem_budget = 0;
@@ -314,6 +332,14 @@ struct BodyWave {
 // SoA points, incident right-hand sides (X, zero-filled, stride nmax) and
 // optional guesses X0 of every body: three launches for the batch
 // per system, the largest modulus of the LHS diagonal (d_out: nsys doubles)
+// capi.hip: per-call staging of the points (SoA, context buffer) and of the
+// systems' parameters (host copy hp, device copy *dp)
+int stage_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double** soa);
+int stage_params(ccab_ctx* ctx, const ccab_physics* ph, int64_t batch, std::vector<SysParams>& hp,
+                 SysParams** dp);
+// tune.hip: the tuned entries for (prec, n), or null
+const AsmTune* asm_tune_of(const ccab_ctx* ctx, int prec, int64_t n);
+const GemvTune* gemv_tune_of(const ccab_ctx* ctx, int prec, int64_t n);
 int launch_diag_max(ccab_ctx* ctx, int64_t nsys, const SysParams* d_params, const double* d_cpts,
                     const BodyDesc* d_desc, int64_t n, double* d_out);
 int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,

@@ -193,6 +193,28 @@ int ccab_set_graphs(ccab_ctx* ctx, int on);
 int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, int* best_variant,
                   int* best_outer, double* ms_table);
 
+/* Sweep autotuner (no reference counterpart): times, on the caller's mesh and
+ * frequencies (ph[nfreq], d_cpts), the batched sweep assembly for 8, 16 and
+ * 32 systems per thread x 64, 128 and 256 columns per workgroup, and the
+ * batched GEMV of the GMRES operator (precision op_prec: CCAB_C32, the
+ * FP16 operator, or CCAB_C64) and of the refinement matrix (the FP32
+ * remainder with CCAB_C32, c128 with CCAB_C64) for workgroup targets 256 ...
+ * 4096 x ordinary / non-temporal loads; keeps the fastest in the context for
+ * size n (ccab_sweep, ccab_sweep_warm and ccab_gmres_ir_batched of that size
+ * use them).  best (optional, 6 ints): systems per thread, column chunk,
+ * operator GEMV target, operator non-temporal, refinement GEMV target,
+ * refinement non-temporal.  ms_table (optional, 29 doubles): assembly times
+ * [kf index * 3 + chunk index], then 10 operator GEMV times [target index *
+ * 2 + nt], then 10 refinement GEMV times.  The choices change only how the
+ * work is split (the order of partial sums), so results agree to rounding. */
+int ccab_autotune_sweep(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq,
+                        const double* d_cpts, int64_t n, int op_prec, int* best,
+                        double* ms_table);
+/* Set the sweep tuning of size n by hand (cfg: the 6 ints of
+ * ccab_autotune_sweep's best), or drop it (cfg = NULL: the defaults 16, 64
+ * and the context's GEMV settings). */
+int ccab_set_sweep_tuning(ccab_ctx* ctx, int op_prec, int64_t n, const int* cfg);
+
 /* ---- geometry (host) --------------------------------------------------- */
 /* One centroid collocation point per linear tri/quad element:
  * elements.rs:49-181 (TRIGP1 / QUADGP1, detj incl. the quad 0.25 factor at

@@ -80,3 +80,73 @@ def test_lu_graph_replay_matches_eager(prec):
         torch.cuda.synchronize()
     ctx.set_graphs(False)
     ctx.close()
+
+
+def _sweep_case(level=3, nfreq=16, bc=capi.BC_NORMAL_VELOCITY, bcv=(0.0, 0.0)):
+    import math
+    from cc_abstemious_amd import mesh
+    from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+    fs = list(np.linspace(0.5, 8.0, nfreq) * 343.0 / (2 * math.pi))
+    return PreData(mesh.icosphere(level), fs, 343.0, 1.21, capi.BURTON_MILLER, capi.EXTERIOR, bc,
+                   bcv, IncidentWave(capi.WAVE_PLANE, (0.3, -0.4, 0.866), (1.0, 0.0)),
+                   Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-10, restart=30,
+                          ir_tolerance=1e-3, ir_max_steps=60, ref_mgs=False,
+                          stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C32,
+                          residual_matrix=capi.C32))
+
+
+def test_sweep_autotune_and_results_invariant():
+    """ccab_autotune_sweep keeps the fastest measured settings; the sweep's
+    solutions with the tuned and with extreme hand-set settings agree with
+    the untuned ones to the solve tolerance (the settings only change how
+    the work is split)."""
+    pd = _sweep_case()
+    n = pd.get_num_eqn()
+    _note(f"sweep autotune n={n} nfreq=16 (FP16 operator, FP32 remainder)")
+    ctx = capi.Context()
+    phi0, _, st0 = solve.sweep(pd, ctx=ctx)
+    best, table = ctx.autotune_sweep(pd, capi.C32)
+    print(best)
+    for key, t in table.items():
+        assert all(np.isfinite(v) and v > 0 for v in t.values()), key
+    assert table["assembly_ms"][(best["systems_per_thread"], best["column_chunk"])] == \
+        min(table["assembly_ms"].values())
+    assert table["op_gemv_ms"][(best["op_wg_target"], best["op_nt"])] == \
+        min(table["op_gemv_ms"].values())
+    assert table["ref_gemv_ms"][(best["ref_wg_target"], best["ref_nt"])] == \
+        min(table["ref_gemv_ms"].values())
+    sols = {"tuned": solve.sweep(pd, ctx=ctx)[0]}
+    for label, cfg in (("8/256/256/0/4096/1", (8, 256, 256, 0, 4096, 1)),
+                       ("32/128/4096/1/256/0", (32, 128, 4096, 1, 256, 0))):
+        ctx.set_sweep_tuning(capi.C32, n, cfg)
+        sols[label] = solve.sweep(pd, ctx=ctx)[0]
+    ctx.set_sweep_tuning(capi.C32, n, None)
+    for label, phi in sols.items():
+        e = float(torch.linalg.vector_norm(phi - phi0) / torch.linalg.vector_norm(phi0))
+        print(f"{label}: {e:.2e}")
+        assert e < 1e-9, label
+    ctx.close()
+
+
+def test_sweep_assembly_invariant_to_tuning():
+    """The batched fused assembly (ccab_assemble_system, the sweep kernel)
+    with other systems per thread and column chunks: LHS entries equal to
+    FP64 rounding (the phasor steps), right-hand sides (row sums over other
+    chunks) equal to rounding."""
+    from cc_abstemious_amd import influence_matrix
+    pd = _sweep_case(level=3, nfreq=12, bc=capi.BC_NORMAL_VELOCITY, bcv=(0.7, -0.2))
+    n = pd.get_num_eqn()
+    _note(f"sweep assembly vs tuning n={n} nfreq=12")
+    ctx = capi.Context()
+    fs = pd.get_frequencies()
+    rhs_inc = torch.zeros((len(fs), n), dtype=torch.complex128, device="cuda")
+    l0, _, r0 = influence_matrix.assemble_system(pd, rhs_inc, fs, ctx=ctx)
+    for cfg in ((8, 128, 1024, 1, 1024, 1), (32, 256, 1024, 1, 1024, 1)):
+        ctx.set_sweep_tuning(capi.C64, n, cfg)
+        l1, _, r1 = influence_matrix.assemble_system(pd, rhs_inc, fs, ctx=ctx)
+        el = float((l1 - l0).abs().max() / l0.abs().max())
+        er = float((r1 - r0).abs().max() / r0.abs().max())
+        print(cfg, el, er)
+        assert el < 1e-14 and er < 1e-13, cfg
+    ctx.set_sweep_tuning(capi.C64, n, None)
+    ctx.close()
