@@ -529,6 +529,22 @@ def main():
     (phi_c64, _, st_c64), c64_rate = timed(Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol,
                                                   restart=50, ir_tolerance=1e-5, ir_max_steps=20,
                                                   ref_mgs=False))
+    # the headline sweep with every GMRES-IR cycle replayed as a captured HIP
+    # graph (captured by an untimed sweep first)
+    pd.solver = headline
+    ctx.set_graphs(True)
+    solve.sweep(pd, ctx=ctx)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        phi_g, _, st_g = solve.sweep(pd, ctx=ctx)
+    torch.cuda.synchronize()
+    graphs = {"solves_per_s": solves / (time.perf_counter() - t1),
+              "gmres_iters_total": int(sum(x["iterations"] for x in st_g)),
+              "rel_err_vs_eager": float((torch.linalg.vector_norm(phi_g - phi, dim=1) /
+                                         torch.linalg.vector_norm(phi, dim=1)).max())}
+    ctx.set_graphs(False)
+    del phi_g
     strict = None
     if args.strict_tol > 0:
         (phi_s, _, st_s), strict_rate = timed(Solver(kind=capi.SOLVER_ITERATIVE_MIXED,
@@ -621,6 +637,7 @@ def main():
         "c64_operator_gmres_iters": iter_stats(st_c64, 50, 1e-5, "c64"),
         "lu_tuning": tuning,
         "sweep_tuning": sweep_tuning,
+        "graph_cycles": graphs,
         "assembly_gentries_per_s": gentries,
         "assembly_store_gbs": asm_gbs,
         "gmres_iters": gmres_iters,

@@ -386,7 +386,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const int32_t* d_active, const void* d_A, int64_t lda, int64_t strideA,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
                 int64_t stridey, const double* d_inv_scale, const Ragged* rg, const c64* d_Alo,
-                const int32_t* d_skip, int prof_cls) {
+                const int32_t* d_skip, int prof_cls, bool reserve_only) {
     if (M <= 0 || nact <= 0) return CCAB_OK;
     if (prec == CCAB_C32 && !d_inv_scale) return ctx_fail(ctx, CCAB_EINVAL, "c32 GEMV needs scales");
     if (d_Alo && prec != CCAB_C32)
@@ -419,6 +419,7 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;
     c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
+    if (reserve_only) return CCAB_OK;
     dim3 grid((unsigned)rowblocks, (unsigned)nsplit, (unsigned)nact);
     ProfScope ps(ctx, prof_cls >= 0 ? prof_cls : (prec == CCAB_C128 || d_Alo) ? PROF_GEMV128 : PROF_GEMV,
                  (rg ? rg->elems : double(M) * double(N) * double(nact)) *

@@ -50,6 +50,8 @@ void graph_store(ccab_ctx* ctx, const GraphKey& k, hipGraphExec_t e) {
 void graph_clear(ccab_ctx* ctx) {
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     ctx->graphs.clear();
+    for (auto& g : ctx->gir_graphs) (void)hipGraphExecDestroy(g.second);
+    ctx->gir_graphs.clear();
 }
 
 void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes) {
@@ -80,6 +82,7 @@ int pinned_reserve(ccab_ctx* ctx, size_t bytes) {
     if (ctx->pinned_bytes >= bytes) return CCAB_OK;
     if (ctx->pinned) {
         (void)hipStreamSynchronize(ctx->stream);
+        graph_clear(ctx);  // captured GMRES-IR cycles write flags into it
         (void)hipHostFree(ctx->pinned);
         ctx->pinned = nullptr;
         ctx->pinned_bytes = 0;

@@ -141,6 +141,26 @@ struct GraphKey {
     }
 };
 
+// identity of a captured GMRES-IR cycle (see gmres_ir_run): sizes, every
+// buffer and scalar its kernels take, and the settings that shape launches
+struct GirKey {
+    int64_t n, B, na;
+    int m, op_prec, ref_mgs, mgs_coop, wg_target, nt, has_tune;
+    const void *Aop, *inv_scale, *act, *Q, *Hd, *gv, *Yd, *X, *Zd, *Dv, *R, *nrm, *bn, *pinned,
+        *rg_n, *rg_aoff;
+    int64_t lda, sA;
+    double inner_tol, tol;
+    bool operator==(const GirKey& o) const {
+        return n == o.n && B == o.B && na == o.na && m == o.m && op_prec == o.op_prec &&
+               ref_mgs == o.ref_mgs && mgs_coop == o.mgs_coop && wg_target == o.wg_target &&
+               nt == o.nt && has_tune == o.has_tune && Aop == o.Aop && inv_scale == o.inv_scale &&
+               act == o.act && Q == o.Q && Hd == o.Hd && gv == o.gv && Yd == o.Yd && X == o.X &&
+               Zd == o.Zd && Dv == o.Dv && R == o.R && nrm == o.nrm && bn == o.bn &&
+               pinned == o.pinned && rg_n == o.rg_n && rg_aoff == o.rg_aoff && lda == o.lda &&
+               sA == o.sA && inner_tol == o.inner_tol && tol == o.tol;
+    }
+};
+
 struct ccab_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -199,6 +219,7 @@ struct ccab_ctx {
     // replay captured HIP graphs of fixed launch sequences (the LU)
     bool use_graphs = false;
     std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+    std::vector<std::pair<GirKey, hipGraphExec_t>> gir_graphs;  // GMRES-IR cycles
     // in-process kernel timing (HIP events on ctx->stream), see prof_begin
     bool prof_on = false;
     struct ProfRec {
@@ -404,7 +425,8 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
                 const c128* d_x, int64_t stridex, c128 alpha, c128 beta, c128* d_y,
                 int64_t stridey, const double* d_inv_scale = nullptr,
                 const Ragged* rg = nullptr, const c64* d_Alo = nullptr,
-                const int32_t* d_skip = nullptr, int prof_cls = -1);
+                const int32_t* d_skip = nullptr, int prof_cls = -1,
+                bool reserve_only = false);  // only grow the workspace this call needs
 int gemv_general(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t batch, const void* d_A,
                  int64_t lda, int64_t strideA, const c128* d_x, int64_t stridex, c128 alpha,
                  c128 beta, c128* d_y, int64_t stridey);

@@ -236,7 +236,8 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
 // null): one workgroup per system, or, for few large systems that would
 // leave most CUs idle, mgs_coop_kernel over up to one workgroup per CU.
 int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ,
-               c128* h, int64_t sh, const int32_t* act, int64_t nact, const int32_t* skip = nullptr) {
+               c128* h, int64_t sh, const int32_t* act, int64_t nact, const int32_t* skip = nullptr,
+               bool reserve_only = false) {
     hipStream_t s = ctx->stream;
     if (ctx->mgs_coop && n >= 4096) {
         int rq = 0;
@@ -268,6 +269,7 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
                                    int64_t(1));
                 ctx->coop_abort = st;
             }
+            if (reserve_only) return CCAB_OK;
             const dim3 grid((unsigned)G, (unsigned)nact);
 #define CCAB_MGS_COOP(R)                                                                        \
     do {                                                                                        \
@@ -291,6 +293,7 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
             return CCAB_OK;
         }
     }
+    if (reserve_only) return CCAB_OK;
     if (ref)
         hipLaunchKernelGGL(mgs_kernel<true>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k, Q,
                            ldq, sQ, h, sh, act, skip);
@@ -981,6 +984,80 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         for (int32_t b : a) e += double(rg->hn[b]) * double(rg->hn[b]);
         rg->elems = e;
     };
+    // one Arnoldi step over the first ni systems of the device's active list
+    auto arnoldi_step = [&](int k, int ni) -> int {
+        CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq, one,
+                             zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr, kdone_d));
+        {
+            ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d));
+        }
+        hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act, ni,
+                           Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d, inner_tol, tol,
+                           Hbar, ring_done_dev(k));
+        CCAB_HIP(hipGetLastError());
+        return CCAB_OK;
+    };
+    // y = R^{-1} beta and x += Q y over each system's own steps
+    auto finish_cycle = [&](int na_c) -> int {
+        if (m <= kSolveYLdsM)
+            hipLaunchKernelGGL(gir_solve_y_wave_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m, act,
+                               Hg, bed, kdone_d, Yd, Hbar, Zd);
+        else
+            hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
+                               m, act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
+        hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
+                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X, Zd, Dv, R);
+        CCAB_HIP(hipGetLastError());
+        return CCAB_OK;
+    };
+    // HIP graphs (ccab_set_graphs / CCAB_GRAPHS, not while profiling): a
+    // cycle over na systems is captured once per key and replayed
+    const bool graph_cycles = ctx->use_graphs && !ctx->prof_on && ctx->stream;
+    auto run_cycle_graph = [&](int na_c) -> int {
+        const GemvTune* tn = gemv_tune_of(ctx, op_prec, n);
+        GirKey key{n, B, na_c, m, op_prec, ref_mgs ? 1 : 0, ctx->mgs_coop ? 1 : 0,
+                   tn ? tn->wg_target : ctx->gemv_wg_target, tn ? tn->nt : (ctx->gemv_nt ? 1 : 0),
+                   tn ? 1 : 0, Aop, d_inv_scale, act, Q, Hd, gv, Yd, X, Zd, Dv, R, nrm, bn_d,
+                   ctx->pinned, rg ? rg->n : nullptr, rg ? rg->aoff : nullptr, lda, sA,
+                   inner_tol, tol};
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : ctx->gir_graphs)
+            if (g.first == key) exec = g.second;
+        if (!exec) {
+            // grow every workspace the captured launches use before capturing
+            // (a growing buffer would drop the graphs and allocate mid-capture)
+            CCAB_TRY(gemv_active(ctx, op_prec, n, n, na_c, act, Aop, lda, sA, Q, sq, one, zero, Q + n,
+                                 sq, d_inv_scale, rg, nullptr, kdone_d, -1, true));
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, 0, Q, n, sq, Hd, m + 2, act, na_c, kdone_d, true));
+            // (the active list and every buffer key.* names stay valid for the graph)
+            CCAB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rc = CCAB_OK;
+            hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
+                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
+            hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0,
+                               s, m, act, na_c, kdone_d, bed, nrm);
+            for (int k = 0; k < m && rc == CCAB_OK; ++k) rc = arnoldi_step(k, na_c);
+            if (rc == CCAB_OK) rc = finish_cycle(na_c);
+            hipGraph_t graph = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &graph);
+            if (rc != CCAB_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            CCAB_HIP(ec);
+            const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            CCAB_HIP(ei);
+            if (ctx->gir_graphs.size() >= 16) {
+                (void)hipGraphExecDestroy(ctx->gir_graphs.front().second);
+                ctx->gir_graphs.erase(ctx->gir_graphs.begin());
+            }
+            ctx->gir_graphs.emplace_back(key, exec);
+        }
+        CCAB_HIP(hipGraphLaunch(exec, s));
+        return CCAB_OK;
+    };
     std::vector<double> bn(B), rn(B), rel(B, 1.0);
     std::vector<int64_t> iters(B, 0), nmv(B, 0), cycles(B, 0);
     std::vector<char> conv(B, 0);
@@ -1027,72 +1104,64 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         if (active.empty() || cyc == max_cycles) break;
         CCAB_TRY(upload_active(active, act));
         // one GMRES(m) cycle for A64 d = r on the active systems
-        hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
-                           (unsigned)active.size()), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
         const int na_c = (int)active.size();
-        hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
-                           m, act, na_c, kdone_d, bed, nrm);
         for (int32_t b : active) cycles[b]++;
-        std::vector<int32_t> inner = active;
         std::vector<int> launched(B, 0);  // Arnoldi steps enqueued per system (profiler)
-        int known = -1;  // the latest step whose done flags the host has applied
-        for (int k = 0; k < m && !inner.empty(); ++k) {
-            if (inner != uploaded) {  // a fresh pinned slot per step: no host wait
-                std::memcpy(ring_act(k), inner.data(), inner.size() * 4);
-                CCAB_HIP(hipMemcpyAsync(act, ring_act(k), inner.size() * 4, hipMemcpyHostToDevice, s));
-                uploaded = inner;
-            }
-            const unsigned ni = (unsigned)inner.size();
-            set_elems(inner);
-            for (int32_t b : inner) launched[b]++;
-            // systems already done on the device skip the product, the
-            // Gram-Schmidt step and the rotation
-            CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq,
-                                 one, zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr,
-                                 kdone_d));
-            {
-                ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
-                CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d));
-            }
-            hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act,
-                               (int)ni, Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d,
-                               inner_tol, tol, Hbar, ring_done_dev(k));
-            CCAB_HIP(hipGetLastError());
-            CCAB_HIP(hipEventRecord(ctx->gir_events[k], s));
-            // newest finished flags: wait for step k - kLag, take a later one if ready
-            int target = -1;
-            if (k - kLag > known) {
-                CCAB_HIP(hipEventSynchronize(ctx->gir_events[k - kLag]));
-                target = k - kLag;
-            }
-            for (int t = k; t > std::max(target, known); --t) {
-                const hipError_t q = hipEventQuery(ctx->gir_events[t]);
-                if (q == hipSuccess) {
-                    target = t;
-                    break;
+        if (graph_cycles) {
+            // the whole cycle (m Arnoldi steps over the cycle's systems, the
+            // back substitution and the update) as one captured graph: the
+            // device's done flags make finished systems skip the later steps,
+            // so the launch sequence does not depend on the data
+            CCAB_TRY(run_cycle_graph(na_c));
+        } else {
+            hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
+                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
+            hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0,
+                               s, m, act, na_c, kdone_d, bed, nrm);
+            std::vector<int32_t> inner = active;
+            int known = -1;  // the latest step whose done flags the host has applied
+            for (int k = 0; k < m && !inner.empty(); ++k) {
+                if (inner != uploaded) {  // a fresh pinned slot per step: no host wait
+                    std::memcpy(ring_act(k), inner.data(), inner.size() * 4);
+                    CCAB_HIP(hipMemcpyAsync(act, ring_act(k), inner.size() * 4,
+                                            hipMemcpyHostToDevice, s));
+                    uploaded = inner;
                 }
-                if (q != hipErrorNotReady) CCAB_HIP(q);
+                const unsigned ni = (unsigned)inner.size();
+                set_elems(inner);
+                for (int32_t b : inner) launched[b]++;
+                // systems already done on the device skip the product, the
+                // Gram-Schmidt step and the rotation
+                CCAB_TRY(arnoldi_step(k, (int)ni));
+                CCAB_HIP(hipEventRecord(ctx->gir_events[k], s));
+                // newest finished flags: wait for step k - kLag, take a later one if ready
+                int target = -1;
+                if (k - kLag > known) {
+                    CCAB_HIP(hipEventSynchronize(ctx->gir_events[k - kLag]));
+                    target = k - kLag;
+                }
+                for (int t = k; t > std::max(target, known); --t) {
+                    const hipError_t q = hipEventQuery(ctx->gir_events[t]);
+                    if (q == hipSuccess) {
+                        target = t;
+                        break;
+                    }
+                    if (q != hipErrorNotReady) CCAB_HIP(q);
+                }
+                if (target > known) {
+                    known = target;
+                    const int32_t* fl = ring_done(target);
+                    std::vector<int32_t> next;
+                    for (int32_t b : inner)
+                        if (fl[b] == 0) next.push_back(b);
+                    inner.swap(next);
+                }
             }
-            if (target > known) {
-                known = target;
-                const int32_t* fl = ring_done(target);
-                std::vector<int32_t> next;
-                for (int32_t b : inner)
-                    if (fl[b] == 0) next.push_back(b);
-                inner.swap(next);
-            }
+            // every system of the cycle is done on the device (step m - 1 ends
+            // the cycle): y = R^{-1} beta, then x += Q y
+            CCAB_TRY(upload_active(active, act));
+            CCAB_TRY(finish_cycle(na_c));
         }
-        // every system of the cycle is done on the device (step m - 1 ends
-        // the cycle): y = R^{-1} beta, then x += Q y
-        CCAB_TRY(upload_active(active, act));
-        if (m <= kSolveYLdsM)
-            hipLaunchKernelGGL(gir_solve_y_wave_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m, act,
-                               Hg, bed, kdone_d, Yd, Hbar, Zd);
-        else
-            hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
-                               m, act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
-        hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
-                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X, Zd, Dv, R);
         CCAB_HIP(hipMemcpyAsync(hAct, kdone_d, size_t(B) * 4, hipMemcpyDeviceToHost, s));
         CCAB_HIP(hipStreamSynchronize(s));
         uploaded.clear();  // hAct now holds flags, not the list
