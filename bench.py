@@ -81,6 +81,11 @@ def parse():
                     help="1: also time the H-matrix case (n = 81920, and 20480 vs dense)")
     ap.add_argument("--bodies", type=int, default=256,
                     help="uneven bodies of the batched-bodies line (0 skips it)")
+    ap.add_argument("--hm-levels", default="6,5",
+                    help="icosphere levels of the H-matrix sub-record (6: 81920 triangles; "
+                         "5: 20480, with the dense-LU comparison)")
+    ap.add_argument("--only", choices=["hmatrix", "large"], default=None,
+                    help="run only that sub-record's workload and print it (for profiling)")
     ap.add_argument("--tune-sweep", type=int, default=1,
                     help="1: run the sweep autotuner (assembly split, GEMV workgroup target and "
                          "loads) on the workload before timing (untimed)")
@@ -348,10 +353,11 @@ def hmatrix_block(args, ctx):
     d = np.array([0.3, -0.4, 0.866])
     d /= np.linalg.norm(d)
     out = {}
-    for level in (6, 5):
+    for level in [int(v) for v in args.hm_levels.split(",")]:
         m = mesh.icosphere(level)
         rec = {}
         for key, prec in (("c128", capi.C128), ("fp32_storage", capi.C64)):
+            print(f"[hmatrix] level {level} {key}", file=sys.stderr, flush=True)
             pd = PreData(m, [f], c, rho, capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY,
                          (0.0, 0.0), IncidentWave(capi.WAVE_PLANE, tuple(d), (1.0, 0.0)),
                          Solver(kind=capi.SOLVER_HIERARCHICAL, tolerance=1e-6, max_iterations=3000,
@@ -371,19 +377,27 @@ def hmatrix_block(args, ctx):
             y = torch.zeros_like(x)
             H.gemv(1.0, x, 0.0, y)
             torch.cuda.synchronize()
+            ctx.profile(True)
+            ctx.profile_read(reset=True)
             t0 = time.perf_counter()
             for _ in range(10):
                 H.gemv(1.0, x, 0.0, y)
             torch.cuda.synchronize()
             tm = (time.perf_counter() - t0) / 10
+            pr = ctx.profile_read(reset=True).get("hmatrix_gemv", {})
+            ctx.profile(False)
             del H
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             phi, _ = solve.solve_for_surface(pd, rhs, ctx=ctx)
             torch.cuda.synchronize()
             ts = time.perf_counter() - t0
+            ev_ms = pr["ms"] / pr["count"] if pr.get("count") else None
             rec[key] = {"build_ms": tb * 1e3, "matvec_ms": tm * 1e3,
                         "matvec_gbs": st["storage"] * (16 if prec == capi.C128 else 8) / tm / 1e9,
+                        # HIP events around the matvec's launches on the library's stream
+                        "matvec_event_ms": ev_ms,
+                        "matvec_bytes": pr["work"] / pr["count"] if pr.get("count") else None,
                         "solve_ms_incl_build": ts * 1e3, "gmres_matvecs": int(pd.last_stats["num_mv"]),
                         "rel_residual": pd.last_stats["rel_residual"]}
             rec["compression_ratio_pct"] = st["compression_ratio"]
@@ -421,6 +435,17 @@ def main():
     import torch
     from cc_abstemious_amd import capi, mesh, solve
     from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
+
+    if args.only:
+        ctx = capi.default_context()
+        blk = hmatrix_block(args, ctx) if args.only == "hmatrix" else large_dense_block(args, ctx)
+        out = {args.only: blk}
+        if args.only == "hmatrix":
+            out["roofline_hmatvec"] = roofline_hmatvec(blk)
+        if args.calibrate:
+            out["calibration"] = calibrate()
+        print(json.dumps(out), flush=True)
+        return
 
     rng = np.random.default_rng(2)
     d = rng.standard_normal(3)
@@ -674,20 +699,51 @@ def main():
     result["bodies"] = bodies_block(args, ctx) if args.bodies > 0 else None
     result["large_dense"] = large_dense_block(args, ctx) if args.large else None
     result["hmatrix"] = hmatrix_block(args, ctx) if args.hmatrix else None
+    result["roofline_hmatvec"] = roofline_hmatvec(result["hmatrix"])
     if not args.skip_cpu:
         result["cpu_baseline"] = cpu_baseline(pd.cpts, (freqs[-1], c, rho, 1, 0, 1, (0.0, 0.0)),
                                               args.cpu_lu_n)
     else:
         result["cpu_baseline"] = None
     if args.calibrate:
-        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
-        x.fill_(1.0)
-        torch.cuda.synchronize()
-        float(x.sum())
-        torch.cuda.synchronize()
-        result["calibration"] = {"read_kernel": "torch sum over 1 GiB", "write_kernel":
-                                 "torch fill of 1 GiB", "bytes": 1 << 30}
+        result["calibration"] = calibrate()
     print(json.dumps(result), flush=True)
+
+
+def roofline_hmatvec(hm):
+    """HBM roofline of the H-matrix matvec at n = 81920 (c128 and FP32 block
+    storage): algorithmic bytes = the stored block entries one product reads
+    (sum r (m + n) + sum of the dense blocks' m n, times 16 or 8 B), over the
+    HIP-event time of one product (its three launches)."""
+    if not hm:
+        return None
+    rec = hm.get("icosphere_81920_tri")
+    if not rec:
+        return None
+    out = {}
+    for key in ("c128", "fp32_storage"):
+        r = rec[key]
+        if not r.get("matvec_event_ms"):
+            continue
+        gbs = r["matvec_bytes"] / (r["matvec_event_ms"] * 1e-3) / 1e9
+        out[key] = {"bound": "hbm", "kernel": "hmv_vt_kernel + hmv_node_kernel + hmv_reduce_kernel "
+                    "(one H-matrix product)", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": r["matvec_bytes"],
+                    "avg_launch_ms": r["matvec_event_ms"], "traffic": None}
+    return out
+
+
+def calibrate():
+    """A write of 1 GiB (torch fill) and a read of 1 GiB (torch sum): known
+    byte counts that scale the hardware traffic counters."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    float(x.sum())
+    torch.cuda.synchronize()
+    return {"read_kernel": "torch sum over 1 GiB", "write_kernel": "torch fill of 1 GiB",
+            "bytes": 1 << 30}
 
 
 if __name__ == "__main__":

@@ -383,6 +383,11 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* gr = getenv("CCAB_GRAPHS")) ctx->use_graphs = atoi(gr) != 0;
     if (const char* nt = getenv("CCAB_GEMV_NT")) ctx->gemv_nt = atoi(nt) != 0;
     if (const char* rc = getenv("CCAB_GEMV_RAGGED_CHUNK")) ctx->gemv_ragged_chunk = std::max(0, atoi(rc));
+    if (const char* hb = getenv("CCAB_HMV_NODE_BUDGET")) ctx->hmv_node_budget = std::max(0L, atol(hb));
+    if (const char* hs = getenv("CCAB_HMV_SORT")) ctx->hmv_sort = atoi(hs) != 0;
+    if (const char* hp = getenv("CCAB_HMV_PAIRED")) ctx->hmv_paired = atoi(hp) & 3;
+    if (const char* hw = getenv("CCAB_HMV_WAVES_PER_CU"))
+        ctx->hmv_waves_per_cu = std::max(0, std::min(64, atoi(hw)));
     if (const char* gw = getenv("CCAB_GEMV_WG_TARGET"))
         ctx->gemv_wg_target = std::max(64, std::min(65536, atoi(gw)));
     // the grid-barrier kernels assume one resident workgroup per CU: without

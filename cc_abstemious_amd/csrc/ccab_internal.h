@@ -206,6 +206,16 @@ struct ccab_ctx {
     // environment CCAB_GEMV_NT=0 turns it off
     bool gemv_nt = true;
     int gemv_ragged_chunk = 512;  // columns per workgroup of ragged products (0: from the workgroup target)
+    // H-matrix product: persistent waves per CU (0: one item per wave);
+    // environment CCAB_HMV_WAVES_PER_CU
+    int hmv_waves_per_cu = 0;
+    // H-matrix build: columns per product segment (0: whole clusters) and
+    // items by decreasing work (CCAB_HMV_NODE_BUDGET, CCAB_HMV_SORT)
+    int64_t hmv_node_budget = 512;
+    bool hmv_sort = true;
+    // FP32 H-matrix storage: bit 0 two columns per 16-byte load in V^T x,
+    // bit 1 two rows per 16-byte load in the product (CCAB_HMV_PAIRED)
+    int hmv_paired = 3;
     std::vector<AsmTune> asm_tune;
     std::vector<GemvTune> gemv_tune;
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -39,9 +40,11 @@ struct HBlkDev {
     int64_t w_off;                 // ACA workspace (vectors), in c128 units
     int64_t f_off;                 // ACA exclusion flags, in bytes
     int64_t u2_off, v2_off;        // recompressed U, V (packed to the ACA rank)
+    int64_t clo;                   // first position of the column cluster in the
+                                   // depth-first order of its tree (the matvec's x)
     int32_t adm, rank;
     int32_t status;                // ACA: 0 ok, 1 rank capacity reached, 2 rng exhausted
-    int32_t pad_;
+    int32_t vtc;                   // matvec: column chunks of V^T x (partials at p_off + c rank)
     double norm;                   // Frobenius norm estimate / dense Frobenius norm
 };
 
@@ -868,86 +871,253 @@ __device__ __forceinline__ c128 ldx(const c64* p) {
     return c128{double(v.re), double(v.im)};
 }
 
-// t_b = V_b^T x_cols, rank entries per low-rank block at t + p_off: one
-// wave per block, lanes over the block's columns, four rank columns of V per
-// pass, then fixed-order wave reductions (one-wave workgroups measured faster
-// than four blocks per workgroup)
+// The matvec's two passes work on balanced items: V^T x per (low-rank block,
+// chunk of at most kVtChunk columns), and per (segment of a row cluster's
+// blocks worth at most kNodeBudget columns, 64-row tile); each item writes
+// its own partial (t, or a slot), so sums keep a fixed order.  The segment
+// budget (CCAB_HMV_NODE_BUDGET columns, 0 = a whole cluster) and the item
+// order (CCAB_HMV_SORT: by decreasing work, or tree order) are set when the
+// H matrix is built; a launch gives each wave one item, or with a cap on
+// resident waves (CCAB_HMV_WAVES_PER_CU) wave w takes items w, w + W, ...
+// (persistent waves).
+constexpr int64_t kVtChunk = 512;
+
+// A V^T x item with everything its wave needs (one load, no chain through
+// the block descriptors): the chunk's first V entry and x entry, its t slot
+struct VtItem {
+    int64_t v;    // V offset of the chunk's first column (rank 0)
+    int64_t x;    // xp offset of the chunk's first column
+    int64_t t;    // t offset of the chunk's rank partials
+    int64_t ld;   // rows of V (the block's column count)
+    int32_t len;  // columns in the chunk
+    int32_t rank;
+};
+
+// t[item.t + k] = sum over the chunk's columns j of V[item.v + j + k ld] xp[item.x + j]
 template <typename T>
-__global__ __launch_bounds__(64) void hmv_vt_kernel(const HBlkDev* __restrict__ blks,
-                                                    const int32_t* __restrict__ list,
-                                                    const int64_t* __restrict__ cols,
-                                                    const T* __restrict__ V,
-                                                    const c128* __restrict__ x, c128* t) {
-    const HBlkDev bk = blks[list[blockIdx.x]];
-    const int r = bk.rank;
-    const int64_t nc = bk.nc;
-    const T* Vb = V + bk.v_off;
-    const int64_t* cj = cols + bk.c_off;
-    const int lane = threadIdx.x;
-    for (int k0 = 0; k0 < r; k0 += 4) {
-        c128 acc[4] = {};
-        for (int64_t j = lane; j < nc; j += 64) {
-            const c128 xj = x[cj[j]];
+__global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ items, int64_t cnt,
+                                                     const T* __restrict__ V,
+                                                     const c128* __restrict__ xp, c128* t,
+                                                     int paired) {
+    const int lane = threadIdx.x & 63;
+    const int wpg = blockDim.x >> 6;
+    const int64_t W = int64_t(gridDim.x) * wpg;
+    for (int64_t it = int64_t(blockIdx.x) * wpg + (threadIdx.x >> 6); it < cnt; it += W) {
+        const VtItem q = items[it];
+        const int r = q.rank;
+        const T* Vb = V + q.v;
+        const c128* xc = xp + q.x;  // the column cluster's x, contiguous
+        if constexpr (sizeof(T) == 8) {
+            // complex FP32 with even offsets and an even column count: columns
+            // (j, j + 1) of a rank are one aligned 16-byte load, 128 columns
+            // per pass
+            if (paired && (q.ld & 1) == 0 && (q.v & 1) == 0) {
+                for (int k0 = 0; k0 < r; k0 += 8) {
+                    c128 acc[8] = {};
+                    for (int j = 2 * lane; j < q.len; j += 128) {
+                        const c128 x0 = xc[j];
+                        if (j + 1 < q.len) {
+                            const c128 x1 = xc[j + 1];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (k0 + u < r) acc[u] += ldx(Vb + j + (k0 + u) * nc) * xj;
+                            for (int u = 0; u < 8; ++u)
+                                if (k0 + u < r) {
+                                    const float4 v = *reinterpret_cast<const float4*>(
+                                        Vb + j + int64_t(k0 + u) * q.ld);
+                                    acc[u] += c128{v.x, v.y} * x0 + c128{v.z, v.w} * x1;
+                                }
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                if (k0 + u < r) acc[u] += ldx(Vb + j + int64_t(k0 + u) * q.ld) * x0;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (k0 + u >= r) break;  // wave-uniform
+                        const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
+                        if (lane == 0) t[q.t + k0 + u] = c128{re, im};
+                    }
+                }
+                continue;
+            }
         }
+        for (int k0 = 0; k0 < r; k0 += 8) {
+            c128 acc[8] = {};
+            for (int j = lane; j < q.len; j += 64) {
+                const c128 xj = xc[j];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
-            if (lane == 0 && k0 + u < r) t[bk.p_off + k0 + u] = c128{re, im};
+                for (int u = 0; u < 8; ++u)
+                    if (k0 + u < r) acc[u] += ldx(Vb + j + int64_t(k0 + u) * q.ld) * xj;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                if (k0 + u >= r) break;  // wave-uniform
+                const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
+                if (lane == 0) t[q.t + k0 + u] = c128{re, im};
+            }
         }
     }
 }
 
-// One wave per 64-row tile of a row cluster: acc = sum over the cluster's
-// blocks of D_b x_cols (dense) and U_b t_b (low-rank), nacc[slot] = alpha acc
+// One product work entry per (row cluster, block) in the clusters' block
+// order (dense first, then low-rank): the segment's entries are read with
+// one vector load (a lane per entry) and broadcast, instead of a dependent
+// chain of scalar loads per block.
+struct HmvWork {
+    int64_t off;   // dense: d_off; low-rank: u_off
+    int64_t xoff;  // dense: clo (xp); low-rank: p_off (t)
+    int32_t cnt;   // dense: columns; low-rank: rank
+    int32_t vtc;   // 0: dense; low-rank: chunks of V^T x
+};
+
+__device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), src);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), src);
+    return (int64_t(hi) << 32) | int64_t(uint32_t(lo));
+}
+
+// Segment s of a row cluster (work entries seg_e0[s] .. seg_e1[s]), rows
+// r0 .. r0 + 63 of the cluster (seg_nr[s] rows): acc = sum of D_b xp (dense)
+// and U_b t_b (low-rank, t_b summed over its chunks), slot nacc[seg_off[s] +
+// row] = alpha acc
 template <typename T>
-__global__ __launch_bounds__(64) void hmv_node_kernel(
-    const int32_t* __restrict__ tile_node, const int32_t* __restrict__ tile_r0,
-    const int64_t* __restrict__ nptr, const int32_t* __restrict__ nlist,
-    const int64_t* __restrict__ node_off, const HBlkDev* __restrict__ blks,
-    const int64_t* __restrict__ cols, const T* __restrict__ D, const T* __restrict__ U,
-    const c128* __restrict__ t, const c128* __restrict__ x, c128 alpha, c128* nacc) {
-    const int c = tile_node[blockIdx.x];
-    const int64_t p = tile_r0[blockIdx.x] + threadIdx.x;
-    const int64_t e0 = nptr[c], e1 = nptr[c + 1];
-    const int64_t nr = blks[nlist[e0]].nr;  // every block of the cluster has its rows
-    const bool on = p < nr;
-    const int64_t pp = on ? p : 0;          // lanes past the cluster read row 0
-    c128 acc{0.0, 0.0};
-    for (int64_t e = e0; e < e1; ++e) {
-        const HBlkDev bk = blks[nlist[e]];
-        if (!bk.adm) {
-            const T* d = D + bk.d_off + pp;
-            const int64_t* cj = cols + bk.c_off;
-            int64_t j = 0;
-            for (; j + 7 < bk.nc; j += 8) {
-                c128 a[8], xv[8];
+__global__ __launch_bounds__(256) void hmv_node_kernel(
+    const int32_t* __restrict__ item_seg, const int32_t* __restrict__ item_r0, int64_t cnt,
+    const int32_t* __restrict__ seg_nr, const int32_t* __restrict__ seg_e0,
+    const int32_t* __restrict__ seg_e1, const int64_t* __restrict__ seg_off,
+    const HmvWork* __restrict__ work, const T* __restrict__ D, const T* __restrict__ U,
+    const c128* __restrict__ t, const c128* __restrict__ xp, c128 alpha, c128* nacc,
+    int paired) {
+    const int lane = threadIdx.x & 63;
+    const int wpg = blockDim.x >> 6;
+    const int64_t W = int64_t(gridDim.x) * wpg;
+    for (int64_t it = int64_t(blockIdx.x) * wpg + (threadIdx.x >> 6); it < cnt; it += W) {
+        const int sg = item_seg[it];
+        const int64_t p = item_r0[it] + lane;
+        const int64_t e0 = seg_e0[sg], e1 = seg_e1[sg];
+        const int64_t nr = seg_nr[sg];
+        if constexpr (sizeof(T) == 8) {
+            // complex FP32, an even row count (offsets are even): lane l of
+            // half h takes rows r0 + 2(l mod 32) and the next (one aligned
+            // 16-byte load per column), columns (or ranks) congruent to h
+            // mod 2; the halves are added at the end
+            if (paired && (nr & 1) == 0) {
+                const int h = lane >> 5;
+                const int64_t p0 = item_r0[it] + 2 * (lane & 31);
+                const bool on2 = p0 < nr;
+                const int64_t q0 = on2 ? p0 : 0;
+                c128 a0{0.0, 0.0}, a1{0.0, 0.0};
+                for (int64_t eb = e0; eb < e1; eb += 64) {
+                    const int nq = (int)min<int64_t>(64, e1 - eb);
+                    HmvWork mine{0, 0, 0, 0};
+                    if (lane < nq) mine = work[eb + lane];
+                    for (int qi = 0; qi < nq; ++qi) {
+                        const int64_t off = bcast64(mine.off, qi);
+                        const int64_t xoff = bcast64(mine.xoff, qi);
+                        const int c = __builtin_amdgcn_readlane(mine.cnt, qi);
+                        const int vtc = __builtin_amdgcn_readlane(mine.vtc, qi);
+                        if (vtc == 0) {
+                            const T* d = D + off + q0 + int64_t(h) * nr;
+                            const c128* xc = xp + xoff;  // wave-uniform: both parities loaded, one kept
+                            int jb = 0;
+                            for (; jb + 15 < c; jb += 16) {
+                                float4 v[8];
+                                c128 xv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    a[u] = ldx(d + (j + u) * nr);
-                    xv[u] = x[cj[j + u]];
+                                for (int u = 0; u < 8; ++u) {
+                                    v[u] = *reinterpret_cast<const float4*>(d + int64_t(jb + 2 * u) * nr);
+                                    const c128 xa = xc[jb + 2 * u], xb = xc[jb + 2 * u + 1];
+                                    xv[u] = h ? xb : xa;
+                                }
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) {
+                                    a0 += c128{v[u].x, v[u].y} * xv[u];
+                                    a1 += c128{v[u].z, v[u].w} * xv[u];
+                                }
+                            }
+                            for (; jb < c; jb += 2) {
+                                if (jb + h < c) {
+                                    const float4 v = *reinterpret_cast<const float4*>(d + int64_t(jb) * nr);
+                                    const c128 xv = xc[jb + h];
+                                    a0 += c128{v.x, v.y} * xv;
+                                    a1 += c128{v.z, v.w} * xv;
+                                }
+                            }
+                        } else {
+                            const T* uu = U + off + q0 + int64_t(h) * nr;
+                            const c128* tb = t + xoff;
+                            for (int kb = 0; kb < c; kb += 2) {
+                                c128 ta = tb[kb];
+                                c128 tc = kb + 1 < c ? tb[kb + 1] : c128{0.0, 0.0};
+                                for (int ch = 1; ch < vtc; ++ch) {
+                                    ta += tb[int64_t(ch) * c + kb];
+                                    if (kb + 1 < c) tc += tb[int64_t(ch) * c + kb + 1];
+                                }
+                                if (kb + h < c) {
+                                    const c128 tk = h ? tc : ta;
+                                    const float4 v = *reinterpret_cast<const float4*>(uu + int64_t(kb) * nr);
+                                    a0 += c128{v.x, v.y} * tk;
+                                    a1 += c128{v.z, v.w} * tk;
+                                }
+                            }
+                        }
+                    }
                 }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) acc += a[u] * xv[u];
+                a0.re += __shfl_xor(a0.re, 32, 64);
+                a0.im += __shfl_xor(a0.im, 32, 64);
+                a1.re += __shfl_xor(a1.re, 32, 64);
+                a1.im += __shfl_xor(a1.im, 32, 64);
+                if (h == 0 && on2) {
+                    nacc[seg_off[sg] + p0] = alpha * a0;
+                    nacc[seg_off[sg] + p0 + 1] = alpha * a1;
+                }
+                continue;
             }
-            for (; j < bk.nc; ++j) acc += ldx(d + j * nr) * x[cj[j]];
-        } else {
-            const T* uu = U + bk.u_off + pp;
-            const c128* tb = t + bk.p_off;
-            int k = 0;
-            for (; k + 3 < bk.rank; k += 4) {
-                c128 a[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) a[u] = ldx(uu + (k + u) * nr);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) acc += a[u] * tb[k + u];
-            }
-            for (; k < bk.rank; ++k) acc += ldx(uu + k * nr) * tb[k];
         }
+        const bool on = p < nr;
+        const int64_t pp = on ? p : 0;          // lanes past the cluster read row 0
+        c128 acc{0.0, 0.0};
+        for (int64_t eb = e0; eb < e1; eb += 64) {
+            const int nq = (int)min<int64_t>(64, e1 - eb);
+            HmvWork mine{0, 0, 0, 0};
+            if (lane < nq) mine = work[eb + lane];
+            for (int q = 0; q < nq; ++q) {
+                const int64_t off = bcast64(mine.off, q);
+                const int64_t xoff = bcast64(mine.xoff, q);
+                const int c = __builtin_amdgcn_readlane(mine.cnt, q);
+                const int vtc = __builtin_amdgcn_readlane(mine.vtc, q);
+                if (vtc == 0) {
+                    const T* d = D + off + pp;
+                    const c128* xc = xp + xoff;  // wave-uniform, contiguous
+                    int j = 0;
+                    for (; j + 15 < c; j += 16) {
+                        c128 a[16];
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) a[u] = ldx(d + int64_t(j + u) * nr);
+#pragma unroll
+                        for (int u = 0; u < 16; ++u) acc += a[u] * xc[j + u];
+                    }
+                    for (; j + 3 < c; j += 4) {
+                        c128 a[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) a[u] = ldx(d + int64_t(j + u) * nr);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) acc += a[u] * xc[j + u];
+                    }
+                    for (; j < c; ++j) acc += ldx(d + int64_t(j) * nr) * xc[j];
+                } else {
+                    const T* uu = U + off + pp;
+                    const c128* tb = t + xoff;
+                    for (int k = 0; k < c; ++k) {
+                        c128 tk = tb[k];
+                        for (int ch = 1; ch < vtc; ++ch) tk += tb[int64_t(ch) * c + k];
+                        acc += ldx(uu + int64_t(k) * nr) * tk;
+                    }
+                }
+            }
+        }
+        if (on) nacc[seg_off[sg] + p] = alpha * acc;
     }
-    if (on) nacc[node_off[c] + p] = alpha * acc;
 }
 
 // y[i] = beta y[i] + sum of the contributions of row i (fixed order)
@@ -987,9 +1157,36 @@ __global__ void hmat_full_kernel(const HBlkDev* __restrict__ blks, const int64_t
     }
 }
 
+// low-rank factors to the depth-first row order of their clusters:
+// row i of a factor moves to pos[list offset + i]
+__global__ void reorder_lr_kernel(const HBlkDev* __restrict__ blks, const int32_t* __restrict__ list,
+                                  const int64_t* __restrict__ pos, const c128* __restrict__ U,
+                                  const c128* __restrict__ V, c128* U2, c128* V2) {
+    const HBlkDev bk = blks[list[blockIdx.x]];
+    const int64_t cu = bk.nr * bk.rank, cv = bk.nc * bk.rank;
+    for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cu;
+         e += int64_t(blockDim.x) * gridDim.y) {
+        const int64_t i = e % bk.nr, k = e / bk.nr;
+        U2[bk.u_off + pos[bk.r_off + i] + k * bk.nr] = U[bk.u_off + e];
+    }
+    for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cv;
+         e += int64_t(blockDim.x) * gridDim.y) {
+        const int64_t j = e % bk.nc, k = e / bk.nc;
+        V2[bk.v_off + pos[bk.c_off + j] + k * bk.nc] = V[bk.v_off + e];
+    }
+}
+
+// xp[p] = x[perm[p]]
+__global__ void hmv_gather_kernel(int64_t n, const int64_t* __restrict__ perm,
+                                  const c128* __restrict__ x, c128* __restrict__ xp) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < n) xp[p] = x[perm[p]];
+}
+
 // FP32 storage: the first rank columns of every low-rank block's U and V
 // (contiguous, nr * rank and nc * rank entries) packed at the new offsets
-// held in w_off / f_off, and every dense block (same offsets), as c64
+// held in w_off / f_off, and every dense block at the offset held in u2_off,
+// as c64
 __global__ void hmat_to_c64_kernel(const HBlkDev* __restrict__ blks, const c128* __restrict__ U,
                                    const c128* __restrict__ V, const c128* __restrict__ D,
                                    c64* U32, c64* V32, c64* D32) {
@@ -1007,7 +1204,7 @@ __global__ void hmat_to_c64_kernel(const HBlkDev* __restrict__ blks, const c128*
         const int64_t cd = bk.nr * bk.nc;
         for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cd;
              e += int64_t(blockDim.x) * gridDim.y)
-            D32[bk.d_off + e] = cvt(D[bk.d_off + e]);
+            D32[bk.u2_off + e] = cvt(D[bk.d_off + e]);
     }
 }
 
@@ -1039,17 +1236,25 @@ struct ccab_hmatrix {
     c64* d_U32 = nullptr;
     c64* d_V32 = nullptr;
     c64* d_D32 = nullptr;
-    // matvec plan: t = V^T x per low-rank block, one slot per (cluster, row)
+    // matvec plan: xp = x in the depth-first order of the column tree
+    // (xp[p] = x[perm[p]]), t = V^T xp per low-rank block, one slot per
+    // (cluster, row)
+    int64_t* d_perm = nullptr;
+    c128* d_xp = nullptr;
     c128* d_t = nullptr;
     c128* d_nacc = nullptr;
-    int32_t* d_tile_node = nullptr;  // 64-row tiles of the clusters that own blocks
-    int32_t* d_tile_r0 = nullptr;
-    int64_t* d_nptr = nullptr;       // cluster -> its blocks (CSR over d_nlist)
-    int32_t* d_nlist = nullptr;
-    int64_t* d_node_off = nullptr;   // cluster -> first slot
+    VtItem* d_vt_items = nullptr;    // V^T x items: (low-rank block, column chunk)
+    int64_t n_vt = 0;
+    int32_t* d_item_seg = nullptr;   // product items: (segment, 64-row tile)
+    int32_t* d_item_r0 = nullptr;
+    int64_t n_items = 0;
+    int32_t* d_seg_nr = nullptr;     // segment -> rows of its cluster, entries [e0, e1) of d_work
+    int32_t* d_seg_e0 = nullptr;
+    int32_t* d_seg_e1 = nullptr;
+    int64_t* d_seg_off = nullptr;    // segment -> first slot
+    HmvWork* d_work = nullptr;       // clusters' blocks (dense first, then low-rank)
     int64_t* d_rowptr = nullptr;     // row -> its slots (CSR over d_ent)
     int64_t* d_ent = nullptr;
-    int64_t n_tiles = 0;
     int32_t* d_dense_list = nullptr;
     int32_t* d_lr_list = nullptr;
     int64_t n_dense = 0, n_lr = 0;
@@ -1089,7 +1294,7 @@ struct BuildTrace {
 // has nrow rows and ncol columns (square for the cluster-tree H matrices; one
 // rectangular admissible block for a plain ACA).
 template <typename E>
-int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>& T,
+int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
                         const std::vector<HostBlock>& bl, int64_t nrow, int64_t ncol, double tol,
                         ccab_hmatrix** out, int prec, BuildTrace& trace) {
     hipStream_t s = ctx->stream;
@@ -1305,11 +1510,14 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>&
     std::sort(dl.begin(), dl.end());
     H->n_dense = (int64_t)dl.size();
     H->n_lr = (int64_t)ll2.size();
-    // V^T x of every low-rank block: rank entries at p_off
+    // V^T x of every low-rank block: one rank-vector of partials per chunk
+    // of kVtChunk columns, at p_off
     int64_t t_total = 0;
     for (int32_t b : ll2) {
-        H->blk[b].p_off = t_total;
-        t_total += H->blk[b].rank;
+        HBlkDev& d = H->blk[b];
+        d.vtc = (int32_t)((d.nc + kVtChunk - 1) / kVtChunk);
+        d.p_off = t_total;
+        t_total += int64_t(d.vtc) * d.rank;
     }
     H->d_D = dalloc<c128>(H->owned, d_off);
     H->d_t = dalloc<c128>(H->owned, t_total);
@@ -1317,6 +1525,97 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>&
     CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
+    // Storage order of the matvec: every cluster's points in the depth-first
+    // leaf order of its tree, so that a cluster is one contiguous range of
+    // positions (the product permutes x once and reads it contiguously).  The
+    // ACA above ran on the cluster lists in the order of cluster.rs (its
+    // pivots depend on it); the low-rank factors' rows move to the new order
+    // here and the dense blocks are built in it.
+    {
+        std::vector<int> parent(T.size(), -1);
+        for (size_t c = 0; c < T.size(); ++c)
+            for (int q : T[c].son)
+                if (q >= 0) parent[q] = (int)c;
+        std::vector<int64_t> lo(T.size(), 0);
+        std::vector<int> root_of(T.size(), 0);
+        std::vector<std::vector<int64_t>> perm_of;   // per root: position -> point
+        std::vector<std::vector<int64_t>> inv_of;    // per root: point -> position
+        std::vector<int> roots;
+        for (size_t c = 0; c < T.size(); ++c)
+            if (parent[c] < 0) roots.push_back((int)c);
+        for (size_t ri = 0; ri < roots.size(); ++ri) {
+            std::vector<int64_t> perm;
+            // depth-first, son 0 before son 1; lo of a cluster = its first leaf's start
+            std::function<void(int)> walk = [&](int c) {
+                root_of[c] = (int)ri;
+                lo[c] = (int64_t)perm.size();
+                if (T[c].son[0] < 0) {
+                    perm.insert(perm.end(), T[c].idx.begin(), T[c].idx.end());
+                } else {
+                    walk(T[c].son[0]);
+                    walk(T[c].son[1]);
+                }
+            };
+            walk(roots[ri]);
+            int64_t mx = 0;
+            for (int64_t g : perm) mx = std::max(mx, g + 1);
+            std::vector<int64_t> inv(mx, -1);
+            for (size_t p = 0; p < perm.size(); ++p) inv[perm[p]] = (int64_t)p;
+            perm_of.push_back(std::move(perm));
+            inv_of.push_back(std::move(inv));
+        }
+        // position of every list entry in its cluster's new order, then the new lists
+        std::vector<int64_t> pos(nidx);
+        for (size_t c = 0; c < T.size(); ++c) {
+            const auto& inv = inv_of[root_of[c]];
+            for (size_t i = 0; i < T[c].idx.size(); ++i)
+                pos[node_off[c] + i] = inv[T[c].idx[i]] - lo[c];
+            const auto& perm = perm_of[root_of[c]];
+            std::copy(perm.begin() + lo[c], perm.begin() + lo[c] + (int64_t)T[c].idx.size(),
+                      T[c].idx.begin());
+            std::copy(T[c].idx.begin(), T[c].idx.end(), rows.begin() + node_off[c]);
+        }
+        for (int64_t b = 0; b < nb; ++b) H->blk[b].clo = lo[bl[b].c];
+        const int col_root = root_of[bl.empty() ? 0 : bl[0].c];
+        H->d_perm = dalloc<int64_t>(H->owned, perm_of[col_root].size());
+        H->d_xp = dalloc<c128>(H->owned, perm_of[col_root].size());
+        int64_t* d_pos = dalloc<int64_t>(H->owned, nidx);
+        if (!H->d_perm || !H->d_xp || !d_pos) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        CCAB_HIP(hipMemcpy(H->d_perm, perm_of[col_root].data(), perm_of[col_root].size() * 8,
+                           hipMemcpyHostToDevice));
+        CCAB_HIP(hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
+        CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+        if (!ll2.empty()) {
+            int64_t cu = 0, cv = 0;
+            for (int32_t b : ll2) {
+                cu = std::max(cu, H->blk[b].u_off + H->blk[b].nr * H->blk[b].rank);
+                cv = std::max(cv, H->blk[b].v_off + H->blk[b].nc * H->blk[b].rank);
+            }
+            c128* U2 = dalloc<c128>(H->owned, cu);
+            c128* V2 = dalloc<c128>(H->owned, cv);
+            if (!U2 || !V2) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+            hipLaunchKernelGGL(reorder_lr_kernel, dim3((unsigned)ll2.size(), 4), dim3(256), 0, s,
+                               H->d_blk, H->d_lr_list, d_pos, H->d_U, H->d_V, U2, V2);
+            CCAB_HIP(hipGetLastError());
+            CCAB_HIP(hipStreamSynchronize(s));
+            for (void* old : {static_cast<void*>(H->d_U), static_cast<void*>(H->d_V)}) {
+                auto it = std::find(H->owned.begin(), H->owned.end(), old);
+                if (it != H->owned.end()) {
+                    (void)hipFree(*it);
+                    H->owned.erase(it);
+                }
+            }
+            H->d_U = U2;
+            H->d_V = V2;
+        }
+        CCAB_HIP(hipMemcpy(H->d_rows, rows.data(), rows.size() * 8, hipMemcpyHostToDevice));
+        auto it = std::find(H->owned.begin(), H->owned.end(), static_cast<void*>(d_pos));
+        if (it != H->owned.end()) {
+            (void)hipFree(*it);
+            H->owned.erase(it);
+        }
+        trace.mark("depth-first storage order");
+    }
     if (!dl.empty()) {
         hipLaunchKernelGGL(dense_block_kernel<E>, dim3((unsigned)dl.size(), 4), dim3(256), 0, s,
                            entry, H->d_blk, H->d_dense_list, H->d_rows, H->d_cols, H->d_D);
@@ -1348,62 +1647,141 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>&
             for (int64_t b = 0; b < nb; ++b)
                 if ((pass == 0) == (H->blk[b].adm == 0)) nlist[fill[bl[b].r]++] = (int32_t)b;
     }
-    std::vector<int32_t> tile_node, tile_r0;
-    std::vector<int64_t> cnt(n + 1, 0);
+    // segments: runs of a cluster's blocks (in nlist order) worth at most
+    // kNodeBudget columns each (at least one block), one slot per (segment,
+    // row); items: (segment, 64-row tile)
+    std::vector<int32_t> seg_node, seg_e0, seg_e1, item_seg, item_r0;
+    std::vector<int64_t> seg_off, seg_work;
+    int64_t nslots = 0;
+    const int64_t budget = ctx->hmv_node_budget > 0 ? ctx->hmv_node_budget : INT64_MAX;
     for (int64_t c = 0; c < nn; ++c) {
-        if (nptr[c + 1] == nptr[c]) continue;
-        const int64_t sz = (int64_t)T[c].idx.size();
-        for (int64_t r0 = 0; r0 < sz; r0 += 64) {
-            tile_node.push_back((int32_t)c);
-            tile_r0.push_back((int32_t)r0);
+        int64_t e = nptr[c];
+        while (e < nptr[c + 1]) {
+            const int64_t ea = e;
+            int64_t w = 0;
+            do {
+                const HBlkDev& d = H->blk[nlist[e]];
+                w += d.adm ? d.rank : d.nc;
+                ++e;
+            } while (e < nptr[c + 1] && w + (H->blk[nlist[e]].adm ? H->blk[nlist[e]].rank
+                                                                    : H->blk[nlist[e]].nc) <=
+                                            budget);
+            seg_node.push_back((int32_t)c);
+            seg_e0.push_back((int32_t)ea);
+            seg_e1.push_back((int32_t)e);
+            seg_off.push_back(nslots);
+            seg_work.push_back(w);
+            nslots += (int64_t)T[c].idx.size();
         }
-        for (int64_t q = 0; q < sz; ++q) cnt[T[c].idx[q] + 1]++;
     }
+    const int64_t nseg = (int64_t)seg_node.size();
+    std::vector<int32_t> seg_nr(nseg);
+    for (int64_t sg = 0; sg < nseg; ++sg) seg_nr[sg] = (int32_t)T[seg_node[sg]].idx.size();
+    for (int64_t sg = 0; sg < nseg; ++sg)
+        for (int64_t r0 = 0; r0 < (int64_t)T[seg_node[sg]].idx.size(); r0 += 64) {
+            item_seg.push_back((int32_t)sg);
+            item_r0.push_back((int32_t)r0);
+        }
+    // every row's slots, in segment order (a fixed summation order)
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (int64_t sg = 0; sg < nseg; ++sg)
+        for (int64_t g : T[seg_node[sg]].idx) cnt[g + 1]++;
     for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
     std::vector<int64_t> ent(cnt[n]);
     {
         std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-        for (int64_t c = 0; c < nn; ++c) {
-            if (nptr[c + 1] == nptr[c]) continue;
-            for (int64_t q = 0; q < (int64_t)T[c].idx.size(); ++q)
-                ent[fill[T[c].idx[q]]++] = node_off[c] + q;
+        for (int64_t sg = 0; sg < nseg; ++sg) {
+            const auto& idx = T[seg_node[sg]].idx;
+            for (int64_t q = 0; q < (int64_t)idx.size(); ++q) ent[fill[idx[q]]++] = seg_off[sg] + q;
         }
     }
-    H->n_tiles = (int64_t)tile_node.size();
-    H->d_nacc = dalloc<c128>(H->owned, nidx);
-    H->d_tile_node = dalloc<int32_t>(H->owned, tile_node.size());
-    H->d_tile_r0 = dalloc<int32_t>(H->owned, tile_r0.size());
-    H->d_nptr = dalloc<int64_t>(H->owned, nn + 1);
-    H->d_nlist = dalloc<int32_t>(H->owned, nb);
-    H->d_node_off = dalloc<int64_t>(H->owned, nn);
+    // items by decreasing work (stable: equal work keeps the tree order)
+    if (ctx->hmv_sort) {
+        std::vector<size_t> ord(item_seg.size());
+        for (size_t q = 0; q < ord.size(); ++q) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+            return seg_work[item_seg[x]] > seg_work[item_seg[y]];
+        });
+        std::vector<int32_t> a2(ord.size()), b2(ord.size());
+        for (size_t q = 0; q < ord.size(); ++q) {
+            a2[q] = item_seg[ord[q]];
+            b2[q] = item_r0[ord[q]];
+        }
+        item_seg.swap(a2);
+        item_r0.swap(b2);
+    }
+    std::vector<int32_t> vt_blk, vt_chunk;
+    std::vector<int64_t> vt_work;
+    for (int32_t b : ll2) {
+        const HBlkDev& d = H->blk[b];
+        for (int32_t ch = 0; ch < d.vtc; ++ch) {
+            vt_blk.push_back(b);
+            vt_chunk.push_back(ch);
+            vt_work.push_back(std::min<int64_t>(kVtChunk, d.nc - int64_t(ch) * kVtChunk) * d.rank);
+        }
+    }
+    if (ctx->hmv_sort) {
+        std::vector<size_t> ord(vt_blk.size());
+        for (size_t q = 0; q < ord.size(); ++q) ord[q] = q;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](size_t x, size_t y) { return vt_work[x] > vt_work[y]; });
+        std::vector<int32_t> a2(ord.size()), b2(ord.size());
+        for (size_t q = 0; q < ord.size(); ++q) {
+            a2[q] = vt_blk[ord[q]];
+            b2[q] = vt_chunk[ord[q]];
+        }
+        vt_blk.swap(a2);
+        vt_chunk.swap(b2);
+    }
+    H->n_vt = (int64_t)vt_blk.size();
+    H->n_items = (int64_t)item_seg.size();
+    H->d_nacc = dalloc<c128>(H->owned, nslots);
+    H->d_vt_items = dalloc<VtItem>(H->owned, vt_blk.size());
+    H->d_item_seg = dalloc<int32_t>(H->owned, item_seg.size());
+    H->d_item_r0 = dalloc<int32_t>(H->owned, item_r0.size());
+    H->d_seg_nr = dalloc<int32_t>(H->owned, nseg);
+    H->d_work = dalloc<HmvWork>(H->owned, nb);
+    H->d_seg_e0 = dalloc<int32_t>(H->owned, nseg);
+    H->d_seg_e1 = dalloc<int32_t>(H->owned, nseg);
+    H->d_seg_off = dalloc<int64_t>(H->owned, nseg);
     H->d_rowptr = dalloc<int64_t>(H->owned, n + 1);
     H->d_ent = dalloc<int64_t>(H->owned, ent.size());
-    if (!H->d_nacc || !H->d_tile_node || !H->d_tile_r0 || !H->d_nptr || !H->d_nlist ||
-        !H->d_node_off || !H->d_rowptr || !H->d_ent)
+    if (!H->d_nacc || !H->d_vt_items || !H->d_item_seg || !H->d_item_r0 ||
+        !H->d_seg_nr || !H->d_work || !H->d_seg_e0 || !H->d_seg_e1 || !H->d_seg_off ||
+        !H->d_rowptr || !H->d_ent)
         return fail(CCAB_ENOMEM, "H-matrix allocation failed");
-    CCAB_HIP(hipMemcpy(H->d_tile_node, tile_node.data(), tile_node.size() * 4, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_tile_r0, tile_r0.data(), tile_r0.size() * 4, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_nptr, nptr.data(), (nn + 1) * 8, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_nlist, nlist.data(), nb * 4, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_node_off, node_off.data(), nn * 8, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_item_seg, item_seg.data(), item_seg.size() * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_item_r0, item_r0.data(), item_r0.size() * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_nr, seg_nr.data(), nseg * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_e0, seg_e0.data(), nseg * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_e1, seg_e1.data(), nseg * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_off, seg_off.data(), nseg * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), ent.size() * 8, hipMemcpyHostToDevice));
     trace.mark("matvec plan");
     std::vector<void*> release;
     if (prec == CCAB_C64) {
         // FP32 storage: U and V packed to their ranks at new offsets (staged
-        // in w_off / f_off for the conversion), dense blocks at their offsets
-        int64_t cu = 0, cv = 0;
+        // in w_off / f_off for the conversion), dense blocks at new offsets
+        // (staged in u2_off); every offset even, so that two consecutive
+        // complex-FP32 entries of a column are one aligned 16-byte load
+        int64_t cu = 0, cv = 0, cd = 0;
+        auto even = [](int64_t v) { return (v + 1) & ~int64_t(1); };
         for (int32_t b : ll2) {
             HBlkDev& d = H->blk[b];
             d.w_off = cu;
-            cu += d.nr * d.rank;
+            cu = even(cu + d.nr * d.rank);
             d.f_off = cv;
-            cv += d.nc * d.rank;
+            cv = even(cv + d.nc * d.rank);
+        }
+        for (int32_t b : dl) {
+            HBlkDev& d = H->blk[b];
+            d.u2_off = cd;
+            cd = even(cd + d.nr * d.nc);
         }
         H->d_U32 = dalloc<c64>(H->owned, cu);
         H->d_V32 = dalloc<c64>(H->owned, cv);
-        H->d_D32 = dalloc<c64>(H->owned, d_off);
+        H->d_D32 = dalloc<c64>(H->owned, cd);
         if (!H->d_U32 || !H->d_V32 || !H->d_D32) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
         CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
         if (nb)
@@ -1415,6 +1793,7 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>&
             d.u_off = d.w_off;
             d.v_off = d.f_off;
         }
+        for (int32_t b : dl) H->blk[b].d_off = H->blk[b].u2_off;
         CCAB_HIP(hipMemcpyAsync(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice, s));
         CCAB_HIP(hipStreamSynchronize(s));
         release.insert(release.end(), {H->d_U, H->d_V, H->d_D});
@@ -1430,6 +1809,27 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, const std::vector<HNode>&
             (void)hipFree(*it);
             H->owned.erase(it);
         }
+    }
+    // the product's work entries, with the final storage offsets (FP32
+    // storage repacks U)
+    {
+        std::vector<HmvWork> work(nb);
+        for (int64_t e = 0; e < nb; ++e) {
+            const HBlkDev& d = H->blk[nlist[e]];
+            work[e] = d.adm ? HmvWork{d.u_off, d.p_off, d.rank, d.vtc}
+                            : HmvWork{d.d_off, d.clo, (int32_t)d.nc, 0};
+        }
+        CCAB_HIP(hipMemcpy(H->d_work, work.data(), size_t(nb) * sizeof(HmvWork),
+                           hipMemcpyHostToDevice));
+        std::vector<VtItem> items(vt_blk.size());
+        for (size_t q = 0; q < items.size(); ++q) {
+            const HBlkDev& d = H->blk[vt_blk[q]];
+            const int64_t j0 = int64_t(vt_chunk[q]) * kVtChunk;
+            items[q] = VtItem{d.v_off + j0, d.clo + j0, d.p_off + int64_t(vt_chunk[q]) * d.rank, d.nc,
+                              (int32_t)std::min<int64_t>(kVtChunk, d.nc - j0), d.rank};
+        }
+        CCAB_HIP(hipMemcpy(H->d_vt_items, items.data(), items.size() * sizeof(VtItem),
+                           hipMemcpyHostToDevice));
     }
     trace.mark("workspace release");
     *out = owner.release();
@@ -1457,7 +1857,7 @@ int hmatrix_build(ccab_ctx* ctx, const SysParams& p, const double* d_soa, const 
     trace.mark("cluster and block trees");
     const BemEntry entry{p, d_soa, n, side == CCAB_LHS ? p.lhs_a : p.rhs_a,
                          side == CCAB_LHS ? p.lhs_b : p.rhs_b};
-    return hmatrix_from_blocks(ctx, entry, T, bl, n, n, tol, out, prec, trace);
+    return hmatrix_from_blocks(ctx, entry, std::move(T), bl, n, n, tol, out, prec, trace);
 }
 
 int hmatrix_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, const double* h_xyz,
@@ -1468,7 +1868,7 @@ int hmatrix_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, const doub
     std::vector<HostBlock> bl;
     trees(h_xyz, 3, n, leaf, eta, T, bl);
     trace.mark("cluster and block trees");
-    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, T, bl, n, n, tol, out, prec, trace);
+    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, std::move(T), bl, n, n, tol, out, prec, trace);
 }
 
 int aca_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, int64_t m, int64_t ncol,
@@ -1481,29 +1881,40 @@ int aca_build_matrix(ccab_ctx* ctx, const c128* d_A, int64_t lda, int64_t m, int
     for (int64_t i = 0; i < m; ++i) T[0].idx[i] = i;
     for (int64_t j = 0; j < ncol; ++j) T[1].idx[j] = j;
     const std::vector<HostBlock> bl{{0, 1, true}};
-    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, T, bl, m, ncol, tol, out, prec, trace);
+    return hmatrix_from_blocks(ctx, MatEntry{d_A, lda}, std::move(T), bl, m, ncol, tol, out, prec, trace);
 }
 
 int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x, c128 beta,
                  c128* y) {
     hipStream_t s = ctx->stream;
     ProfScope ps(ctx, PROF_HMAT, H->mv_bytes);
+    hipLaunchKernelGGL(hmv_gather_kernel, dim3((unsigned)((H->ncol + 255) / 256)), dim3(256), 0, s,
+                       H->ncol, H->d_perm, x, H->d_xp);
+    // one item per wave (64-thread workgroups), or persistent waves capped
+    // per CU (CCAB_HMV_WAVES_PER_CU, four per workgroup)
+    const int64_t cap = int64_t(std::max(ctx->num_cus, 1)) * std::max(ctx->hmv_waves_per_cu, 0);
+    auto grid_of = [&](int64_t items) {
+        return cap > 0 ? dim3((unsigned)((std::min(items, cap) + 3) / 4)) : dim3((unsigned)items);
+    };
+    const dim3 blk(cap > 0 ? 256 : 64);
     if (H->fp32) {
-        if (H->n_lr)
-            hipLaunchKernelGGL(hmv_vt_kernel<c64>, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
-                               H->d_lr_list, H->d_cols, H->d_V32, x, H->d_t);
-        if (H->n_tiles)
-            hipLaunchKernelGGL(hmv_node_kernel<c64>, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
-                               H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
-                               H->d_blk, H->d_cols, H->d_D32, H->d_U32, H->d_t, x, alpha, H->d_nacc);
+        if (H->n_vt)
+            hipLaunchKernelGGL(hmv_vt_kernel<c64>, grid_of(H->n_vt), blk, 0, s, H->d_vt_items,
+                               H->n_vt, H->d_V32, H->d_xp, H->d_t, ctx->hmv_paired & 1);
+        if (H->n_items)
+            hipLaunchKernelGGL(hmv_node_kernel<c64>, grid_of(H->n_items), blk, 0, s, H->d_item_seg,
+                               H->d_item_r0, H->n_items, H->d_seg_nr, H->d_seg_e0, H->d_seg_e1,
+                               H->d_seg_off, H->d_work, H->d_D32, H->d_U32, H->d_t,
+                               H->d_xp, alpha, H->d_nacc, (ctx->hmv_paired >> 1) & 1);
     } else {
-        if (H->n_lr)
-            hipLaunchKernelGGL(hmv_vt_kernel<c128>, dim3((unsigned)H->n_lr), dim3(64), 0, s, H->d_blk,
-                               H->d_lr_list, H->d_cols, H->d_V, x, H->d_t);
-        if (H->n_tiles)
-            hipLaunchKernelGGL(hmv_node_kernel<c128>, dim3((unsigned)H->n_tiles), dim3(64), 0, s,
-                               H->d_tile_node, H->d_tile_r0, H->d_nptr, H->d_nlist, H->d_node_off,
-                               H->d_blk, H->d_cols, H->d_D, H->d_U, H->d_t, x, alpha, H->d_nacc);
+        if (H->n_vt)
+            hipLaunchKernelGGL(hmv_vt_kernel<c128>, grid_of(H->n_vt), blk, 0, s, H->d_vt_items,
+                               H->n_vt, H->d_V, H->d_xp, H->d_t, 0);
+        if (H->n_items)
+            hipLaunchKernelGGL(hmv_node_kernel<c128>, grid_of(H->n_items), blk, 0, s, H->d_item_seg,
+                               H->d_item_r0, H->n_items, H->d_seg_nr, H->d_seg_e0, H->d_seg_e1,
+                               H->d_seg_off, H->d_work, H->d_D, H->d_U, H->d_t,
+                               H->d_xp, alpha, H->d_nacc, 0);
     }
     hipLaunchKernelGGL(hmv_reduce_kernel, dim3((unsigned)((H->n + 255) / 256)), dim3(256), 0, s,
                        H->n, H->d_rowptr, H->d_ent, H->d_nacc, beta, y);
