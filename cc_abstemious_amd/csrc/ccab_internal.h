@@ -216,6 +216,8 @@ struct ccab_ctx {
     // FP32 H-matrix storage: bit 0 two columns per 16-byte load in V^T x,
     // bit 1 two rows per 16-byte load in the product (CCAB_HMV_PAIRED)
     int hmv_paired = 3;
+    // non-temporal loads of the H-matrix blocks in the product (CCAB_HMV_NT)
+    bool hmv_nt = true;
     std::vector<AsmTune> asm_tune;
     std::vector<GemvTune> gemv_tune;
     // device bytes a sweep or a batch of bodies may hold at once (0: 70% of

@@ -871,6 +871,40 @@ __device__ __forceinline__ c128 ldx(const c64* p) {
     return c128{double(v.re), double(v.im)};
 }
 
+// block entries streamed once per product: non-temporal loads (NT) keep x,
+// t and the slots in the caches
+typedef double hf64x2_t __attribute__((ext_vector_type(2)));
+typedef float hf32x2_t __attribute__((ext_vector_type(2)));
+typedef float hf32x4_t __attribute__((ext_vector_type(4)));
+template <bool NT>
+__device__ __forceinline__ c128 ldm(const c128* p) {
+    if constexpr (NT) {
+        const hf64x2_t v = __builtin_nontemporal_load(reinterpret_cast<const hf64x2_t*>(p));
+        return c128{v.x, v.y};
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ c128 ldm(const c64* p) {
+    if constexpr (NT) {
+        const hf32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const hf32x2_t*>(p));
+        return c128{double(v.x), double(v.y)};
+    } else {
+        return ldx(p);
+    }
+}
+// two consecutive complex-FP32 entries (16-byte aligned)
+template <bool NT>
+__device__ __forceinline__ float4 ldm2(const c64* p) {
+    if constexpr (NT) {
+        const hf32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const hf32x4_t*>(p));
+        return float4{v.x, v.y, v.z, v.w};
+    } else {
+        return *reinterpret_cast<const float4*>(p);
+    }
+}
+
 // The matvec's two passes work on balanced items: V^T x per (low-rank block,
 // chunk of at most kVtChunk columns), and per (segment of a row cluster's
 // blocks worth at most kNodeBudget columns, 64-row tile); each item writes
@@ -894,7 +928,7 @@ struct VtItem {
 };
 
 // t[item.t + k] = sum over the chunk's columns j of V[item.v + j + k ld] xp[item.x + j]
-template <typename T>
+template <typename T, bool NT>
 __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ items, int64_t cnt,
                                                      const T* __restrict__ V,
                                                      const c128* __restrict__ xp, c128* t,
@@ -921,14 +955,13 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
                                 if (k0 + u < r) {
-                                    const float4 v = *reinterpret_cast<const float4*>(
-                                        Vb + j + int64_t(k0 + u) * q.ld);
+                                    const float4 v = ldm2<NT>(Vb + j + int64_t(k0 + u) * q.ld);
                                     acc[u] += c128{v.x, v.y} * x0 + c128{v.z, v.w} * x1;
                                 }
                         } else {
 #pragma unroll
                             for (int u = 0; u < 8; ++u)
-                                if (k0 + u < r) acc[u] += ldx(Vb + j + int64_t(k0 + u) * q.ld) * x0;
+                                if (k0 + u < r) acc[u] += ldm<NT>(Vb + j + int64_t(k0 + u) * q.ld) * x0;
                         }
                     }
 #pragma unroll
@@ -947,7 +980,7 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
                 const c128 xj = xc[j];
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
-                    if (k0 + u < r) acc[u] += ldx(Vb + j + int64_t(k0 + u) * q.ld) * xj;
+                    if (k0 + u < r) acc[u] += ldm<NT>(Vb + j + int64_t(k0 + u) * q.ld) * xj;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -980,7 +1013,7 @@ __device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
 // r0 .. r0 + 63 of the cluster (seg_nr[s] rows): acc = sum of D_b xp (dense)
 // and U_b t_b (low-rank, t_b summed over its chunks), slot nacc[seg_off[s] +
 // row] = alpha acc
-template <typename T>
+template <typename T, bool NT>
 __global__ __launch_bounds__(256) void hmv_node_kernel(
     const int32_t* __restrict__ item_seg, const int32_t* __restrict__ item_r0, int64_t cnt,
     const int32_t* __restrict__ seg_nr, const int32_t* __restrict__ seg_e0,
@@ -1025,7 +1058,7 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                                 c128 xv[8];
 #pragma unroll
                                 for (int u = 0; u < 8; ++u) {
-                                    v[u] = *reinterpret_cast<const float4*>(d + int64_t(jb + 2 * u) * nr);
+                                    v[u] = ldm2<NT>(d + int64_t(jb + 2 * u) * nr);
                                     const c128 xa = xc[jb + 2 * u], xb = xc[jb + 2 * u + 1];
                                     xv[u] = h ? xb : xa;
                                 }
@@ -1037,7 +1070,7 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                             }
                             for (; jb < c; jb += 2) {
                                 if (jb + h < c) {
-                                    const float4 v = *reinterpret_cast<const float4*>(d + int64_t(jb) * nr);
+                                    const float4 v = ldm2<NT>(d + int64_t(jb) * nr);
                                     const c128 xv = xc[jb + h];
                                     a0 += c128{v.x, v.y} * xv;
                                     a1 += c128{v.z, v.w} * xv;
@@ -1055,7 +1088,7 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                                 }
                                 if (kb + h < c) {
                                     const c128 tk = h ? tc : ta;
-                                    const float4 v = *reinterpret_cast<const float4*>(uu + int64_t(kb) * nr);
+                                    const float4 v = ldm2<NT>(uu + int64_t(kb) * nr);
                                     a0 += c128{v.x, v.y} * tk;
                                     a1 += c128{v.z, v.w} * tk;
                                 }
@@ -1093,25 +1126,25 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                     for (; j + 15 < c; j += 16) {
                         c128 a[16];
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) a[u] = ldx(d + int64_t(j + u) * nr);
+                        for (int u = 0; u < 16; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
 #pragma unroll
                         for (int u = 0; u < 16; ++u) acc += a[u] * xc[j + u];
                     }
                     for (; j + 3 < c; j += 4) {
                         c128 a[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) a[u] = ldx(d + int64_t(j + u) * nr);
+                        for (int u = 0; u < 4; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
 #pragma unroll
                         for (int u = 0; u < 4; ++u) acc += a[u] * xc[j + u];
                     }
-                    for (; j < c; ++j) acc += ldx(d + int64_t(j) * nr) * xc[j];
+                    for (; j < c; ++j) acc += ldm<NT>(d + int64_t(j) * nr) * xc[j];
                 } else {
                     const T* uu = U + off + pp;
                     const c128* tb = t + xoff;
                     for (int k = 0; k < c; ++k) {
                         c128 tk = tb[k];
                         for (int ch = 1; ch < vtc; ++ch) tk += tb[int64_t(ch) * c + k];
-                        acc += ldx(uu + int64_t(k) * nr) * tk;
+                        acc += ldm<NT>(uu + int64_t(k) * nr) * tk;
                     }
                 }
             }
@@ -1897,25 +1930,26 @@ int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x
         return cap > 0 ? dim3((unsigned)((std::min(items, cap) + 3) / 4)) : dim3((unsigned)items);
     };
     const dim3 blk(cap > 0 ? 256 : 64);
+    const int pv = ctx->hmv_paired & 1, pn = (ctx->hmv_paired >> 1) & 1;
+#define CCAB_HMV_LAUNCH(T, NTF, Vp, Dp, Up, PV, PN)                                              \
+    do {                                                                                       \
+        if (H->n_vt)                                                                           \
+            hipLaunchKernelGGL((hmv_vt_kernel<T, NTF>), grid_of(H->n_vt), blk, 0, s,           \
+                               H->d_vt_items, H->n_vt, Vp, H->d_xp, H->d_t, PV);               \
+        if (H->n_items)                                                                        \
+            hipLaunchKernelGGL((hmv_node_kernel<T, NTF>), grid_of(H->n_items), blk, 0, s,      \
+                               H->d_item_seg, H->d_item_r0, H->n_items, H->d_seg_nr,           \
+                               H->d_seg_e0, H->d_seg_e1, H->d_seg_off, H->d_work, Dp, Up,      \
+                               H->d_t, H->d_xp, alpha, H->d_nacc, PN);                         \
+    } while (0)
     if (H->fp32) {
-        if (H->n_vt)
-            hipLaunchKernelGGL(hmv_vt_kernel<c64>, grid_of(H->n_vt), blk, 0, s, H->d_vt_items,
-                               H->n_vt, H->d_V32, H->d_xp, H->d_t, ctx->hmv_paired & 1);
-        if (H->n_items)
-            hipLaunchKernelGGL(hmv_node_kernel<c64>, grid_of(H->n_items), blk, 0, s, H->d_item_seg,
-                               H->d_item_r0, H->n_items, H->d_seg_nr, H->d_seg_e0, H->d_seg_e1,
-                               H->d_seg_off, H->d_work, H->d_D32, H->d_U32, H->d_t,
-                               H->d_xp, alpha, H->d_nacc, (ctx->hmv_paired >> 1) & 1);
+        if (ctx->hmv_nt) CCAB_HMV_LAUNCH(c64, true, H->d_V32, H->d_D32, H->d_U32, pv, pn);
+        else CCAB_HMV_LAUNCH(c64, false, H->d_V32, H->d_D32, H->d_U32, pv, pn);
     } else {
-        if (H->n_vt)
-            hipLaunchKernelGGL(hmv_vt_kernel<c128>, grid_of(H->n_vt), blk, 0, s, H->d_vt_items,
-                               H->n_vt, H->d_V, H->d_xp, H->d_t, 0);
-        if (H->n_items)
-            hipLaunchKernelGGL(hmv_node_kernel<c128>, grid_of(H->n_items), blk, 0, s, H->d_item_seg,
-                               H->d_item_r0, H->n_items, H->d_seg_nr, H->d_seg_e0, H->d_seg_e1,
-                               H->d_seg_off, H->d_work, H->d_D, H->d_U, H->d_t,
-                               H->d_xp, alpha, H->d_nacc, 0);
+        if (ctx->hmv_nt) CCAB_HMV_LAUNCH(c128, true, H->d_V, H->d_D, H->d_U, 0, 0);
+        else CCAB_HMV_LAUNCH(c128, false, H->d_V, H->d_D, H->d_U, 0, 0);
     }
+#undef CCAB_HMV_LAUNCH
     hipLaunchKernelGGL(hmv_reduce_kernel, dim3((unsigned)((H->n + 255) / 256)), dim3(256), 0, s,
                        H->n, H->d_rowptr, H->d_ent, H->d_nacc, beta, y);
     CCAB_HIP(hipGetLastError());
