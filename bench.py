@@ -84,7 +84,9 @@ def parse():
     ap.add_argument("--hm-levels", default="6,5",
                     help="icosphere levels of the H-matrix sub-record (6: 81920 triangles; "
                          "5: 20480, with the dense-LU comparison)")
-    ap.add_argument("--only", choices=["hmatrix", "large"], default=None,
+    ap.add_argument("--bodies-baseline", type=int, default=16,
+                    help="bodies also solved body by body (same solver, FP64 LU) as baselines")
+    ap.add_argument("--only", choices=["hmatrix", "large", "bodies"], default=None,
                     help="run only that sub-record's workload and print it (for profiling)")
     ap.add_argument("--tune-sweep", type=int, default=1,
                     help="1: run the sweep autotuner (assembly split, GEMV workgroup target and "
@@ -188,23 +190,25 @@ def bodies_block(args, ctx):
     dt = time.perf_counter() - t0
     prof = ctx.profile_read(reset=True)
     ctx.profile(False)
-    nsub = min(16, len(bodies))
-    for pd in bodies[:nsub]:  # (baselines warmed the same way)
-        solve.sweep(pd, ctx=ctx)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for pd in bodies[:nsub]:
-        solve.sweep(pd, ctx=ctx)
-    torch.cuda.synchronize()
-    one_rate = nsub / (time.perf_counter() - t1)
-    solve.solve_bodies(bodies[:nsub], solver=Solver(kind=capi.SOLVER_DIRECT), ctx=ctx)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    lu, _, _ = solve.solve_bodies(bodies[:nsub], solver=Solver(kind=capi.SOLVER_DIRECT), ctx=ctx)
-    torch.cuda.synchronize()
-    lu_rate = nsub / (time.perf_counter() - t1)
-    err = max(float(torch.linalg.vector_norm(phis[i] - lu[i]) / torch.linalg.vector_norm(lu[i]))
-              for i in range(nsub))
+    nsub = min(args.bodies_baseline, len(bodies))
+    one_rate = lu_rate = err = None
+    if nsub > 0:
+        for pd in bodies[:nsub]:  # (baselines warmed the same way)
+            solve.sweep(pd, ctx=ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for pd in bodies[:nsub]:
+            solve.sweep(pd, ctx=ctx)
+        torch.cuda.synchronize()
+        one_rate = nsub / (time.perf_counter() - t1)
+        solve.solve_bodies(bodies[:nsub], solver=Solver(kind=capi.SOLVER_DIRECT), ctx=ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lu, _, _ = solve.solve_bodies(bodies[:nsub], solver=Solver(kind=capi.SOLVER_DIRECT), ctx=ctx)
+        torch.cuda.synchronize()
+        lu_rate = nsub / (time.perf_counter() - t1)
+        err = max(float(torch.linalg.vector_norm(phis[i] - lu[i]) / torch.linalg.vector_norm(lu[i]))
+                  for i in range(nsub))
     its = [s_["iterations"] for s_ in st]
     warm = None
     if not args.no_surrogate:
@@ -251,7 +255,7 @@ def bodies_block(args, ctx):
                                 "worst_body_change": int(max(a - b for a, b in zip(itw, its)))},
                 "rel_err_vs_fp64_lu_max": max(
                     float(torch.linalg.vector_norm(phw[i] - lu[i]) / torch.linalg.vector_norm(lu[i]))
-                    for i in range(nsub)),
+                    for i in range(nsub)) if nsub else None,
                 "train_systems": 2000, "train_steps": 8000, "final_loss": losses[-1][1],
                 "train_s": train_s}
     gemv = prof.get("gemv", {"count": 0, "ms": float("nan"), "work": 0.0})
@@ -262,10 +266,12 @@ def bodies_block(args, ctx):
             "all_converged": all(bool(s_["converged"]) for s_ in st),
             "gmres_iters": {"mean": float(np.mean(its)), "max": int(max(its))},
             "body_by_body_solves_per_s": one_rate, "fp64_lu_body_by_body_solves_per_s": lu_rate,
-            "speedup_vs_fp64_lu": (len(bodies) / dt) / lu_rate,
+            "speedup_vs_fp64_lu": (len(bodies) / dt) / lu_rate if lu_rate else None,
             "rel_err_vs_fp64_lu_max": err, "baseline_bodies": nsub,
             "warm_start": warm,
             "ragged_gemv_gbs": (gemv["work"] / (gemv["ms"] * 1e-3) / 1e9) if gemv["count"] else None,
+            "assembly_gentries_per_s": (sum(n * n for n in ns) / (prof["assembly"]["ms"] * 1e-3) / 1e9
+                                        if prof.get("assembly", {}).get("count") else None),
             "kernel_ms": {k: round(v["ms"], 3) for k, v in prof.items()}}
 
 
@@ -438,7 +444,8 @@ def main():
 
     if args.only:
         ctx = capi.default_context()
-        blk = hmatrix_block(args, ctx) if args.only == "hmatrix" else large_dense_block(args, ctx)
+        blk = {"hmatrix": hmatrix_block, "large": large_dense_block,
+               "bodies": bodies_block}[args.only](args, ctx)
         out = {args.only: blk}
         if args.only == "hmatrix":
             out["roofline_hmatvec"] = roofline_hmatvec(blk)

@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "ccab_internal.h"
 #include "greens.h"
@@ -131,6 +132,23 @@ __device__ __forceinline__ void pair_coeffs(const PairGeom<double>& q, double dh
     }
 }
 
+// c with lhs_a H + lhs_b G = w e^{ikr} c off the diagonal, from the
+// coefficients of pair_coeffs and the per-system products lhs_a k,
+// lhs_a / k, lhs_b / k of SysParams: one complex product per entry instead of
+// forming H and G (used where no row sum of G or H is needed)
+template <bool BM>
+__device__ __forceinline__ c128 lhs_coef(const SysParams& p, double a0, double a1, double a2,
+                                         double g0, double g1) {
+    if constexpr (BM)
+        return c128{p.lhs_a.re * a0 - p.lhs_ak.im * a1 - p.lhs_aik.im * a2 + p.lhs_b.re * g0 -
+                        p.lhs_bik.im * g1,
+                    p.lhs_a.im * a0 + p.lhs_ak.re * a1 + p.lhs_aik.re * a2 + p.lhs_b.im * g0 +
+                        p.lhs_bik.re * g1};
+    else  // a2 = 0, g0 = 1, g1 = 0
+        return c128{p.lhs_a.re * a0 - p.lhs_ak.im * a1 + p.lhs_b.re,
+                    p.lhs_a.im * a0 + p.lhs_ak.re * a1 + p.lhs_b.im};
+}
+
 // one LHS entry L to the outputs: out0 c128; out1 OutT (c64, or c32 scaled by
 // h_scale); with OutT = c32 and out2, out2 the complex-FP32 remainder
 // L - fp16(L h_scale) / h_scale
@@ -143,9 +161,9 @@ __device__ __forceinline__ void store_lhs(const SysParams& p, int64_t off, c128 
             const uint16_t hr = to_half_sat(L.re * p.h_scale);
             const uint16_t hi = to_half_sat(L.im * p.h_scale);
             reinterpret_cast<c32*>(out1)[off] = c32{hr, hi};
-            const double inv = 1.0 / p.h_scale;  // a power of two: exact
-            const double br = double(__half2float(__ushort_as_half(hr))) * inv;
-            const double bi = double(__half2float(__ushort_as_half(hi))) * inv;
+            // h_inv = 1 / h_scale, a power of two: the products are exact
+            const double br = double(__half2float(__ushort_as_half(hr))) * p.h_inv;
+            const double bi = double(__half2float(__ushort_as_half(hi))) * p.h_inv;
             reinterpret_cast<c64*>(out2)[off] =
                 c64{static_cast<float>(L.re - br), static_cast<float>(L.im - bi)};
         } else if (out1) {
@@ -198,14 +216,14 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
         const bool diag = j == ic;
         // the diagonal lane evaluates a unit-distance pair and discards it
         const double rx = diag ? 1.0 : s.x[j] - yx;
-        const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
-                                                         s.ny[j], s.nz[j], nyx, nyy, nyz);
+        const PairGeom<double> q = pair_geom_fast<BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
+                                                      s.ny[j], s.nz[j], nyx, nyy, nyz);
         const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
         double a0, a1, a2, g0, g1;
         pair_coeffs<BM>(q, dh_sign, a0, a1, a2, g0, g1);
         double sn, cs, sd = 0.0, cd = 1.0;
-        sincos(k0 * q.rd, &sn, &cs);
-        if (dk != 0.0) sincos(dk * q.rd, &sd, &cd);
+        sincos_fast(k0 * q.rd, &sn, &cs);
+        if (dk != 0.0) sincos_fast(dk * q.rd, &sd, &cd);
         // the phasor carries the weight
         cs *= w;
         sn *= w;
@@ -218,28 +236,38 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     sn = sn * cd + cs * sd;
                     cs = c2;
                 } else {
-                    sincos(p.k * q.rd, &sn, &cs);
+                    sincos_fast(p.k * q.rd, &sn, &cs);
                     cs *= w;
                     sn *= w;
                 }
             }
-            c128 G, H;
-            if (diag) {
-                diag_entry(p, s.area[j], s.wt[j], G, H);
-            } else {
-                const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
-                const c128 ph{cs, sn};
-                H = ph * c128{a0, p.k * a1 + a2 * ik};
-                G = ph * c128{g0, g1 * ik};
-            }
-            if (active)
-                store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, p.lhs_a * H + p.lhs_b * G, out0,
-                                out1, out2);
+            const c128 ph{cs, sn};
             if (rowsum_part && p.rowsum_of != 0) {
+                c128 G, H;
+                if (diag) {
+                    diag_entry(p, s.area[j], s.wt[j], G, H);
+                } else {
+                    const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
+                    H = ph * c128{a0, p.k * a1 + a2 * ik};
+                    G = ph * c128{g0, g1 * ik};
+                }
+                if (active)
+                    store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld,
+                                    p.lhs_a * H + p.lhs_b * G, out0, out1, out2);
                 const c128 a = p.rowsum_of == 1 ? G : H;
                 double2& r = rs(f);
                 r.x += a.re;
                 r.y += a.im;
+            } else {
+                c128 L;
+                if (diag) {
+                    c128 G, H;
+                    diag_entry(p, s.area[j], s.wt[j], G, H);
+                    L = p.lhs_a * H + p.lhs_b * G;
+                } else {
+                    L = ph * lhs_coef<BM>(p, a0, a1, a2, g0, g1);
+                }
+                if (active) store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, L, out0, out1, out2);
             }
         }
     }
@@ -429,7 +457,9 @@ __global__ void bodies_incident_kernel(const BodyDesc* __restrict__ desc,
 
 // Fused LHS assembly of every body (one system each, FP64 arithmetic): the
 // 1-D grid is every body's (row block of 256, column chunk of 64) tiles back
-// to back; the pair as in the sweep kernel with the phasor from sincos.
+// to back; the pair as in the sweep kernel, with the phasor from sincos_fast.
+// A body whose right-hand side needs no row sum (rowsum_of 0) takes the
+// column loop that forms lhs_a H + lhs_b G as one complex product per entry.
 template <bool BM, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const SysParams* __restrict__ params, const BodyDesc* __restrict__ desc, int nbody,
@@ -437,7 +467,7 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     double* __restrict__ rowsum_part) {
     const int c = body_of_wg(desc, nbody, blockIdx.x);
     const BodyDesc d = desc[c];
-    const SysParams& p = params[c];
+    const SysParams p = params[c];  // a private copy: not reloaded after each store
     const int64_t n = d.n;
     const int64_t rowblocks = (n + kRows - 1) / kRows;
     const int64_t local = int64_t(blockIdx.x) - d.wg0;
@@ -452,33 +482,47 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
     const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
     double rs_re = 0.0, rs_im = 0.0;
-    for (int64_t j = j0; j < j1; ++j) {
-        const double wa = s.wt[j] * s.area[j];
-        const bool diag = j == ic;
-        const double rx = diag ? 1.0 : s.x[j] - yx;
-        const PairGeom<double> q = pair_geom<double, BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
-                                                         s.ny[j], s.nz[j], nyx, nyy, nyz);
-        c128 G, H;
-        if (diag) {
-            diag_entry(p, s.area[j], s.wt[j], G, H);
-        } else {
-            double a0, a1, a2, g0, g1;
-            pair_coeffs<BM>(q, p.dh_sign, a0, a1, a2, g0, g1);
-            const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
-            double sn, cs;
-            sincos(p.k * q.rd, &sn, &cs);
-            const c128 ph{cs * w, sn * w};
-            H = ph * c128{a0, p.k * a1 + a2 * ik};
-            G = ph * c128{g0, g1 * ik};
+    auto columns = [&](auto row_sums) {
+        constexpr bool RS = decltype(row_sums)::value;
+        for (int64_t j = j0; j < j1; ++j) {
+            const double wa = s.wt[j] * s.area[j];
+            const bool diag = j == ic;
+            const double rx = diag ? 1.0 : s.x[j] - yx;
+            const PairGeom<double> q = pair_geom_fast<BM>(rx, s.y[j] - yy, s.z[j] - yz, s.nx[j],
+                                                          s.ny[j], s.nz[j], nyx, nyy, nyz);
+            c128 L, G, H;
+            if (diag) {
+                diag_entry(p, s.area[j], s.wt[j], G, H);
+                L = p.lhs_a * H + p.lhs_b * G;
+            } else {
+                double a0, a1, a2, g0, g1;
+                pair_coeffs<BM>(q, p.dh_sign, a0, a1, a2, g0, g1);
+                const double w = wa * q.inv_r * (1.0 / (4.0 * kPi));
+                double sn, cs;
+                sincos_fast(p.k * q.rd, &sn, &cs);
+                const c128 ph{cs * w, sn * w};
+                if constexpr (RS) {
+                    H = ph * c128{a0, p.k * a1 + a2 * ik};
+                    G = ph * c128{g0, g1 * ik};
+                    L = p.lhs_a * H + p.lhs_b * G;
+                } else {
+                    L = ph * lhs_coef<BM>(p, a0, a1, a2, g0, g1);
+                }
+            }
+            if (active) store_lhs<OutT>(p, d.aoff + i + j * n, L, out0, out1, out2);
+            if constexpr (RS) {
+                const c128 a = p.rowsum_of == 1 ? G : H;
+                rs_re += a.re;
+                rs_im += a.im;
+            }
         }
-        if (active) store_lhs<OutT>(p, d.aoff + i + j * n, p.lhs_a * H + p.lhs_b * G, out0, out1, out2);
-        if (p.rowsum_of != 0) {
-            const c128 a = p.rowsum_of == 1 ? G : H;
-            rs_re += a.re;
-            rs_im += a.im;
-        }
-    }
-    if (rowsum_part != nullptr && active && p.rowsum_of != 0) {
+    };
+    const bool rs_on = rowsum_part != nullptr && p.rowsum_of != 0;
+    if (rs_on)
+        columns(std::true_type{});
+    else
+        columns(std::false_type{});
+    if (rs_on && active) {
         double* dst = rowsum_part + d.part_off + 2 * (cb * n + i);
         dst[0] = rs_re;
         dst[1] = rs_im;

@@ -170,7 +170,10 @@ SysParams make_params(const ccab_physics& ph) {
         p.rhs_scale = {0, 0};
         p.rhs_inc_sign = -1.0;
     }
-    p.h_scale = 1.0;
+    set_h_scale(p, 1.0);
+    p.lhs_ak = c128{p.lhs_a.re * p.k, p.lhs_a.im * p.k};
+    p.lhs_aik = c128{p.lhs_a.re * p.beta.im, p.lhs_a.im * p.beta.im};
+    p.lhs_bik = c128{p.lhs_b.re * p.beta.im, p.lhs_b.im * p.beta.im};
     p.corrections = ph.corrections;
     p.dh_sign = (ph.corrections & CCAB_FIX_BM_DH) ? 1.0 : -1.0;
     p.vn_factor = (ph.corrections & CCAB_FIX_IMPEDANCE_VN) ? c128{-p.lhs_b.re, -p.lhs_b.im}
@@ -790,7 +793,7 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         for (int64_t f = 0; f < nfreq; ++f) {
             const double mx = isc[f];
             CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
-            hp[f].h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
+            set_h_scale(hp[f], std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx))));
             isc[f] = 1.0 / hp[f].h_scale;
         }
         CCAB_HIP(hipMemcpy(dp, hp.data(), sizeof(SysParams) * nfreq, hipMemcpyHostToDevice));
@@ -1007,7 +1010,7 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
                 SysParams& p = hp[b0 + c];
                 const double mx = isc[c];
                 CCAB_CHECK(mx > 0.0 && std::isfinite(mx), "degenerate diagonal for the FP16 operator");
-                p.h_scale = std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx)));
+                set_h_scale(p, std::ldexp(1.0, (int)std::floor(std::log2(4096.0 / mx))));
                 isc[c] = 1.0 / p.h_scale;
             }
             CCAB_HIP(hipMemcpy(dp + b0, hp.data() + b0, sizeof(SysParams) * nc, hipMemcpyHostToDevice));

@@ -327,12 +327,20 @@ struct SysParams {
     int32_t rowsum_of;   // 0 none, 1 sum of G, 2 sum of H
     double rhs_inc_sign; // sign applied to rhs_inc
     double h_scale;      // factor of the FP16 (c32) copy of the LHS (power of two)
+    double h_inv;        // 1 / h_scale (exact: set with h_scale by set_h_scale)
     double dh_sign;      // sign of the (k^2 r + 3 f1)(e.n_x)(e.n_y) term of dh
     c128 vn_factor;      // impedance BC: vn = vn_factor * phi
     int32_t corrections; // CCAB_FIX_* bits
     int32_t pad_;
+    // lhs_a k, lhs_a beta.im, lhs_b beta.im (set by make_params): the LHS
+    // entry lhs_a H + lhs_b G as one complex product per entry
+    c128 lhs_ak, lhs_aik, lhs_bik;
 };
 SysParams make_params(const ccab_physics& ph);
+inline void set_h_scale(SysParams& p, double s) {
+    p.h_scale = s;
+    p.h_inv = 1.0 / s;
+}
 int check_physics(ccab_ctx* ctx, const ccab_physics& ph);
 
 constexpr double kPi = 3.14159265358979323846;

@@ -39,6 +39,70 @@ __device__ __forceinline__ PairGeom<T> pair_geom(T rx, T ry, T rz, T nxx, T nxy,
     return q;
 }
 
+// pair_geom in FP64 for the assembly kernels, with 1/r from the hardware
+// reciprocal square root (relative error e0 below 2^-22) and one third-order
+// Newton correction y <- y (1 + e/2 + 3e^2/8), e = 1 - r^2 y^2, which leaves
+// about 2.5 e0^3 (under an FP64 ulp) and takes 7 instructions instead of the
+// 26 of an IEEE sqrt and divide; then r = r^2 y.  Needs r > 0 (the diagonal
+// lane of the assembly kernels evaluates a unit-distance pair).
+template <bool BM>
+__device__ __forceinline__ PairGeom<double> pair_geom_fast(double rx, double ry, double rz,
+                                                           double nxx, double nxy, double nxz,
+                                                           double nyx, double nyy, double nyz) {
+    PairGeom<double> q;
+    const double r2 = fma(rx, rx, fma(ry, ry, rz * rz));
+    double y = __builtin_amdgcn_rsq(r2);
+    const double e = fma(-(r2 * y), y, 1.0);
+    y = fma(y, fma(e, 0.375, 0.5) * e, y);
+    q.inv_r = y;
+    q.rd = r2 * y;
+    q.rdoty = fma(rx, nyx, fma(ry, nyy, rz * nyz)) * y;
+    q.rdotx = BM ? fma(rx, nxx, fma(ry, nxy, rz * nxz)) * y : 0.0;
+    q.nn = BM ? fma(nyx, nxx, fma(nyy, nxy, nyz * nxz)) : 0.0;
+    return q;
+}
+
+// sin and cos of a: a = q pi/2 + r by Cody and Waite's reduction with
+// pi/2 = P1 + P2 + P3 (three doubles; each product is exact inside the fused
+// multiply-add, so r carries an absolute error of about 1.5e-16 for
+// |a| <= 2^16), then sin r = r + r z S(z) and cos r = 1 - z/2 + z^2 C(z),
+// z = r^2, with S and C of degree 5 from Chebyshev interpolation of
+// (sin(sqrt z)/sqrt z - 1)/z and (cos(sqrt z) - 1 + z/2)/z^2 on
+// 0 <= z <= (pi/4)^2 (interpolation error 2e-17 and 1.3e-18; measured
+// against long double: 1.3e-16 absolute for |a| <= 1e5), and the quadrant
+// q mod 4.  About 40 instructions against about 70 for the library sincos,
+// which takes |a| > 2^16 and non-finite a.
+__device__ __forceinline__ void sincos_fast(double a, double* s, double* c) {
+    if (!(fabs(a) <= 65536.0)) {
+        sincos(a, s, c);
+        return;
+    }
+    const double q = rint(a * 0.6366197723675814);  // 2/pi
+    double r = fma(-q, 0x1.921fb54442d18p+0, a);    // P1 = pi/2 rounded to a double
+    r = fma(-q, 0x1.1a62633145c07p-54, r);          // P2 = pi/2 - P1 rounded
+    r = fma(-q, -0x1.f1976b7ed8fbcp-110, r);        // P3 = pi/2 - P1 - P2 rounded
+    const double z = r * r;
+    double ps = 1.5918115263265974e-10;
+    ps = fma(ps, z, -2.505113165023518e-08);
+    ps = fma(ps, z, 2.7557316101617874e-06);
+    ps = fma(ps, z, -0.00019841269836756774);
+    ps = fma(ps, z, 0.008333333333330948);
+    ps = fma(ps, z, -0.16666666666666666);
+    const double sr = fma(r * z, ps, r);
+    double pc = -1.1382623647474604e-11;
+    pc = fma(pc, z, 2.087614614655861e-09);
+    pc = fma(pc, z, -2.7557317271145144e-07);
+    pc = fma(pc, z, 2.480158729876456e-05);
+    pc = fma(pc, z, -0.0013888888888887398);
+    pc = fma(pc, z, 0.041666666666666664);
+    const double cr = fma(z, fma(z, pc, -0.5), 1.0);
+    const int n = int(q);
+    const double sv = (n & 1) ? cr : sr;
+    const double cv = (n & 1) ? sr : cr;
+    *s = (n & 2) ? -sv : sv;
+    *c = ((n + 1) & 2) ? -cv : cv;
+}
+
 // Kernel pair (g, h) at wavenumber k from the geometry and the phasor
 // (c, s) = (cos kr, sin kr).  dh_sign: -1 as the reference
 // (influence_matrix.rs:30), +1 for d^2 G / dn_x dn_y.
