@@ -75,8 +75,12 @@ __global__ __launch_bounds__(kRows) void assemble_kernel(const SysParams* __rest
             diag_entry(p, s.area[j], s.wt[j], G, H);
         } else {
             Cx<T> g, h;
-            greens<T, BM>(k, beta, T(s.x[j] - yx), T(s.y[j] - yy), T(s.z[j] - yz), T(s.nx[j]),
-                          T(s.ny[j]), T(s.nz[j]), nyx, nyy, nyz, g, h, dh_sign);
+            if constexpr (std::is_same<T, double>::value)
+                greens_fast<BM>(k, beta, s.x[j] - yx, s.y[j] - yy, s.z[j] - yz, s.nx[j], s.ny[j],
+                                s.nz[j], nyx, nyy, nyz, g, h, dh_sign);
+            else
+                greens<T, BM>(k, beta, T(s.x[j] - yx), T(s.y[j] - yy), T(s.z[j] - yz),
+                              T(s.nx[j]), T(s.ny[j]), T(s.nz[j]), nyx, nyy, nyz, g, h, dh_sign);
             G = c128{double(g.re) * wa, double(g.im) * wa};
             H = c128{double(h.re) * wa, double(h.im) * wa};
         }
@@ -395,8 +399,8 @@ __global__ void rowcol_kernel(SysParams p, const double* __restrict__ soa_base, 
         }
     } else {
         c128 g, h;
-        greens<double, BM>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
-                           s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h, p.dh_sign);
+        greens_fast<BM>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
+                        s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h, p.dh_sign);
         const double wa = s.wt[j] * s.area[j];
         G = g * wa;
         H = h * wa;

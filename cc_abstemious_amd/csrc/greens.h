@@ -138,6 +138,19 @@ __device__ __forceinline__ void greens(T k, Cx<T> beta, T rx, T ry, T rz, T nxx,
     greens_at<T, BM>(q, k, beta, c, s, g, h, dh_sign);
 }
 
+// greens in FP64 with pair_geom_fast and sincos_fast (the assembly, row /
+// column and H-matrix block kernels; r > 0)
+template <bool BM>
+__device__ __forceinline__ void greens_fast(double k, c128 beta, double rx, double ry, double rz,
+                                            double nxx, double nxy, double nxz, double nyx,
+                                            double nyy, double nyz, c128& g, c128& h,
+                                            double dh_sign = -1.0) {
+    const PairGeom<double> q = pair_geom_fast<BM>(rx, ry, rz, nxx, nxy, nxz, nyx, nyy, nyz);
+    double s, c;
+    sincos_fast(k * q.rd, &s, &c);
+    greens_at<double, BM>(q, k, beta, c, s, g, h, dh_sign);
+}
+
 // Singular diagonal (influence_matrix.rs:42-50) incl. the global shifts of
 // get_dense_surface_matrices (influence_matrix.rs:78-79).  Always FP64.
 __host__ __device__ __forceinline__ void diag_entry(const SysParams& p, double area, double wt,
@@ -165,12 +178,12 @@ __device__ __forceinline__ c128 side_entry(const SysParams& p, const Soa& s, int
     } else {
         c128 g, h;
         if (p.bm)
-            greens<double, true>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
-                                 s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h,
-                                 p.dh_sign);
+            greens_fast<true>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
+                              s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h,
+                              p.dh_sign);
         else
-            greens<double, false>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
-                                  s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);
+            greens_fast<false>(p.k, p.beta, s.x[j] - s.x[i], s.y[j] - s.y[i], s.z[j] - s.z[i],
+                               s.nx[j], s.ny[j], s.nz[j], s.nx[i], s.ny[i], s.nz[i], g, h);
         const double wa = s.wt[j] * s.area[j];
         G = g * wa;
         H = h * wa;
