@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--hm-levels", default="6,5",
                     help="icosphere levels of the H-matrix sub-record (6: 81920 triangles; "
                          "5: 20480, with the dense-LU comparison)")
+    ap.add_argument("--bodies-train", type=int, default=2000,
+                    help="solver solutions (other perturbed spheres, 10 (ka, direction) each) "
+                         "that train the bodies line's warm start")
+    ap.add_argument("--bodies-train-steps", type=int, default=8000)
     ap.add_argument("--bodies-baseline", type=int, default=16,
                     help="bodies also solved body by body (same solver, FP64 LU) as baselines")
     ap.add_argument("--only", choices=["hmatrix", "large", "bodies"], default=None,
@@ -212,14 +216,14 @@ def bodies_block(args, ctx):
     its = [s_["iterations"] for s_ in st]
     warm = None
     if not args.no_surrogate:
-        # learned warm start: trained (untimed) on 2000 solver solutions of
-        # other bodies (200 perturbed spheres x 10 (ka, direction)); the
-        # timed call includes the network inference over all bodies
+        # learned warm start: trained (untimed) on --bodies-train solver
+        # solutions of other bodies (perturbed spheres x 10 (ka, direction)
+        # each); the timed call includes the network inference over all bodies
         import copy
         from cc_abstemious_amd.surrogate import Surrogate
         t_tr = time.perf_counter()
         train = []
-        for _ in range(200):
+        for _ in range(max(1, args.bodies_train // 10)):
             cf = rng.standard_normal(81) * 0.03
             cf[0] = 0.0
             base = PreData(mesh.perturbed_sphere(int(rng.integers(8, 17)), cf), [100.0], 343.0,
@@ -238,7 +242,8 @@ def bodies_block(args, ctx):
         sur = Surrogate(width=256, seed=0, lr=2e-3, ctx=ctx)
         data = sur.dataset([(pt, pt.frequency_list, pt.incident_wave, ph[None])
                             for pt, ph in zip(train, sols)])
-        losses = sur.fit(data, steps=8000, batch=16384, seed=1)
+        losses = sur.fit(data, steps=args.bodies_train_steps, batch=16384, seed=1)
+        sols_n = len(sols)
         del data, sols, train
         train_s = time.perf_counter() - t_tr
         solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
@@ -247,6 +252,8 @@ def bodies_block(args, ctx):
         phw, _, stw = solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
         torch.cuda.synchronize()
         dtw = time.perf_counter() - t1
+        gerr = [float(torch.linalg.vector_norm(g - p) / torch.linalg.vector_norm(p))
+                for g, p in zip(sur.predict_bodies(bodies), phw)]
         itw = [s_["iterations"] for s_ in stw]
         warm = {"solves_per_s": len(bodies) / dtw, "seconds": dtw,
                 "all_converged": all(bool(s_["converged"]) for s_ in stw),
@@ -256,8 +263,9 @@ def bodies_block(args, ctx):
                 "rel_err_vs_fp64_lu_max": max(
                     float(torch.linalg.vector_norm(phw[i] - lu[i]) / torch.linalg.vector_norm(lu[i]))
                     for i in range(nsub)) if nsub else None,
-                "train_systems": 2000, "train_steps": 8000, "final_loss": losses[-1][1],
-                "train_s": train_s}
+                "guess_rel_err": {"median": float(np.median(gerr)), "max": float(max(gerr))},
+                "train_systems": sols_n, "train_steps": args.bodies_train_steps,
+                "final_loss": losses[-1][1], "train_s": train_s}
     gemv = prof.get("gemv", {"count": 0, "ms": float("nan"), "work": 0.0})
     return {"workload": (f"{len(bodies)} uneven bodies (perturbed spheres, {min(ns)}-{max(ns)} "
                          "unknowns), Burton-Miller rigid, plane wave, ka in [1, 6], GMRES-IR "
