@@ -641,7 +641,10 @@ def main():
     lu_roof = None
     if upd:
         tf = upd["work"] / (upd["ms"] * 1e-3) / 1e12
-        lu_roof = {"bound": "mfma", "kernel": "cgemm_kernel<float,0> (LU trailing update)",
+        gv = tuning.get("c64", {}).get("gemm_variant")
+        kname = ("rocblas_cgemm_strided_batched" if gv == capi.GEMM_LIBRARY
+                 else "cgemm_kernel<float,0>")
+        lu_roof = {"bound": "mfma", "kernel": f"{kname} (LU trailing update, GEMM variant {gv})",
                    "achieved": tf, "peak": PEAK_FP32_MFMA_TFLOPS, "unit": "TFLOP/s",
                    "frac": tf / PEAK_FP32_MFMA_TFLOPS, "launches": upd["count"],
                    "avg_launch_ms": upd["ms"] / upd["count"]}

@@ -25,6 +25,8 @@ STOP_BACKWARD_ERROR, STOP_RELATIVE_RESIDUAL = 0, 1
 FIX_BM_DH, FIX_FIELD_SIGN, FIX_IMPEDANCE_VN, FIX_SPHERICAL_ORIGIN, FIX_QUAD_AREA = 1, 2, 4, 8, 16
 FIX_PRESSURE_SIGN = 32
 FIX_ALL = 63
+# LU GEMM variants (ccab_set_tuning): 0..7 the MFMA kernel, 8 rocBLAS
+GEMM_VARIANTS, GEMM_LIBRARY = 9, 8
 
 
 class CcabError(RuntimeError):
@@ -292,9 +294,10 @@ class Context:
         """Pick the fastest LU GEMM variant and outer panel width by timing;
         returns (variant, outer, {(outer, variant): ms})."""
         v, o = ct.c_int(), ct.c_int()
-        tab = (ct.c_double * 16)()
+        nv = GEMM_VARIANTS
+        tab = (ct.c_double * (2 * nv))()
         self.call("ccab_autotune", int(prec), int(n), int(batch), ct.byref(v), ct.byref(o), tab)
-        times = {(64 if i < 8 else 128, i % 8): tab[i] for i in range(16)}
+        times = {(64 if i < nv else 128, i % nv): tab[i] for i in range(2 * nv)}
         return v.value, o.value, times
 
     def autotune_sweep(self, predata, op_prec: int = C32, frequencies=None):

@@ -375,7 +375,7 @@ int ccab_create(int device, ccab_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     if (const char* gv = getenv("CCAB_GEMM_VARIANT")) {
         const int v = atoi(gv);
-        ctx->gemm_variant[0] = ctx->gemm_variant[1] = v & 7;
+        ctx->gemm_variant[0] = ctx->gemm_variant[1] = (v >= 0 && v <= kGemmLibVariant) ? v : 3;
     }
     if (const char* lo = getenv("CCAB_LU_OUTER")) ctx->lu_outer = atoi(lo) == 64 ? 64 : 128;
     if (const char* lc = getenv("CCAB_LU_COOP_ROWS")) ctx->lu_coop_rows = std::max(0, atoi(lc));
@@ -429,7 +429,7 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
 int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer) {
     if (!ctx) return CCAB_EINVAL;
     CCAB_CHECK(prec == CCAB_C128 || prec == CCAB_C64, "bad precision");
-    CCAB_CHECK(gemm_variant >= 0 && gemm_variant < 8, "gemm variant must be 0..7");
+    CCAB_CHECK(gemm_variant >= 0 && gemm_variant <= kGemmLibVariant, "gemm variant must be 0..8");
     CCAB_CHECK(lu_outer == 64 || lu_outer == 128, "lu_outer must be 64 or 128");
     ctx->gemm_variant[prec == CCAB_C128 ? 0 : 1] = gemm_variant;
     ctx->lu_outer = lu_outer;
@@ -476,6 +476,7 @@ int ccab_destroy(ccab_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     graph_clear(ctx);
+    blas_release(ctx);
     if (ctx->wait_event) (void)hipEventDestroy(ctx->wait_event);
     if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
     for (hipEvent_t e : ctx->gir_events) (void)hipEventDestroy(e);

@@ -181,6 +181,9 @@ struct ccab_ctx {
     // complex-GEMM variant per precision [c128, c64] (see lu.hip); set by
     // ccab_set_tuning / ccab_autotune, initialised from CCAB_GEMM_VARIANT
     int gemm_variant[2] = {3, 0};
+    // rocBLAS handle of the library trailing update (GEMM variant 8), made on
+    // first use; a void* so that this header needs no rocBLAS include
+    void* blas = nullptr;
     // LU outer panel width (64 or 128; 128 halves the trailing-update traffic)
     int lu_outer = 128;
     // panels of at least this many rows are factored by many workgroups per
@@ -308,6 +311,8 @@ void* ctx_buf(ccab_ctx* ctx, int slot, size_t bytes);
 hipGraphExec_t graph_find(ccab_ctx* ctx, const GraphKey& k);
 void graph_store(ccab_ctx* ctx, const GraphKey& k, hipGraphExec_t e);
 void graph_clear(ccab_ctx* ctx);
+// lu.hip: destroys ctx->blas
+void blas_release(ccab_ctx* ctx);
 inline size_t ws_align(size_t b) { return (b + 255) & ~size_t(255); }
 int pinned_reserve(ccab_ctx* ctx, size_t bytes);
 
@@ -344,6 +349,9 @@ inline void set_h_scale(SysParams& p, double s) {
 int check_physics(ccab_ctx* ctx, const ccab_physics& ph);
 
 constexpr double kPi = 3.14159265358979323846;
+// GEMM variants 0..7 are lu.hip's MFMA kernel (tile order, epilogue, tile
+// height); 8 sends the LU trailing update C -= A B to rocBLAS
+constexpr int kGemmLibVariant = 8;
 
 // ---------------------------------------------------------------------------
 // Internal launchers (all enqueue on ctx->stream)

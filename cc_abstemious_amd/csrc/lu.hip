@@ -15,6 +15,8 @@
 // (f32 for c64, f64 for c128):  [C_re; C_im] interleaved = A' B' where B' and
 // C' are the interleaved memory itself and A' is the 2x2 real expansion
 // [[Re, -Im], [Im, Re]] built on the fly from LDS.
+#include <rocblas/rocblas.h>
+
 #include "ccab_internal.h"
 #include "coop.h"
 
@@ -959,6 +961,54 @@ void gemm_variant(ccab_ctx* ctx, int64_t grid, int64_t M, int64_t N, int64_t K, 
                        ldb, sB, C, ldc, sC, batch);
 }
 
+// Variant 8: the trailing update C -= A B as rocBLAS's strided-batched
+// complex GEMM on the context's stream (column-major, as the LU stores its
+// systems).  It is never captured into a HIP graph: under capture, and for
+// C = A B (whose C may alias B), the MFMA kernel's variant 3 runs instead.
+rocblas_handle blas_of(ccab_ctx* ctx) {
+    if (!ctx->blas) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+        ctx->blas = h;
+    }
+    return static_cast<rocblas_handle>(ctx->blas);
+}
+
+template <typename T>
+int gemm_library(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, int64_t lda,
+                 int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C, int64_t ldc,
+                 int64_t sC, int64_t batch) {
+    constexpr int64_t kMax = 0x7fffffff;
+    CCAB_CHECK(M <= kMax && N <= kMax && K <= kMax && lda <= kMax && ldb <= kMax && ldc <= kMax &&
+                   batch <= kMax,
+               "GEMM dimensions beyond rocBLAS's 32-bit sizes");
+    rocblas_handle h = blas_of(ctx);
+    if (!h) return ctx_fail(ctx, CCAB_EHIP, "rocblas_create_handle failed");
+    if (rocblas_set_stream(h, ctx->stream) != rocblas_status_success)
+        return ctx_fail(ctx, CCAB_EHIP, "rocblas_set_stream failed");
+    rocblas_status st;
+    if constexpr (sizeof(T) == 8) {
+        const rocblas_double_complex alpha{-1.0, 0.0}, beta{1.0, 0.0};
+        st = rocblas_zgemm_strided_batched(
+            h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N,
+            (rocblas_int)K, &alpha, reinterpret_cast<const rocblas_double_complex*>(A),
+            (rocblas_int)lda, sA, reinterpret_cast<const rocblas_double_complex*>(B),
+            (rocblas_int)ldb, sB, &beta, reinterpret_cast<rocblas_double_complex*>(C),
+            (rocblas_int)ldc, sC, (rocblas_int)batch);
+    } else {
+        const rocblas_float_complex alpha{-1.0f, 0.0f}, beta{1.0f, 0.0f};
+        st = rocblas_cgemm_strided_batched(
+            h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, (rocblas_int)N,
+            (rocblas_int)K, &alpha, reinterpret_cast<const rocblas_float_complex*>(A),
+            (rocblas_int)lda, sA, reinterpret_cast<const rocblas_float_complex*>(B),
+            (rocblas_int)ldb, sB, &beta, reinterpret_cast<rocblas_float_complex*>(C),
+            (rocblas_int)ldc, sC, (rocblas_int)batch);
+    }
+    if (st != rocblas_status_success)
+        return ctx_fail(ctx, CCAB_EHIP, "rocBLAS complex GEMM failed (status %d)", (int)st);
+    return CCAB_OK;
+}
+
 template <typename T, int MODE>
 int gemm_launch(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, int64_t lda,
                 int64_t sA, const Cx<T>* B, int64_t ldb, int64_t sB, Cx<T>* C, int64_t ldc,
@@ -966,6 +1016,13 @@ int gemm_launch(ccab_ctx* ctx, int64_t M, int64_t N, int64_t K, const Cx<T>* A, 
     if (M <= 0 || N <= 0 || K <= 0) return CCAB_OK;
     const int pi = sizeof(T) == 8 ? 0 : 1;
     int v = ctx->gemm_variant[pi];
+    if (v == kGemmLibVariant) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        CCAB_HIP(hipStreamIsCapturing(ctx->stream, &cs));
+        if (MODE == 0 && cs == hipStreamCaptureStatusNone)
+            return gemm_library<T>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+        v = 3;
+    }
     if (MODE == 1 || M <= 64) v &= ~4;  // in-place U12 needs one row tile
     const int64_t tmv = (v & 4) ? 128 : 64;
     const int64_t total = ((M + tmv - 1) / tmv) * ((N + TN - 1) / TN) * batch;
@@ -1461,4 +1518,9 @@ bool lu_coop_fits() {
            fits(reinterpret_cast<const void*>(&coop_slab_kernel<double, coop_width<double, 2>(), 2>)) &&
            fits(reinterpret_cast<const void*>(&coop_slab_kernel<float, coop_width<float, 1>(), 1>)) &&
            fits(reinterpret_cast<const void*>(&coop_slab_kernel<float, coop_width<float, 2>(), 2>));
+}
+
+void blas_release(ccab_ctx* ctx) {
+    if (ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+    ctx->blas = nullptr;
 }

@@ -73,7 +73,7 @@ extern "C" int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, 
     int rc = CCAB_OK;
     for (int oi = 0; oi < 2 && rc == CCAB_OK; ++oi) {
         const int outer = oi == 0 ? 64 : 128;
-        for (int v = 0; v < 8 && rc == CCAB_OK; ++v) {
+        for (int v = 0; v <= kGemmLibVariant && rc == CCAB_OK; ++v) {
             ctx->gemm_variant[pi] = v;
             ctx->lu_outer = outer;
             double tmin = DBL_MAX;
@@ -93,7 +93,7 @@ extern "C" int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, 
                 }
                 if (rep > 0) tmin = std::min(tmin, double(ms));
             }
-            if (ms_table) ms_table[oi * 8 + v] = tmin;
+            if (ms_table) ms_table[oi * (kGemmLibVariant + 1) + v] = tmin;
             if (rc == CCAB_OK && tmin < best) {
                 best = tmin;
                 bv = v;

@@ -168,9 +168,11 @@ int ccab_profile_read(ccab_ctx* ctx, int cls, int64_t* count, double* ms, double
                       int reset);
 
 /* LU tuning (no reference counterpart): the complex-GEMM variant for
- * precision prec (bit 0 XCD-aware tile order, bit 1 LDS-staged epilogue,
- * bit 2 128-row tiles) and the outer panel width (64 or 128) that sets the
- * K of the trailing update. */
+ * precision prec (0..7: the library's MFMA kernel with bit 0 XCD-aware tile
+ * order, bit 1 LDS-staged epilogue, bit 2 128-row tiles; 8: the trailing
+ * update C -= A B by rocBLAS's strided-batched complex GEMM, except inside a
+ * captured HIP graph, where variant 3 runs) and the outer panel width (64
+ * or 128) that sets the K of the trailing update. */
 int ccab_set_tuning(ccab_ctx* ctx, int prec, int gemm_variant, int lu_outer);
 /* LU panel placement (no reference counterpart): panels of at least
  * min_rows rows are factored by up to one workgroup per CU per system
@@ -186,10 +188,11 @@ int ccab_set_lu_coop(ccab_ctx* ctx, int64_t min_rows);
  * workspace buffer is reallocated. */
 int ccab_set_graphs(ccab_ctx* ctx, int on);
 /* The autotuner: times the batched LU of `batch` synthetic n x n systems in
- * precision prec for every GEMM variant (0..7) and outer width (64, 128),
+ * precision prec for every GEMM variant (0..8) and outer width (64, 128),
  * keeps the fastest in the context (as ccab_set_tuning) and returns it;
- * ms_table (optional, 16 doubles) receives the time of variant v at width 64
- * (index v) and 128 (index 8 + v).  Factors do not depend on the variant. */
+ * ms_table (optional, 18 doubles) receives the time of variant v at width 64
+ * (index v) and 128 (index 9 + v).  Factors do not depend on variants 0..7;
+ * variant 8 (rocBLAS) agrees with them to rounding. */
 int ccab_autotune(ccab_ctx* ctx, int prec, int64_t n, int64_t batch, int* best_variant,
                   int* best_outer, double* ms_table);
 

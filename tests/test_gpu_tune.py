@@ -19,7 +19,8 @@ def test_autotune_picks_fastest(prec):
     _note(f"autotune prec={prec} n=1536 batch=2")
     ctx = capi.Context()
     v, o, times = ctx.autotune(prec, 1536, 2)
-    assert 0 <= v < 8 and o in (64, 128)
+    assert 0 <= v < capi.GEMM_VARIANTS and o in (64, 128)
+    assert len(times) == 2 * capi.GEMM_VARIANTS
     assert all(np.isfinite(t) and t > 0 for t in times.values())
     assert times[(o, v)] == min(times.values())
     print({k: round(t, 3) for k, t in times.items()})
@@ -55,19 +56,57 @@ def test_factors_invariant_to_gemm_variant(prec):
 
 
 @pytest.mark.parametrize("prec", [capi.C64, capi.C128])
-def test_lu_graph_replay_matches_eager(prec):
+def test_library_variant_matches_kernel(prec):
+    """GEMM variant 8 (the trailing update by rocBLAS): the factorisation
+    solves the systems as the MFMA kernel's does, to rounding (the pivots may
+    differ only where a column holds near-ties)."""
+    n, batch = 1000, 3
+    _note(f"LU library variant vs MFMA kernel prec={prec} n={n} batch={batch}")
+    dt = torch.complex64 if prec == capi.C64 else torch.complex128
+    g = torch.Generator(device="cpu").manual_seed(7)
+    A64 = torch.randn(batch, n, n, dtype=torch.complex128, generator=g) + \
+        n ** 0.5 * torch.eye(n, dtype=torch.complex128)
+    b64 = torch.randn(batch, n, dtype=torch.complex128, generator=g)
+    ctx = capi.Context()
+    xs = {}
+    for v in (3, capi.GEMM_LIBRARY):
+        for outer in (64, 128):
+            ctx.set_tuning(prec, v, outer)
+            A = A64.to(dt).cuda().transpose(1, 2).contiguous().transpose(1, 2)
+            piv = solve.lu_factor_batched(A, ctx=ctx)
+            x = solve.lu_solve_batched(A, piv, b64.to(dt).cuda(), ctx=ctx)
+            xs[(v, outer)] = x.to(torch.complex128).cpu()
+    eps = 1e-4 if prec == capi.C64 else 1e-12
+    for key, x in xs.items():
+        r = torch.linalg.vector_norm(A64 @ x.unsqueeze(-1) - b64.unsqueeze(-1)) / \
+            torch.linalg.vector_norm(b64)
+        d = torch.linalg.vector_norm(x - xs[(3, 128)]) / torch.linalg.vector_norm(xs[(3, 128)])
+        print(key, float(r), float(d))
+        assert float(r) < eps and float(d) < eps, key
+    ctx.close()
+
+
+@pytest.mark.parametrize("variant", [None, capi.GEMM_LIBRARY])
+@pytest.mark.parametrize("prec", [capi.C64, capi.C128])
+def test_lu_graph_replay_matches_eager(prec, variant):
     """Captured-and-replayed LU (HIP graph) equals the eager launches, over
-    several replays with eager work and synchronisations in between."""
+    several replays with eager work and synchronisations in between.  With
+    the library variant selected the graph holds the MFMA kernel's variant 3
+    (rocBLAS is not captured), so it equals the eager variant-3 factors."""
     n, batch = 700, 3
-    _note(f"LU graph replay prec={prec} n={n} batch={batch}")
+    _note(f"LU graph replay prec={prec} n={n} batch={batch} variant={variant}")
     dt = torch.complex64 if prec == capi.C64 else torch.complex128
     g = torch.Generator(device="cpu").manual_seed(11)
     A0 = (torch.randn(batch, n, n, dtype=torch.complex128, generator=g)
           + n ** 0.5 * torch.eye(n, dtype=torch.complex128)).to(dt).cuda()
     ctx = capi.Context()
+    if variant is not None:
+        ctx.set_tuning(prec, 3, 128)
     work = A0.clone().transpose(1, 2).contiguous().transpose(1, 2)
     piv_e = solve.lu_factor_batched(work, ctx=ctx)
     LU_e = work.clone()
+    if variant is not None:
+        ctx.set_tuning(prec, variant, 128)
     ctx.set_graphs(True)
     for rep in range(4):
         work.copy_(A0.transpose(1, 2).contiguous().transpose(1, 2))
