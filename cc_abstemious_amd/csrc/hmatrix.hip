@@ -1558,6 +1558,7 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
+    trace.mark("norms, dense storage");
     // Storage order of the matvec: every cluster's points in the depth-first
     // leaf order of its tree, so that a cluster is one contiguous range of
     // positions (the product permutes x once and reads it contiguously).  The
@@ -1618,6 +1619,7 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
                            hipMemcpyHostToDevice));
         CCAB_HIP(hipMemcpy(d_pos, pos.data(), pos.size() * 8, hipMemcpyHostToDevice));
         CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+        trace.mark("depth-first order (host)");
         if (!ll2.empty()) {
             int64_t cu = 0, cv = 0;
             for (int32_t b : ll2) {
@@ -1631,6 +1633,7 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
                                H->d_blk, H->d_lr_list, d_pos, H->d_U, H->d_V, U2, V2);
             CCAB_HIP(hipGetLastError());
             CCAB_HIP(hipStreamSynchronize(s));
+            trace.mark("factor rows reordered");
             for (void* old : {static_cast<void*>(H->d_U), static_cast<void*>(H->d_V)}) {
                 auto it = std::find(H->owned.begin(), H->owned.end(), old);
                 if (it != H->owned.end()) {
