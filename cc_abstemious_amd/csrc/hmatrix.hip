@@ -512,6 +512,85 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(E entry, HBlkDev* blks
     }
 }
 
+// One-sided complex Jacobi SVD of the r x r matrix W (LDS, [row][col]),
+// rotations accumulated in Vs (identity on entry), with the parallel
+// (round-robin) ordering: the r columns (one padding column when r is odd)
+// meet in r' - 1 rounds of r'/2 disjoint pairs, so NTHR / SW pairs rotate at
+// once, SW lanes (one per row, SW >= r, a power of two <= 64) per pair, the
+// norms and the inner product of a pair summed over its SW lanes.  The
+// rotation of a pair is that of aca.rs:137-179's SVD in one-sided form
+// (tau = (a_qq - a_pp) / 2|a_pq|); sweeps until every pair is below 1e-15
+// relative (at most 60).  NTHR threads take part (tid < NTHR); with
+// NTHR > 64 the caller's whole workgroup runs it (barriers between rounds).
+template <int RM, int SW, int NTHR>
+__device__ __forceinline__ void jacobi_rr(c128 (*Wm)[RM + 1], c128 (*Vs)[RM + 1], int r, int tid,
+                                          double* red_off) {
+    constexpr int NSLOT = NTHR / SW;
+    const int slot = tid / SW, row = tid % SW;
+    const int m = r + (r & 1);         // columns incl. the padding one
+    const int npair = m / 2;
+    auto sync = [&]() {
+        if constexpr (NTHR > 64) __syncthreads();
+        else __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    };
+    auto ssum = [](double v) {
+#pragma unroll
+        for (int off = SW / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
+    };
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0.0;
+        for (int k = 0; k < m - 1; ++k) {
+            for (int p0 = 0; p0 < npair; p0 += NSLOT) {
+                const int pi = p0 + slot;
+                // circle method: column 0 fixed, the others rotate by k
+                int a = 0, b = 0;
+                if (pi < npair) {
+                    auto rot = [&](int j) { return 1 + (j + k) % (m - 1); };
+                    a = pi == 0 ? 0 : rot(pi);
+                    b = pi == 0 ? rot(0) : rot(m - 1 - pi);
+                }
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                const bool live = pi < npair && q < r;  // no padding column
+                const bool mine = live && row < r;
+                const c128 wp = mine ? Wm[row][p] : c128{0.0, 0.0};
+                const c128 wq = mine ? Wm[row][q] : c128{0.0, 0.0};
+                const double app = ssum(norm_sqr(wp));
+                const double aqq = ssum(norm_sqr(wq));
+                const c128 z = conj(wp) * wq;
+                const c128 apq{ssum(z.re), ssum(z.im)};
+                const double mag = hypot(apq.re, apq.im);
+                if (live && mag != 0.0 && mag > 1e-15 * sqrt(app * aqq)) {
+                    off = fmax(off, mag / sqrt(app * aqq));
+                    const c128 ph{apq.re / mag, apq.im / mag};
+                    const double tau = (aqq - app) / (2.0 * mag);
+                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
+                    if (row < r) {
+                        Wm[row][p] = wp * cs - (conj(ph) * wq) * sn;
+                        Wm[row][q] = (ph * wp) * sn + wq * cs;
+                        const c128 vp = Vs[row][p], vq = Vs[row][q];
+                        Vs[row][p] = vp * cs - (conj(ph) * vq) * sn;
+                        Vs[row][q] = (ph * vp) * sn + vq * cs;
+                    }
+                }
+            }
+            sync();  // the next round pairs the columns differently
+        }
+        // the largest off-diagonal measure of the sweep over all threads
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) off = fmax(off, __shfl_xor(off, o, 64));
+        if constexpr (NTHR > 64) {
+            if ((tid & 63) == 0) red_off[tid >> 6] = off;
+            __syncthreads();
+            off = 0.0;
+            for (int w = 0; w < NTHR / 64; ++w) off = fmax(off, red_off[w]);
+            __syncthreads();
+        }
+        if (off < 1e-15) break;
+    }
+}
+
 // --- recompression (aca.rs:137-179): QR(U), QR(V) by CGS2, SVD of R_u R_v^T
 //     by one-sided Jacobi (one wavefront, LDS), truncation, new U and V -------
 template <int RM>
@@ -596,42 +675,13 @@ __global__ __launch_bounds__(kAcaThreads) void recompress_kernel(HBlkDev* blks,
         Vs[i][c] = c128{i == c ? 1.0 : 0.0, 0.0};
     }
     __syncthreads();
-    // one-sided Jacobi on the columns of Wm (wave 0, lane = row)
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        for (int sweep = 0; sweep < 60; ++sweep) {
-            double off = 0.0;
-            for (int p = 0; p < r - 1; ++p)
-                for (int q = p + 1; q < r; ++q) {
-                    const c128 wp = lane < r ? Wm[lane][p] : c128{0, 0};
-                    const c128 wq = lane < r ? Wm[lane][q] : c128{0, 0};
-                    const double app = wave_sum(norm_sqr(wp));
-                    const double aqq = wave_sum(norm_sqr(wq));
-                    const c128 z = conj(wp) * wq;
-                    const c128 apq{wave_sum(z.re), wave_sum(z.im)};
-                    const double mag = hypot(apq.re, apq.im);
-                    if (mag == 0.0 || mag <= 1e-15 * sqrt(app * aqq)) continue;
-                    off = fmax(off, mag / sqrt(app * aqq));
-                    const c128 ph{apq.re / mag, apq.im / mag};
-                    const double tau = (aqq - app) / (2.0 * mag);
-                    const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-                    if (lane < r) {
-                        Wm[lane][p] = wp * cs - (conj(ph) * wq) * sn;
-                        Wm[lane][q] = (ph * wp) * sn + wq * cs;
-                        const c128 vp = Vs[lane][p], vq = Vs[lane][q];
-                        Vs[lane][p] = vp * cs - (conj(ph) * vq) * sn;
-                        Vs[lane][q] = (ph * vp) * sn + vq * cs;
-                    }
-                }
-            if (off < 1e-15) break;
-        }
-        // singular values = column norms of the rotated W
-        if (lane < r) {
-            double sq = 0.0;
-            for (int i = 0; i < r; ++i) sq += norm_sqr(Wm[i][lane]);
-            sv[lane] = sqrt(sq);
-        }
+    // one-sided Jacobi on the columns of Wm, all four wavefronts
+    jacobi_rr<RM, (RM <= 16 ? 16 : RM <= 32 ? 32 : 64), kAcaThreads>(Wm, Vs, r, threadIdx.x, red);
+    // singular values = column norms of the rotated W
+    if (threadIdx.x < r) {
+        double sq = 0.0;
+        for (int i = 0; i < r; ++i) sq += norm_sqr(Wm[i][threadIdx.x]);
+        sv[threadIdx.x] = sqrt(sq);
     }
     __syncthreads();
     if (threadIdx.x == 0) {  // descending order, stable
@@ -759,34 +809,8 @@ __global__ __launch_bounds__(256) void recompress_wave_kernel(HBlkDev* blks,
         Vs[i][c] = c128{i == c ? 1.0 : 0.0, 0.0};
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    // one-sided Jacobi on the columns of Wm (lane = row)
-    for (int sweep = 0; sweep < 60; ++sweep) {
-        double off = 0.0;
-        for (int p = 0; p < r - 1; ++p)
-            for (int q = p + 1; q < r; ++q) {
-                const c128 wp = lane < r ? Wm[lane][p] : c128{0, 0};
-                const c128 wq = lane < r ? Wm[lane][q] : c128{0, 0};
-                const double app = wave_sum(norm_sqr(wp));
-                const double aqq = wave_sum(norm_sqr(wq));
-                const c128 z = conj(wp) * wq;
-                const c128 apq{wave_sum(z.re), wave_sum(z.im)};
-                const double mag = hypot(apq.re, apq.im);
-                if (mag == 0.0 || mag <= 1e-15 * sqrt(app * aqq)) continue;
-                off = fmax(off, mag / sqrt(app * aqq));
-                const c128 ph{apq.re / mag, apq.im / mag};
-                const double tau = (aqq - app) / (2.0 * mag);
-                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-                const double cs = 1.0 / sqrt(1.0 + t * t), sn = cs * t;
-                if (lane < r) {
-                    Wm[lane][p] = wp * cs - (conj(ph) * wq) * sn;
-                    Wm[lane][q] = (ph * wp) * sn + wq * cs;
-                    const c128 vp = Vs[lane][p], vq = Vs[lane][q];
-                    Vs[lane][p] = vp * cs - (conj(ph) * vq) * sn;
-                    Vs[lane][q] = (ph * vp) * sn + vq * cs;
-                }
-            }
-        if (off < 1e-15) break;
-    }
+    // one-sided Jacobi on the columns of Wm, four column pairs at once
+    jacobi_rr<RM, 16, 64>(Wm, Vs, r, lane, nullptr);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     if (lane < r) {  // singular values = column norms of the rotated W
         double sq = 0.0;
