@@ -415,7 +415,13 @@ int gemv_active(ccab_ctx* ctx, int prec, int64_t M, int64_t N, int64_t nact,
     int64_t chunk = N > 0 ? (N + nsplit - 1) / nsplit : 1;
     // ragged: a fixed column chunk, so that a large body's workgroups stream
     // no more than a small body's and the launch has no long tail
-    if (rg && ctx->gemv_ragged_chunk > 0 && N > 0) chunk = std::min<int64_t>(N, ctx->gemv_ragged_chunk);
+    if (rg && ctx->gemv_ragged_chunk > 0 && N > 0) {
+        chunk = std::min<int64_t>(N, ctx->gemv_ragged_chunk);
+        // the iteration tail (few bodies left): halve the chunk (down to 64
+        // columns) while the launch would have fewer than 4 workgroups per CU
+        const int64_t want = 4 * int64_t(std::max(ctx->num_cus, 1));
+        while (chunk > 64 && rowblocks * ((N + chunk - 1) / chunk) * nact < want) chunk /= 2;
+    }
     nsplit = N > 0 ? (N + chunk - 1) / chunk : 1;
     c128* part = static_cast<c128*>(ctx_buf(ctx, SLOT_GEMV_PART, size_t(nact) * nsplit * M * 16));
     if (!part) return CCAB_ENOMEM;
