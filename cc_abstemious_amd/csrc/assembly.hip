@@ -153,6 +153,13 @@ __device__ __forceinline__ c128 lhs_coef(const SysParams& p, double a0, double a
                     p.lhs_a.im * a0 + p.lhs_ak.re * a1 + p.lhs_b.im};
 }
 
+// lhs_coef for lhs_a = 1, lhs_b = 0 (H alone: the normal-velocity BC)
+template <bool BM>
+__device__ __forceinline__ c128 lhs_coef_h(const SysParams& p, double a0, double a1, double a2) {
+    if constexpr (BM) return c128{a0, fma(p.lhs_aik.re, a2, p.lhs_ak.re * a1)};
+    else return c128{a0, p.lhs_ak.re * a1};
+}
+
 // one LHS entry L to the outputs: out0 c128; out1 OutT (c64, or c32 scaled by
 // h_scale); with OutT = c32 and out2, out2 the complex-FP32 remainder
 // L - fp16(L h_scale) / h_scale
@@ -463,7 +470,8 @@ __global__ void bodies_incident_kernel(const BodyDesc* __restrict__ desc,
 // 1-D grid is every body's (row block of 256, column chunk of 64) tiles back
 // to back; the pair as in the sweep kernel, with the phasor from sincos_fast.
 // A body whose right-hand side needs no row sum (rowsum_of 0) takes the
-// column loop that forms lhs_a H + lhs_b G as one complex product per entry.
+// column loop that forms lhs_a H + lhs_b G as one complex product per entry
+// (H alone for lhs_a = 1, lhs_b = 0).
 template <bool BM, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const SysParams* __restrict__ params, const BodyDesc* __restrict__ desc, int nbody,
@@ -486,8 +494,11 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
     const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
     double rs_re = 0.0, rs_im = 0.0;
-    auto columns = [&](auto row_sums) {
-        constexpr bool RS = decltype(row_sums)::value;
+    // column loop variants: 0 fused lhs_a H + lhs_b G, 1 H alone, 2 H and G
+    // separately with row sums
+    auto columns = [&](auto variant) {
+        constexpr int VAR = decltype(variant)::value;
+        constexpr bool RS = VAR == 2;
         for (int64_t j = j0; j < j1; ++j) {
             const double wa = s.wt[j] * s.area[j];
             const bool diag = j == ic;
@@ -509,6 +520,8 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
                     H = ph * c128{a0, p.k * a1 + a2 * ik};
                     G = ph * c128{g0, g1 * ik};
                     L = p.lhs_a * H + p.lhs_b * G;
+                } else if constexpr (VAR == 1) {
+                    L = ph * lhs_coef_h<BM>(p, a0, a1, a2);
                 } else {
                     L = ph * lhs_coef<BM>(p, a0, a1, a2, g0, g1);
                 }
@@ -522,10 +535,14 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
         }
     };
     const bool rs_on = rowsum_part != nullptr && p.rowsum_of != 0;
+    const bool h_only = p.lhs_b.re == 0.0 && p.lhs_b.im == 0.0 && p.lhs_a.re == 1.0 &&
+                        p.lhs_a.im == 0.0;
     if (rs_on)
-        columns(std::true_type{});
+        columns(std::integral_constant<int, 2>{});
+    else if (h_only)
+        columns(std::integral_constant<int, 1>{});
     else
-        columns(std::false_type{});
+        columns(std::integral_constant<int, 0>{});
     if (rs_on && active) {
         double* dst = rowsum_part + d.part_off + 2 * (cb * n + i);
         dst[0] = rs_re;
