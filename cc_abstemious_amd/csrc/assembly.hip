@@ -222,6 +222,10 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     auto rs = [&](int f) -> double2& { return rs_lds[f * kRows + threadIdx.x]; };
     if (rowsum_part)
         for (int f = 0; f < KF; ++f) rs(f) = double2{0.0, 0.0};
+    // column loop variants: 0 general, 1 every system of the group H alone
+    // (lhs_a = 1, lhs_b = 0, no row sums): only H's coefficient is formed
+    auto columns = [&](auto variant) {
+    constexpr int VAR = decltype(variant)::value;
     for (int64_t j = j0; j < j1; ++j) {
         const double wa = s.wt[j] * s.area[j];
         const bool diag = j == ic;
@@ -253,6 +257,18 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                 }
             }
             const c128 ph{cs, sn};
+            if constexpr (VAR == 1) {
+                c128 L;
+                if (diag) {
+                    c128 G, H;
+                    diag_entry(p, s.area[j], s.wt[j], G, H);
+                    L = p.lhs_a * H + p.lhs_b * G;
+                } else {
+                    L = ph * lhs_coef_h<BM>(p, a0, a1, a2);
+                }
+                if (active) store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, L, out0, out1, out2);
+                continue;
+            }
             if (rowsum_part && p.rowsum_of != 0) {
                 c128 G, H;
                 if (diag) {
@@ -282,6 +298,17 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
             }
         }
     }
+    };
+    bool h_group = true;
+    for (int f = 0; f < nf; ++f) {
+        const SysParams& p = params[b0 + f];
+        h_group = h_group && p.lhs_b.re == 0.0 && p.lhs_b.im == 0.0 && p.lhs_a.re == 1.0 &&
+                  p.lhs_a.im == 0.0 && !(rowsum_part && p.rowsum_of != 0);
+    }
+    if (h_group)
+        columns(std::integral_constant<int, 1>{});
+    else
+        columns(std::integral_constant<int, 0>{});
     if (rowsum_part == nullptr || !active) return;
     for (int f = 0; f < nf; ++f) {
         if (params[b0 + f].rowsum_of == 0) continue;
