@@ -20,6 +20,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "ccab_internal.h"
@@ -951,7 +952,58 @@ struct VtItem {
     int32_t rank;
 };
 
+// The wave's sums of N complex accumulators (N = 2, 4 or 8) by recursive
+// halving: the exchanges between lanes 32, 16, ... apart (log2 N of them)
+// each pass on half of what a lane holds, so that afterwards lane l holds
+// accumulator wave_sum_index<N>(l) summed over the 64 / N lanes that share
+// its low bits; the remaining xor exchanges sum those.  For N = 8: 40 lane
+// exchanges of 32 bits instead of 192 for eight wave_sums of the real and
+// imaginary parts.  All 64 / N lanes of an index hold the same sum.
+__device__ __forceinline__ c128 shfl_xor_c(c128 v, int off) {
+    return c128{__shfl_xor(v.re, off, 64), __shfl_xor(v.im, off, 64)};
+}
+template <int N>
+__device__ __forceinline__ int wave_sum_index(int lane) {
+    int idx = 0;
+#pragma unroll
+    for (int off = 32, m = N; m > 1; m >>= 1, off >>= 1) idx = 2 * idx + ((lane & off) ? 1 : 0);
+    return idx;
+}
+template <int N>
+__device__ __forceinline__ c128 wave_sum_many(const c128 (&a)[N], int lane) {
+    c128 cur[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) cur[u] = a[u];
+    int off = 32;
+#pragma unroll
+    for (int m = N; m > 1; m >>= 1, off >>= 1) {
+        const bool b = lane & off;
+#pragma unroll
+        for (int u = 0; u < m / 2; ++u)
+            cur[u] = (b ? cur[u + m / 2] : cur[u]) + shfl_xor_c(b ? cur[u] : cur[u + m / 2], off);
+    }
+#pragma unroll
+    for (; off > 0; off >>= 1) cur[0] += shfl_xor_c(cur[0], off);
+    return cur[0];
+}
+
+// f(std::integral_constant<int, n>) for a wave-uniform count 1 <= n <= MAXN:
+// the loops over n loads become straight-line code without branches
+template <int K, int MAXN, typename F>
+__device__ __forceinline__ void with_count(int n, F& f) {
+    if constexpr (K <= MAXN) {
+        if (n == K) f(std::integral_constant<int, K>{});
+        else with_count<K + 1, MAXN>(n, f);
+    }
+}
+template <int NU>
+constexpr int pow2_ceil() { return NU <= 1 ? 1 : NU <= 2 ? 2 : NU <= 4 ? 4 : 8; }
+
 // t[item.t + k] = sum over the chunk's columns j of V[item.v + j + k ld] xp[item.x + j]
+// Up to RB ranks per pass (8 for c128 storage; 4 for complex FP32, whose
+// paired loads carry two columns each): the pass over NU ranks is compiled
+// for each NU, so a column's NU loads are issued together without branches,
+// and its NU sums are formed by one wave_sum_many.
 template <typename T, bool NT>
 __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ items, int64_t cnt,
                                                      const T* __restrict__ V,
@@ -965,54 +1017,44 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
         const int r = q.rank;
         const T* Vb = V + q.v;
         const c128* xc = xp + q.x;  // the column cluster's x, contiguous
-        if constexpr (sizeof(T) == 8) {
-            // complex FP32 with even offsets and an even column count: columns
-            // (j, j + 1) of a rank are one aligned 16-byte load, 128 columns
-            // per pass
-            if (paired && (q.ld & 1) == 0 && (q.v & 1) == 0) {
-                for (int k0 = 0; k0 < r; k0 += 8) {
-                    c128 acc[8] = {};
+        int k0 = 0;
+        // complex FP32 with even offsets and even column counts: columns
+        // (j, j + 1) of a rank are one aligned 16-byte load, 128 columns per
+        // pass
+        const bool pair = sizeof(T) == 8 && paired && (q.ld & 1) == 0 && (q.v & 1) == 0 &&
+                          (q.len & 1) == 0;
+        auto pass = [&](auto nuc) {
+            constexpr int NU = decltype(nuc)::value, N = pow2_ceil<NU>();
+            c128 acc[N] = {};
+            if constexpr (sizeof(T) == 8) {
+                if (pair) {
                     for (int j = 2 * lane; j < q.len; j += 128) {
-                        const c128 x0 = xc[j];
-                        if (j + 1 < q.len) {
-                            const c128 x1 = xc[j + 1];
+                        const c128 x0 = xc[j], x1 = xc[j + 1];
+                        float4 v[NU];
 #pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                if (k0 + u < r) {
-                                    const float4 v = ldm2<NT>(Vb + j + int64_t(k0 + u) * q.ld);
-                                    acc[u] += c128{v.x, v.y} * x0 + c128{v.z, v.w} * x1;
-                                }
-                        } else {
+                        for (int u = 0; u < NU; ++u) v[u] = ldm2<NT>(Vb + j + int64_t(k0 + u) * q.ld);
 #pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                if (k0 + u < r) acc[u] += ldm<NT>(Vb + j + int64_t(k0 + u) * q.ld) * x0;
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        if (k0 + u >= r) break;  // wave-uniform
-                        const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
-                        if (lane == 0) t[q.t + k0 + u] = c128{re, im};
+                        for (int u = 0; u < NU; ++u)
+                            acc[u] += c128{v[u].x, v[u].y} * x0 + c128{v[u].z, v[u].w} * x1;
                     }
                 }
-                continue;
             }
-        }
-        for (int k0 = 0; k0 < r; k0 += 8) {
-            c128 acc[8] = {};
-            for (int j = lane; j < q.len; j += 64) {
-                const c128 xj = xc[j];
+            if (!pair) {
+                for (int j = lane; j < q.len; j += 64) {
+                    const c128 xj = xc[j];
+                    c128 v[NU];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (k0 + u < r) acc[u] += ldm<NT>(Vb + j + int64_t(k0 + u) * q.ld) * xj;
-            }
+                    for (int u = 0; u < NU; ++u) v[u] = ldm<NT>(Vb + j + int64_t(k0 + u) * q.ld);
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                if (k0 + u >= r) break;  // wave-uniform
-                const double re = wave_sum(acc[u].re), im = wave_sum(acc[u].im);
-                if (lane == 0) t[q.t + k0 + u] = c128{re, im};
+                    for (int u = 0; u < NU; ++u) acc[u] += v[u] * xj;
+                }
             }
-        }
+            const c128 s = wave_sum_many<N>(acc, lane);
+            const int su = wave_sum_index<N>(lane);
+            if ((lane & (64 / N - 1)) == 0 && su < NU) t[q.t + k0 + su] = s;
+        };
+        constexpr int RB = sizeof(T) == 8 ? 4 : 8;
+        for (; k0 < r; k0 += RB) with_count<1, RB>(min(r - k0, RB), pass);
     }
 }
 
@@ -1101,21 +1143,35 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                                 }
                             }
                         } else {
-                            const T* uu = U + off + q0 + int64_t(h) * nr;
+                            // half h takes the ranks of parity h, up to four of
+                            // them per pass, compiled per count (half 0's) so
+                            // their loads issue together; half 1's rank past
+                            // the block reads rank c - 1 again (the same
+                            // addresses as half 0), times t = 0
+                            const T* uu = U + off + q0;
                             const c128* tb = t + xoff;
-                            for (int kb = 0; kb < c; kb += 2) {
-                                c128 ta = tb[kb];
-                                c128 tc = kb + 1 < c ? tb[kb + 1] : c128{0.0, 0.0};
-                                for (int ch = 1; ch < vtc; ++ch) {
-                                    ta += tb[int64_t(ch) * c + kb];
-                                    if (kb + 1 < c) tc += tb[int64_t(ch) * c + kb + 1];
-                                }
-                                if (kb + h < c) {
-                                    const c128 tk = h ? tc : ta;
-                                    const float4 v = ldm2<NT>(uu + int64_t(kb) * nr);
-                                    a0 += c128{v.x, v.y} * tk;
-                                    a1 += c128{v.z, v.w} * tk;
-                                }
+                            for (int k0 = 0; k0 < c; k0 += 8) {
+                                auto lr = [&](auto nuc) {
+                                    constexpr int NU = decltype(nuc)::value;
+                                    float4 v[NU];
+                                    c128 tk[NU];
+#pragma unroll
+                                    for (int u = 0; u < NU; ++u)
+                                        v[u] = ldm2<NT>(uu + int64_t(min(k0 + 2 * u + h, c - 1)) * nr);
+#pragma unroll
+                                    for (int u = 0; u < NU; ++u) {
+                                        const int k = min(k0 + 2 * u + h, c - 1);
+                                        c128 s = tb[k];
+                                        for (int ch = 1; ch < vtc; ++ch) s += tb[int64_t(ch) * c + k];
+                                        tk[u] = k0 + 2 * u + h < c ? s : c128{0.0, 0.0};
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < NU; ++u) {
+                                        a0 += c128{v[u].x, v[u].y} * tk[u];
+                                        a1 += c128{v[u].z, v[u].w} * tk[u];
+                                    }
+                                };
+                                with_count<1, 4>(min(4, (c - k0 + 1) / 2), lr);
                             }
                         }
                     }
@@ -1163,12 +1219,26 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                     }
                     for (; j < c; ++j) acc += ldm<NT>(d + int64_t(j) * nr) * xc[j];
                 } else {
+                    // up to eight ranks per pass, compiled per count so
+                    // their loads issue together
                     const T* uu = U + off + pp;
                     const c128* tb = t + xoff;
-                    for (int k = 0; k < c; ++k) {
-                        c128 tk = tb[k];
-                        for (int ch = 1; ch < vtc; ++ch) tk += tb[int64_t(ch) * c + k];
-                        acc += ldm<NT>(uu + int64_t(k) * nr) * tk;
+                    for (int k0 = 0; k0 < c; k0 += 8) {
+                        auto lr = [&](auto nuc) {
+                            constexpr int NU = decltype(nuc)::value;
+                            c128 v[NU], tk[NU];
+#pragma unroll
+                            for (int u = 0; u < NU; ++u) v[u] = ldm<NT>(uu + int64_t(k0 + u) * nr);
+#pragma unroll
+                            for (int u = 0; u < NU; ++u) {
+                                c128 s = tb[k0 + u];
+                                for (int ch = 1; ch < vtc; ++ch) s += tb[int64_t(ch) * c + k0 + u];
+                                tk[u] = s;
+                            }
+#pragma unroll
+                            for (int u = 0; u < NU; ++u) acc += v[u] * tk[u];
+                        };
+                        with_count<1, 8>(min(c - k0, 8), lr);
                     }
                 }
             }
