@@ -219,6 +219,9 @@ struct ccab_ctx {
     // FP32 H-matrix storage: bit 0 two columns per 16-byte load in V^T x,
     // bit 1 two rows per 16-byte load in the product (CCAB_HMV_PAIRED)
     int hmv_paired = 3;
+    // FP32 H-matrix storage: ranks per pass of V^T x over paired loads, 4 or
+    // 8 (CCAB_HMV_VT_RANKS)
+    int hmv_vt_ranks = 8;
     // non-temporal loads of the H-matrix blocks in the product (CCAB_HMV_NT)
     bool hmv_nt = true;
     std::vector<AsmTune> asm_tune;

@@ -1000,11 +1000,11 @@ template <int NU>
 constexpr int pow2_ceil() { return NU <= 1 ? 1 : NU <= 2 ? 2 : NU <= 4 ? 4 : 8; }
 
 // t[item.t + k] = sum over the chunk's columns j of V[item.v + j + k ld] xp[item.x + j]
-// Up to RB ranks per pass (8 for c128 storage; 4 for complex FP32, whose
+// Up to RB ranks per pass (8 for c128 storage; 4 or 8 for complex FP32, whose
 // paired loads carry two columns each): the pass over NU ranks is compiled
 // for each NU, so a column's NU loads are issued together without branches,
 // and its NU sums are formed by one wave_sum_many.
-template <typename T, bool NT>
+template <typename T, bool NT, int RB>
 __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ items, int64_t cnt,
                                                      const T* __restrict__ V,
                                                      const c128* __restrict__ xp, c128* t,
@@ -1053,7 +1053,6 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
             const int su = wave_sum_index<N>(lane);
             if ((lane & (64 / N - 1)) == 0 && su < NU) t[q.t + k0 + su] = s;
         };
-        constexpr int RB = sizeof(T) == 8 ? 4 : 8;
         for (; k0 < r; k0 += RB) with_count<1, RB>(min(r - k0, RB), pass);
     }
 }
@@ -1064,10 +1063,29 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
 // chain of scalar loads per block.
 struct HmvWork {
     int64_t off;   // dense: d_off; low-rank: u_off
-    int64_t xoff;  // dense: clo (xp); low-rank: p_off (t)
+    int64_t xoff;  // dense: clo (xp); low-rank: p_off (t, summed over the chunks)
     int32_t cnt;   // dense: columns; low-rank: rank
     int32_t vtc;   // 0: dense; low-rank: chunks of V^T x
 };
+
+// The rank partials of a low-rank block's V^T x chunks summed in chunk order
+// into the first chunk's slots (one thread per rank of a block with more than
+// one chunk), so that the product reads one t per rank: summed inside the
+// product, every 64-row tile of the block repeated the sum as a chain of
+// dependent scalar loads.
+struct TsumItem {
+    int64_t p;       // t offset of the rank's first-chunk partial
+    int32_t stride;  // the block's rank (distance between its chunks' partials)
+    int32_t cnt;     // chunks
+};
+__global__ void hmv_tsum_kernel(const TsumItem* __restrict__ items, int64_t cnt, c128* t) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const TsumItem q = items[i];
+    c128 s = t[q.p];
+    for (int ch = 1; ch < q.cnt; ++ch) s += t[q.p + int64_t(ch) * q.stride];
+    t[q.p] = s;
+}
 
 __device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
     const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffff), src);
@@ -1160,9 +1178,7 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                                         v[u] = ldm2<NT>(uu + int64_t(min(k0 + 2 * u + h, c - 1)) * nr);
 #pragma unroll
                                     for (int u = 0; u < NU; ++u) {
-                                        const int k = min(k0 + 2 * u + h, c - 1);
-                                        c128 s = tb[k];
-                                        for (int ch = 1; ch < vtc; ++ch) s += tb[int64_t(ch) * c + k];
+                                        const c128 s = tb[min(k0 + 2 * u + h, c - 1)];
                                         tk[u] = k0 + 2 * u + h < c ? s : c128{0.0, 0.0};
                                     }
 #pragma unroll
@@ -1230,11 +1246,7 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
 #pragma unroll
                             for (int u = 0; u < NU; ++u) v[u] = ldm<NT>(uu + int64_t(k0 + u) * nr);
 #pragma unroll
-                            for (int u = 0; u < NU; ++u) {
-                                c128 s = tb[k0 + u];
-                                for (int ch = 1; ch < vtc; ++ch) s += tb[int64_t(ch) * c + k0 + u];
-                                tk[u] = s;
-                            }
+                            for (int u = 0; u < NU; ++u) tk[u] = tb[k0 + u];
 #pragma unroll
                             for (int u = 0; u < NU; ++u) acc += v[u] * tk[u];
                         };
@@ -1372,6 +1384,8 @@ struct ccab_hmatrix {
     c128* d_nacc = nullptr;
     VtItem* d_vt_items = nullptr;    // V^T x items: (low-rank block, column chunk)
     int64_t n_vt = 0;
+    TsumItem* d_tsum = nullptr;      // chunk sums: (block of several chunks, rank)
+    int64_t n_tsum = 0;
     int32_t* d_item_seg = nullptr;   // product items: (segment, 64-row tile)
     int32_t* d_item_r0 = nullptr;
     int64_t n_items = 0;
@@ -1865,6 +1879,19 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     }
     H->n_vt = (int64_t)vt_blk.size();
     H->n_items = (int64_t)item_seg.size();
+    std::vector<TsumItem> tsum;
+    for (int32_t b : ll2) {
+        const HBlkDev& d = H->blk[b];
+        if (d.vtc > 1)
+            for (int32_t k = 0; k < d.rank; ++k) tsum.push_back(TsumItem{d.p_off + k, d.rank, d.vtc});
+    }
+    H->n_tsum = (int64_t)tsum.size();
+    if (H->n_tsum) {
+        H->d_tsum = dalloc<TsumItem>(H->owned, tsum.size());
+        if (!H->d_tsum) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        CCAB_HIP(hipMemcpy(H->d_tsum, tsum.data(), tsum.size() * sizeof(TsumItem),
+                           hipMemcpyHostToDevice));
+    }
     H->d_nacc = dalloc<c128>(H->owned, nslots);
     H->d_vt_items = dalloc<VtItem>(H->owned, vt_blk.size());
     H->d_item_seg = dalloc<int32_t>(H->owned, item_seg.size());
@@ -2030,9 +2057,15 @@ int hmatrix_gemv(ccab_ctx* ctx, const ccab_hmatrix* H, c128 alpha, const c128* x
     const int pv = ctx->hmv_paired & 1, pn = (ctx->hmv_paired >> 1) & 1;
 #define CCAB_HMV_LAUNCH(T, NTF, Vp, Dp, Up, PV, PN)                                              \
     do {                                                                                       \
-        if (H->n_vt)                                                                           \
-            hipLaunchKernelGGL((hmv_vt_kernel<T, NTF>), grid_of(H->n_vt), blk, 0, s,           \
+        if (H->n_vt && (sizeof(T) == 16 || ctx->hmv_vt_ranks == 8))                            \
+            hipLaunchKernelGGL((hmv_vt_kernel<T, NTF, 8>), grid_of(H->n_vt), blk, 0, s,        \
                                H->d_vt_items, H->n_vt, Vp, H->d_xp, H->d_t, PV);               \
+        else if (H->n_vt)                                                                      \
+            hipLaunchKernelGGL((hmv_vt_kernel<T, NTF, 4>), grid_of(H->n_vt), blk, 0, s,        \
+                               H->d_vt_items, H->n_vt, Vp, H->d_xp, H->d_t, PV);               \
+        if (H->n_tsum)                                                                         \
+            hipLaunchKernelGGL(hmv_tsum_kernel, dim3((unsigned)((H->n_tsum + 255) / 256)),     \
+                               dim3(256), 0, s, H->d_tsum, H->n_tsum, H->d_t);                 \
         if (H->n_items)                                                                        \
             hipLaunchKernelGGL((hmv_node_kernel<T, NTF>), grid_of(H->n_items), blk, 0, s,      \
                                H->d_item_seg, H->d_item_r0, H->n_items, H->d_seg_nr,           \
