@@ -127,6 +127,7 @@ SIGNATURES = {
     "ccab_stream_join": (_I, [_P, _P]),
     "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
     "ccab_set_graphs": (_I, [_P, _I]),
+    "ccab_set_warm_ax": (_I, [_P, _I]),
     "ccab_stream_wait": (_I, [_P, _P]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_collocation_points_ex": (_I, [_P, _P, _P, _I64, _I, _P]),
@@ -329,6 +330,11 @@ class Context:
     def set_graphs(self, on: bool):
         """Replay captured HIP graphs of the LU factorisation."""
         self.call("ccab_set_graphs", int(bool(on)))
+
+    def set_warm_ax(self, on: bool):
+        """Form a warm start's A x0 inside the assembly (default) or as a
+        product over the stored matrices."""
+        self.call("ccab_set_warm_ax", int(bool(on)))
 
     def set_tuning(self, prec: int, variant: int, outer: int):
         self.call("ccab_set_tuning", int(prec), int(variant), int(outer))

@@ -196,14 +196,18 @@ __device__ __forceinline__ void store_lhs(const SysParams& p, int64_t off, c128 
 // Same layout, outputs and row-sum partials as assemble_kernel; with OutT =
 // c32 and out2 set, out2 also receives the complex-FP32 remainder
 // L - fp16(L h_scale) / h_scale (the FP16 copy plus the remainder stand for
-// the FP64 matrix to about 2^-35 relative per entry).
+// the FP64 matrix to about 2^-35 relative per entry).  With ax_part set,
+// every entry L_ij is also multiplied by x0_j as it is formed: the partial
+// sums of A x0 per column chunk (the first residual of a warm start, b - A
+// x0, in FP64 from the FP64 entries, without reading the stored matrix again).
 constexpr int kF = 16;
 
 template <bool BM, typename OutT, int KF>
 __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     const SysParams* __restrict__ params, const double* __restrict__ soa_base, int64_t n,
     int64_t batch, void* out0, void* out1, void* out2, int64_t ld, int64_t stride,
-    double* __restrict__ rowsum_part, int64_t nchunks, double dk, int cc) {
+    double* __restrict__ rowsum_part, int64_t nchunks, double dk, int cc,
+    const c128* __restrict__ x0, double* __restrict__ ax_part) {
     const int64_t b0 = int64_t(blockIdx.z) * KF;
     const int nf = (int)min<int64_t>(KF, batch - b0);
     const Soa s = soa_of(soa_base, n);
@@ -222,6 +226,28 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
     auto rs = [&](int f) -> double2& { return rs_lds[f * kRows + threadIdx.x]; };
     if (rowsum_part)
         for (int f = 0; f < KF; ++f) rs(f) = double2{0.0, 0.0};
+    // per-lane accumulators of (A x0)_i of the kF systems, in the dynamic
+    // LDS after the row sums' slots, then the guess over the workgroup's
+    // columns (kF x cc, staged once: read per entry, it would be a scalar
+    // load the loop waits for)
+    double2* ax_lds = rs_lds + (rowsum_part ? KF * kRows : 0);
+    c128* x0s = reinterpret_cast<c128*>(ax_lds + KF * kRows);
+    auto axs = [&](int f) -> double2& { return ax_lds[f * kRows + threadIdx.x]; };
+    if (ax_part) {
+        for (int f = 0; f < KF; ++f) axs(f) = double2{0.0, 0.0};
+        const int ncol = (int)(j1 - j0);
+        for (int t = threadIdx.x; t < nf * cc; t += kRows) {
+            const int f = t / cc, jj = t - f * cc;
+            x0s[t] = jj < ncol ? x0[(b0 + f) * n + j0 + jj] : c128{0.0, 0.0};
+        }
+        __syncthreads();
+    }
+    auto add_ax = [&](int f, int64_t j, const c128& L) {
+        const c128 xv = x0s[f * cc + (j - j0)];
+        double2& a = axs(f);
+        a.x += L.re * xv.re - L.im * xv.im;
+        a.y += L.re * xv.im + L.im * xv.re;
+    };
     // column loop variants: 0 general, 1 every system of the group H alone
     // (lhs_a = 1, lhs_b = 0, no row sums): only H's coefficient is formed
     auto columns = [&](auto variant) {
@@ -267,6 +293,7 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     L = ph * lhs_coef_h<BM>(p, a0, a1, a2);
                 }
                 if (active) store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, L, out0, out1, out2);
+                if (ax_part) add_ax(f, j, L);
                 continue;
             }
             if (rowsum_part && p.rowsum_of != 0) {
@@ -278,9 +305,9 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     H = ph * c128{a0, p.k * a1 + a2 * ik};
                     G = ph * c128{g0, g1 * ik};
                 }
-                if (active)
-                    store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld,
-                                    p.lhs_a * H + p.lhs_b * G, out0, out1, out2);
+                const c128 L = p.lhs_a * H + p.lhs_b * G;
+                if (active) store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, L, out0, out1, out2);
+                if (ax_part) add_ax(f, j, L);
                 const c128 a = p.rowsum_of == 1 ? G : H;
                 double2& r = rs(f);
                 r.x += a.re;
@@ -295,6 +322,7 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
                     L = ph * lhs_coef<BM>(p, a0, a1, a2, g0, g1);
                 }
                 if (active) store_lhs<OutT>(p, (b0 + f) * stride + i + j * ld, L, out0, out1, out2);
+                if (ax_part) add_ax(f, j, L);
             }
         }
     }
@@ -309,6 +337,12 @@ __global__ __launch_bounds__(kRows) void assemble_sweep_kernel(
         columns(std::integral_constant<int, 1>{});
     else
         columns(std::integral_constant<int, 0>{});
+    if (ax_part && active)
+        for (int f = 0; f < nf; ++f) {
+            double* dst = ax_part + 2 * (((b0 + f) * nchunks + blockIdx.y) * n + i);
+            dst[0] = axs(f).x;
+            dst[1] = axs(f).y;
+        }
     if (rowsum_part == nullptr || !active) return;
     for (int f = 0; f < nf; ++f) {
         if (params[b0 + f].rowsum_of == 0) continue;
@@ -337,6 +371,21 @@ __global__ void rhs_finish_kernel(const SysParams* __restrict__ params, int64_t 
     }
     const ccab_c128 ri = rhs_inc[b * n + i];
     rhs[b * n + i] = ccab_c128{acc.re + p.rhs_inc_sign * ri.re, acc.im + p.rhs_inc_sign * ri.im};
+}
+
+// ax[b n + i] = sum of the chunk partials of (A x0)_i (fixed chunk order)
+__global__ void ax_finish_kernel(int64_t n, const double* __restrict__ part, int64_t nchunks,
+                                 c128* ax) {
+    const int64_t b = blockIdx.y;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    c128 acc{0.0, 0.0};
+    for (int64_t c = 0; c < nchunks; ++c) {
+        const double* src = part + 2 * ((b * nchunks + c) * n + i);
+        acc.re += src[0];
+        acc.im += src[1];
+    }
+    ax[b * n + i] = acc;
 }
 
 __global__ void cpts_to_soa_kernel(const double* __restrict__ aos, int64_t n, double* soa) {
@@ -498,12 +547,15 @@ __global__ void bodies_incident_kernel(const BodyDesc* __restrict__ desc,
 // to back; the pair as in the sweep kernel, with the phasor from sincos_fast.
 // A body whose right-hand side needs no row sum (rowsum_of 0) takes the
 // column loop that forms lhs_a H + lhs_b G as one complex product per entry
-// (H alone for lhs_a = 1, lhs_b = 0).
+// (H alone for lhs_a = 1, lhs_b = 0).  With ax_part set, the partial sums of
+// A x0 per column chunk as in the sweep kernel (x0 of body c at c nmax, laid
+// out as the row-sum partials).
 template <bool BM, typename OutT>
 __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const SysParams* __restrict__ params, const BodyDesc* __restrict__ desc, int nbody,
     const double* __restrict__ soa_all, void* out0, void* out1, void* out2,
-    double* __restrict__ rowsum_part) {
+    double* __restrict__ rowsum_part, const c128* __restrict__ x0, int64_t nmax,
+    double* __restrict__ ax_part) {
     const int c = body_of_wg(desc, nbody, blockIdx.x);
     const BodyDesc d = desc[c];
     const SysParams p = params[c];  // a private copy: not reloaded after each store
@@ -521,6 +573,15 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
     const double nyx = s.nx[ic], nyy = s.ny[ic], nyz = s.nz[ic];
     const double ik = p.beta.im;  // 1/k (Burton-Miller; unused otherwise)
     double rs_re = 0.0, rs_im = 0.0;
+    c128 ax{0.0, 0.0};
+    // the guess over the workgroup's columns, staged once in LDS
+    __shared__ c128 x0s[kColChunk];
+    if (x0) {
+        if (threadIdx.x < kColChunk)
+            x0s[threadIdx.x] = j0 + threadIdx.x < j1 ? x0[int64_t(c) * nmax + j0 + threadIdx.x]
+                                                     : c128{0.0, 0.0};
+        __syncthreads();
+    }
     // column loop variants: 0 fused lhs_a H + lhs_b G, 1 H alone, 2 H and G
     // separately with row sums
     auto columns = [&](auto variant) {
@@ -554,6 +615,7 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
                 }
             }
             if (active) store_lhs<OutT>(p, d.aoff + i + j * n, L, out0, out1, out2);
+            if (x0) ax += L * x0s[j - j0];
             if constexpr (RS) {
                 const c128 a = p.rowsum_of == 1 ? G : H;
                 rs_re += a.re;
@@ -575,6 +637,31 @@ __global__ __launch_bounds__(kRows) void assemble_bodies_kernel(
         dst[0] = rs_re;
         dst[1] = rs_im;
     }
+    if (ax_part && active) {
+        double* dst = ax_part + d.part_off + 2 * (cb * n + i);
+        dst[0] = ax.re;
+        dst[1] = ax.im;
+    }
+}
+
+// grid (rows / 256, bodies): AX[c nmax + i] = the sum of body c's A x0
+// partials (fixed chunk order), zero past the body's size
+__global__ void ax_finish_bodies_kernel(const BodyDesc* __restrict__ desc,
+                                        const double* __restrict__ part, int64_t nmax, c128* AX) {
+    const int64_t c = blockIdx.y;
+    const BodyDesc d = desc[c];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nmax) return;
+    c128 acc{0.0, 0.0};
+    if (i < d.n) {
+        const int64_t nch = (d.n + kColChunk - 1) / kColChunk;
+        for (int64_t k = 0; k < nch; ++k) {
+            const double* src = part + d.part_off + 2 * (k * d.n + i);
+            acc.re += src[0];
+            acc.im += src[1];
+        }
+    }
+    AX[c * nmax + i] = acc;
 }
 
 // grid (rows / 256, bodies): X = rhs_inc_sign X + rhs_scale sum of the row
@@ -659,7 +746,10 @@ int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDe
 int launch_assemble_bodies(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
                            const BodyDesc* h_desc, const SysParams* d_params,
                            const SysParams* h_params, const double* soa, int prec, void* out0,
-                           void* out1, void* out2, double* part, ccab_c128* X) {
+                           void* out1, void* out2, double* part, ccab_c128* X, const c128* X0,
+                           double* ax_part, c128* AX) {
+    CCAB_CHECK((X0 == nullptr) == (ax_part == nullptr) && (AX == nullptr) == (ax_part == nullptr),
+               "A x0 needs the guess, its partials and the output");
     CCAB_CHECK(nbody >= 1 && nbody <= 65535, "bodies per launch out of range");
     CCAB_CHECK(prec == PREC_F64 || prec == PREC_F64_H, "bodies assemble in FP64");
     CCAB_CHECK(out2 == nullptr || (prec == PREC_F64_H && out1), "the FP32 remainder goes with the FP16 copy");
@@ -682,15 +772,18 @@ int launch_assemble_bodies(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const Bod
         const dim3 grid((unsigned)wgs);
         const int nb = (int)nbody;
         if (prec == PREC_F64_H) {
-            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
-            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part, X0, nmax, ax_part);
+            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c32>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part, X0, nmax, ax_part);
         } else {
-            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
-            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part);
+            if (bm) hipLaunchKernelGGL((assemble_bodies_kernel<true, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part, X0, nmax, ax_part);
+            else hipLaunchKernelGGL((assemble_bodies_kernel<false, c64>), grid, dim3(kRows), 0, st, d_params, d_desc, nb, soa, out0, out1, out2, part, X0, nmax, ax_part);
         }
     }
     hipLaunchKernelGGL(rhs_finish_bodies_kernel, dim3((unsigned)((nmax + 255) / 256), (unsigned)nbody),
                        dim3(256), 0, st, d_desc, d_params, part, nmax, X);
+    if (ax_part)
+        hipLaunchKernelGGL(ax_finish_bodies_kernel, dim3((unsigned)((nmax + 255) / 256), (unsigned)nbody),
+                           dim3(256), 0, st, d_desc, ax_part, nmax, AX);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }
@@ -705,8 +798,9 @@ int launch_cpts_to_soa(ccab_ctx* ctx, const double* d_cpts, int64_t n, double* d
 int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                     int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
                     void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
-                    int64_t* nchunks_out, void* out2) {
+                    int64_t* nchunks_out, void* out2, const c128* d_x0, double* d_ax_part) {
     const int64_t nchunks = (n + kColChunk - 1) / kColChunk;
+    CCAB_CHECK((d_x0 == nullptr) == (d_ax_part == nullptr), "A x0 needs the guess and its partials");
     if (nchunks_out) *nchunks_out = nchunks;
     CCAB_CHECK(batch >= 1 && batch <= 65535, "batch out of range");
     const bool bm = h_params[0].bm != 0;
@@ -729,7 +823,10 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         const char* e = std::getenv("CCAB_ASM_SWEEP");
         return !(e && e[0] == '0');
     }();
-    if (mode == ASM_LHS && prec_compute != PREC_F32 && ((batch >= 2 && sweep_kernel) || out2)) {
+    CCAB_CHECK(!d_ax_part || (mode == ASM_LHS && prec_compute != PREC_F32),
+               "A x0 is formed by the FP64 LHS assembly");
+    if (mode == ASM_LHS && prec_compute != PREC_F32 &&
+        ((batch >= 2 && sweep_kernel) || out2 || d_ax_part)) {
         // equally spaced wavenumbers (to 4 ulp of the largest) let the sweep
         // kernel step the phasor instead of calling sincos per system
         const double k0 = h_params[0].k;
@@ -751,22 +848,39 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
         // chunk is never narrower than kColChunk, so callers size the row-sum
         // partials for (n + kColChunk - 1) / kColChunk chunks
         const AsmTune* tn = asm_tune_of(ctx, prec_compute == PREC_F64_H ? CCAB_C32 : CCAB_C64, n);
-        // (32 systems with row sums would need 128 KiB of dynamic LDS: 16 then)
-        const int kf = tn ? (tn->kf == 32 && d_rowsum_part ? 16 : tn->kf) : kF;
-        const int cc = tn ? std::max(tn->col_chunk, kColChunk) : kColChunk;
+        // (32 systems with row sums or A x0 would need 128 KiB of dynamic
+        // LDS: 16 then; 8 with both)
+        // (A x0: 8 systems per thread, 32 KiB of accumulators, so that the
+        // LDS does not lower the occupancy the registers allow; measured
+        // faster than 16)
+        static const int ax_kf = [] {
+            const char* e = std::getenv("CCAB_ASM_AX_KF");
+            return e && atoi(e) == 16 ? 16 : 8;
+        }();
+        int kf = tn ? tn->kf : kF;
+        if (d_ax_part) kf = std::min(kf, d_rowsum_part ? 8 : ax_kf);
+        else if (d_rowsum_part) kf = std::min(kf, 16);
+        // (the staged guess takes kf x cc x 16 bytes more: 64 columns then)
+        const int cc = d_ax_part ? kColChunk
+                                 : tn ? std::max(tn->col_chunk, kColChunk) : kColChunk;
         const int64_t nch = (n + cc - 1) / cc;
         if (nchunks_out) *nchunks_out = nch;
         dim3 g2(grid.x, (unsigned)nch, (unsigned)((batch + kf - 1) / kf));
-        // row sums in dynamic LDS, 16 B per frequency and lane (64 KiB at kf = 16)
-        const size_t lds = d_rowsum_part ? size_t(kf) * kRows * sizeof(double2) : 0;
+        // row sums and A x0 in dynamic LDS, 16 B per frequency and lane each
+        // (64 KiB at kf = 16)
+        const size_t lds = size_t((d_rowsum_part ? 1 : 0) + (d_ax_part ? 1 : 0)) * kf * kRows *
+                               sizeof(double2) +
+                           (d_ax_part ? size_t(kf) * cc * sizeof(double2) : 0);
+        CCAB_CHECK(lds <= ctx->lds_per_block, "assembly LDS (%zu bytes) exceeds the device's %zu",
+                   lds, ctx->lds_per_block);
 #define CCAB_ASM_SWEEP_LAUNCH(KF)                                                                  \
     do {                                                                                           \
         if (prec_compute == PREC_F64_H) {                                                          \
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc, d_x0, d_ax_part); \
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c32, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc, d_x0, d_ax_part); \
         } else {                                                                                   \
-            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
-            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc); \
+            if (bm) hipLaunchKernelGGL((assemble_sweep_kernel<true, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc, d_x0, d_ax_part); \
+            else hipLaunchKernelGGL((assemble_sweep_kernel<false, c64, KF>), g2, dim3(kRows), lds, st, d_params, d_soa, n, batch, out0, out1, out2, ld, stride, d_rowsum_part, nch, dk, cc, d_x0, d_ax_part); \
         }                                                                                          \
     } while (0)
         if (kf == 8) CCAB_ASM_SWEEP_LAUNCH(8);
@@ -794,6 +908,14 @@ int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h
             else launch_one<double, false, ASM_LHS, c64>(grid, st, d_params, d_soa, n, out0, out1, ld, stride, d_rowsum_part, nchunks);
         }
     }
+    CCAB_HIP(hipGetLastError());
+    return CCAB_OK;
+}
+
+int launch_ax_finish(ccab_ctx* ctx, int64_t batch, int64_t n, const double* d_ax_part,
+                     int64_t nchunks, c128* d_ax) {
+    hipLaunchKernelGGL(ax_finish_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)batch),
+                       dim3(256), 0, ctx->stream, n, d_ax_part, nchunks, d_ax);
     CCAB_HIP(hipGetLastError());
     return CCAB_OK;
 }

@@ -18,7 +18,7 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
                  bool ref_mgs, ccab_solve_stats* st, Ragged* rg = nullptr,
-                 const c64* Alo = nullptr);
+                 const c64* Alo = nullptr, const c128* AX0 = nullptr);
 
 // ---------------------------------------------------------------------------
 // context plumbing
@@ -372,7 +372,10 @@ int ccab_create(int device, ccab_ctx** out) {
     }
     ctx->stream = ctx->own_stream;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->num_cus = prop.multiProcessorCount;
+        ctx->lds_per_block = prop.sharedMemPerBlock;
+    }
     if (const char* gv = getenv("CCAB_GEMM_VARIANT")) {
         const int v = atoi(gv);
         ctx->gemm_variant[0] = ctx->gemm_variant[1] = (v >= 0 && v <= kGemmLibVariant) ? v : 3;
@@ -390,6 +393,7 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* hs = getenv("CCAB_HMV_SORT")) ctx->hmv_sort = atoi(hs) != 0;
     if (const char* hp = getenv("CCAB_HMV_PAIRED")) ctx->hmv_paired = atoi(hp) & 3;
     if (const char* hn = getenv("CCAB_HMV_NT")) ctx->hmv_nt = atoi(hn) != 0;
+    if (const char* wa = getenv("CCAB_WARM_AX")) ctx->warm_ax = atoi(wa) != 0;
     if (const char* hr = getenv("CCAB_HMV_VT_RANKS")) ctx->hmv_vt_ranks = atoi(hr) == 4 ? 4 : 8;
     if (const char* hw = getenv("CCAB_HMV_WAVES_PER_CU"))
         ctx->hmv_waves_per_cu = std::max(0, std::min(64, atoi(hw)));
@@ -469,6 +473,12 @@ int ccab_set_graphs(ccab_ctx* ctx, int on) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->use_graphs = on != 0;
     if (!ctx->use_graphs) graph_clear(ctx);
+    return CCAB_OK;
+}
+
+int ccab_set_warm_ax(ccab_ctx* ctx, int on) {
+    if (!ctx) return CCAB_EINVAL;
+    ctx->warm_ax = on != 0;
     return CCAB_OK;
 }
 
@@ -806,12 +816,27 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
         piv = static_cast<int32_t*>(ctx_buf(ctx, SLOT_SOLVE_PIV, size_t(chunk) * n * 4));
         if (!piv) return CCAB_ENOMEM;
     }
+    // a warm start's A x0 from the assembly: partials per column chunk, then
+    // one vector per system
+    // (with row sums as well: 8 systems per thread, 72 KiB of LDS per
+    // workgroup, where the device allows it)
+    const bool warm_ax = d_x0 && solver == CCAB_SOLVER_ITERATIVE_MIXED && ctx->warm_ax &&
+                         (!need_rowsum || ctx->lds_per_block >= 73728);
+    double* ax_part = nullptr;
+    c128* AX = nullptr;
+    if (warm_ax) {
+        ax_part = static_cast<double*>(ctx_buf(ctx, SLOT_AX_PART, size_t(chunk) * nchunks * n * 16));
+        AX = static_cast<c128*>(ctx_buf(ctx, SLOT_AX, size_t(chunk) * n * 16));
+        if (!ax_part || !AX) return CCAB_ENOMEM;
+    }
     for (int64_t f0 = 0; f0 < nfreq; f0 += chunk) {
         const int64_t nb = std::min(chunk, nfreq - f0);
         int64_t nch = 0;
+        const c128* x0 = d_x0 ? reinterpret_cast<const c128*>(d_x0 + f0 * n) : nullptr;
         CCAB_TRY(launch_assemble(ctx, dp + f0, hp.data() + f0, nb, soa, n,
                                  half_op ? PREC_F64_H : PREC_F64, ASM_LHS, A, A64, n, int64_t(n2),
-                                 part, &nch, Alo));
+                                 part, &nch, Alo, warm_ax ? x0 : nullptr, ax_part));
+        if (warm_ax) CCAB_TRY(launch_ax_finish(ctx, nb, n, ax_part, nch, AX));
         ccab_c128* x = rhs_c + f0 * n;
         CCAB_TRY(launch_rhs_finish(ctx, dp + f0, nb, n, part, nch, x, x));
         if (solver == CCAB_SOLVER_DIRECT) {
@@ -825,13 +850,12 @@ static int sweep_impl(ccab_ctx* ctx, const ccab_physics* ph, int64_t nfreq, cons
                     stats[f0 + b].rel_residual = -1.0;
                 }
         } else if (solver == CCAB_SOLVER_ITERATIVE_MIXED) {
-            const c128* x0 = d_x0 ? reinterpret_cast<const c128*>(d_x0 + f0 * n) : nullptr;
             CCAB_TRY(gmres_ir_run(ctx, n, nb, static_cast<const c128*>(A), A64,
                                   half_op ? CCAB_C32 : CCAB_C64, half_op ? inv_scale + f0 : nullptr,
                                   n, int64_t(n2), reinterpret_cast<c128*>(x), x0, cfg->gmres.restart,
                                   cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
                                   cfg->gmres.ref_mgs != 0, stats ? stats + f0 : nullptr, nullptr,
-                                  Alo));
+                                  Alo, warm_ax ? AX : nullptr));
         } else if (solver == CCAB_SOLVER_DIRECT_MIXED) {
             CCAB_TRY(mixed_run(ctx, n, nb, static_cast<const c128*>(A), A64, n, int64_t(n2),
                                reinterpret_cast<c128*>(x), cfg->ir_tolerance, cfg->ir_max_steps,
@@ -1022,13 +1046,23 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
             part = static_cast<double*>(ctx_buf(ctx, SLOT_ROWSUM, size_t(std::max<int64_t>(poff, 1)) * 8));
             if (!part) return CCAB_ENOMEM;
         }
+        // a warm start's A x0 from the assembly (partials laid out as the
+        // row sums', then one padded vector per body)
+        double* ax_part = nullptr;
+        c128* AX = nullptr;
+        if (X0 && ctx->warm_ax) {
+            ax_part = static_cast<double*>(ctx_buf(ctx, SLOT_AX_PART, size_t(std::max<int64_t>(poff, 1)) * 8));
+            AX = static_cast<c128*>(ctx_buf(ctx, SLOT_AX, size_t(nc) * nmax * 16));
+            if (!ax_part || !AX) return CCAB_ENOMEM;
+        }
         CCAB_TRY(launch_bodies_setup(ctx, nc, nmax, d_desc, d_cpts + 8 * coff[b0], soa, dp + b0,
                                      d_waves, reinterpret_cast<ccab_c128*>(X),
                                      d_x0 ? d_x0 + coff[b0] : nullptr,
                                      reinterpret_cast<ccab_c128*>(X0)));
         CCAB_TRY(launch_assemble_bodies(ctx, nc, nmax, d_desc, desc.data(), dp + b0, hp.data() + b0,
                                         soa, half_op ? PREC_F64_H : PREC_F64, A128, Aop, Alo, part,
-                                        reinterpret_cast<ccab_c128*>(X)));
+                                        reinterpret_cast<ccab_c128*>(X), ax_part ? X0 : nullptr,
+                                        ax_part, AX));
         Ragged rg;
         rg.n = dr;
         rg.aoff = dr + nc;
@@ -1038,7 +1072,8 @@ static int bodies_impl(ccab_ctx* ctx, int64_t nbody, const int64_t* nh, const do
         CCAB_TRY(gmres_ir_run(ctx, nmax, nc, A128, Aop, half_op ? CCAB_C32 : CCAB_C64,
                               half_op ? inv_scale : nullptr, nmax, 0, X, X0, cfg->gmres.restart,
                               cfg->ir_tolerance, cfg->gmres.tolerance, cfg->ir_max_steps,
-                              cfg->gmres.ref_mgs != 0, stats ? stats + b0 : nullptr, &rg, Alo));
+                              cfg->gmres.ref_mgs != 0, stats ? stats + b0 : nullptr, &rg, Alo,
+                              AX));
         for (int64_t c = 0; c < nc; ++c) {
             const int64_t b = b0 + c;
             CCAB_TRY(finish_surface(ctx, hp.data() + b, dp + b, bc, hn[c], 1,

@@ -108,7 +108,7 @@ enum BufSlot {
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
     SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
-    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_COUNT
+    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_AX_PART, SLOT_AX, SLOT_COUNT
 };
 // Sweep tuning (ccab_autotune_sweep / ccab_set_sweep_tuning), per size n:
 // the batched assembly's systems per thread and column chunk for an
@@ -178,6 +178,7 @@ struct ccab_ctx {
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
     int num_cus = 256;
+    size_t lds_per_block = 65536;  // the device's largest workgroup LDS allocation
     // complex-GEMM variant per precision [c128, c64] (see lu.hip); set by
     // ccab_set_tuning / ccab_autotune, initialised from CCAB_GEMM_VARIANT
     int gemm_variant[2] = {3, 0};
@@ -219,6 +220,10 @@ struct ccab_ctx {
     // FP32 H-matrix storage: bit 0 two columns per 16-byte load in V^T x,
     // bit 1 two rows per 16-byte load in the product (CCAB_HMV_PAIRED)
     int hmv_paired = 3;
+    // warm starts of the batched GMRES-IR (sweeps, uneven bodies): A x0 formed
+    // by the assembly as it forms the entries (CCAB_WARM_AX=0: a product
+    // over the stored matrices after the assembly)
+    bool warm_ax = true;
     // FP32 H-matrix storage: ranks per pass of V^T x over paired loads, 4 or
     // 8 (CCAB_HMV_VT_RANKS)
     int hmv_vt_ranks = 8;
@@ -363,7 +368,11 @@ constexpr int kGemmLibVariant = 8;
 int launch_assemble(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                     int64_t batch, const double* d_soa, int64_t n, int prec_compute, int mode,
                     void* out0, void* out1, int64_t ld, int64_t stride, double* d_rowsum_part,
-                    int64_t* nchunks_out, void* out2 = nullptr);
+                    int64_t* nchunks_out, void* out2 = nullptr, const c128* d_x0 = nullptr,
+                    double* d_ax_part = nullptr);
+// ax[b n + i] = the sum over column chunks of the assembly's A x0 partials
+int launch_ax_finish(ccab_ctx* ctx, int64_t batch, int64_t n, const double* d_ax_part,
+                     int64_t nchunks, c128* d_ax);
 int launch_rhs_finish(ccab_ctx* ctx, const SysParams* d_params, int64_t batch, int64_t n,
                       const double* d_rowsum_part, int64_t nchunks, const ccab_c128* d_rhs_inc,
                       ccab_c128* d_rhs);
@@ -403,7 +412,9 @@ int launch_bodies_setup(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDe
 int launch_assemble_bodies(ccab_ctx* ctx, int64_t nbody, int64_t nmax, const BodyDesc* d_desc,
                            const BodyDesc* h_desc, const SysParams* d_params,
                            const SysParams* h_params, const double* soa, int prec, void* out0,
-                           void* out1, void* out2, double* part, ccab_c128* X);
+                           void* out1, void* out2, double* part, ccab_c128* X,
+                           const c128* X0 = nullptr, double* ax_part = nullptr,
+                           c128* AX = nullptr);
 // every frequency of a sweep in one launch (d_params: the device copy of h_params)
 int launch_incident_sweep(ccab_ctx* ctx, const SysParams* d_params, const SysParams* h_params,
                           const ccab_physics* ph, int64_t nfreq, const ccab_wave& w,

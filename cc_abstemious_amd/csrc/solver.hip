@@ -372,6 +372,14 @@ __global__ void copy_active_kernel(int64_t n, const c128* __restrict__ src, c128
     if (i < n) dst[b * n + i] = src[b * n + i];
 }
 
+// dst_b -= src_b for the active systems
+__global__ void sub_active_kernel(int64_t n, const c128* __restrict__ src, c128* dst,
+                                  const int32_t* __restrict__ active) {
+    const int64_t b = active[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[b * n + i] = dst[b * n + i] - src[b * n + i];
+}
+
 // Warm start judged by the residual: with u = A x0 = b - r (r the FP64
 // residual of x0), alpha = (u^H b) / (u^H u) minimises ||b - alpha A x0||,
 // so the scaled guess is never worse than the zero start (alpha = 0 when u
@@ -871,12 +879,17 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 // inner_tol ||r||, and updates x += d.  All systems advance together: one
 // batched c64 matvec and one batched MGS launch per Arnoldi step, the Givens
 // rotations of every system on the host; converged systems leave the active
-// set.  Stops when ||b - A x|| <= tol ||b|| (FP64).
+// set.  Stops when ||b - A x|| <= tol ||b|| (FP64).  AX0 (with X0): A x0
+// formed by the caller (the assembly multiplies each FP64 entry by the guess
+// as it forms it), so the warm start's first residual b - A x0 reads no
+// matrix.
 // ---------------------------------------------------------------------------
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
                  const c128* X0, int m, double inner_tol, double tol, int max_cycles,
-                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg, const c64* Alo) {
+                 bool ref_mgs, ccab_solve_stats* st, Ragged* rg, const c64* Alo,
+                 const c128* AX0) {
+    if (AX0 && !X0) return ctx_fail(ctx, CCAB_EINVAL, "A x0 without the guess x0");
     if (op_prec == CCAB_C32 && (!(A128 || Alo) || !d_inv_scale))
         return ctx_fail(ctx, CCAB_EINVAL,
                         "the FP16 operator needs scales and the c128 matrix or its FP32 remainder");
@@ -1077,7 +1090,10 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         } else {
             hipLaunchKernelGGL(copy_active_kernel, dim3((unsigned)((n + 255) / 256), na), dim3(256),
                                0, s, n, Bv, R, act);
-            if (cyc > 0 || X0) {
+            if (cyc == 0 && AX0) {
+                hipLaunchKernelGGL(sub_active_kernel, dim3((unsigned)((n + 255) / 256), na),
+                                   dim3(256), 0, s, n, AX0, R, act);
+            } else if (cyc > 0 || X0) {
                 set_elems(active);
                 // (a warm start's first residual with the FP32 remainder reads
                 // operator + remainder: leaving A_l x0 to a later cycle costs

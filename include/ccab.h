@@ -187,6 +187,14 @@ int ccab_set_lu_coop(ccab_ctx* ctx, int64_t min_rows);
  * while ccab_profile_enable is on.  Captured graphs are dropped when a
  * workspace buffer is reallocated. */
 int ccab_set_graphs(ccab_ctx* ctx, int on);
+/* Warm starts of the batched GMRES-IR (no reference counterpart: the
+ * reference always starts from zero, gmres.rs:38-39): with on != 0 (the
+ * default; environment CCAB_WARM_AX=0 turns it off) ccab_sweep_warm and
+ * ccab_solve_bodies with d_x0 form A x0 inside the assembly, each FP64
+ * entry times the guess as it is formed, so the first residual b - A x0
+ * reads no stored matrix; with 0 it is a product over the stored matrices
+ * after the assembly.  The two agree to rounding. */
+int ccab_set_warm_ax(ccab_ctx* ctx, int on);
 /* The autotuner: times the batched LU of `batch` synthetic n x n systems in
  * precision prec for every GEMM variant (0..8) and outer width (64, 128),
  * keeps the fastest in the context (as ccab_set_tuning) and returns it;

@@ -424,8 +424,8 @@ def test_sweep_gmres_ir_fp16_operator_matches_fp64(level, residual):
             print(f"frequency {k}: reported {st[k]['rel_residual']:.3e} true {true_rel:.3e}")
             # tolerance 1e-10 plus the remainder's 2^-35 representation error
             assert true_rel <= 3e-10
-        # warm start: the first residual uses the FP16 operator alone and the
-        # remainder's share of x0 joins the first carried update
+        # warm start: the first residual b - A x0 with A x0 formed by the
+        # assembly from the FP64 entries, then carried as above
         g = torch.Generator().manual_seed(11)
         x0 = phi_d * (1 + 0.01 * torch.randn(phi_d.shape, generator=g, dtype=torch.float64).to(phi_d.device))
         phi_w, _, st_w = solve.sweep(pd, x0=x0)
