@@ -43,9 +43,13 @@ struct HBlkDev {
     int64_t u2_off, v2_off;        // recompressed U, V (packed to the ACA rank)
     int64_t clo;                   // first position of the column cluster in the
                                    // depth-first order of its tree (the matvec's x)
+    int64_t s_off;                 // low-rank: U in the product's storage (the blocks of a
+                                   // segment one after another)
+    int64_t ts_off;                // low-rank: its summed V^T x in the matvec's t buffer
     int32_t adm, rank;
     int32_t status;                // ACA: 0 ok, 1 rank capacity reached, 2 rng exhausted
-    int32_t vtc;                   // matvec: column chunks of V^T x (partials at p_off + c rank)
+    int32_t vtc;                   // matvec: column chunks of V^T x (partials at p_off + c
+                                   // rank when there are several, else at ts_off)
     double norm;                   // Frobenius norm estimate / dense Frobenius norm
 };
 
@@ -1057,24 +1061,29 @@ __global__ __launch_bounds__(256) void hmv_vt_kernel(const VtItem* __restrict__ 
     }
 }
 
-// One product work entry per (row cluster, block) in the clusters' block
-// order (dense first, then low-rank): the segment's entries are read with
-// one vector load (a lane per entry) and broadcast, instead of a dependent
-// chain of scalar loads per block.
+// One product work entry per dense block and per low-rank run of a segment
+// (a segment lists its dense blocks, then at most one run): the segment's
+// entries are read with one vector load (a lane per entry) and broadcast,
+// instead of a dependent chain of scalar loads per block.  A run is the
+// segment's low-rank blocks stored one after another: U of all of them one
+// nr x R column-major matrix, their V^T x one vector of R values in t, so
+// the product reads it as it reads a dense block (16 columns in flight per
+// lane, no per-block rank loop).
 struct HmvWork {
-    int64_t off;   // dense: d_off; low-rank: u_off
-    int64_t xoff;  // dense: clo (xp); low-rank: p_off (t, summed over the chunks)
-    int32_t cnt;   // dense: columns; low-rank: rank
-    int32_t vtc;   // 0: dense; low-rank: chunks of V^T x
+    int64_t off;   // dense: d_off (D); run: its first U entry (U)
+    int64_t xoff;  // dense: clo (xp); run: its first value in t
+    int32_t cnt;   // dense: columns; run: the sum of its blocks' ranks
+    int32_t kind;  // 0: dense; 1: low-rank run
 };
 
 // The rank partials of a low-rank block's V^T x chunks summed in chunk order
-// into the first chunk's slots (one thread per rank of a block with more than
+// into the block's place in t (one thread per rank of a block with more than
 // one chunk), so that the product reads one t per rank: summed inside the
 // product, every 64-row tile of the block repeated the sum as a chain of
 // dependent scalar loads.
 struct TsumItem {
     int64_t p;       // t offset of the rank's first-chunk partial
+    int64_t dst;     // t offset of the summed value
     int32_t stride;  // the block's rank (distance between its chunks' partials)
     int32_t cnt;     // chunks
 };
@@ -1084,7 +1093,7 @@ __global__ void hmv_tsum_kernel(const TsumItem* __restrict__ items, int64_t cnt,
     const TsumItem q = items[i];
     c128 s = t[q.p];
     for (int ch = 1; ch < q.cnt; ++ch) s += t[q.p + int64_t(ch) * q.stride];
-    t[q.p] = s;
+    t[q.dst] = s;
 }
 
 __device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
@@ -1094,9 +1103,10 @@ __device__ __forceinline__ int64_t bcast64(int64_t v, int src) {
 }
 
 // Segment s of a row cluster (work entries seg_e0[s] .. seg_e1[s]), rows
-// r0 .. r0 + 63 of the cluster (seg_nr[s] rows): acc = sum of D_b xp (dense)
-// and U_b t_b (low-rank, t_b summed over its chunks), slot nacc[seg_off[s] +
-// row] = alpha acc
+// r0 .. r0 + 63 of the cluster (seg_nr[s] rows): acc = sum of D_b xp over its
+// dense blocks and U_run t_run over its low-rank run, slot nacc[seg_off[s] +
+// row] = alpha acc.  Both are the same loop: a column-major nr x c matrix
+// times c values (xp of the block's column cluster, or the run's t).
 template <typename T, bool NT>
 __global__ __launch_bounds__(256) void hmv_node_kernel(
     const int32_t* __restrict__ item_seg, const int32_t* __restrict__ item_r0, int64_t cnt,
@@ -1116,8 +1126,8 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
         if constexpr (sizeof(T) == 8) {
             // complex FP32, an even row count (offsets are even): lane l of
             // half h takes rows r0 + 2(l mod 32) and the next (one aligned
-            // 16-byte load per column), columns (or ranks) congruent to h
-            // mod 2; the halves are added at the end
+            // 16-byte load per column), columns congruent to h mod 2; the
+            // halves are added at the end
             if (paired && (nr & 1) == 0) {
                 const int h = lane >> 5;
                 const int64_t p0 = item_r0[it] + 2 * (lane & 31);
@@ -1132,62 +1142,32 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                         const int64_t off = bcast64(mine.off, qi);
                         const int64_t xoff = bcast64(mine.xoff, qi);
                         const int c = __builtin_amdgcn_readlane(mine.cnt, qi);
-                        const int vtc = __builtin_amdgcn_readlane(mine.vtc, qi);
-                        if (vtc == 0) {
-                            const T* d = D + off + q0 + int64_t(h) * nr;
-                            const c128* xc = xp + xoff;  // wave-uniform: both parities loaded, one kept
-                            int jb = 0;
-                            for (; jb + 15 < c; jb += 16) {
-                                float4 v[8];
-                                c128 xv[8];
+                        const bool run = __builtin_amdgcn_readlane(mine.kind, qi) != 0;
+                        const T* d = (run ? U : D) + off + q0 + int64_t(h) * nr;
+                        // wave-uniform: both parities loaded, one kept
+                        const c128* xc = (run ? t : xp) + xoff;
+                        int jb = 0;
+                        for (; jb + 15 < c; jb += 16) {
+                            float4 v[8];
+                            c128 xv[8];
 #pragma unroll
-                                for (int u = 0; u < 8; ++u) {
-                                    v[u] = ldm2<NT>(d + int64_t(jb + 2 * u) * nr);
-                                    const c128 xa = xc[jb + 2 * u], xb = xc[jb + 2 * u + 1];
-                                    xv[u] = h ? xb : xa;
-                                }
-#pragma unroll
-                                for (int u = 0; u < 8; ++u) {
-                                    a0 += c128{v[u].x, v[u].y} * xv[u];
-                                    a1 += c128{v[u].z, v[u].w} * xv[u];
-                                }
+                            for (int u = 0; u < 8; ++u) {
+                                v[u] = ldm2<NT>(d + int64_t(jb + 2 * u) * nr);
+                                const c128 xa = xc[jb + 2 * u], xb = xc[jb + 2 * u + 1];
+                                xv[u] = h ? xb : xa;
                             }
-                            for (; jb < c; jb += 2) {
-                                if (jb + h < c) {
-                                    const float4 v = ldm2<NT>(d + int64_t(jb) * nr);
-                                    const c128 xv = xc[jb + h];
-                                    a0 += c128{v.x, v.y} * xv;
-                                    a1 += c128{v.z, v.w} * xv;
-                                }
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) {
+                                a0 += c128{v[u].x, v[u].y} * xv[u];
+                                a1 += c128{v[u].z, v[u].w} * xv[u];
                             }
-                        } else {
-                            // half h takes the ranks of parity h, up to four of
-                            // them per pass, compiled per count (half 0's) so
-                            // their loads issue together; half 1's rank past
-                            // the block reads rank c - 1 again (the same
-                            // addresses as half 0), times t = 0
-                            const T* uu = U + off + q0;
-                            const c128* tb = t + xoff;
-                            for (int k0 = 0; k0 < c; k0 += 8) {
-                                auto lr = [&](auto nuc) {
-                                    constexpr int NU = decltype(nuc)::value;
-                                    float4 v[NU];
-                                    c128 tk[NU];
-#pragma unroll
-                                    for (int u = 0; u < NU; ++u)
-                                        v[u] = ldm2<NT>(uu + int64_t(min(k0 + 2 * u + h, c - 1)) * nr);
-#pragma unroll
-                                    for (int u = 0; u < NU; ++u) {
-                                        const c128 s = tb[min(k0 + 2 * u + h, c - 1)];
-                                        tk[u] = k0 + 2 * u + h < c ? s : c128{0.0, 0.0};
-                                    }
-#pragma unroll
-                                    for (int u = 0; u < NU; ++u) {
-                                        a0 += c128{v[u].x, v[u].y} * tk[u];
-                                        a1 += c128{v[u].z, v[u].w} * tk[u];
-                                    }
-                                };
-                                with_count<1, 4>(min(4, (c - k0 + 1) / 2), lr);
+                        }
+                        for (; jb < c; jb += 2) {
+                            if (jb + h < c) {
+                                const float4 v = ldm2<NT>(d + int64_t(jb) * nr);
+                                const c128 xv = xc[jb + h];
+                                a0 += c128{v.x, v.y} * xv;
+                                a1 += c128{v.z, v.w} * xv;
                             }
                         }
                     }
@@ -1214,45 +1194,25 @@ __global__ __launch_bounds__(256) void hmv_node_kernel(
                 const int64_t off = bcast64(mine.off, q);
                 const int64_t xoff = bcast64(mine.xoff, q);
                 const int c = __builtin_amdgcn_readlane(mine.cnt, q);
-                const int vtc = __builtin_amdgcn_readlane(mine.vtc, q);
-                if (vtc == 0) {
-                    const T* d = D + off + pp;
-                    const c128* xc = xp + xoff;  // wave-uniform, contiguous
-                    int j = 0;
-                    for (; j + 15 < c; j += 16) {
-                        c128 a[16];
+                const bool run = __builtin_amdgcn_readlane(mine.kind, q) != 0;
+                const T* d = (run ? U : D) + off + pp;
+                const c128* xc = (run ? t : xp) + xoff;  // wave-uniform, contiguous
+                int j = 0;
+                for (; j + 15 < c; j += 16) {
+                    c128 a[16];
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
+                    for (int u = 0; u < 16; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) acc += a[u] * xc[j + u];
-                    }
-                    for (; j + 3 < c; j += 4) {
-                        c128 a[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) acc += a[u] * xc[j + u];
-                    }
-                    for (; j < c; ++j) acc += ldm<NT>(d + int64_t(j) * nr) * xc[j];
-                } else {
-                    // up to eight ranks per pass, compiled per count so
-                    // their loads issue together
-                    const T* uu = U + off + pp;
-                    const c128* tb = t + xoff;
-                    for (int k0 = 0; k0 < c; k0 += 8) {
-                        auto lr = [&](auto nuc) {
-                            constexpr int NU = decltype(nuc)::value;
-                            c128 v[NU], tk[NU];
-#pragma unroll
-                            for (int u = 0; u < NU; ++u) v[u] = ldm<NT>(uu + int64_t(k0 + u) * nr);
-#pragma unroll
-                            for (int u = 0; u < NU; ++u) tk[u] = tb[k0 + u];
-#pragma unroll
-                            for (int u = 0; u < NU; ++u) acc += v[u] * tk[u];
-                        };
-                        with_count<1, 8>(min(c - k0, 8), lr);
-                    }
+                    for (int u = 0; u < 16; ++u) acc += a[u] * xc[j + u];
                 }
+                for (; j + 3 < c; j += 4) {
+                    c128 a[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) a[u] = ldm<NT>(d + int64_t(j + u) * nr);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc += a[u] * xc[j + u];
+                }
+                for (; j < c; ++j) acc += ldm<NT>(d + int64_t(j) * nr) * xc[j];
             }
         }
         if (on) nacc[seg_off[sg] + p] = alpha * acc;
@@ -1313,6 +1273,17 @@ __global__ void reorder_lr_kernel(const HBlkDev* __restrict__ blks, const int32_
         const int64_t j = e % bk.nc, k = e / bk.nc;
         V2[bk.v_off + pos[bk.c_off + j] + k * bk.nc] = V[bk.v_off + e];
     }
+}
+
+// U of every low-rank block (nr x rank, contiguous) copied to its place in
+// the runs' storage (s_off)
+__global__ void repack_u_kernel(const HBlkDev* __restrict__ blks, const int32_t* __restrict__ list,
+                                const c128* __restrict__ U, c128* U3) {
+    const HBlkDev bk = blks[list[blockIdx.x]];
+    const int64_t cu = bk.nr * bk.rank;
+    for (int64_t e = threadIdx.x + int64_t(blockIdx.y) * blockDim.x; e < cu;
+         e += int64_t(blockDim.x) * gridDim.y)
+        U3[bk.s_off + e] = U[bk.u_off + e];
 }
 
 // xp[p] = x[perm[p]]
@@ -1652,17 +1623,13 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     H->n_dense = (int64_t)dl.size();
     H->n_lr = (int64_t)ll2.size();
     // V^T x of every low-rank block: one rank-vector of partials per chunk
-    // of kVtChunk columns, at p_off
-    int64_t t_total = 0;
+    // of kVtChunk columns (offsets with the matvec plan below)
     for (int32_t b : ll2) {
         HBlkDev& d = H->blk[b];
         d.vtc = (int32_t)((d.nc + kVtChunk - 1) / kVtChunk);
-        d.p_off = t_total;
-        t_total += int64_t(d.vtc) * d.rank;
     }
     H->d_D = dalloc<c128>(H->owned, d_off);
-    H->d_t = dalloc<c128>(H->owned, t_total);
-    if (!H->d_D || !H->d_t) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+    if (!H->d_D) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
     CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_dense_list, dl.data(), dl.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_lr_list, ll2.data(), ll2.size() * 4, hipMemcpyHostToDevice));
@@ -1821,6 +1788,59 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     const int64_t nseg = (int64_t)seg_node.size();
     std::vector<int32_t> seg_nr(nseg);
     for (int64_t sg = 0; sg < nseg; ++sg) seg_nr[sg] = (int32_t)T[seg_node[sg]].idx.size();
+    // Low-rank runs: the low-rank blocks of a segment (its last entries; a
+    // cluster lists its dense blocks first) are stored one after another, so
+    // that their U form one nr x R column-major matrix (R the sum of their
+    // ranks; the segment's blocks share the row cluster) and their summed
+    // V^T x one vector of R values: the product treats the run as one dense
+    // block over t.  Work entries: a dense block (>= 0) or a run (-1 - its
+    // first block, with R in wk_cnt); every run's U at an even offset.
+    std::vector<int64_t> wk_blk;
+    std::vector<int32_t> wk_cnt, seg_w0(nseg), seg_w1(nseg);
+    int64_t s_total = 0, ts_total = 0;
+    auto even = [](int64_t v) { return (v + 1) & ~int64_t(1); };
+    for (int64_t sg = 0; sg < nseg; ++sg) {
+        seg_w0[sg] = (int32_t)wk_blk.size();
+        int32_t first = -1;
+        int64_t R = 0;
+        for (int64_t e = seg_e0[sg]; e < seg_e1[sg]; ++e) {
+            const int32_t b = nlist[e];
+            HBlkDev& d = H->blk[b];
+            if (!d.adm) {
+                wk_blk.push_back(b);
+                wk_cnt.push_back(0);
+                continue;
+            }
+            if (first < 0) {
+                first = b;
+                s_total = even(s_total);
+            }
+            d.s_off = s_total;
+            s_total += d.nr * d.rank;
+            d.ts_off = ts_total;
+            ts_total += d.rank;
+            R += d.rank;
+        }
+        if (first >= 0) {
+            wk_blk.push_back(-1 - int64_t(first));
+            wk_cnt.push_back((int32_t)R);
+        }
+        seg_w1[sg] = (int32_t)wk_blk.size();
+    }
+    // t: the summed V^T x of every block (ts_off), then the chunk partials
+    // of the blocks with several chunks (p_off)
+    int64_t t_total = ts_total;
+    for (int32_t b : ll2) {
+        HBlkDev& d = H->blk[b];
+        if (d.vtc > 1) {
+            d.p_off = t_total;
+            t_total += int64_t(d.vtc) * d.rank;
+        } else {
+            d.p_off = d.ts_off;
+        }
+    }
+    H->d_t = dalloc<c128>(H->owned, std::max<int64_t>(t_total, 1));
+    if (!H->d_t) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
     for (int64_t sg = 0; sg < nseg; ++sg)
         for (int64_t r0 = 0; r0 < (int64_t)T[seg_node[sg]].idx.size(); r0 += 64) {
             item_seg.push_back((int32_t)sg);
@@ -1883,7 +1903,8 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     for (int32_t b : ll2) {
         const HBlkDev& d = H->blk[b];
         if (d.vtc > 1)
-            for (int32_t k = 0; k < d.rank; ++k) tsum.push_back(TsumItem{d.p_off + k, d.rank, d.vtc});
+            for (int32_t k = 0; k < d.rank; ++k)
+                tsum.push_back(TsumItem{d.p_off + k, d.ts_off + k, d.rank, d.vtc});
     }
     H->n_tsum = (int64_t)tsum.size();
     if (H->n_tsum) {
@@ -1897,7 +1918,7 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     H->d_item_seg = dalloc<int32_t>(H->owned, item_seg.size());
     H->d_item_r0 = dalloc<int32_t>(H->owned, item_r0.size());
     H->d_seg_nr = dalloc<int32_t>(H->owned, nseg);
-    H->d_work = dalloc<HmvWork>(H->owned, nb);
+    H->d_work = dalloc<HmvWork>(H->owned, std::max<size_t>(wk_blk.size(), 1));
     H->d_seg_e0 = dalloc<int32_t>(H->owned, nseg);
     H->d_seg_e1 = dalloc<int32_t>(H->owned, nseg);
     H->d_seg_off = dalloc<int64_t>(H->owned, nseg);
@@ -1910,8 +1931,8 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     CCAB_HIP(hipMemcpy(H->d_item_seg, item_seg.data(), item_seg.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_item_r0, item_r0.data(), item_r0.size() * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_seg_nr, seg_nr.data(), nseg * 4, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_seg_e0, seg_e0.data(), nseg * 4, hipMemcpyHostToDevice));
-    CCAB_HIP(hipMemcpy(H->d_seg_e1, seg_e1.data(), nseg * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_e0, seg_w0.data(), nseg * 4, hipMemcpyHostToDevice));
+    CCAB_HIP(hipMemcpy(H->d_seg_e1, seg_w1.data(), nseg * 4, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_seg_off, seg_off.data(), nseg * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_rowptr, cnt.data(), (n + 1) * 8, hipMemcpyHostToDevice));
     CCAB_HIP(hipMemcpy(H->d_ent, ent.data(), ent.size() * 8, hipMemcpyHostToDevice));
@@ -1922,12 +1943,10 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
         // in w_off / f_off for the conversion), dense blocks at new offsets
         // (staged in u2_off); every offset even, so that two consecutive
         // complex-FP32 entries of a column are one aligned 16-byte load
-        int64_t cu = 0, cv = 0, cd = 0;
-        auto even = [](int64_t v) { return (v + 1) & ~int64_t(1); };
+        int64_t cu = s_total, cv = 0, cd = 0;
         for (int32_t b : ll2) {
             HBlkDev& d = H->blk[b];
-            d.w_off = cu;
-            cu = even(cu + d.nr * d.rank);
+            d.w_off = d.s_off;
             d.f_off = cv;
             cv = even(cv + d.nc * d.rank);
         }
@@ -1958,6 +1977,20 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
         H->fp32 = true;
         H->mv_bytes *= 0.5;
         trace.mark("FP32 storage");
+    } else if (!ll2.empty()) {
+        // c128 storage: U moved to the runs' order
+        c128* U3 = dalloc<c128>(H->owned, std::max<int64_t>(s_total, 1));
+        if (!U3) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        CCAB_HIP(hipMemcpy(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(repack_u_kernel, dim3((unsigned)ll2.size(), 4), dim3(256), 0, s, H->d_blk,
+                           H->d_lr_list, H->d_U, U3);
+        CCAB_HIP(hipGetLastError());
+        for (int32_t b : ll2) H->blk[b].u_off = H->blk[b].s_off;
+        CCAB_HIP(hipMemcpyAsync(H->d_blk, H->blk.data(), nb * sizeof(HBlkDev), hipMemcpyHostToDevice, s));
+        CCAB_HIP(hipStreamSynchronize(s));
+        release.push_back(H->d_U);
+        H->d_U = U3;
+        trace.mark("low-rank runs");
     }
     // release the build workspace
     for (void* w : release) {
@@ -1970,14 +2003,19 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     // the product's work entries, with the final storage offsets (FP32
     // storage repacks U)
     {
-        std::vector<HmvWork> work(nb);
-        for (int64_t e = 0; e < nb; ++e) {
-            const HBlkDev& d = H->blk[nlist[e]];
-            work[e] = d.adm ? HmvWork{d.u_off, d.p_off, d.rank, d.vtc}
-                            : HmvWork{d.d_off, d.clo, (int32_t)d.nc, 0};
+        std::vector<HmvWork> work(wk_blk.size());
+        for (size_t e = 0; e < wk_blk.size(); ++e) {
+            if (wk_blk[e] >= 0) {
+                const HBlkDev& d = H->blk[wk_blk[e]];
+                work[e] = HmvWork{d.d_off, d.clo, (int32_t)d.nc, 0};
+            } else {
+                const HBlkDev& d = H->blk[-1 - wk_blk[e]];
+                work[e] = HmvWork{d.u_off, d.ts_off, wk_cnt[e], 1};
+            }
         }
-        CCAB_HIP(hipMemcpy(H->d_work, work.data(), size_t(nb) * sizeof(HmvWork),
-                           hipMemcpyHostToDevice));
+        if (!work.empty())
+            CCAB_HIP(hipMemcpy(H->d_work, work.data(), work.size() * sizeof(HmvWork),
+                               hipMemcpyHostToDevice));
         std::vector<VtItem> items(vt_blk.size());
         for (size_t q = 0; q < items.size(); ++q) {
             const HBlkDev& d = H->blk[vt_blk[q]];
