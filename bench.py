@@ -97,8 +97,9 @@ def parse():
     ap.add_argument("--tune-sweep", type=int, default=1,
                     help="1: run the sweep autotuner (assembly split, GEMV workgroup target and "
                          "loads) on the workload before timing (untimed)")
-    ap.add_argument("--train-dirs", type=int, default=12)
-    ap.add_argument("--train-steps", type=int, default=6000)
+    ap.add_argument("--train-dirs", type=int, default=64,
+                    help="incident directions (8 random frequencies each) the sweep warm start trains on")
+    ap.add_argument("--train-steps", type=int, default=32000)
     return ap.parse_args()
 
 
@@ -252,11 +253,15 @@ def bodies_block(args, ctx):
         train_s = time.perf_counter() - t_tr
         solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
         torch.cuda.synchronize()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
         t1 = time.perf_counter()
         for _ in range(reps):
             phw, _, stw = solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
         torch.cuda.synchronize()
         dtw = (time.perf_counter() - t1) / reps
+        prof_w = ctx.profile_read(reset=True)
+        ctx.profile(False)
         gerr = [float(torch.linalg.vector_norm(g - p) / torch.linalg.vector_norm(p))
                 for g, p in zip(sur.predict_bodies(bodies), phw)]
         itw = [s_["iterations"] for s_ in stw]
@@ -270,7 +275,8 @@ def bodies_block(args, ctx):
                     for i in range(nsub)) if nsub else None,
                 "guess_rel_err": {"median": float(np.median(gerr)), "max": float(max(gerr))},
                 "train_systems": sols_n, "train_steps": args.bodies_train_steps,
-                "final_loss": losses[-1][1], "train_s": train_s}
+                "final_loss": losses[-1][1], "train_s": train_s,
+                "kernel_ms": {k: round(v["ms"] / reps, 3) for k, v in prof_w.items()}}
     gemv = prof.get("gemv", {"count": 0, "ms": float("nan"), "work": 0.0})
     return {"workload": (f"{len(bodies)} uneven bodies (perturbed spheres, {min(ns)}-{max(ns)} "
                          "unknowns), Burton-Miller rigid, plane wave, ka in [1, 6], GMRES-IR "
@@ -555,7 +561,8 @@ def main():
                     "inference_ms_per_sweep": prof_w.get("surrogate", {}).get("ms", 0.0) / args.steps,
                     "train_systems": 8 * args.train_dirs, "train_steps": args.train_steps,
                     "final_loss": losses[-1][1], "train_s": train_s,
-                    "network": "32-256-256-256-2 SiLU, BF16 MFMA GEMMs, FP32 Adam"}
+                    "network": "32-256-256-256-2 SiLU, BF16 MFMA GEMMs, FP32 Adam",
+                    "kernel_ms": {k: round(v["ms"] / args.steps, 3) for k, v in prof_w.items()}}
 
     gmres_iters = iter_stats(st, args.restart, args.inner_tol,
                              "complex FP16, scaled (FP64 residual "
