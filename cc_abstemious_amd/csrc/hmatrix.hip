@@ -517,6 +517,276 @@ __global__ __launch_bounds__(kAcaThreads) void aca_kernel(E entry, HBlkDev* blks
     }
 }
 
+// The ACA of aca_kernel for blocks of at most 64 PL rows and 64 PL columns
+// (PL = 1, 2 or 4): one wavefront per block, four blocks per workgroup.
+// Lane l holds entries l, l + 64, ... of the residual rows (column indices)
+// and of the residual columns (row indices) in registers, the exclusion sets
+// are bit masks and every reduction is a wave reduction, so the steps wait
+// on no workgroup barrier and on no workspace round trip.  The steps, pivot
+// rules (first index of the largest |r| on ties) and expressions are those
+// of aca_kernel; only the sums of the norm estimate are formed in another
+// order.
+__device__ __forceinline__ c128 shfl_c(c128 v, int src) {
+    return c128{__shfl(v.re, src, 64), __shfl(v.im, src, 64)};
+}
+
+template <int PL>
+struct LaneVec {
+    c128 v[PL];
+    // entry t (wave-uniform t < 64 PL) to every lane
+    __device__ __forceinline__ c128 get(int64_t t) const {
+        c128 r = v[0];
+#pragma unroll
+        for (int p = 1; p < PL; ++p)
+            if (t >> 6 == p) r = v[p];
+        return shfl_c(r, int(t & 63));
+    }
+};
+
+template <int PL>
+struct LaneMask {
+    uint64_t w[PL];
+    __device__ __forceinline__ bool test(int64_t t) const {
+        uint64_t x = w[0];
+#pragma unroll
+        for (int p = 1; p < PL; ++p)
+            if (t >> 6 == p) x = w[p];
+        return (x >> (t & 63)) & 1;
+    }
+    __device__ __forceinline__ void set(int64_t t) {
+#pragma unroll
+        for (int p = 0; p < PL; ++p)
+            if (t >> 6 == p) w[p] |= uint64_t(1) << (t & 63);
+    }
+};
+
+template <typename E, int PL>
+__global__ __launch_bounds__(256) void aca_wave_kernel(E entry, HBlkDev* blks,
+                                                       const int32_t* __restrict__ list,
+                                                       int64_t cnt,
+                                                       const int64_t* __restrict__ rows,
+                                                       const int64_t* __restrict__ cols,
+                                                       c128* U, c128* V,
+                                                       const uint32_t* __restrict__ rng,
+                                                       double tol) {
+    using Vec = LaneVec<PL>;
+    const int lane = threadIdx.x & 63;
+    const int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    if (w >= cnt) return;  // the whole wavefront
+    HBlkDev& bk = blks[list[w]];
+    const int64_t nr = bk.nr, nc = bk.nc;
+    const int64_t* R = rows + bk.r_off;
+    const int64_t* C = cols + bk.c_off;
+    c128* Ub = U + bk.u_off;  // nr x capacity
+    c128* Vb = V + bk.v_off;  // nc x capacity
+    int64_t gi[PL], gj[PL];
+#pragma unroll
+    for (int p = 0; p < PL; ++p) {
+        const int64_t t = lane + 64 * p;
+        gi[p] = t < nr ? R[t] : 0;
+        gj[p] = t < nc ? C[t] : 0;
+    }
+    const int64_t nmin = nr < nc ? nr : nc;
+    // residual row i over the columns, residual column j over the rows
+    // (aca_row, aca_col)
+    auto res_row = [&](int64_t i, int rank) {
+        Vec r;
+        const int64_t g = R[i];
+#pragma unroll
+        for (int p = 0; p < PL; ++p) {
+            const int64_t t = lane + 64 * p;
+            c128 v{0.0, 0.0};
+            if (t < nc) {
+                v = entry(g, gj[p]);
+                for (int k = 0; k < rank; ++k) v -= Ub[i + k * nr] * Vb[t + k * nc];
+            }
+            r.v[p] = v;
+        }
+        return r;
+    };
+    auto res_col = [&](int64_t j, int rank) {
+        Vec r;
+        const int64_t g = C[j];
+#pragma unroll
+        for (int p = 0; p < PL; ++p) {
+            const int64_t t = lane + 64 * p;
+            c128 v{0.0, 0.0};
+            if (t < nr) {
+                v = entry(gi[p], g);
+                for (int k = 0; k < rank; ++k) v -= Vb[j + k * nc] * Ub[t + k * nr];
+            }
+            r.v[p] = v;
+        }
+        return r;
+    };
+    // argmax_excl: len when every candidate is excluded or NaN
+    auto argmax = [&](const Vec& r, int64_t len, const LaneMask<PL>& excl, double& bv) {
+        double best = -1.0;
+        int64_t bi = len;
+#pragma unroll
+        for (int p = 0; p < PL; ++p) {
+            const int64_t t = lane + 64 * p;
+            if (t < len && !((excl.w[p] >> lane) & 1)) {
+                const double c = cabs(r.v[p]);
+                if (best < c) {
+                    best = c;
+                    bi = t;
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off, 64);
+            const int64_t oi = __shfl_xor(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) {
+                best = ov;
+                bi = oi;
+            }
+        }
+        bv = best;
+        return bi;
+    };
+    int pos = 0;
+    int status = 0;
+    int64_t iref, jref;
+    {
+        bool ok = true;
+        iref = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+        jref = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+        if (!ok) status = 2;
+    }
+    Vec riref = res_row(iref, 0), rjref = res_col(jref, 0);
+    LaneMask<PL> exI, exJ;
+#pragma unroll
+    for (int p = 0; p < PL; ++p) exI.w[p] = exJ.w[p] = 0;
+    double total = 0.0;
+    int rank = 0;
+    for (int64_t k = 0; k < nmin; ++k) {
+        if (rank == RMAX) {
+            status = 1;
+            break;
+        }
+        double bvj, bvi;
+        int64_t jstar = argmax(riref, nc, exJ, bvj);
+        int64_t istar = argmax(rjref, nr, exI, bvi);
+        if (jstar >= nc || istar >= nr) {
+            status = 3;
+            break;
+        }
+        Vec ristar, rjstar;
+        if (bvi > bvj) {  // |rjref[istar]| > |riref[jstar]|
+            ristar = res_row(istar, rank);
+            jstar = argmax(ristar, nc, exJ, bvj);
+            if (jstar >= nc) {
+                status = 3;
+                break;
+            }
+            rjstar = res_col(jstar, rank);
+        } else {
+            rjstar = res_col(jstar, rank);
+            istar = argmax(rjstar, nr, exI, bvi);
+            if (istar >= nr) {
+                status = 3;
+                break;
+            }
+            ristar = res_row(istar, rank);
+        }
+        exI.set(istar);
+        exJ.set(jstar);
+        // u = rjstar, v = ristar / ristar[jstar]
+        const c128 factor = crecip(ristar.get(jstar));
+        Vec vk;
+        const Vec& uk = rjstar;
+        double un = 0.0, vn = 0.0;
+#pragma unroll
+        for (int p = 0; p < PL; ++p) {
+            const int64_t t = lane + 64 * p;
+            vk.v[p] = ristar.v[p] * factor;
+            if (t < nc) {
+                Vb[t + rank * nc] = vk.v[p];
+                vn += norm_sqr(vk.v[p]);
+            }
+            if (t < nr) {
+                Ub[t + rank * nr] = uk.v[p];
+                un += norm_sqr(uk.v[p]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        // norm estimate (aca.rs:224-241): |u|^2 |v|^2 + sum_i 2 |u_i.u| |v_i.v|
+        un = wave_sum(un);
+        vn = wave_sum(vn);
+        const double nk2 = un * vn;
+        total += nk2;
+        for (int i = 0; i < rank; ++i) {
+            double ur = 0.0, ui = 0.0, vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int p = 0; p < PL; ++p) {
+                const int64_t t = lane + 64 * p;
+                if (t < nr) {
+                    const c128 z = Ub[t + i * nr] * uk.v[p];
+                    ur += z.re;
+                    ui += z.im;
+                }
+                if (t < nc) {
+                    const c128 z = Vb[t + i * nc] * vk.v[p];
+                    vr += z.re;
+                    vi += z.im;
+                }
+            }
+            ur = wave_sum(ur);
+            ui = wave_sum(ui);
+            vr = wave_sum(vr);
+            vi = wave_sum(vi);
+            total += 2.0 * hypot(ur, ui) * hypot(vr, vi);
+        }
+        ++rank;
+        const double step = sqrt(nk2) / sqrt(total);
+        if (step < tol / 50.0) break;
+        // next reference row / column (aca.rs:116-132)
+        if (iref == istar) {
+            bool ok = true;
+            int64_t ii = iref;
+            if (exI.test(ii)) {
+                ii = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+                for (int64_t t = 0; t < nmin - 1; ++t) {
+                    ii = (ii + 1) % nmin;
+                    if (!exI.test(ii)) break;
+                }
+            }
+            iref = ii;
+            if (!ok) status = 2;
+            riref = res_row(iref, rank);
+        } else {
+            const c128 ui = uk.get(iref);  // U[iref, rank - 1]
+#pragma unroll
+            for (int p = 0; p < PL; ++p) riref.v[p] -= ui * vk.v[p];
+        }
+        if (jref == jstar) {
+            bool ok = true;
+            int64_t jj = jref;
+            if (exJ.test(jj)) {
+                jj = rng_uniform(rng, &pos, uint32_t(nmin), &ok);
+                for (int64_t t = 0; t < nmin - 1; ++t) {
+                    jj = (jj + 1) % nmin;
+                    if (!exJ.test(jj)) break;
+                }
+            }
+            jref = jj;
+            if (!ok) status = 2;
+            rjref = res_col(jref, rank);
+        } else {
+            const c128 vj = vk.get(jref);  // V[jref, rank - 1]
+#pragma unroll
+            for (int p = 0; p < PL; ++p) rjref.v[p] -= vj * uk.v[p];
+        }
+        if (status) break;
+    }
+    if (lane == 0) {
+        bk.rank = rank;
+        bk.status = status;
+        bk.norm = sqrt(total);
+    }
+}
+
 // One-sided complex Jacobi SVD of the r x r matrix W (LDS, [row][col]),
 // rotations accumulated in Vs (identity on entry), with the parallel
 // (round-robin) ordering: the r columns (one padding column when r is odd)
@@ -1487,9 +1757,45 @@ int hmatrix_from_blocks(ccab_ctx* ctx, const E& entry, std::vector<HNode> T,
     trace.mark("allocation and upload");
     // batched ACA + recompression of the admissible blocks
     if (!ll.empty()) {
-        hipLaunchKernelGGL(aca_kernel<E>, dim3((unsigned)ll.size()), dim3(kAcaThreads), 0, s, entry,
-                           H->d_blk, H->d_lr_list, H->d_rows, H->d_cols, Ua, Va, Wws, Fws, d_rng,
-                           tol);
+        // blocks of at most 64 PL rows and columns (PL = 1, 2, 4 up to
+        // CCAB_HM_WAVE_ACA, default 2; 0: none): one wavefront each; the
+        // larger ones a workgroup each (n = 81920 ACA: 56 ms with a workgroup
+        // per block, 51.5 with wavefronts up to 64 or 128, 54 up to 256,
+        // where the registers leave one wavefront per SIMD)
+        const char* wac = std::getenv("CCAB_HM_WAVE_ACA");
+        const int wave_pl = wac ? std::atoi(wac) : 2;
+        std::vector<int32_t> grp[4];  // PL = 1, 2, 4; workgroup
+        for (int32_t b : ll) {
+            const int64_t m = std::max(H->blk[b].nr, H->blk[b].nc);
+            const int g = m <= 64 && wave_pl >= 1 ? 0 : m <= 128 && wave_pl >= 2 ? 1
+                        : m <= 256 && wave_pl >= 4 ? 2 : 3;
+            grp[g].push_back(b);
+        }
+        int32_t* d_aca = dalloc<int32_t>(H->owned, ll.size());
+        if (!d_aca) return fail(CCAB_ENOMEM, "H-matrix allocation failed");
+        std::vector<int32_t> aca_order;
+        for (auto& g : grp) aca_order.insert(aca_order.end(), g.begin(), g.end());
+        CCAB_HIP(hipMemcpy(d_aca, aca_order.data(), aca_order.size() * 4, hipMemcpyHostToDevice));
+        int64_t off = 0;
+        for (int g = 0; g < 4; ++g) {
+            const int64_t c = (int64_t)grp[g].size();
+            if (c == 0) continue;
+            const dim3 wg((unsigned)((c + 3) / 4));
+            if (g == 0)
+                hipLaunchKernelGGL((aca_wave_kernel<E, 1>), wg, dim3(256), 0, s, entry, H->d_blk,
+                                   d_aca + off, c, H->d_rows, H->d_cols, Ua, Va, d_rng, tol);
+            else if (g == 1)
+                hipLaunchKernelGGL((aca_wave_kernel<E, 2>), wg, dim3(256), 0, s, entry, H->d_blk,
+                                   d_aca + off, c, H->d_rows, H->d_cols, Ua, Va, d_rng, tol);
+            else if (g == 2)
+                hipLaunchKernelGGL((aca_wave_kernel<E, 4>), wg, dim3(256), 0, s, entry, H->d_blk,
+                                   d_aca + off, c, H->d_rows, H->d_cols, Ua, Va, d_rng, tol);
+            else
+                hipLaunchKernelGGL(aca_kernel<E>, dim3((unsigned)c), dim3(kAcaThreads), 0, s, entry,
+                                   H->d_blk, d_aca + off, H->d_rows, H->d_cols, Ua, Va, Wws, Fws,
+                                   d_rng, tol);
+            off += c;
+        }
         // recompression with LDS sized by rank: blocks bucketed by their ACA
         // rank (most are far below RMAX, and the RMAX-sized kernel holds one
         // workgroup per CU)
