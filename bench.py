@@ -47,6 +47,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM traffic / algorithmic bytes of gemv_part_vec_kernel<c32,4,true>, measured by
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB read and write
 GEMV_C32_TRAFFIC_RATIO = 1.003
+# the same for one H-matrix product at n = 81920 (all its launches), c128 and
+# FP32 block storage
+HMATVEC_TRAFFIC_RATIO = {"c128": 1.115, "fp32_storage": 1.21}
 
 
 def parse():
@@ -745,7 +748,9 @@ def roofline_hmatvec(hm):
     """HBM roofline of the H-matrix matvec at n = 81920 (c128 and FP32 block
     storage): algorithmic bytes = the stored block entries one product reads
     (sum r (m + n) + sum of the dense blocks' m n, times 16 or 8 B), over the
-    HIP-event time of one product (its three launches)."""
+    HIP-event time of one product (its launches); traffic = those bytes times
+    the FETCH_SIZE + WRITE_SIZE ratio measured with rocprofv3 on the same
+    workload."""
     if not hm:
         return None
     rec = hm.get("icosphere_81920_tri")
@@ -757,10 +762,12 @@ def roofline_hmatvec(hm):
         if not r.get("matvec_event_ms"):
             continue
         gbs = r["matvec_bytes"] / (r["matvec_event_ms"] * 1e-3) / 1e9
-        out[key] = {"bound": "hbm", "kernel": "hmv_vt_kernel + hmv_node_kernel + hmv_reduce_kernel "
-                    "(one H-matrix product)", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        out[key] = {"bound": "hbm", "kernel": "hmv_gather + hmv_vt + hmv_tsum + hmv_node + hmv_reduce "
+                    "kernels (one H-matrix product)", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": r["matvec_bytes"],
-                    "avg_launch_ms": r["matvec_event_ms"], "traffic": None}
+                    "avg_launch_ms": r["matvec_event_ms"],
+                    "traffic": r["matvec_bytes"] * HMATVEC_TRAFFIC_RATIO[key],
+                    "traffic_ratio_measured": HMATVEC_TRAFFIC_RATIO[key]}
     return out
 
 
