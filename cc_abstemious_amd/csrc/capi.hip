@@ -394,6 +394,10 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* hp = getenv("CCAB_HMV_PAIRED")) ctx->hmv_paired = atoi(hp) & 3;
     if (const char* hn = getenv("CCAB_HMV_NT")) ctx->hmv_nt = atoi(hn) != 0;
     if (const char* wa = getenv("CCAB_WARM_AX")) ctx->warm_ax = atoi(wa) != 0;
+    if (const char* gb = getenv("CCAB_GS_BLOCK")) {
+        const int v = atoi(gb);
+        ctx->gs_block = v == 1 ? 1 : v == 4 ? 4 : 8;
+    }
     if (const char* hr = getenv("CCAB_HMV_VT_RANKS")) ctx->hmv_vt_ranks = atoi(hr) == 4 ? 4 : 8;
     if (const char* hw = getenv("CCAB_HMV_WAVES_PER_CU"))
         ctx->hmv_waves_per_cu = std::max(0, std::min(64, atoi(hw)));

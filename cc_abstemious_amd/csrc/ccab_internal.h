@@ -224,6 +224,10 @@ struct ccab_ctx {
     // by the assembly as it forms the entries (CCAB_WARM_AX=0: a product
     // over the stored matrices after the assembly)
     bool warm_ax = true;
+    // batched GMRES-IR: Gram-Schmidt over basis vectors 8 at a time (classical
+    // within a block, modified across blocks) on the multi-workgroup path;
+    // 4, or 1: one at a time (CCAB_GS_BLOCK)
+    int gs_block = 8;
     // FP32 H-matrix storage: ranks per pass of V^T x over paired loads, 4 or
     // 8 (CCAB_HMV_VT_RANKS)
     int hmv_vt_ranks = 8;

@@ -232,12 +232,150 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_coop_kernel(
     leave();
 }
 
+// The textbook step (coefficient conj(q_i) w) of mgs_coop_kernel with the
+// basis vectors taken B at a time: the B coefficients of a block are formed
+// against the same w (classical Gram-Schmidt within a block, modified across
+// blocks), so a step waits at ceil((k + 1) / B) + 1 grid barriers instead of
+// k + 2 and loads B vectors at a time (each read twice: for the coefficients
+// and for the update).  The Arnoldi relation A q_k = Q h holds to rounding
+// either way; within a block the basis vectors are orthonormal to working
+// accuracy, so orthogonality is kept to the same order.  Used by the
+// batched GMRES-IR (its inner solves stop at 1e-3 relative).
+template <int RQ, int B>
+__global__ __launch_bounds__(kMgsCoopThreads) void mgs_block_coop_kernel(
+    int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ, c128* h, int64_t sh,
+    const int32_t* __restrict__ active, uint64_t* __restrict__ xbuf, int64_t sys_words,
+    unsigned spin_limit, const int32_t* __restrict__ skip, uint64_t* __restrict__ sticky) {
+    constexpr int NW = kMgsCoopThreads / 64;
+    constexpr int NV = 2 * B;
+    __shared__ double red[NV][NW];
+    __shared__ double tot[NV];
+    __shared__ int s_ok;
+    const int G = gridDim.x, g = blockIdx.x;
+    const int64_t z = blockIdx.y;
+    const int64_t b = active ? active[z] : z;
+    // every workgroup of a skipped system leaves before its first barrier
+    if (skip && skip[b]) return;
+    uint64_t* hdr = xbuf + z * sys_words;
+    c128* q = Q + b * sQ;
+    c128* w = q + (k + 1) * ldq;
+    c128* hb = h + b * sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r0 = int64_t(g) * kMgsCoopThreads * RQ;
+    c128 wv[RQ];
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+        const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+        wv[u] = row < n ? w[row] : c128{0.0, 0.0};
+    }
+    // the grid-wide sums of v[0..NV) over the workgroups' partials, barrier
+    // number nb; every workgroup adds the G partials in the same order
+    auto grid_sum = [&](double (&v)[NV], int64_t nb) -> bool {
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] = wave_sum(v[t]);
+        if (lane == 0)
+#pragma unroll
+            for (int t = 0; t < NV; ++t) red[t][wave] = v[t];
+        __syncthreads();
+        uint64_t* par = hdr + kCoopHeaderWords + (nb & 1) * NV * int64_t(G);
+        if (tid < NV) {
+            double pa = 0.0;
+            for (int t = 0; t < NW; ++t) pa += red[tid][t];
+            __hip_atomic_store(par + NV * int64_t(g) + tid, uint64_t(__double_as_longlong(pa)),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!coop_barrier(hdr, uint64_t(nb + 1) * uint64_t(G), spin_limit, &s_ok)) return false;
+        if (tid < NV) {
+            double sa = 0.0;
+            for (int t = 0; t < G; ++t)
+                sa += __longlong_as_double((long long)__hip_atomic_load(
+                    par + NV * int64_t(t) + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            tot[tid] = sa;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] = tot[t];
+        __syncthreads();  // tot and red are reused by the next sum
+        return true;
+    };
+    auto leave = [&]() {
+        __syncthreads();
+        if (tid == 0) {
+            const uint64_t before = __hip_atomic_fetch_add(hdr + 16, uint64_t(1), __ATOMIC_ACQ_REL,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            if (before == uint64_t(G - 1)) {
+                __hip_atomic_store(hdr, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(hdr + 8, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(hdr + 16, uint64_t(0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    };
+    bool ok = true;
+    int64_t nb = 0;
+    for (int64_t i0 = 0; i0 <= k && ok; i0 += B) {
+        const int nbk = (int)min<int64_t>(B, k + 1 - i0);
+        double v[NV];
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] = 0.0;
+        // (vectors past k repeat vector k; their sums are not used)
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb) {
+            const c128* qi = q + min<int64_t>(i0 + bb, k) * ldq;
+#pragma unroll
+            for (int u = 0; u < RQ; ++u) {
+                const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+                const c128 a = row < n ? qi[row] : c128{0.0, 0.0};
+                const c128 x = wv[u];
+                v[2 * bb] += a.re * x.re + a.im * x.im;
+                v[2 * bb + 1] += a.re * x.im - a.im * x.re;
+            }
+        }
+        ok = grid_sum(v, nb++);
+        if (!ok) break;
+        for (int bb = 0; bb < nbk; ++bb) {
+            const c128 hi{v[2 * bb], v[2 * bb + 1]};
+            const c128* qi = q + (i0 + bb) * ldq;
+#pragma unroll
+            for (int u = 0; u < RQ; ++u) {
+                const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+                if (row < n) wv[u] = wv[u] - hi * qi[row];
+            }
+            if (g == 0 && tid == 0) hb[i0 + bb] = hi;
+        }
+    }
+    double nv[NV];
+#pragma unroll
+    for (int t = 0; t < NV; ++t) nv[t] = 0.0;
+    if (ok) {
+#pragma unroll
+        for (int u = 0; u < RQ; ++u) nv[0] += wv[u].re * wv[u].re + wv[u].im * wv[u].im;
+        ok = grid_sum(nv, nb++);
+    }
+    if (!ok) {
+        if (g == 0 && tid == 0) {
+            hb[k + 1] = c128{__builtin_nan(""), 0.0};
+            __hip_atomic_store(sticky, uint64_t(1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        leave();
+        return;
+    }
+    const double hn = sqrt(nv[0]);
+    if (g == 0 && tid == 0) hb[k + 1] = c128{hn, 0.0};
+    const c128 sc = crecip(c128{hn, 0.0});
+#pragma unroll
+    for (int u = 0; u < RQ; ++u) {
+        const int64_t row = r0 + tid + int64_t(u) * kMgsCoopThreads;
+        if (row < n) w[row] = wv[u] * sc;
+    }
+    leave();
+}
+
 // Modified Gram-Schmidt step of nact systems (active list, or all when
 // null): one workgroup per system, or, for few large systems that would
 // leave most CUs idle, mgs_coop_kernel over up to one workgroup per CU.
 int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t ldq, int64_t sQ,
                c128* h, int64_t sh, const int32_t* act, int64_t nact, const int32_t* skip = nullptr,
-               bool reserve_only = false) {
+               bool reserve_only = false, int block = 1) {
     hipStream_t s = ctx->stream;
     if (ctx->mgs_coop && n >= 4096) {
         int rq = 0;
@@ -250,7 +388,10 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
             }
         }
         if (rq) {
-            const int64_t sys_words = kCoopHeaderWords + 4 * int64_t(std::max(ctx->num_cus, 1));
+            // (room for the exchange of mgs_block_coop_kernel<., 8>: two
+            // buffers of 16 values per workgroup; the same layout for every
+            // launch of the context, whose headers sit at multiples of it)
+            const int64_t sys_words = kCoopHeaderWords + 32 * int64_t(std::max(ctx->num_cus, 1));
             uint64_t* xbuf = static_cast<uint64_t*>(
                 ctx_buf(ctx, SLOT_MGS_COOP, size_t(nact) * sys_words * 8));
             if (!xbuf) return CCAB_ENOMEM;
@@ -273,7 +414,15 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
             const dim3 grid((unsigned)G, (unsigned)nact);
 #define CCAB_MGS_COOP(R)                                                                        \
     do {                                                                                        \
-        if (ref)                                                                                \
+        if (!ref && block == 4)                                                                 \
+            hipLaunchKernelGGL((mgs_block_coop_kernel<R, 4>), grid, dim3(kMgsCoopThreads), 0, s, \
+                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit,    \
+                               skip, ctx->coop_abort);                                 \
+        else if (!ref && block == 8)                                                            \
+            hipLaunchKernelGGL((mgs_block_coop_kernel<R, 8>), grid, dim3(kMgsCoopThreads), 0, s, \
+                               n, k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit,    \
+                               skip, ctx->coop_abort);                                 \
+        else if (ref)                                                                           \
             hipLaunchKernelGGL((mgs_coop_kernel<true, R>), grid, dim3(kMgsCoopThreads), 0, s, n, \
                                k, Q, ldq, sQ, h, sh, act, xbuf, sys_words, kCoopSpinLimit, skip, \
                                ctx->coop_abort);                                       \
@@ -325,7 +474,11 @@ bool mgs_coop_fits() {
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kMgsCoopThreads, 0) ==
                    hipSuccess && nb >= 1;
     };
-    return fits(reinterpret_cast<const void*>(&mgs_coop_kernel<true, 1>)) &&
+    return fits(reinterpret_cast<const void*>(&mgs_block_coop_kernel<1, 4>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_block_coop_kernel<8, 4>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_block_coop_kernel<1, 8>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_block_coop_kernel<8, 8>)) &&
+           fits(reinterpret_cast<const void*>(&mgs_coop_kernel<true, 1>)) &&
            fits(reinterpret_cast<const void*>(&mgs_coop_kernel<true, 8>)) &&
            fits(reinterpret_cast<const void*>(&mgs_coop_kernel<false, 1>)) &&
            fits(reinterpret_cast<const void*>(&mgs_coop_kernel<false, 8>));
@@ -1003,7 +1156,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                              zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr, kdone_d));
         {
             ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
-            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d));
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d, false,
+                                ctx->gs_block));
         }
         hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act, ni,
                            Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d, inner_tol, tol,
@@ -1042,7 +1196,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
             // (a growing buffer would drop the graphs and allocate mid-capture)
             CCAB_TRY(gemv_active(ctx, op_prec, n, n, na_c, act, Aop, lda, sA, Q, sq, one, zero, Q + n,
                                  sq, d_inv_scale, rg, nullptr, kdone_d, -1, true));
-            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, 0, Q, n, sq, Hd, m + 2, act, na_c, kdone_d, true));
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, 0, Q, n, sq, Hd, m + 2, act, na_c, kdone_d, true,
+                                ctx->gs_block));
             // (the active list and every buffer key.* names stay valid for the graph)
             CCAB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = CCAB_OK;
