@@ -87,6 +87,70 @@ __global__ __launch_bounds__(kMgsThreads) void mgs_kernel(int64_t n, int64_t k, 
     for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) w[r] = w[r] * sc;
 }
 
+// The textbook step of mgs_kernel with the basis vectors taken B at a time
+// (classical Gram-Schmidt within a block, modified across blocks, as
+// mgs_block_coop_kernel): per block one pass over w and the block's vectors
+// for the B coefficients, one block reduction, one pass for the update.
+template <int B>
+__global__ __launch_bounds__(kMgsThreads) void mgs_block_kernel(int64_t n, int64_t k, c128* Q,
+                                                                int64_t ldq, int64_t sQ, c128* h,
+                                                                int64_t sh,
+                                                                const int32_t* __restrict__ active,
+                                                                const int32_t* __restrict__ skip) {
+    constexpr int NV = 2 * B;
+    constexpr int NW = kMgsThreads / 64;
+    __shared__ double red[NV][NW];
+    const int64_t b = active ? active[blockIdx.x] : blockIdx.x;
+    if (skip && skip[b]) return;
+    c128* q = Q + b * sQ;
+    c128* w = q + (k + 1) * ldq;
+    c128* hb = h + b * sh;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t i0 = 0; i0 <= k; i0 += B) {
+        const int nbk = (int)min<int64_t>(B, k + 1 - i0);
+        double v[NV];
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] = 0.0;
+        for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) {
+            const c128 x = w[r];
+            // (vectors past k repeat vector k; their sums are not used)
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) {
+                const c128 a = q[min<int64_t>(i0 + bb, k) * ldq + r];
+                v[2 * bb] += a.re * x.re + a.im * x.im;
+                v[2 * bb + 1] += a.re * x.im - a.im * x.re;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NV; ++t) v[t] = wave_sum(v[t]);
+        if (lane == 0)
+#pragma unroll
+            for (int t = 0; t < NV; ++t) red[t][wave] = v[t];
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NV; ++t) {
+            v[t] = 0.0;
+            for (int ww = 0; ww < NW; ++ww) v[t] += red[t][ww];
+        }
+        __syncthreads();  // red is reused by the next block
+        for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) {
+            c128 x = w[r];
+            for (int bb = 0; bb < nbk; ++bb) x = x - c128{v[2 * bb], v[2 * bb + 1]} * q[(i0 + bb) * ldq + r];
+            w[r] = x;
+        }
+        if (threadIdx.x == 0)
+            for (int bb = 0; bb < nbk; ++bb) hb[i0 + bb] = c128{v[2 * bb], v[2 * bb + 1]};
+        __syncthreads();
+    }
+    double nn = 0.0, dummy = 0.0;
+    for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) nn += w[r].re * w[r].re + w[r].im * w[r].im;
+    block_sum2(nn, dummy, reinterpret_cast<double*>(red));
+    const double hn = sqrt(nn);
+    if (threadIdx.x == 0) hb[k + 1] = c128{hn, 0.0};
+    const c128 sc = crecip(c128{hn, 0.0});
+    for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) w[r] = w[r] * sc;
+}
+
 // The same step for few large systems: G workgroups per system (grid.x), each
 // holding rows g*256*RQ .. of w in registers.  Per basis vector every
 // workgroup publishes its partial dot product, one grid barrier, and all
@@ -443,7 +507,13 @@ int launch_mgs(ccab_ctx* ctx, bool ref, int64_t n, int64_t k, c128* Q, int64_t l
         }
     }
     if (reserve_only) return CCAB_OK;
-    if (ref)
+    if (!ref && block == 8)
+        hipLaunchKernelGGL(mgs_block_kernel<8>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k,
+                           Q, ldq, sQ, h, sh, act, skip);
+    else if (!ref && block == 4)
+        hipLaunchKernelGGL(mgs_block_kernel<4>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k,
+                           Q, ldq, sQ, h, sh, act, skip);
+    else if (ref)
         hipLaunchKernelGGL(mgs_kernel<true>, dim3((unsigned)nact), dim3(kMgsThreads), 0, s, n, k, Q,
                            ldq, sQ, h, sh, act, skip);
     else
