@@ -93,3 +93,39 @@ def test_hmatrix_gmres(ctx_pair):
     r2 = solve.sweep(pd, ctx=coop)
     assert r1[2][0]["num_mv"] == r2[2][0]["num_mv"]
     assert _rel(r2[0][0], r1[0][0]) < 1e-9
+
+
+@pytest.mark.parametrize("coop_on", [True, False])
+def test_gmres_ir_blocked_gram_schmidt(coop_on):
+    """The batched GMRES-IR's Gram-Schmidt with the basis vectors 8 (the
+    default) or 4 at a time per reduction (classical within a block, modified
+    across blocks) against one at a time (CCAB_GS_BLOCK=1), on the
+    multi-workgroup path (3 systems of 5120 unknowns) and on one workgroup
+    per system (CCAB_MGS_COOP=0): the same iterations to within one per
+    system and the same solutions to the solver's tolerance."""
+    ka = np.array([1.0, 3.5, 7.0])
+    pd = PreData(mesh.icosphere(4), ka * 343.0 / (2 * math.pi), 343.0, 1.21,
+                 capi.BURTON_MILLER, capi.EXTERIOR, capi.BC_NORMAL_VELOCITY, (0.0, 0.0),
+                 IncidentWave(capi.WAVE_PLANE, (0.3, -0.5, 0.8), (1.0, 0.0)),
+                 Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-10, restart=30,
+                        ir_tolerance=1e-3, ir_max_steps=30, ref_mgs=False,
+                        matrix_prec=capi.C32))
+    _note(f"GMRES-IR blocked Gram-Schmidt, multi-workgroup={coop_on}")
+    res = {}
+    for blk in ("1", "4", "8"):
+        os.environ["CCAB_GS_BLOCK"] = blk
+        if not coop_on:
+            os.environ["CCAB_MGS_COOP"] = "0"
+        try:
+            ctx = capi.Context()
+        finally:
+            del os.environ["CCAB_GS_BLOCK"]
+            os.environ.pop("CCAB_MGS_COOP", None)
+        res[blk] = solve.sweep(pd, ctx=ctx)
+        ctx.close()
+    for blk in ("4", "8"):
+        for a, b, sa, sb in zip(res["1"][0], res[blk][0], res["1"][2], res[blk][2]):
+            assert sa["converged"] and sb["converged"]
+            assert sb["rel_residual"] <= 1e-10
+            assert abs(sa["iterations"] - sb["iterations"]) <= 1, (blk, sa, sb)
+            assert _rel(b, a) < 1e-8
