@@ -33,6 +33,7 @@ constexpr int GT = 64;          // GEMM tile (rows and columns)
 constexpr int GK = 32;          // GEMM k step (one MFMA)
 constexpr int LDSK = GK + 8;    // padded LDS row, in bf16
 constexpr int kWgradSplit = 32; // reduction splits of the weight-gradient GEMMs
+constexpr int kSumSplit = 256;  // row blocks of the column sums (bias gradients, output layer)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -445,28 +446,55 @@ __global__ void last_bwd_kernel(const float* __restrict__ dout, const float* __r
 // partial column sums over a row chunk: bias gradients of the hidden layers
 // (colsum of dZ) or, with A3, the output layer's weight gradient.
 // out layout per split: [ncols] (bias) or [2][W] + [2] (output layer)
+// column sums of dZ (P x W BF16, row-major) over the rows of row block
+// blockIdx.x (chunk rows), one thread per column, 16 rows in flight
 __global__ void colsum_kernel(const uint16_t* __restrict__ dZ, int64_t P, int W, int64_t chunk,
                               float* part) {
     const int k = threadIdx.x;
     const int64_t p0 = int64_t(blockIdx.x) * chunk, p1 = std::min(P, p0 + chunk);
     float s = 0.f;
-    for (int64_t p = p0; p < p1; ++p) s += bf2f(dZ[p * W + k]);
+    int64_t p = p0;
+    for (; p + 15 < p1; p += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = bf2f(dZ[(p + u) * W + k]);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += v[u];
+    }
+    for (; p < p1; ++p) s += bf2f(dZ[p * W + k]);
     part[int64_t(blockIdx.x) * W + k] = s;
 }
 
+// output layer over the rows of row block blockIdx.x: dW4[c][k] partials
+// sum_p dout[p][c] A3[p][k] and db4[c] partials sum_p dout[p][c]
 __global__ void last_wgrad_kernel(const float* __restrict__ dout, const uint16_t* __restrict__ A3,
                                   int64_t P, int W, int64_t chunk, float* part) {
     const int k = threadIdx.x;
     const int64_t p0 = int64_t(blockIdx.x) * chunk, p1 = std::min(P, p0 + chunk);
     float s0 = 0.f, s1 = 0.f, d0 = 0.f, d1 = 0.f;
-    for (int64_t p = p0; p < p1; ++p) {
+    int64_t p = p0;
+    for (; p + 7 < p1; p += 8) {
+        float a[8];
+        float2 g[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = bf2f(A3[(p + u) * W + k]);
+            g[u] = reinterpret_cast<const float2*>(dout)[p + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            s0 += g[u].x * a[u];
+            s1 += g[u].y * a[u];
+            d0 += g[u].x;
+            d1 += g[u].y;
+        }
+    }
+    for (; p < p1; ++p) {
         const float a = bf2f(A3[p * W + k]);
         s0 += dout[2 * p] * a;
         s1 += dout[2 * p + 1] * a;
-        if (k == 0) {
-            d0 += dout[2 * p];
-            d1 += dout[2 * p + 1];
-        }
+        d0 += dout[2 * p];
+        d1 += dout[2 * p + 1];
     }
     float* o = part + int64_t(blockIdx.x) * (2 * W + 2);
     o[k] = s0;
@@ -477,7 +505,21 @@ __global__ void last_wgrad_kernel(const float* __restrict__ dout, const uint16_t
     }
 }
 
-// dst[e] = sum over splits of part[s][e], fixed order
+// dst[e] = the sum of the nsplit partials part[q count + e]: one workgroup
+// per entry, a fixed-order tree
+__global__ __launch_bounds__(256) void reduce_splits_tree_kernel(const float* __restrict__ part,
+                                                                 int nsplit, int64_t count,
+                                                                 float* dst) {
+    __shared__ float red[4];
+    const int64_t e = blockIdx.x;
+    float s = 0.f;
+    for (int q = threadIdx.x; q < nsplit; q += 256) s += part[int64_t(q) * count + e];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) dst[e] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 __global__ void reduce_splits_kernel(const float* __restrict__ part, int nsplit, int64_t count,
                                      float* dst) {
     const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -648,7 +690,8 @@ int sur_forward(ccab_ctx* ctx, ccab_surrogate* S, int64_t P, const float* T, flo
 int sur_backward(ccab_ctx* ctx, ccab_surrogate* S, int64_t P) {
     const int W = S->W;
     const int64_t chunk = (P + kWgradSplit - 1) / kWgradSplit;
-    const int64_t need = int64_t(kWgradSplit) * W * W;
+    const int64_t schunk = (P + kSumSplit - 1) / kSumSplit;
+    const int64_t need = std::max(int64_t(kWgradSplit) * W * W, int64_t(kSumSplit) * (2 * W + 2));
     if (S->part_cap < need) {
         CCAB_HIP(hipStreamSynchronize(ctx->stream));
         if (S->part) S->release(S->part);
@@ -658,10 +701,10 @@ int sur_backward(ccab_ctx* ctx, ccab_surrogate* S, int64_t P) {
     }
     hipStream_t s = ctx->stream;
     // output layer: dW4, db4; dZ3
-    hipLaunchKernelGGL(last_wgrad_kernel, dim3(kWgradSplit), dim3(W), 0, s, S->dout, S->A[2], P, W,
-                       chunk, S->part);
-    hipLaunchKernelGGL(reduce_splits_kernel, grid1d(2 * W + 2), dim3(256), 0, s, S->part,
-                       kWgradSplit, int64_t(2 * W + 2), S->grad + S->off_w[3]);
+    hipLaunchKernelGGL(last_wgrad_kernel, dim3(kSumSplit), dim3(W), 0, s, S->dout, S->A[2], P, W,
+                       schunk, S->part);
+    hipLaunchKernelGGL(reduce_splits_tree_kernel, dim3(unsigned(2 * W + 2)), dim3(256), 0, s,
+                       S->part, kSumSplit, int64_t(2 * W + 2), S->grad + S->off_w[3]);
     uint16_t* dZ = S->dZa;
     uint16_t* dZn = S->dZb;
     hipLaunchKernelGGL(last_bwd_kernel, grid1d(P * W), dim3(256), 0, s, S->dout,
@@ -688,9 +731,9 @@ int sur_backward(ccab_ctx* ctx, ccab_surrogate* S, int64_t P) {
         hipLaunchKernelGGL(reduce_splits_kernel, grid1d(W * K), dim3(256), 0, s, S->part, nsp,
                            int64_t(W) * K, S->grad + S->off_w[l]);
         // db_l = column sums of dZ (the part buffer is free again after the reduction)
-        hipLaunchKernelGGL(colsum_kernel, dim3(kWgradSplit), dim3(W), 0, s, dZ, P, W, chunk, S->part);
-        hipLaunchKernelGGL(reduce_splits_kernel, grid1d(W), dim3(256), 0, s, S->part, kWgradSplit,
-                           int64_t(W), S->grad + S->off_b[l]);
+        hipLaunchKernelGGL(colsum_kernel, dim3(kSumSplit), dim3(W), 0, s, dZ, P, W, schunk, S->part);
+        hipLaunchKernelGGL(reduce_splits_tree_kernel, dim3(unsigned(W)), dim3(256), 0, s, S->part,
+                           kSumSplit, int64_t(W), S->grad + S->off_b[l]);
         if (l > 0) {
             // dZ_{l-1} = (dZ_l W_l) * SiLU'(Z_{l-1})
             GemmArgs h{};
