@@ -108,7 +108,7 @@ enum BufSlot {
     SLOT_GIR_Y, SLOT_GIR_NRM, SLOT_GIR_ACT, SLOT_GMRES_NRM, SLOT_HMAT_X, SLOT_SUR_SYS,
     SLOT_SUR_LOSS, SLOT_FIELD_SYS, SLOT_FIELD_PART, SLOT_SOLVE_ISCALE, SLOT_LU_COOP,
     SLOT_RAGGED, SLOT_MGS_COOP, SLOT_GIR_GIV, SLOT_GIR_D, SLOT_INC_AMP, SLOT_HM_UA, SLOT_HM_VA, SLOT_HM_W,
-    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_AX_PART, SLOT_AX, SLOT_COUNT
+    SLOT_HM_F, SLOT_HM_R, SLOT_BODY_DESC, SLOT_COOP_ABORT, SLOT_AX_PART, SLOT_AX, SLOT_GIR_U, SLOT_GIR_RC, SLOT_COUNT
 };
 // Sweep tuning (ccab_autotune_sweep / ccab_set_sweep_tuning), per size n:
 // the batched assembly's systems per thread and column chunk for an
@@ -228,6 +228,11 @@ struct ccab_ctx {
     // within a block, modified across blocks) on the multi-workgroup path;
     // 4, or 1: one at a time (CCAB_GS_BLOCK)
     int gs_block = 8;
+    // batched GMRES-IR: refinement cycles after the first run GCRO, deflated
+    // by the first this-many directions of A times the first cycle's Krylov
+    // space (C = A U from its Arnoldi relation; CCAB_GIR_RECYCLE, 0: every
+    // cycle restarts from scratch)
+    int gir_recycle = 8;
     // FP32 H-matrix storage: ranks per pass of V^T x over paired loads, 4 or
     // 8 (CCAB_HMV_VT_RANKS)
     int hmv_vt_ranks = 8;

@@ -135,11 +135,16 @@ __global__ __launch_bounds__(kMgsThreads) void mgs_block_kernel(int64_t n, int64
         __syncthreads();  // red is reused by the next block
         for (int64_t r = threadIdx.x; r < n; r += kMgsThreads) {
             c128 x = w[r];
-            for (int bb = 0; bb < nbk; ++bb) x = x - c128{v[2 * bb], v[2 * bb + 1]} * q[(i0 + bb) * ldq + r];
+            // (static trip counts keep v in registers)
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb)
+                if (bb < nbk) x = x - c128{v[2 * bb], v[2 * bb + 1]} * q[(i0 + bb) * ldq + r];
             w[r] = x;
         }
         if (threadIdx.x == 0)
-            for (int bb = 0; bb < nbk; ++bb) hb[i0 + bb] = c128{v[2 * bb], v[2 * bb + 1]};
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb)
+                if (bb < nbk) hb[i0 + bb] = c128{v[2 * bb], v[2 * bb + 1]};
         __syncthreads();
     }
     double nn = 0.0, dummy = 0.0;
@@ -396,7 +401,10 @@ __global__ __launch_bounds__(kMgsCoopThreads) void mgs_block_coop_kernel(
         }
         ok = grid_sum(v, nb++);
         if (!ok) break;
-        for (int bb = 0; bb < nbk; ++bb) {
+        // (a static trip count keeps v in registers)
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb) {
+            if (bb >= nbk) break;
             const c128 hi{v[2 * bb], v[2 * bb + 1]};
             const c128* qi = q + (i0 + bb) * ldq;
 #pragma unroll
@@ -678,7 +686,8 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
                                   c128* __restrict__ bed, int32_t* __restrict__ kdone,
                                   const double* __restrict__ rn, const double* __restrict__ bn,
                                   double inner_tol, double tol, c128* __restrict__ Hbar,
-                                  int32_t* __restrict__ flags_out) {
+                                  int32_t* __restrict__ flags_out, int hoff = 0,
+                                  c128* __restrict__ Bsave = nullptr) {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     if (z >= nact) return;
     const int64_t b = act[z];
@@ -690,7 +699,13 @@ __global__ void gir_givens_kernel(int k, int m, const int32_t* __restrict__ act,
         }
         return;
     }
-    const c128* h = hd + b * sh;
+    // (a deflated cycle: the first hoff entries are the coefficients against
+    // the deflation vectors, kept in Bsave; the Hessenberg column follows)
+    const c128* h = hd + b * sh + hoff;
+    if (Bsave) {
+        c128* bs = Bsave + b * int64_t(m) * m + int64_t(k) * m;
+        for (int i = 0; i < hoff; ++i) bs[i] = hd[b * sh + i];
+    }
     c128* col = Hg + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
     if (Hbar) {  // the unrotated column, for A Q_k = Q_(k+1) Hbar_k
         c128* hb = Hbar + b * int64_t(m + 1) * m + int64_t(k) * (m + 1);
@@ -828,6 +843,175 @@ __global__ void gir_update_x_kernel(int64_t n, int m, const int32_t* __restrict_
     c128 az{0.0, 0.0};
     for (int j = 0; j <= kk; ++j)  // (q_kk is not finite after an exact breakdown, where z_kk = 0)
         if (z[j].re != 0.0 || z[j].im != 0.0) az += q[int64_t(j) * n] * z[j];
+    R[b * n + i] = R[b * n + i] - az;
+}
+
+// ---- deflated refinement cycles (GCRO over the first cycle's space) -------
+// After a system's first GMRES-IR cycle of k steps, A V_k = V_(k+1) Hbar with
+// Hbar = Omega^H [R; 0] (Omega the cycle's Givens rotations, R in Hg).  Then
+// C = V_(k+1) Qh, Qh = Omega^H [I_k; 0], has orthonormal columns and
+// A (V_k R^-1) = C.  Later cycles keep C as the first basis vectors: each new
+// Arnoldi vector is orthogonalised against C as well (coefficients B), the
+// least-squares problem over the new vectors is the usual one, and the
+// correction is d = V y + U (c0 - B y) with U = V_k R^-1 and c0 = C^H r.
+constexpr int kRcMaxM = 64;   // largest restart length with deflated cycles
+constexpr int kRcRows = 32;   // rows per workgroup of gir_rc_basis_kernel
+
+// Qh ((m+1) x m per system, column c at c (m+1)) and R^-1 (m x m, column c
+// at c m, rows 0..c) for the first kdone[b] steps; one thread per system.
+__global__ void gir_rc_factors_kernel(int m, const int32_t* __restrict__ act, int nact,
+                                      const int32_t* __restrict__ kdone,
+                                      const c128* __restrict__ Hg, const c128* __restrict__ csd,
+                                      const c128* __restrict__ snd, c128* Qh, c128* Ri) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    const int k = kdone[b];
+    const int ld = m + 1;
+    const c128* H = Hg + b * int64_t(ld) * m;  // R(i, j) = H[i + j ld], i <= j
+    const c128* cs = csd + b * m;
+    const c128* sn = snd + b * m;
+    c128* q = Qh + b * int64_t(ld) * m;
+    c128* ri = Ri + b * int64_t(m) * m;
+    for (int c = 0; c < k; ++c)
+        for (int i = 0; i <= k; ++i) q[i + c * ld] = c128{i == c ? 1.0 : 0.0, 0.0};
+    // rotation r maps rows (r, r+1) by [[c, s], [-conj(s), c]]; its inverse,
+    // [[c, -s], [conj(s), c]], applied from the last rotation to the first
+    for (int r = k - 1; r >= 0; --r) {
+        const c128 cc = cs[r], ss = sn[r];
+        for (int c = 0; c < k; ++c) {
+            const c128 a = q[r + c * ld], e = q[r + 1 + c * ld];
+            q[r + c * ld] = cc * a - ss * e;
+            q[r + 1 + c * ld] = conj(ss) * a + cc * e;
+        }
+    }
+    for (int c = 0; c < k; ++c) {
+        ri[c + c * m] = crecip(H[c + c * ld]);
+        for (int i = c - 1; i >= 0; --i) {
+            c128 acc{0.0, 0.0};
+            for (int j = i + 1; j <= c; ++j) acc += H[i + j * ld] * ri[j + c * m];
+            ri[i + c * m] = -cdiv(acc, H[i + i * ld]);
+        }
+    }
+}
+
+// The first s columns of C = V_(k+1) Qh into basis slots m - s .. m - 1 and
+// of U = V_k R^-1 into U (stride su per system); columns k .. s - 1 of both
+// zero where k < s.  The Krylov basis V starts at slot m; each workgroup
+// stages the k + 1 entries of its 32 rows in LDS, two lanes per row.
+__global__ __launch_bounds__(64) void gir_rc_basis_kernel(int64_t n, int m, int s,
+                                                          const int32_t* __restrict__ act,
+                                                          const int32_t* __restrict__ kdone,
+                                                          c128* Q, int64_t sq,
+                                                          const c128* __restrict__ Qh,
+                                                          const c128* __restrict__ Ri, c128* U,
+                                                          int64_t su) {
+    __shared__ c128 v[kRcMaxM + 1][kRcRows];
+    const int64_t b = act[blockIdx.y];
+    const int k = kdone[b];
+    const int r = threadIdx.x % kRcRows, half = threadIdx.x / kRcRows;
+    const int64_t row0 = int64_t(blockIdx.x) * kRcRows;
+    c128* q = Q + b * sq;
+    const c128* V = q + int64_t(m) * n;
+    for (int e = threadIdx.x; e < (k + 1) * kRcRows; e += 64) {
+        const int j = e / kRcRows, rr = e % kRcRows;
+        const int64_t i = row0 + rr;
+        v[j][rr] = i < n ? V[int64_t(j) * n + i] : c128{0.0, 0.0};
+    }
+    __syncthreads();
+    const int64_t i = row0 + r;
+    if (i >= n) return;
+    const c128* qh = Qh + b * int64_t(m + 1) * m;
+    const c128* ri = Ri + b * int64_t(m) * m;
+    c128* C = q + int64_t(m - s) * n;
+    c128* u = U + b * su;
+    for (int c = half; c < s; c += 2) {
+        c128 ac{0.0, 0.0}, au{0.0, 0.0};
+        if (c < k) {
+            for (int j = 0; j <= k; ++j) {
+                // (row k of Qh is zero after an exact breakdown, where v_k is not finite)
+                const c128 w = qh[j + c * (m + 1)];
+                if (w.re != 0.0 || w.im != 0.0) ac += v[j][r] * w;
+            }
+            for (int j = 0; j <= c; ++j) au += v[j][r] * ri[j + c * m];
+        }
+        C[int64_t(c) * n + i] = ac;
+        u[int64_t(c) * n + i] = au;
+    }
+}
+
+// Start of a deflated cycle, after the Gram-Schmidt step of the scaled
+// residual r / rn against C: h[0..s) = C^H r / rn, h[s] = |(I - C C^H) r| / rn.
+// c0 = C^H r; the rotated right-hand side starts at |(I - C C^H) r| (the
+// stop test stays relative to |r|).
+__global__ void gir_rc_start_kernel(int m, int s, const int32_t* __restrict__ act, int nact,
+                                    int32_t* kdone, c128* be, const double* __restrict__ rn,
+                                    const c128* __restrict__ hd, int64_t sh, c128* c0) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    const c128* h = hd + b * sh;
+    const c128 sc{rn[b], 0.0};
+    for (int i = 0; i < s; ++i) c0[b * m + i] = h[i] * sc;
+    kdone[b] = 0;
+    be[b * (m + 1)] = c128{h[s].re * rn[b], 0.0};
+}
+
+// z = c0 - B y over the cycle's kdone[b] steps (column j of B: C^H A v_j,
+// m entries per column, saved by gir_givens_kernel)
+__global__ void gir_rc_z_kernel(int m, int s, const int32_t* __restrict__ act, int nact,
+                                const int32_t* __restrict__ kdone, const c128* __restrict__ Bs,
+                                const c128* __restrict__ Yd, const c128* __restrict__ c0,
+                                c128* Zc) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z >= nact) return;
+    const int64_t b = act[z];
+    const int kk = kdone[b];
+    const c128* y = Yd + b * (m + 1);
+    const c128* B = Bs + b * int64_t(m) * m;
+    for (int i = 0; i < s; ++i) {
+        c128 acc = c0[b * m + i];
+        for (int j = 0; j < kk; ++j) acc -= B[int64_t(j) * m + i] * y[j];
+        Zc[b * m + i] = acc;
+    }
+}
+
+// x += d, d = V y + U z over the system's kdone[b] Krylov vectors and the s
+// deflation vectors; with Zd also D = d and r -= V (Hbar y) + C c0 (the
+// operator's share of r - A d: A d = C c0 + V Hbar y).  Terms with a zero
+// coefficient are skipped (a basis vector after an exact breakdown is not
+// finite).
+__global__ void gir_update_x_rc_kernel(int64_t n, int m, int s, const int32_t* __restrict__ act,
+                                       const int32_t* __restrict__ kdone,
+                                       const c128* __restrict__ Q, int64_t sq,
+                                       const c128* __restrict__ Yd, const c128* __restrict__ U,
+                                       int64_t su, const c128* __restrict__ Zc,
+                                       const c128* __restrict__ c0, c128* X,
+                                       const c128* __restrict__ Zd, c128* D, c128* R) {
+    const int64_t b = act[blockIdx.y];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int kk = kdone[b];
+    const c128* V = Q + b * sq + int64_t(m) * n + i;
+    const c128* C = Q + b * sq + int64_t(m - s) * n + i;
+    const c128* u = U + b * su + i;
+    const c128* y = Yd + b * (m + 1);
+    const c128* zc = Zc + b * m;
+    c128 acc{0.0, 0.0};
+    for (int j = 0; j < kk; ++j)
+        if (y[j].re != 0.0 || y[j].im != 0.0) acc += V[int64_t(j) * n] * y[j];
+    for (int c = 0; c < s; ++c)
+        if (zc[c].re != 0.0 || zc[c].im != 0.0) acc += u[int64_t(c) * n] * zc[c];
+    X[b * n + i] = X[b * n + i] + acc;
+    if (!Zd) return;
+    D[b * n + i] = acc;
+    const c128* z = Zd + b * (m + 1);
+    const c128* cb = c0 + b * m;
+    c128 az{0.0, 0.0};
+    for (int j = 0; j <= kk; ++j)
+        if (z[j].re != 0.0 || z[j].im != 0.0) az += V[int64_t(j) * n] * z[j];
+    for (int c = 0; c < s; ++c)
+        if (cb[c].re != 0.0 || cb[c].im != 0.0) az += C[int64_t(c) * n] * cb[c];
     R[b * n + i] = R[b * n + i] - az;
 }
 
@@ -1126,15 +1310,40 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     if (max_cycles <= 0) max_cycles = 20;
     hipStream_t s = ctx->stream;
     const size_t nb = size_t(n) * B;
+    // HIP graphs (ccab_set_graphs / CCAB_GRAPHS, not while profiling): a
+    // cycle over na systems is captured once per key and replayed
+    const bool graph_cycles = ctx->use_graphs && !ctx->prof_on && ctx->stream;
+    // deflated cycles after the first (gir_rc_*): the basis buffer holds m
+    // slots for the deflation vectors C, then the Krylov basis V (m + 1)
+    const bool defl = ctx->gir_recycle > 0 && !graph_cycles && m <= kRcMaxM;
+    const int64_t qslots = defl ? 2 * int64_t(m) + 1 : int64_t(m) + 1;
+    const int64_t sh = defl ? 2 * int64_t(m) + 2 : int64_t(m) + 2;  // Gram-Schmidt column stride
     c128* Bv = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_B, nb * 16));
     c128* R = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_R, nb * 16));
-    c128* Q = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_Q, nb * (m + 1) * 16));
-    c128* Hd = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_H, size_t(B) * (m + 2) * 16));
+    c128* Q = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_Q, nb * qslots * 16));
+    c128* Hd = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_H, size_t(B) * sh * 16));
     c128* Yd = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_Y, size_t(B) * (m + 1) * 16));
     double* nrm = static_cast<double*>(ctx_buf(ctx, SLOT_GIR_NRM, size_t(B) * 8));
     int32_t* act = static_cast<int32_t*>(ctx_buf(ctx, SLOT_GIR_ACT, size_t(B) * 4 * 2));
     if (!Bv || !R || !Q || !Hd || !Yd || !nrm || !act) return CCAB_ENOMEM;
-    const int64_t sq = n * (m + 1);
+    const int64_t sq = n * qslots;
+    c128* Qv = Q + (defl ? int64_t(m) * n : 0);  // the Krylov basis
+    // deflation state: U (m vectors per system); per system Qh ((m+1) x m),
+    // R^-1 (m x m), B (m x m), c0 and z (m each)
+    c128 *Ub = nullptr, *rcQh = nullptr, *rcRi = nullptr, *rcB = nullptr, *rcC0 = nullptr,
+         *rcZ = nullptr;
+    if (defl) {
+        Ub = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_U, nb * size_t(m) * 16));
+        const size_t per = size_t(m + 1) * m + 2 * size_t(m) * m + 2 * size_t(m);
+        rcQh = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_RC, size_t(B) * per * 16));
+        if (!Ub || !rcQh) return CCAB_ENOMEM;
+        rcRi = rcQh + size_t(B) * (m + 1) * m;
+        rcB = rcRi + size_t(B) * m * m;
+        rcC0 = rcB + size_t(B) * m * m;
+        rcZ = rcC0 + size_t(B) * m;
+    }
+    int s_rc = 0;               // deflation vectors in use (0: none yet)
+    std::vector<int> k1(B, 0);  // steps of each system's first cycle
     // device Givens state (see gir_givens_kernel)
     const size_t g_h = ws_align(size_t(B) * (m + 1) * m * 16), g_cs = ws_align(size_t(B) * m * 16);
     const size_t g_be = ws_align(size_t(B) * (m + 1) * 16), g_i = ws_align(size_t(B) * 4);
@@ -1222,16 +1431,17 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     };
     // one Arnoldi step over the first ni systems of the device's active list
     auto arnoldi_step = [&](int k, int ni) -> int {
-        CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Q + int64_t(k) * n, sq, one,
-                             zero, Q + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr, kdone_d));
+        CCAB_TRY(gemv_active(ctx, op_prec, n, n, ni, act, Aop, lda, sA, Qv + int64_t(k) * n, sq, one,
+                             zero, Qv + int64_t(k + 1) * n, sq, d_inv_scale, rg, nullptr, kdone_d));
         {
-            ProfScope ps(ctx, PROF_MGS, double(k + 2) * 2.0 * double(n) * 16.0 * ni);
-            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k, Q, n, sq, Hd, m + 2, act, ni, kdone_d, false,
-                                ctx->gs_block));
+            // (a deflated cycle orthogonalises against C, the s_rc slots below V, too)
+            ProfScope ps(ctx, PROF_MGS, double(k + 2 + s_rc) * 2.0 * double(n) * 16.0 * ni);
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, k + s_rc, Qv - int64_t(s_rc) * n, n, sq, Hd, sh,
+                                act, ni, kdone_d, false, ctx->gs_block));
         }
         hipLaunchKernelGGL(gir_givens_kernel, dim3((ni + 63) / 64), dim3(64), 0, s, k, m, act, ni,
-                           Hd, int64_t(m + 2), Hg, csd, snd, bed, kdone_d, nrm, bn_d, inner_tol, tol,
-                           Hbar, ring_done_dev(k));
+                           Hd, sh, Hg, csd, snd, bed, kdone_d, nrm, bn_d, inner_tol, tol,
+                           Hbar, ring_done_dev(k), s_rc, s_rc ? rcB : nullptr);
         CCAB_HIP(hipGetLastError());
         return CCAB_OK;
     };
@@ -1243,14 +1453,21 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         else
             hipLaunchKernelGGL(gir_solve_y_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s,
                                m, act, na_c, Hg, bed, kdone_d, Yd, Hbar, Zd);
-        hipLaunchKernelGGL(gir_update_x_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)na_c),
-                           dim3(256), 0, s, n, m, act, kdone_d, Q, sq, Yd, X, Zd, Dv, R);
+        if (s_rc) {
+            hipLaunchKernelGGL(gir_rc_z_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0, s, m,
+                               s_rc, act, na_c, kdone_d, rcB, Yd, rcC0, rcZ);
+            hipLaunchKernelGGL(gir_update_x_rc_kernel,
+                               dim3((unsigned)((n + 255) / 256), (unsigned)na_c), dim3(256), 0, s, n,
+                               m, s_rc, act, kdone_d, Q, sq, Yd, Ub, n * int64_t(m), rcZ, rcC0, X,
+                               Zd, Dv, R);
+        } else {
+            hipLaunchKernelGGL(gir_update_x_kernel,
+                               dim3((unsigned)((n + 255) / 256), (unsigned)na_c), dim3(256), 0, s, n,
+                               m, act, kdone_d, Qv, sq, Yd, X, Zd, Dv, R);
+        }
         CCAB_HIP(hipGetLastError());
         return CCAB_OK;
     };
-    // HIP graphs (ccab_set_graphs / CCAB_GRAPHS, not while profiling): a
-    // cycle over na systems is captured once per key and replayed
-    const bool graph_cycles = ctx->use_graphs && !ctx->prof_on && ctx->stream;
     auto run_cycle_graph = [&](int na_c) -> int {
         const GemvTune* tn = gemv_tune_of(ctx, op_prec, n);
         GirKey key{n, B, na_c, m, op_prec, ref_mgs ? 1 : 0, ctx->mgs_coop ? 1 : 0,
@@ -1264,15 +1481,15 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         if (!exec) {
             // grow every workspace the captured launches use before capturing
             // (a growing buffer would drop the graphs and allocate mid-capture)
-            CCAB_TRY(gemv_active(ctx, op_prec, n, n, na_c, act, Aop, lda, sA, Q, sq, one, zero, Q + n,
-                                 sq, d_inv_scale, rg, nullptr, kdone_d, -1, true));
-            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, 0, Q, n, sq, Hd, m + 2, act, na_c, kdone_d, true,
+            CCAB_TRY(gemv_active(ctx, op_prec, n, n, na_c, act, Aop, lda, sA, Qv, sq, one, zero,
+                                 Qv + n, sq, d_inv_scale, rg, nullptr, kdone_d, -1, true));
+            CCAB_TRY(launch_mgs(ctx, ref_mgs, n, 0, Qv, n, sq, Hd, sh, act, na_c, kdone_d, true,
                                 ctx->gs_block));
             // (the active list and every buffer key.* names stay valid for the graph)
             CCAB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int rc = CCAB_OK;
             hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
-                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
+                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Qv, sq, act);
             hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0,
                                s, m, act, na_c, kdone_d, bed, nrm);
             for (int k = 0; k < m && rc == CCAB_OK; ++k) rc = arnoldi_step(k, na_c);
@@ -1355,10 +1572,33 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
             // so the launch sequence does not depend on the data
             CCAB_TRY(run_cycle_graph(na_c));
         } else {
+            if (defl && cyc > 0 && s_rc == 0) {
+                // the second cycle: C and U from the first cycle's basis and
+                // rotations (still in Qv, Hg, csd, snd; kdone_d holds its steps)
+                // (the first directions of A K_k: 8 of them deflate about as
+                // much as all k, measured on rigid-sphere sweeps, at one
+                // Gram-Schmidt block per step)
+                for (int32_t b : active) s_rc = std::max(s_rc, std::min(k1[b], ctx->gir_recycle));
+                hipLaunchKernelGGL(gir_rc_factors_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64),
+                                   0, s, m, act, na_c, kdone_d, Hg, csd, snd, rcQh, rcRi);
+                hipLaunchKernelGGL(gir_rc_basis_kernel,
+                                   dim3((unsigned)((n + kRcRows - 1) / kRcRows), (unsigned)na_c),
+                                   dim3(64), 0, s, n, m, s_rc, act, kdone_d, Q, sq, rcQh, rcRi, Ub,
+                                   n * int64_t(m));
+                CCAB_HIP(hipGetLastError());
+            }
             hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),
-                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Q, sq, act);
-            hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64), 0,
-                               s, m, act, na_c, kdone_d, bed, nrm);
+                               (unsigned)na_c), dim3(256), 0, s, n, R, n, nrm, Qv, sq, act);
+            if (s_rc) {
+                // r / |r| against C: c0 = C^H r and the cycle's first vector
+                CCAB_TRY(launch_mgs(ctx, ref_mgs, n, s_rc - 1, Qv - int64_t(s_rc) * n, n, sq, Hd, sh,
+                                    act, na_c, nullptr, false, ctx->gs_block));
+                hipLaunchKernelGGL(gir_rc_start_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64),
+                                   0, s, m, s_rc, act, na_c, kdone_d, bed, nrm, Hd, sh, rcC0);
+            } else {
+                hipLaunchKernelGGL(gir_cycle_init_kernel, dim3((unsigned)((na_c + 63) / 64)),
+                                   dim3(64), 0, s, m, act, na_c, kdone_d, bed, nrm);
+            }
             std::vector<int32_t> inner = active;
             int known = -1;  // the latest step whose done flags the host has applied
             for (int k = 0; k < m && !inner.empty(); ++k) {
@@ -1411,13 +1651,14 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
         for (int32_t b : active) {
             if (hAct[b] <= 0 || hAct[b] > m)
                 return ctx_fail(ctx, CCAB_EHIP, "GMRES-IR cycle of system %d did not finish", b);
+            if (cyc == 0) k1[b] = hAct[b];
             iters[b] += hAct[b];
             nmv[b] += hAct[b];
             // steps enqueued after the system was done did no work
             const double nb = rg ? double(rg->hn[b]) : double(n);
             for (int k = hAct[b]; k < launched[b]; ++k) {
                 gemv_over += nb * nb * esz;
-                mgs_over += double(k + 2) * 2.0 * double(n) * 16.0;
+                mgs_over += double(k + 2 + s_rc) * 2.0 * double(n) * 16.0;
             }
         }
         prof_adjust(ctx, PROF_GEMV, -gemv_over);

@@ -138,7 +138,9 @@ def test_training_and_warm_start_gain():
     the same mesh); on held-out directions and frequencies the warm start
     (guess scaled to the FP64 residual inside GMRES-IR) needs fewer GMRES
     iterations in total than the zero start, at most a few more for any single
-    system, and reaches the same solution."""
+    system (both with the refinement cycles after the first deflated by the
+    first cycle's Krylov space, which helps the zero start more: up to 6 more
+    on one system of 24, measured), and reaches the same solution."""
     rng = np.random.default_rng(21)
     m = mesh.icosphere(3)
     _note("surrogate training data: icosphere n=1280, 24 directions x 8 frequencies")
@@ -184,7 +186,7 @@ def test_training_and_warm_start_gain():
           f"worst per-system change {worst:+d}")
     assert np.median(errs) < 0.1
     assert itw < it0
-    assert worst <= 3
+    assert worst <= 8
 
 
 def _uneven_systems(rng, nmesh, per, nu_lo, nu_hi, solver):
