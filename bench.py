@@ -65,7 +65,7 @@ def parse():
                     help="FP64 relative residual at which a frequency counts as solved (the "
                          "metric asks for <= 1e-6 relative error vs the FP64 LU; 1e-7 leaves "
                          "a 10x margin, checked in rel_err_vs_fp64_lu)")
-    ap.add_argument("--inner-tol", type=float, default=1e-3,
+    ap.add_argument("--inner-tol", type=float, default=5e-3,
                     help="relative tolerance of each inner GMRES cycle (FP16 operator)")
     ap.add_argument("--restart", type=int, default=30, help="inner GMRES restart length")
     ap.add_argument("--residual-matrix", choices=["c128", "split"], default="split",
@@ -171,7 +171,7 @@ def bodies_block(args, ctx):
     from cc_abstemious_amd.preprocess import IncidentWave, PreData, Solver
     rng = np.random.default_rng(2024)
     solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=30,
-                    ir_tolerance=1e-3, ir_max_steps=20, ref_mgs=False,
+                    ir_tolerance=args.inner_tol, ir_max_steps=20, ref_mgs=False,
                     stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C32,
                     residual_matrix=capi.C32 if args.residual_matrix == "split" else capi.C128)
     bodies = []
