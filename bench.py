@@ -67,6 +67,8 @@ def parse():
                          "a 10x margin, checked in rel_err_vs_fp64_lu)")
     ap.add_argument("--inner-tol", type=float, default=5e-3,
                     help="relative tolerance of each inner GMRES cycle (FP16 operator)")
+    ap.add_argument("--warm-inner-tol", type=float, default=1e-3,
+                    help="inner GMRES tolerance of the warm-started sweep and bodies")
     ap.add_argument("--restart", type=int, default=30, help="inner GMRES restart length")
     ap.add_argument("--residual-matrix", choices=["c128", "split"], default="split",
                     help="matrix of the FP64 residuals of the headline and bodies lines: the "
@@ -254,13 +256,18 @@ def bodies_block(args, ctx):
         sols_n = len(sols)
         del data, sols, train
         train_s = time.perf_counter() - t_tr
-        solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
+        wsolver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=1e-7, restart=30,
+                         ir_tolerance=args.warm_inner_tol, ir_max_steps=20, ref_mgs=False,
+                         stop_rule=capi.STOP_RELATIVE_RESIDUAL, matrix_prec=capi.C32,
+                         residual_matrix=solver.residual_matrix)
+        solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), solver=wsolver, ctx=ctx)
         torch.cuda.synchronize()
         ctx.profile(True)
         ctx.profile_read(reset=True)
         t1 = time.perf_counter()
         for _ in range(reps):
-            phw, _, stw = solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies), ctx=ctx)
+            phw, _, stw = solve.solve_bodies(bodies, x0=sur.predict_bodies(bodies),
+                                             solver=wsolver, ctx=ctx)
         torch.cuda.synchronize()
         dtw = (time.perf_counter() - t1) / reps
         prof_w = ctx.profile_read(reset=True)
@@ -547,6 +554,10 @@ def main():
         losses = sur.fit(sur.dataset(items), steps=args.train_steps, batch=16384, seed=1)
         train_s = time.perf_counter() - t_tr
         del items
+        warm_inner = args.warm_inner_tol
+        pd.solver = Solver(kind=capi.SOLVER_ITERATIVE_MIXED, tolerance=args.tol,
+                           restart=args.restart, ir_tolerance=warm_inner, ir_max_steps=30,
+                           ref_mgs=False, matrix_prec=capi.C32, residual_matrix=res_mat)
         solve.sweep(pd, x0=sur.predict(pd), ctx=ctx)
         torch.cuda.synchronize()
         ctx.profile(True)
@@ -559,7 +570,7 @@ def main():
         prof_w = ctx.profile_read(reset=True)
         ctx.profile(False)
         sur_info = {"solves_per_s": warm_rate,
-                    "gmres_iters": iter_stats(st_w, args.restart, args.inner_tol,
+                    "gmres_iters": iter_stats(st_w, args.restart, warm_inner,
                                               "complex FP16, scaled"),
                     "inference_ms_per_sweep": prof_w.get("surrogate", {}).get("ms", 0.0) / args.steps,
                     "train_systems": 8 * args.train_dirs, "train_steps": args.train_steps,
