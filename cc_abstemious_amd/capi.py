@@ -128,6 +128,7 @@ SIGNATURES = {
     "ccab_autotune": (_I, [_P, _I, _I64, _I64, _P, _P, _P]),
     "ccab_set_graphs": (_I, [_P, _I]),
     "ccab_set_warm_ax": (_I, [_P, _I]),
+    "ccab_set_gir_deflation": (_I, [_P, _I]),
     "ccab_stream_wait": (_I, [_P, _P]),
     "ccab_collocation_points": (_I, [_P, _P, _P, _I64, _P]),
     "ccab_collocation_points_ex": (_I, [_P, _P, _P, _I64, _I, _P]),
@@ -335,6 +336,12 @@ class Context:
         """Form a warm start's A x0 inside the assembly (default) or as a
         product over the stored matrices."""
         self.call("ccab_set_warm_ax", int(bool(on)))
+
+    def set_gir_deflation(self, nvec: int):
+        """Refinement cycles of the batched GMRES-IR after each system's first
+        deflated by nvec directions of its first cycle's Krylov space (default
+        8); 0: every cycle restarts from scratch."""
+        self.call("ccab_set_gir_deflation", int(nvec))
 
     def set_tuning(self, prec: int, variant: int, outer: int):
         self.call("ccab_set_tuning", int(prec), int(variant), int(outer))

@@ -487,6 +487,13 @@ int ccab_set_warm_ax(ccab_ctx* ctx, int on) {
     return CCAB_OK;
 }
 
+int ccab_set_gir_deflation(ccab_ctx* ctx, int nvec) {
+    if (!ctx) return CCAB_EINVAL;
+    CCAB_CHECK(nvec >= 0 && nvec <= 64, "deflation vectors must be 0..64");
+    ctx->gir_recycle = nvec;
+    return CCAB_OK;
+}
+
 int ccab_destroy(ccab_ctx* ctx) {
     if (!ctx) return CCAB_OK;
     (void)hipSetDevice(ctx->device);

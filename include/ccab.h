@@ -195,6 +195,14 @@ int ccab_set_graphs(ccab_ctx* ctx, int on);
  * reads no stored matrix; with 0 it is a product over the stored matrices
  * after the assembly.  The two agree to rounding. */
 int ccab_set_warm_ax(ccab_ctx* ctx, int on);
+/* Batched GMRES-IR (no reference counterpart: the reference restarts GMRES
+ * from scratch, gmres.rs:176-190): with nvec > 0 (default 8; environment
+ * CCAB_GIR_RECYCLE) every refinement cycle after a system's first is
+ * deflated by the first nvec directions of A times the first cycle's
+ * Krylov space (GCRO: C = A U from that cycle's Arnoldi relation, no extra
+ * matrix products); 0 restarts every cycle from scratch.  nvec <= 64;
+ * restart lengths above 64 and HIP-graph cycles are never deflated. */
+int ccab_set_gir_deflation(ccab_ctx* ctx, int nvec);
 /* The autotuner: times the batched LU of `batch` synthetic n x n systems in
  * precision prec for every GEMM variant (0..8) and outer width (64, 128),
  * keeps the fastest in the context (as ccab_set_tuning) and returns it;

@@ -31,11 +31,15 @@ def _rel(a, b):
 
 
 def _ctx(vecs: str):
-    os.environ["CCAB_GIR_RECYCLE"] = vecs
-    try:
-        return capi.Context()
-    finally:
-        del os.environ["CCAB_GIR_RECYCLE"]
+    if vecs == "64":     # through the environment at context creation
+        os.environ["CCAB_GIR_RECYCLE"] = vecs
+        try:
+            return capi.Context()
+        finally:
+            del os.environ["CCAB_GIR_RECYCLE"]
+    ctx = capi.Context()
+    ctx.set_gir_deflation(int(vecs))
+    return ctx
 
 
 def _ir(tol, residual):
@@ -108,3 +112,16 @@ def test_bodies_deflated_cycles():
             assert _rel(p, ref) < 1e-8, (vecs, _rel(p, ref))
     _note("Arnoldi steps: deflated %d, restarted %d" % (
         sum(s["iterations"] for s in out["8"][2]), sum(s["iterations"] for s in out["0"][2])))
+
+
+def test_deflation_setter_bounds():
+    ctx = capi.Context()
+    try:
+        with pytest.raises(capi.CcabError, match="deflation vectors"):
+            ctx.set_gir_deflation(65)
+        with pytest.raises(capi.CcabError, match="deflation vectors"):
+            ctx.set_gir_deflation(-1)
+        ctx.set_gir_deflation(0)
+        ctx.set_gir_deflation(64)
+    finally:
+        ctx.close()
