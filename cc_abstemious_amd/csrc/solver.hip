@@ -858,39 +858,39 @@ constexpr int kRcMaxM = 64;   // largest restart length with deflated cycles
 constexpr int kRcRows = 32;   // rows per workgroup of gir_rc_basis_kernel
 
 // Qh ((m+1) x m per system, column c at c (m+1)) and R^-1 (m x m, column c
-// at c m, rows 0..c) for the first kdone[b] steps; one thread per system.
-__global__ void gir_rc_factors_kernel(int m, const int32_t* __restrict__ act, int nact,
-                                      const int32_t* __restrict__ kdone,
-                                      const c128* __restrict__ Hg, const c128* __restrict__ csd,
-                                      const c128* __restrict__ snd, c128* Qh, c128* Ri) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    if (z >= nact) return;
-    const int64_t b = act[z];
+// at c m, rows 0..c) for the first kdone[b] steps; one wavefront per
+// system, a lane per column.  Column c of Qh is the unit vector e_c through
+// the inverse rotations, [[c, -s], [conj(s), c]] on rows (r, r + 1), from
+// the last to the first: rotation c makes rows c, c + 1 nonzero, each later
+// one (r = c - 1 .. 0) moves the value of row r + 1 up one row.
+__global__ __launch_bounds__(64) void gir_rc_factors_kernel(int m, const int32_t* __restrict__ act,
+                                                            const int32_t* __restrict__ kdone,
+                                                            const c128* __restrict__ Hg,
+                                                            const c128* __restrict__ csd,
+                                                            const c128* __restrict__ snd, c128* Qh,
+                                                            c128* Ri) {
+    const int64_t b = act[blockIdx.x];
     const int k = kdone[b];
     const int ld = m + 1;
     const c128* H = Hg + b * int64_t(ld) * m;  // R(i, j) = H[i + j ld], i <= j
     const c128* cs = csd + b * m;
     const c128* sn = snd + b * m;
-    c128* q = Qh + b * int64_t(ld) * m;
-    c128* ri = Ri + b * int64_t(m) * m;
-    for (int c = 0; c < k; ++c)
-        for (int i = 0; i <= k; ++i) q[i + c * ld] = c128{i == c ? 1.0 : 0.0, 0.0};
-    // rotation r maps rows (r, r+1) by [[c, s], [-conj(s), c]]; its inverse,
-    // [[c, -s], [conj(s), c]], applied from the last rotation to the first
-    for (int r = k - 1; r >= 0; --r) {
-        const c128 cc = cs[r], ss = sn[r];
-        for (int c = 0; c < k; ++c) {
-            const c128 a = q[r + c * ld], e = q[r + 1 + c * ld];
-            q[r + c * ld] = cc * a - ss * e;
-            q[r + 1 + c * ld] = conj(ss) * a + cc * e;
+    for (int c = threadIdx.x; c < k; c += 64) {
+        c128* q = Qh + b * int64_t(ld) * m + int64_t(c) * ld;
+        for (int i = c + 2; i <= k; ++i) q[i] = c128{0.0, 0.0};
+        if (c + 1 <= k) q[c + 1] = conj(sn[c]);
+        c128 carry = cs[c];
+        for (int r = c - 1; r >= 0; --r) {
+            q[r + 1] = cs[r] * carry;
+            carry = -(sn[r] * carry);
         }
-    }
-    for (int c = 0; c < k; ++c) {
-        ri[c + c * m] = crecip(H[c + c * ld]);
+        q[0] = carry;
+        c128* ri = Ri + b * int64_t(m) * m + int64_t(c) * m;
+        ri[c] = crecip(H[c + c * ld]);
         for (int i = c - 1; i >= 0; --i) {
             c128 acc{0.0, 0.0};
-            for (int j = i + 1; j <= c; ++j) acc += H[i + j * ld] * ri[j + c * m];
-            ri[i + c * m] = -cdiv(acc, H[i + i * ld]);
+            for (int j = i + 1; j <= c; ++j) acc += H[i + j * ld] * ri[j];
+            ri[i] = -cdiv(acc, H[i + i * ld]);
         }
     }
 }
@@ -1579,8 +1579,8 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                 // much as all k, measured on rigid-sphere sweeps, at one
                 // Gram-Schmidt block per step)
                 for (int32_t b : active) s_rc = std::max(s_rc, std::min(k1[b], ctx->gir_recycle));
-                hipLaunchKernelGGL(gir_rc_factors_kernel, dim3((unsigned)((na_c + 63) / 64)), dim3(64),
-                                   0, s, m, act, na_c, kdone_d, Hg, csd, snd, rcQh, rcRi);
+                hipLaunchKernelGGL(gir_rc_factors_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m,
+                                   act, kdone_d, Hg, csd, snd, rcQh, rcRi);
                 hipLaunchKernelGGL(gir_rc_basis_kernel,
                                    dim3((unsigned)((n + kRcRows - 1) / kRcRows), (unsigned)na_c),
                                    dim3(64), 0, s, n, m, s_rc, act, kdone_d, Q, sq, rcQh, rcRi, Ub,
