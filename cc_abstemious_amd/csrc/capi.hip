@@ -395,6 +395,7 @@ int ccab_create(int device, ccab_ctx** out) {
     if (const char* hn = getenv("CCAB_HMV_NT")) ctx->hmv_nt = atoi(hn) != 0;
     if (const char* wa = getenv("CCAB_WARM_AX")) ctx->warm_ax = atoi(wa) != 0;
     if (const char* gc = getenv("CCAB_GIR_RECYCLE")) ctx->gir_recycle = std::max(0, std::min(64, atoi(gc)));
+    if (const char* gz = getenv("CCAB_GIR_RITZ")) ctx->gir_ritz = atoi(gz) != 0;
     if (const char* gb = getenv("CCAB_GS_BLOCK")) {
         const int v = atoi(gb);
         ctx->gs_block = v == 1 ? 1 : v == 4 ? 4 : 8;

@@ -233,6 +233,10 @@ struct ccab_ctx {
     // space (C = A U from its Arnoldi relation; CCAB_GIR_RECYCLE, 0: every
     // cycle restarts from scratch)
     int gir_recycle = 8;
+    // the deflation directions: harmonic Ritz vectors of the first cycle
+    // (restart lengths up to 32, up to 8 vectors); 0: the first directions
+    // of A K_k (CCAB_GIR_RITZ)
+    bool gir_ritz = true;
     // FP32 H-matrix storage: ranks per pass of V^T x over paired loads, 4 or
     // 8 (CCAB_HMV_VT_RANKS)
     int hmv_vt_ranks = 8;

@@ -886,6 +886,7 @@ __global__ __launch_bounds__(64) void gir_rc_factors_kernel(int m, const int32_t
         }
         q[0] = carry;
         c128* ri = Ri + b * int64_t(m) * m + int64_t(c) * m;
+        for (int i = c + 1; i < k; ++i) ri[i] = c128{0.0, 0.0};
         ri[c] = crecip(H[c + c * ld]);
         for (int i = c - 1; i >= 0; --i) {
             c128 acc{0.0, 0.0};
@@ -895,9 +896,174 @@ __global__ __launch_bounds__(64) void gir_rc_factors_kernel(int m, const int32_t
     }
 }
 
-// The first s columns of C = V_(k+1) Qh into basis slots m - s .. m - 1 and
-// of U = V_k R^-1 into U (stride su per system); columns k .. s - 1 of both
-// zero where k < s.  The Krylov basis V starts at slot m; each workgroup
+// Harmonic Ritz directions of a system's first cycle, for deflation by
+// C = A Y rather than by the first directions of A K_k.  With
+// Hbar = Qh R (gir_rc_factors_kernel), the harmonic Ritz pairs of the cycle,
+// Hbar^H Hbar g = theta H_k^H g, are the eigenpairs of W = R^-1 Qh_top^H with
+// eigenvalues 1 / theta (Qh_top: the first k rows of Qh).  The s = min(8, k)
+// of smallest |theta| span the dominant invariant subspace of W, found by
+// kRitzIters steps of subspace iteration (Z <- orth(W Z) from a fixed
+// pseudo-random start).  With P = Z and R P = Q2 R2 (thin QR), the
+// deflation space is C = V_(k+1) (Qh Q2) (orthonormal) and U = V_k (P R2^-1),
+// A U = C.  Writes Cq = Qh Q2 ((m+1) x m layout) and Uq = P R2^-1 (m x m
+// layout) for columns 0..s-1.  One wavefront per system, a lane per row;
+// k <= kRitzMaxK.
+constexpr int kRitzVecs = 8;
+constexpr int kRitzMaxK = 32;
+constexpr int kRitzIters = 20;
+
+__device__ __forceinline__ c128 wave_csum(c128 v) { return c128{wave_sum(v.re), wave_sum(v.im)}; }
+
+// Modified Gram-Schmidt of the s columns held one row per lane (rows >= k
+// hold zeros); r (may be null) receives the s x s triangular factor.
+// Returns false (the same on every lane: lane 0's norms decide) when a
+// column loses more than all but 1e-10 of its norm.
+__device__ __forceinline__ bool wave_mgs(c128 (&z)[kRitzVecs], int s, c128* r) {
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < kRitzVecs; ++c) {
+        if (c >= s) break;
+        const double n0 = __shfl(sqrt(wave_sum(norm_sqr(z[c]))), 0, 64);
+#pragma unroll
+        for (int c2 = 0; c2 < kRitzVecs; ++c2) {
+            if (c2 >= c) break;
+            const c128 d = wave_csum(conj(z[c2]) * z[c]);
+            z[c] = z[c] - d * z[c2];
+            if (r && threadIdx.x == 0) r[c2 + c * kRitzVecs] = d;
+        }
+        const double nn = __shfl(sqrt(wave_sum(norm_sqr(z[c]))), 0, 64);
+        if (!(nn > 1e-10 * n0) || !(nn < 1e300)) {
+            ok = false;
+            z[c] = c128{0.0, 0.0};
+            if (r && threadIdx.x == 0) r[c + c * kRitzVecs] = c128{1.0, 0.0};
+            continue;
+        }
+        z[c] = z[c] * crecip(c128{nn, 0.0});
+        if (r && threadIdx.x == 0) r[c + c * kRitzVecs] = c128{nn, 0.0};
+    }
+    return ok;
+}
+
+__global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
+                                                         const int32_t* __restrict__ act,
+                                                         const int32_t* __restrict__ kdone,
+                                                         const c128* __restrict__ Hg,
+                                                         const c128* __restrict__ Qh,
+                                                         const c128* __restrict__ Ri, c128* Cq,
+                                                         c128* Uq) {
+    __shared__ c128 sW[kRitzMaxK][kRitzMaxK];
+    __shared__ c128 sZ[kRitzMaxK][kRitzVecs];
+    __shared__ c128 sR2[kRitzVecs * kRitzVecs];
+    const int64_t b = act[blockIdx.x];
+    const int k = kdone[b];
+    const int s = min(min(nvec, kRitzVecs), k);
+    const int i = threadIdx.x;
+    const int ld = m + 1;
+    const c128* H = Hg + b * int64_t(ld) * m;      // R(i, j) = H[i + j ld]
+    const c128* q = Qh + b * int64_t(ld) * m;      // Qh(i, c) = q[i + c ld]
+    const c128* ri = Ri + b * int64_t(m) * m;      // R^-1(i, c) = ri[i + c m], i <= c
+    // W = R^-1 Qh_top^H, row i per lane
+    if (i < k)
+        for (int j = 0; j < k; ++j) {
+            c128 acc{0.0, 0.0};
+            for (int l = i; l < k; ++l) acc += ri[i + l * m] * conj(q[j + l * ld]);
+            sW[i][j] = acc;
+        }
+    c128 z[kRitzVecs];
+#pragma unroll
+    for (int c = 0; c < kRitzVecs; ++c) {
+        // a fixed pseudo-random start (integer hash of row and column)
+        uint32_t h = uint32_t(i) * 2654435761u ^ uint32_t(c + 1) * 40503u;
+        h ^= h >> 13;
+        h *= 0x5bd1e995u;
+        h ^= h >> 15;
+        const double a = double(h & 0xffff) / 32768.0 - 1.0, bb = double(h >> 16) / 32768.0 - 1.0;
+        z[c] = (i < k && c < s) ? c128{a, bb} : c128{0.0, 0.0};
+    }
+    bool ok = wave_mgs(z, s, nullptr);
+    for (int it = 0; it < kRitzIters && ok; ++it) {
+        if (i < k)
+#pragma unroll
+            for (int c = 0; c < kRitzVecs; ++c) sZ[i][c] = z[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c) z[c] = c128{0.0, 0.0};
+        if (i < k)
+            for (int j = 0; j < k; ++j) {
+                const c128 w = sW[i][j];
+#pragma unroll
+                for (int c = 0; c < kRitzVecs; ++c) z[c] += w * sZ[j][c];
+            }
+        __syncthreads();
+        ok = wave_mgs(z, s, nullptr);
+    }
+    // P = z; R P (rows i < k), then its thin QR: Q2 and R2
+    if (i < k)
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c) sZ[i][c] = z[c];
+    __syncthreads();
+    c128 p[kRitzVecs];
+#pragma unroll
+    for (int c = 0; c < kRitzVecs; ++c) {
+        p[c] = z[c];
+        z[c] = c128{0.0, 0.0};
+    }
+    if (i < k)
+        for (int l = i; l < k; ++l) {
+            const c128 w = H[i + l * ld];
+#pragma unroll
+            for (int c = 0; c < kRitzVecs; ++c) z[c] += w * sZ[l][c];
+        }
+    if (ok) ok = wave_mgs(z, s, sR2);   // z = Q2
+    __syncthreads();
+    c128* cq = Cq + b * int64_t(ld) * m;
+    c128* uq = Uq + b * int64_t(m) * m;
+    if (!ok) {
+        // (degenerate: the first s directions of A K_k, Qh and R^-1 with its
+        // zero lower triangle, as without harmonic Ritz vectors)
+        for (int c = 0; c < s; ++c) {
+            if (i <= k) cq[i + c * ld] = q[i + c * ld];
+            if (i < k) uq[i + c * m] = i <= c ? ri[i + c * m] : c128{0.0, 0.0};
+        }
+        return;
+    }
+    if (i < k)
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c) sZ[i][c] = z[c];
+    __syncthreads();
+    // Cq = Qh Q2 (rows 0..k), Uq = P R2^-1 (rows 0..k-1)
+    if (i <= k) {
+        c128 acc[kRitzVecs];
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c) acc[c] = c128{0.0, 0.0};
+        for (int l = 0; l < k; ++l) {
+            const c128 w = q[i + l * ld];
+#pragma unroll
+            for (int c = 0; c < kRitzVecs; ++c) acc[c] += w * sZ[l][c];
+        }
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c)
+            if (c < s) cq[i + c * ld] = acc[c];
+    }
+    if (i < k) {
+        c128 u[kRitzVecs];
+#pragma unroll
+        for (int c = 0; c < kRitzVecs; ++c) {
+            if (c >= s) break;
+            c128 v = p[c];
+#pragma unroll
+            for (int l = 0; l < kRitzVecs; ++l)
+                if (l < c) v = v - u[l] * sR2[l + c * kRitzVecs];
+            u[c] = cdiv(v, sR2[c + c * kRitzVecs]);
+            uq[i + c * m] = u[c];
+        }
+    }
+}
+
+// The first s columns of C = V_(k+1) Cq into basis slots m - s .. m - 1 and
+// of U = V_k Uq into U (stride su per system); columns k .. s - 1 of both
+// zero where k < s.  (Cq, Uq) = (Qh, R^-1) for the first directions of
+// A K_k, or the harmonic Ritz coefficients of gir_rc_ritz_kernel.  The Krylov basis V starts at slot m; each workgroup
 // stages the k + 1 entries of its 32 rows in LDS, two lanes per row.
 __global__ __launch_bounds__(64) void gir_rc_basis_kernel(int64_t n, int m, int s,
                                                           const int32_t* __restrict__ act,
@@ -933,7 +1099,7 @@ __global__ __launch_bounds__(64) void gir_rc_basis_kernel(int64_t n, int m, int 
                 const c128 w = qh[j + c * (m + 1)];
                 if (w.re != 0.0 || w.im != 0.0) ac += v[j][r] * w;
             }
-            for (int j = 0; j <= c; ++j) au += v[j][r] * ri[j + c * m];
+            for (int j = 0; j < k; ++j) au += v[j][r] * ri[j + c * m];
         }
         C[int64_t(c) * n + i] = ac;
         u[int64_t(c) * n + i] = au;
@@ -1331,17 +1497,21 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
     // deflation state: U (m vectors per system); per system Qh ((m+1) x m),
     // R^-1 (m x m), B (m x m), c0 and z (m each)
     c128 *Ub = nullptr, *rcQh = nullptr, *rcRi = nullptr, *rcB = nullptr, *rcC0 = nullptr,
-         *rcZ = nullptr;
+         *rcZ = nullptr, *rcCq = nullptr, *rcUq = nullptr;
     if (defl) {
         Ub = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_U, nb * size_t(m) * 16));
-        const size_t per = size_t(m + 1) * m + 2 * size_t(m) * m + 2 * size_t(m);
+        const size_t per = 2 * size_t(m + 1) * m + 3 * size_t(m) * m + 2 * size_t(m);
         rcQh = static_cast<c128*>(ctx_buf(ctx, SLOT_GIR_RC, size_t(B) * per * 16));
         if (!Ub || !rcQh) return CCAB_ENOMEM;
         rcRi = rcQh + size_t(B) * (m + 1) * m;
         rcB = rcRi + size_t(B) * m * m;
         rcC0 = rcB + size_t(B) * m * m;
         rcZ = rcC0 + size_t(B) * m;
+        rcCq = rcZ + size_t(B) * m;
+        rcUq = rcCq + size_t(B) * (m + 1) * m;
     }
+    // harmonic Ritz deflation directions (gir_rc_ritz_kernel)
+    const bool ritz = defl && ctx->gir_ritz && m <= kRitzMaxK && ctx->gir_recycle <= kRitzVecs;
     int s_rc = 0;               // deflation vectors in use (0: none yet)
     std::vector<int> k1(B, 0);  // steps of each system's first cycle
     // device Givens state (see gir_givens_kernel)
@@ -1581,10 +1751,13 @@ int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const vo
                 for (int32_t b : active) s_rc = std::max(s_rc, std::min(k1[b], ctx->gir_recycle));
                 hipLaunchKernelGGL(gir_rc_factors_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m,
                                    act, kdone_d, Hg, csd, snd, rcQh, rcRi);
+                if (ritz)
+                    hipLaunchKernelGGL(gir_rc_ritz_kernel, dim3((unsigned)na_c), dim3(64), 0, s, m,
+                                       ctx->gir_recycle, act, kdone_d, Hg, rcQh, rcRi, rcCq, rcUq);
                 hipLaunchKernelGGL(gir_rc_basis_kernel,
                                    dim3((unsigned)((n + kRcRows - 1) / kRcRows), (unsigned)na_c),
-                                   dim3(64), 0, s, n, m, s_rc, act, kdone_d, Q, sq, rcQh, rcRi, Ub,
-                                   n * int64_t(m));
+                                   dim3(64), 0, s, n, m, s_rc, act, kdone_d, Q, sq,
+                                   ritz ? rcCq : rcQh, ritz ? rcUq : rcRi, Ub, n * int64_t(m));
                 CCAB_HIP(hipGetLastError());
             }
             hipLaunchKernelGGL(scale_inv_norm_kernel, dim3((unsigned)((n + 255) / 256),

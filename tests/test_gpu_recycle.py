@@ -1,7 +1,8 @@
 """Batched GMRES-IR with the refinement cycles after the first deflated by the
 first cycle's Krylov space (GCRO: C = A U from that cycle's Arnoldi relation;
-its first 8 directions by default, CCAB_GIR_RECYCLE) against cycles restarted
-from scratch (CCAB_GIR_RECYCLE=0): all
+8 harmonic Ritz directions by default, or its first 8 directions
+(CCAB_GIR_RITZ=0), or all of it) against cycles restarted from scratch
+(CCAB_GIR_RECYCLE=0): all
 converge to the requested FP64 relative residual and agree with the FP64 LU
 to it, and the deflated cycles take fewer Arnoldi steps over a sweep.  Cases:
 the FP16 operator with the FP32 remainder (the residual carried through
@@ -31,14 +32,18 @@ def _rel(a, b):
 
 
 def _ctx(vecs: str):
-    if vecs == "64":     # through the environment at context creation
-        os.environ["CCAB_GIR_RECYCLE"] = vecs
-        try:
-            return capi.Context()
-        finally:
-            del os.environ["CCAB_GIR_RECYCLE"]
-    ctx = capi.Context()
-    ctx.set_gir_deflation(int(vecs))
+    """"8": 8 harmonic Ritz directions of the first cycle (the default);
+    "8-first": the first 8 directions of A K_k (CCAB_GIR_RITZ=0); "64": as many
+    as the first cycle's steps (through the environment); "0": none."""
+    env = {"64": {"CCAB_GIR_RECYCLE": "64"}, "8-first": {"CCAB_GIR_RITZ": "0"}}.get(vecs, {})
+    os.environ.update(env)
+    try:
+        ctx = capi.Context()
+    finally:
+        for key in env:
+            del os.environ[key]
+    if vecs in ("8", "0"):
+        ctx.set_gir_deflation(int(vecs))
     return ctx
 
 
@@ -61,11 +66,11 @@ def test_sweep_deflated_cycles(level, nf, tol, residual):
                  Solver(kind=capi.SOLVER_DIRECT))
     phi_d, _, _ = solve.sweep(pd)
     pd.solver = _ir(tol, residual)
-    # 8 deflation vectors (the default), as many as the first cycle's steps
-    # (several Gram-Schmidt blocks; zero vectors for the shorter systems),
-    # none
+    # 8 deflation vectors (harmonic Ritz, the default; or the first
+    # directions), as many as the first cycle's steps (several Gram-Schmidt
+    # blocks; zero vectors for the shorter systems), none
     out = {}
-    for vecs in ("8", "64", "0"):
+    for vecs in ("8", "8-first", "64", "0"):
         ctx = _ctx(vecs)
         try:
             out[vecs] = solve.sweep(pd, ctx=ctx)
@@ -81,7 +86,7 @@ def test_sweep_deflated_cycles(level, nf, tol, residual):
             assert _rel(phi[k], phi_d[k]) < bound, (vecs, k, _rel(phi[k], phi_d[k]))
     its = {v: sum(s["iterations"] for s in o[2]) for v, o in out.items()}
     _note(f"Arnoldi steps by deflation vectors: {its}")
-    assert its["8"] < its["0"] and its["64"] < its["0"], its
+    assert its["8"] < its["0"] and its["8-first"] < its["0"] and its["64"] < its["0"], its
 
 
 def test_bodies_deflated_cycles():
@@ -100,13 +105,13 @@ def test_bodies_deflated_cycles():
                               _ir(1e-10, capi.C32)))
     lu, _, _ = solve.solve_bodies(bodies, solver=Solver(kind=capi.SOLVER_DIRECT))
     out = {}
-    for vecs in ("8", "0"):
+    for vecs in ("8", "8-first", "0"):
         ctx = _ctx(vecs)
         try:
             out[vecs] = solve.solve_bodies(bodies, ctx=ctx)
         finally:
             ctx.close()
-    for vecs in ("8", "0"):
+    for vecs in ("8", "8-first", "0"):
         for p, ref, st in zip(out[vecs][0], lu, out[vecs][2]):
             assert st["converged"] and st["rel_residual"] <= 1e-10
             assert _rel(p, ref) < 1e-8, (vecs, _rel(p, ref))
