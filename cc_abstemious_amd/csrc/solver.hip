@@ -951,8 +951,12 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
                                                          const c128* __restrict__ Qh,
                                                          const c128* __restrict__ Ri, c128* Cq,
                                                          c128* Uq) {
+    // column-major in LDS ([column][row]): lane i reads row i of a column
+    // (consecutive addresses), a fixed entry is read by every lane (broadcast)
+    __shared__ c128 sA[kRitzMaxK][kRitzMaxK];       // R^-1, later R
+    __shared__ c128 sQ[kRitzMaxK][kRitzMaxK + 1];   // Qh, rows 0..k
     __shared__ c128 sW[kRitzMaxK][kRitzMaxK];
-    __shared__ c128 sZ[kRitzMaxK][kRitzVecs];
+    __shared__ c128 sZ[kRitzMaxK][kRitzVecs];       // [row][column]
     __shared__ c128 sR2[kRitzVecs * kRitzVecs];
     const int64_t b = act[blockIdx.x];
     const int k = kdone[b];
@@ -961,13 +965,18 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
     const int ld = m + 1;
     const c128* H = Hg + b * int64_t(ld) * m;      // R(i, j) = H[i + j ld]
     const c128* q = Qh + b * int64_t(ld) * m;      // Qh(i, c) = q[i + c ld]
-    const c128* ri = Ri + b * int64_t(m) * m;      // R^-1(i, c) = ri[i + c m], i <= c
-    // W = R^-1 Qh_top^H, row i per lane
+    const c128* ri = Ri + b * int64_t(m) * m;      // R^-1(i, c) = ri[i + c m]
+    for (int c = 0; c < k; ++c) {
+        if (i < k) sA[c][i] = ri[i + c * m];
+        if (i <= k) sQ[c][i] = q[i + c * ld];
+    }
+    __syncthreads();
+    // W = R^-1 Qh_top^H: W(i, j) = sum_l R^-1(i, l) conj(Qh(j, l)), l >= i
     if (i < k)
         for (int j = 0; j < k; ++j) {
             c128 acc{0.0, 0.0};
-            for (int l = i; l < k; ++l) acc += ri[i + l * m] * conj(q[j + l * ld]);
-            sW[i][j] = acc;
+            for (int l = i; l < k; ++l) acc += sA[l][i] * conj(sQ[l][j]);
+            sW[j][i] = acc;
         }
     c128 z[kRitzVecs];
 #pragma unroll
@@ -981,6 +990,7 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
         z[c] = (i < k && c < s) ? c128{a, bb} : c128{0.0, 0.0};
     }
     bool ok = wave_mgs(z, s, nullptr);
+    __syncthreads();   // sW complete
     for (int it = 0; it < kRitzIters && ok; ++it) {
         if (i < k)
 #pragma unroll
@@ -990,7 +1000,7 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
         for (int c = 0; c < kRitzVecs; ++c) z[c] = c128{0.0, 0.0};
         if (i < k)
             for (int j = 0; j < k; ++j) {
-                const c128 w = sW[i][j];
+                const c128 w = sW[j][i];
 #pragma unroll
                 for (int c = 0; c < kRitzVecs; ++c) z[c] += w * sZ[j][c];
             }
@@ -1001,6 +1011,8 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
     if (i < k)
 #pragma unroll
         for (int c = 0; c < kRitzVecs; ++c) sZ[i][c] = z[c];
+    for (int c = 0; c < k; ++c)
+        if (i < k) sA[c][i] = i <= c ? H[i + c * ld] : c128{0.0, 0.0};
     __syncthreads();
     c128 p[kRitzVecs];
 #pragma unroll
@@ -1010,7 +1022,7 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
     }
     if (i < k)
         for (int l = i; l < k; ++l) {
-            const c128 w = H[i + l * ld];
+            const c128 w = sA[l][i];
 #pragma unroll
             for (int c = 0; c < kRitzVecs; ++c) z[c] += w * sZ[l][c];
         }
@@ -1037,7 +1049,7 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
 #pragma unroll
         for (int c = 0; c < kRitzVecs; ++c) acc[c] = c128{0.0, 0.0};
         for (int l = 0; l < k; ++l) {
-            const c128 w = q[i + l * ld];
+            const c128 w = sQ[l][i];
 #pragma unroll
             for (int c = 0; c < kRitzVecs; ++c) acc[c] += w * sZ[l][c];
         }
