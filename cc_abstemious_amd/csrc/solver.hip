@@ -991,6 +991,8 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
     }
     bool ok = wave_mgs(z, s, nullptr);
     __syncthreads();   // sW complete
+    // (two products per orthonormalisation: the wave reductions of the
+    // Gram-Schmidt steps, not the products, set this kernel's time)
     for (int it = 0; it < kRitzIters && ok; ++it) {
         if (i < k)
 #pragma unroll
@@ -1005,7 +1007,7 @@ __global__ __launch_bounds__(64) void gir_rc_ritz_kernel(int m, int nvec,
                 for (int c = 0; c < kRitzVecs; ++c) z[c] += w * sZ[j][c];
             }
         __syncthreads();
-        ok = wave_mgs(z, s, nullptr);
+        if (it % 2 == 1 || it == kRitzIters - 1) ok = wave_mgs(z, s, nullptr);
     }
     // P = z; R P (rows i < k), then its thin QR: Q2 and R2
     if (i < k)
