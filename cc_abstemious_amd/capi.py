@@ -340,7 +340,7 @@ class Context:
     def set_gir_deflation(self, nvec: int):
         """Refinement cycles of the batched GMRES-IR after each system's first
         deflated by nvec directions of its first cycle's Krylov space (default
-        8); 0: every cycle restarts from scratch."""
+        8, harmonic Ritz directions); 0: every cycle restarts from scratch."""
         self.call("ccab_set_gir_deflation", int(nvec))
 
     def set_tuning(self, prec: int, variant: int, outer: int):

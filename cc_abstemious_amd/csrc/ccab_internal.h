@@ -229,9 +229,9 @@ struct ccab_ctx {
     // 4, or 1: one at a time (CCAB_GS_BLOCK)
     int gs_block = 8;
     // batched GMRES-IR: refinement cycles after the first run GCRO, deflated
-    // by the first this-many directions of A times the first cycle's Krylov
-    // space (C = A U from its Arnoldi relation; CCAB_GIR_RECYCLE, 0: every
-    // cycle restarts from scratch)
+    // by this many directions of the first cycle's Krylov space (C = A U
+    // from its Arnoldi relation; CCAB_GIR_RECYCLE, 0: every cycle restarts
+    // from scratch)
     int gir_recycle = 8;
     // the deflation directions: harmonic Ritz vectors of the first cycle
     // (restart lengths up to 32, up to 8 vectors); 0: the first directions

@@ -198,10 +198,13 @@ int ccab_set_warm_ax(ccab_ctx* ctx, int on);
 /* Batched GMRES-IR (no reference counterpart: the reference restarts GMRES
  * from scratch, gmres.rs:176-190): with nvec > 0 (default 8; environment
  * CCAB_GIR_RECYCLE) every refinement cycle after a system's first is
- * deflated by the first nvec directions of A times the first cycle's
- * Krylov space (GCRO: C = A U from that cycle's Arnoldi relation, no extra
- * matrix products); 0 restarts every cycle from scratch.  nvec <= 64;
- * restart lengths above 64 and HIP-graph cycles are never deflated. */
+ * deflated by nvec directions of the first cycle's Krylov space (GCRO:
+ * C = A U from that cycle's Arnoldi relation, no extra matrix products):
+ * its harmonic Ritz directions of smallest magnitude where the restart
+ * length is at most 32 and nvec at most 8 (environment CCAB_GIR_RITZ=0:
+ * never), else the first nvec directions of A K_k; 0 restarts every cycle
+ * from scratch.  nvec <= 64; restart lengths above 64 and HIP-graph cycles
+ * are never deflated. */
 int ccab_set_gir_deflation(ccab_ctx* ctx, int nvec);
 /* The autotuner: times the batched LU of `batch` synthetic n x n systems in
  * precision prec for every GEMM variant (0..8) and outer width (64, 128),
