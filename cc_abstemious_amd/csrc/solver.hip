@@ -1469,7 +1469,9 @@ int mixed_run(ccab_ctx* ctx, int64_t n, int64_t batch, const c128* A128, c64* A6
 // set.  Stops when ||b - A x|| <= tol ||b|| (FP64).  AX0 (with X0): A x0
 // formed by the caller (the assembly multiplies each FP64 entry by the guess
 // as it forms it), so the warm start's first residual b - A x0 reads no
-// matrix.
+// matrix.  The cycles after a system's first are deflated (GCRO) by a few
+// directions of the first cycle's Krylov space (gir_rc_* kernels,
+// ccab_set_gir_deflation), except in HIP-graph cycles.
 // ---------------------------------------------------------------------------
 int gmres_ir_run(ccab_ctx* ctx, int64_t n, int64_t B, const c128* A128, const void* Aop,
                  int op_prec, const double* d_inv_scale, int64_t lda, int64_t sA, c128* X,
